@@ -1,0 +1,474 @@
+// api.cpp -- the C-ABI (include/stq.h): contexts, options and the host
+// orchestration of one broad-phase call.
+//
+// Replaces the host body of the reference's run_sweep_sharedqueue
+// (src/simulation.cu:555-782): H2D copy, create_ds, sort, output sizing
+// (MemHandler, src/memory.cuh:69-191), twostage_queue, overflow loop.  All
+// errors are returned as status codes; scratch is grow-only and owned by the
+// context; the output is sized exactly by a count pass instead of from free
+// device memory.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "stq_internal.hpp"
+#include "sweep.hpp"
+
+namespace stqamd {
+
+enum Ev { E_START, E_KEYS, E_SORT, E_GATHER, E_PLAN, E_COUNT, E_SCAN, E_FILL, E_END, E_NUM };
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  stq_options opt{};
+  stq_stats stats{};
+  std::string last_error;
+  int cus = 256;
+  hipEvent_t ev[E_NUM] = {};
+
+  DevBuf keys, vals, smin, smax, geo, meta, full, hi, rowlist, groups, nchunks, ioff, items,
+      counts, offsets, scan_tmp, counters, axis_partial, h2d_boxes, d2h_pairs;
+  SortScratch sort;
+
+  ~Context() {
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    DevBuf *bufs[] = {&keys,  &vals,    &smin,    &smax,     &geo,          &meta,
+                      &full,  &hi,      &rowlist, &groups,   &nchunks,      &ioff,
+                      &items, &counts,  &offsets, &scan_tmp, &counters,     &axis_partial,
+                      &h2d_boxes, &d2h_pairs};
+    for (DevBuf *b : bufs) b->release();
+    sort.alt_keys.release();
+    sort.alt_vals.release();
+    sort.counts.release();
+    sort.hist.release();
+    sort.misc.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int pick_axis(const stq_aabb_f64 *boxes, int64_t n);
+  void run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t capacity,
+           stq_pair **alloc_out, int64_t *count);
+};
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int d) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != d) STQ_HIP(hipSetDevice(d));
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+static bool options_valid(const stq_options &o) {
+  return o.precision >= 0 && o.precision <= 1 && o.axis_policy >= 0 && o.axis_policy <= 4 &&
+         o.sizing >= 0 && o.sizing <= 1 && o.chunk >= 0 && o.window >= 0 &&
+         (o.window == 0 || (o.window >= 64 && o.window <= 2048 && (o.window & (o.window - 1)) == 0)) &&
+         o.waves_per_cu >= 0 && o.waves_per_cu <= 64;
+}
+
+int Context::pick_axis(const stq_aabb_f64 *boxes, int64_t n) {
+  const int pol = opt.axis_policy;
+  if (pol <= 2) return pol;
+  const int nb = (int)std::min<int64_t>((n + 255) / 256, 1024);
+  double *part = axis_partial.as<double>((size_t)nb * 12);
+  launch_axis_stats(boxes, n, part, nb, stream);
+  std::vector<double> h((size_t)nb * 12);
+  STQ_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
+  STQ_HIP(hipStreamSynchronize(stream));
+  double s[12] = {0};
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 12; ++k) s[k] += h[(size_t)b * 12 + k];
+  const double dn = (double)n;
+  int best = 0;
+  double bestv = 0;
+  for (int a = 0; a < 3; ++a) {
+    const double s1 = s[4 * a + 0], s2 = s[4 * a + 1], ext = s[4 * a + 2], cc = s[4 * a + 3];
+    double v;
+    if (pol == STQ_AXIS_VARIANCE) {
+      // sum over end points of (x - mu)^2, mu = mean centre (src/sweep.cu:93-123)
+      const double mu = s1 / (2.0 * dn);
+      v = s2 - 2.0 * mu * s1 + 2.0 * dn * mu * mu;
+      if (a == 0 || v > bestv) { best = a; bestv = v; }
+    } else {
+      const double mu = s1 / (2.0 * dn);
+      const double var = std::max(cc / dn - mu * mu, 0.0);
+      const double sd = std::sqrt(var);
+      v = sd > 0 ? (ext / dn) / sd : INFINITY;
+      if (a == 0 || v < bestv) { best = a; bestv = v; }
+    }
+  }
+  return best;
+}
+
+static float ev_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return ms;
+}
+
+void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t capacity,
+                  stq_pair **alloc_out, int64_t *count) {
+  stats = stq_stats{};
+  stats.n_boxes = n;
+  stats.n_scenes = 1;
+  stats.precision = opt.precision;
+  stats.sizing = opt.sizing;
+  const bool f32 = opt.precision == STQ_PRECISION_F32_CONFIRM;
+  hipStream_t st = stream;
+  STQ_HIP(hipEventRecord(ev[E_START], st));
+  if (n <= 1) {
+    *count = 0;
+    if (alloc_out) *alloc_out = static_cast<stq_pair *>(nullptr);
+    for (int e = E_KEYS; e < E_NUM; ++e) STQ_HIP(hipEventRecord(ev[e], st));
+    STQ_HIP(hipStreamSynchronize(st));
+    stats.axis = opt.axis_policy <= 2 ? opt.axis_policy : 0;
+    return;
+  }
+  // counters: [0] error flags, [1] candidates, [2] lane steps, [3] confirms
+  unsigned long long *ctr = counters.as<unsigned long long>(8);
+  STQ_HIP(hipMemsetAsync(ctr, 0, 8 * sizeof(unsigned long long), st));
+
+  const int axis = pick_axis(boxes, n);
+  stats.axis = axis;
+  uint32_t *order = vals.as<uint32_t>((size_t)n);
+  void *kbuf = keys.get((size_t)n * (f32 ? 4 : 8));
+  launch_keys(f32, boxes, n, axis, kbuf, order, (unsigned *)&ctr[0], st);
+  STQ_HIP(hipEventRecord(ev[E_KEYS], st));
+  int passes = 0;
+  if (f32)
+    radix_sort_u32((uint32_t *)kbuf, order, n, 32, sort, st, &passes);
+  else
+    radix_sort_u64((uint64_t *)kbuf, order, n, 64, sort, st, &passes);
+  STQ_HIP(hipEventRecord(ev[E_SORT], st));
+
+  const size_t ts = f32 ? 4 : 8;
+  void *d_smin = smin.get((size_t)n * ts);
+  void *d_smax = smax.get((size_t)n * ts);
+  void *d_geo = geo.get((size_t)n * (f32 ? sizeof(Geo32) : sizeof(Geo64)));
+  int4 *d_meta = meta.as<int4>((size_t)n);
+  Full64 *d_full = f32 ? full.as<Full64>((size_t)n) : nullptr;
+  int *d_hi = hi.as<int>((size_t)n);
+  launch_gather(f32, boxes, order, n, axis, d_smin, d_smax, d_geo, d_meta, d_full, st);
+  launch_chain_end(f32, d_smin, d_smax, n, d_hi, &ctr[1], st);
+  STQ_HIP(hipEventRecord(ev[E_GATHER], st));
+  unsigned long long hc[4];
+  STQ_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipStreamSynchronize(st));
+  if (hc[0]) throw StatusError(STQ_ERR_INVALID_BOX, "input box with NaN or min > max");
+  stats.n_candidates = (int64_t)hc[1];
+
+  // plan
+  int W = opt.window;
+  if (W == 0) {
+    const double mean = (double)hc[1] / (double)n;
+    W = 256;
+    while (W < 2048 && (double)(2 * W) <= mean / 8.0) W *= 2;
+  }
+  const int chunk = opt.chunk > 0 ? opt.chunk : 2048;
+  const int64_t ngroups = (n + 63) / 64;
+  int *d_rowlist = rowlist.as<int>((size_t)n);
+  int4 *d_groups = groups.as<int4>((size_t)ngroups);
+  uint32_t *d_nch = nchunks.as<uint32_t>((size_t)ngroups);
+  int64_t *d_ioff = ioff.as<int64_t>((size_t)ngroups + 1);
+  launch_plan(d_hi, n, W, chunk, d_rowlist, d_groups, d_nch, &ctr[2], st);
+  scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
+  int64_t nitems = 0;
+  STQ_HIP(hipMemcpyAsync(&nitems, d_ioff + ngroups, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipStreamSynchronize(st));
+  stats.n_items = nitems;
+  Item *d_items = items.as<Item>((size_t)std::max<int64_t>(nitems, 1));
+  launch_items(d_groups, d_ioff, ngroups, chunk, d_items, st);
+  STQ_HIP(hipEventRecord(ev[E_PLAN], st));
+
+  const int wpc = opt.waves_per_cu > 0 ? opt.waves_per_cu : 32;
+  const int nblocks = (int)std::max<int64_t>(
+      1, std::min<int64_t>((nitems + 3) / 4, (int64_t)cus * std::max(1, wpc / 4)));
+  uint32_t *d_counts = counts.as<uint32_t>((size_t)std::max<int64_t>(nitems, 1));
+  int64_t *d_off = offsets.as<int64_t>((size_t)nitems + 1);
+  launch_sweep(f32, false, nblocks, d_items, nitems, d_rowlist, d_hi, d_geo, d_meta, d_full,
+               order, d_counts, nullptr, nullptr, &ctr[3], st);
+  STQ_HIP(hipEventRecord(ev[E_COUNT], st));
+  scan_u32_to_i64(d_counts, d_off, nitems, scan_tmp, st);
+  int64_t total = 0;
+  STQ_HIP(hipMemcpyAsync(&total, d_off + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipEventRecord(ev[E_SCAN], st));
+  STQ_HIP(hipStreamSynchronize(st));
+  *count = total;
+  stats.n_pairs = total;
+  stq_pair *dst = out;
+  if (alloc_out) {
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(stq_pair) * (size_t)std::max<int64_t>(total, 1)) != hipSuccess) {
+      (void)hipGetLastError();
+      throw StatusError(STQ_ERR_OUT_OF_MEMORY, "output allocation failed");
+    }
+    dst = static_cast<stq_pair *>(p);
+    *alloc_out = dst;
+  } else if (total > capacity) {
+    throw StatusError(STQ_ERR_CAPACITY, "output capacity too small");
+  }
+  if (total > 0)
+    launch_sweep(f32, true, nblocks, d_items, nitems, d_rowlist, d_hi, d_geo, d_meta, d_full,
+                 order, nullptr, d_off, dst, nullptr, st);
+  STQ_HIP(hipEventRecord(ev[E_FILL], st));
+  STQ_HIP(hipEventRecord(ev[E_END], st));
+  STQ_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipStreamSynchronize(st));
+  stats.n_lane_steps = (int64_t)hc[2];
+  stats.n_confirm = (int64_t)hc[3];
+  stats.ms_total = ev_ms(ev[E_START], ev[E_END]);
+  stats.ms_keys = ev_ms(ev[E_START], ev[E_KEYS]);
+  stats.ms_sort = ev_ms(ev[E_KEYS], ev[E_SORT]);
+  stats.ms_gather = ev_ms(ev[E_SORT], ev[E_GATHER]);
+  stats.ms_plan = ev_ms(ev[E_GATHER], ev[E_PLAN]);
+  stats.ms_sweep_count = ev_ms(ev[E_PLAN], ev[E_COUNT]);
+  stats.ms_scan = ev_ms(ev[E_COUNT], ev[E_SCAN]);
+  stats.ms_sweep_fill = ev_ms(ev[E_SCAN], ev[E_FILL]);
+}
+
+template <class F>
+static int guarded(Context *c, F &&f) {
+  if (!c) return STQ_ERR_INVALID_ARGUMENT;
+  try {
+    DeviceGuard g(c->device);
+    f();
+    c->last_error.clear();
+    return STQ_OK;
+  } catch (const StatusError &e) {
+    c->last_error = e.what;
+    return e.status;
+  } catch (const HipError &e) {
+    c->last_error = e.what;
+    (void)hipGetLastError();
+    return STQ_ERR_HIP;
+  } catch (const std::bad_alloc &) {
+    c->last_error = "host allocation failed";
+    return STQ_ERR_OUT_OF_MEMORY;
+  } catch (...) {
+    c->last_error = "internal error";
+    return STQ_ERR_INTERNAL;
+  }
+}
+
+}  // namespace stqamd
+
+using stqamd::Context;
+
+extern "C" {
+
+int stq_options_default(stq_options *opt) {
+  if (!opt) return STQ_ERR_INVALID_ARGUMENT;
+  std::memset(opt, 0, sizeof(*opt));
+  opt->precision = STQ_PRECISION_F64;
+  opt->axis_policy = STQ_AXIS_X;
+  opt->sizing = STQ_SIZING_COUNT_FILL;
+  return STQ_OK;
+}
+
+int stq_create(void **ctx, int device, const stq_options *opt) {
+  if (!ctx) return STQ_ERR_INVALID_ARGUMENT;
+  *ctx = nullptr;
+  stq_options o;
+  stq_options_default(&o);
+  if (opt) o = *opt;
+  if (!stqamd::options_valid(o)) return STQ_ERR_INVALID_ARGUMENT;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    (void)hipGetLastError();
+    return ndev == 0 ? STQ_ERR_HIP : STQ_ERR_INVALID_ARGUMENT;
+  }
+  Context *c = new (std::nothrow) Context();
+  if (!c) return STQ_ERR_OUT_OF_MEMORY;
+  c->device = device;
+  c->opt = o;
+  int rc = stqamd::guarded(c, [&] {
+    STQ_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto &e : c->ev) STQ_HIP(hipEventCreate(&e));
+    c->cus = stqamd::device_cu_count(device);
+  });
+  if (rc != STQ_OK) {
+    delete c;
+    return rc;
+  }
+  *ctx = c;
+  return STQ_OK;
+}
+
+int stq_destroy(void *ctx) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c) return STQ_ERR_INVALID_ARGUMENT;
+  {
+    stqamd::DeviceGuard g(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c;
+  }
+  return STQ_OK;
+}
+
+int stq_set_options(void *ctx, const stq_options *opt) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !opt || !stqamd::options_valid(*opt)) return STQ_ERR_INVALID_ARGUMENT;
+  c->opt = *opt;
+  return STQ_OK;
+}
+
+int stq_find_pairs_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n, stq_pair *d_out,
+                          int64_t capacity, int64_t *count) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !count || n < 0 || capacity < 0 || (n > 0 && !d_boxes) || (capacity > 0 && !d_out))
+    return STQ_ERR_INVALID_ARGUMENT;
+  if (n > INT32_MAX) return STQ_ERR_TOO_LARGE;
+  return stqamd::guarded(c, [&] { c->run(d_boxes, n, d_out, capacity, nullptr, count); });
+}
+
+int stq_find_pairs_device_alloc(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n,
+                                stq_pair **d_out, int64_t *count) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !count || !d_out || n < 0 || (n > 0 && !d_boxes)) return STQ_ERR_INVALID_ARGUMENT;
+  if (n > INT32_MAX) return STQ_ERR_TOO_LARGE;
+  *d_out = nullptr;
+  return stqamd::guarded(c, [&] {
+    c->run(d_boxes, n, nullptr, 0, d_out, count);
+    if (!*d_out) {
+      void *p = nullptr;
+      STQ_HIP(hipMalloc(&p, sizeof(stq_pair)));
+      *d_out = static_cast<stq_pair *>(p);
+    }
+  });
+}
+
+int stq_find_pairs_host(void *ctx, const stq_aabb_f64 *h_boxes, int64_t n, stq_pair **h_out,
+                        int64_t *count) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !count || !h_out || n < 0 || (n > 0 && !h_boxes)) return STQ_ERR_INVALID_ARGUMENT;
+  if (n > INT32_MAX) return STQ_ERR_TOO_LARGE;
+  *h_out = nullptr;
+  return stqamd::guarded(c, [&] {
+    stq_aabb_f64 *db = c->h2d_boxes.as<stq_aabb_f64>((size_t)std::max<int64_t>(n, 1));
+    if (n > 0)
+      STQ_HIP(hipMemcpyAsync(db, h_boxes, sizeof(stq_aabb_f64) * n, hipMemcpyHostToDevice,
+                             c->stream));
+    // first pass sizes the output exactly, so a retry never happens
+    int64_t cnt = 0;
+    stq_pair *dout = nullptr;
+    c->run(db, n, nullptr, 0, &dout, &cnt);
+    stq_pair *h = static_cast<stq_pair *>(std::malloc(sizeof(stq_pair) * (size_t)std::max<int64_t>(cnt, 1)));
+    if (!h) {
+      if (dout) (void)hipFree(dout);
+      throw std::bad_alloc();
+    }
+    if (cnt > 0) {
+      hipError_t e = hipMemcpyAsync(h, dout, sizeof(stq_pair) * cnt, hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      (void)hipFree(dout);
+      if (e != hipSuccess) {
+        std::free(h);
+        throw stqamd::HipError(e, "hipMemcpyAsync(D2H pairs)", __FILE__, __LINE__);
+      }
+    } else if (dout) {
+      (void)hipFree(dout);
+    }
+    *h_out = h;
+    *count = cnt;
+  });
+}
+
+int stq_find_pairs_batched_device(void *ctx, const stq_aabb_f64 *d_boxes,
+                                  const int64_t *h_scene_offsets, int32_t nscenes,
+                                  stq_pair *d_out, int64_t capacity, int64_t *h_pair_offsets) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || nscenes < 0 || !h_scene_offsets || !h_pair_offsets || capacity < 0 ||
+      (capacity > 0 && !d_out))
+    return STQ_ERR_INVALID_ARGUMENT;
+  for (int s = 0; s < nscenes; ++s) {
+    const int64_t m = h_scene_offsets[s + 1] - h_scene_offsets[s];
+    if (m < 0 || h_scene_offsets[s] < 0) return STQ_ERR_INVALID_ARGUMENT;
+    if (m > INT32_MAX) return STQ_ERR_TOO_LARGE;
+    if (m > 0 && !d_boxes) return STQ_ERR_INVALID_ARGUMENT;
+  }
+  return stqamd::guarded(c, [&] {
+    h_pair_offsets[0] = 0;
+    bool overflow = false;
+    double ms = 0;
+    int64_t nb = 0, cand = 0;
+    for (int s = 0; s < nscenes; ++s) {
+      const int64_t m = h_scene_offsets[s + 1] - h_scene_offsets[s];
+      const int64_t at = h_pair_offsets[s];
+      int64_t cnt = 0;
+      const int64_t room = overflow ? 0 : std::max<int64_t>(capacity - at, 0);
+      try {
+        c->run(d_boxes + h_scene_offsets[s], m, room > 0 ? d_out + at : nullptr, room, nullptr, &cnt);
+      } catch (const stqamd::StatusError &e) {
+        if (e.status != STQ_ERR_CAPACITY) throw;
+        overflow = true;
+        cnt = c->stats.n_pairs;
+      }
+      h_pair_offsets[s + 1] = at + cnt;
+      ms += c->stats.ms_total;
+      nb += m;
+      cand += c->stats.n_candidates;
+    }
+    c->stats.n_scenes = nscenes;
+    c->stats.n_boxes = nb;
+    c->stats.n_pairs = h_pair_offsets[nscenes];
+    c->stats.n_candidates = cand;
+    c->stats.ms_total = ms;
+    if (overflow) throw stqamd::StatusError(STQ_ERR_CAPACITY, "output capacity too small");
+  });
+}
+
+int stq_free_device(void *ctx, void *d_ptr) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c) return STQ_ERR_INVALID_ARGUMENT;
+  if (!d_ptr) return STQ_OK;
+  return stqamd::guarded(c, [&] { STQ_HIP(hipFree(d_ptr)); });
+}
+
+void stq_free_host(void *h_ptr) { std::free(h_ptr); }
+
+int stq_get_stats(void *ctx, stq_stats *stats) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !stats) return STQ_ERR_INVALID_ARGUMENT;
+  *stats = c->stats;
+  return STQ_OK;
+}
+
+const char *stq_last_error(void *ctx) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c) return "null context";
+  return c->last_error.c_str();
+}
+
+const char *stq_status_string(int status) {
+  switch (status) {
+    case STQ_OK: return "ok";
+    case STQ_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case STQ_ERR_INVALID_BOX: return "invalid box (NaN or min > max)";
+    case STQ_ERR_CAPACITY: return "output capacity too small";
+    case STQ_ERR_OUT_OF_MEMORY: return "out of memory";
+    case STQ_ERR_HIP: return "HIP runtime error";
+    case STQ_ERR_TOO_LARGE: return "scene too large";
+    case STQ_ERR_INTERNAL: return "internal error";
+    default: return "unknown status";
+  }
+}
+
+int stq_sizeof_aabb(void) { return (int)sizeof(stq_aabb_f64); }
+int stq_sizeof_options(void) { return (int)sizeof(stq_options); }
+int stq_sizeof_stats(void) { return (int)sizeof(stq_stats); }
+
+}  // extern "C"

@@ -1,0 +1,167 @@
+// stq_internal.hpp -- shared definitions of the gfx950 STQ broad phase.
+//
+// Device-side data layout (all by sorted position s along the sweep axis):
+//   orig[s]    int32   input position of the box at s (the sort's values)
+//   smin[s]    T       sweep-axis minimum (T = double, or float rounded down)
+//   smax[s]    T       sweep-axis maximum (T = double, or float rounded up)
+//   geo[s]     4 x T   {min, max} on the two other axes
+//   meta[s]    int4    {v0, v1, v2, typebit}; typebit 1 vertex, 2 edge, 4 face
+//   full[s]    6 x f64 the FP64 box (FP32-confirm form only)
+//   hi[s]      int32   last sorted position q with smin[q] <= smax[s]
+// The reference keeps the same split as Scalar2 sm[] plus MiniBox mini[]
+// (src/sweep.cu:125-161); here the other-axis bounds and the ids are separate
+// arrays so that the sweep's hot loop fetches 16 or 32 bytes per step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/stq.h"
+
+namespace stqamd {
+
+constexpr int kWave = 64;
+
+// Type bits of a vertex-id triple (src/aabb.cu:12-30).
+__host__ __device__ inline int type_bit(int v1, int v2) {
+  return v2 >= 0 ? 4 : (v1 >= 0 ? 2 : 1);
+}
+// Type bits a row of type `bit` pairs with (is_valid_pair, src/aabb.cu:37-40):
+// vertex-face, face-vertex, edge-edge.
+__host__ __device__ inline int accept_bits(int bit) {
+  return bit == 1 ? 4 : (bit == 4 ? 1 : 2);
+}
+
+// Order-preserving unsigned images of IEEE values (negative values have all
+// bits flipped, non-negative values the sign bit set).  -0.0 is mapped like
+// +0.0 so that equal values have equal keys.
+__host__ __device__ inline uint64_t key_of(double d) {
+  d = d + 0.0;
+  uint64_t u;
+  __builtin_memcpy(&u, &d, 8);
+  return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__host__ __device__ inline uint32_t key_of(float f) {
+  f = f + 0.0f;
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// FP32 outward rounding (src/aabb.cu:56-61): round to nearest, then one ulp
+// toward -inf / +inf, i.e. nextafterf(f, -inf) / nextafterf(f, +inf) written
+// on the bit pattern.  Inputs are never NaN (validated before).
+__host__ __device__ inline float next_down(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) == 0) u = 0x80000001u;          // +-0 -> -denorm_min
+  else if (u == 0xff800000u) {}                          // -inf stays
+  else if (u & 0x80000000u) u += 1;                      // negative: away from 0
+  else u -= 1;                                           // positive (incl. +inf)
+  float r;
+  __builtin_memcpy(&r, &u, 4);
+  return r;
+}
+__host__ __device__ inline float next_up(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) == 0) u = 0x00000001u;          // +-0 -> +denorm_min
+  else if (u == 0x7f800000u) {}                          // +inf stays
+  else if (u & 0x80000000u) u -= 1;                      // negative (incl. -inf)
+  else u += 1;
+  float r;
+  __builtin_memcpy(&r, &u, 4);
+  return r;
+}
+__host__ __device__ inline float round_down_f32(double d) { return next_down((float)d); }
+__host__ __device__ inline float round_up_f32(double d) { return next_up((float)d); }
+
+struct Item {   // one sweep work item: up to 64 rows against a range of q
+  int rowoff;   // first entry of the group's rows in rowlist[]
+  int nrows;    // rows in the group (<= 64)
+  int q0, q1;   // inclusive q range of this item
+};
+
+// Error flags raised by kernels.
+enum : unsigned { kErrNaN = 1u, kErrInverted = 2u };
+
+#define STQ_HIP(call)                                                         \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) throw ::stqamd::HipError(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+struct HipError {
+  hipError_t code;
+  std::string what;
+  HipError(hipError_t c, const char *call, const char *file, int line) : code(c) {
+    char buf[512];
+    snprintf(buf, sizeof(buf), "%s failed: %s (%s:%d)", call, hipGetErrorString(c), file, line);
+    what = buf;
+  }
+};
+
+struct StatusError {
+  int status;
+  std::string what;
+  StatusError(int s, std::string w) : status(s), what(std::move(w)) {}
+};
+
+// Grow-only device buffer.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void *get(size_t need) {
+    if (need == 0) need = 256;
+    if (need > bytes) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      bytes = 0;
+      size_t want = need + need / 8;
+      if (hipMalloc(&p, want) != hipSuccess) {
+        (void)hipGetLastError();
+        if (hipMalloc(&p, need) != hipSuccess) {
+          (void)hipGetLastError();
+          p = nullptr;
+          throw StatusError(STQ_ERR_OUT_OF_MEMORY, "device allocation failed");
+        }
+        want = need;
+      }
+      bytes = want;
+    }
+    return p;
+  }
+  template <class T> T *as(size_t count) { return static_cast<T *>(get(count * sizeof(T))); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// ---- radix sort / scan (radix_sort.hip) ---------------------------------
+// Stable LSD radix sort of (key, value) pairs by `bits` low key bits;
+// digits whose value is the same for every key are skipped.  Result is in
+// keys/vals (the alternate buffers are scratch).
+struct SortScratch {
+  DevBuf alt_keys, alt_vals, counts, hist, misc;
+};
+void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bits,
+                    SortScratch &s, hipStream_t st, int *passes_run);
+void radix_sort_u32(uint32_t *keys, uint32_t *vals, int64_t n, int bits,
+                    SortScratch &s, hipStream_t st, int *passes_run);
+
+// Exclusive prefix sum of n uint32 counts into n+1 int64 offsets (out[n] is
+// the total).  Scratch is taken from `tmp`.
+void scan_u32_to_i64(const uint32_t *in, int64_t *out, int64_t n, DevBuf &tmp,
+                     hipStream_t st);
+// Exclusive prefix sum of uint32 into uint32 (n+1 outputs), for sort counts.
+void scan_u32(const uint32_t *in, uint32_t *out, int64_t n, DevBuf &tmp,
+              hipStream_t st);
+
+int device_cu_count(int device);
+
+}  // namespace stqamd

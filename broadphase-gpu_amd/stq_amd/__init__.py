@@ -1,0 +1,160 @@
+"""Python front of the gfx950 STQ broad phase (libstq_amd.so).
+
+Mirrors the reference entry ``stq::gpu::run_sweep_sharedqueue``
+(src/simulation.cuh:22-27): boxes in the reference FP64 ``Aabb`` layout go in,
+the overlapping box-id pairs come out.  All work happens in the HIP library;
+this module only moves numpy arrays across the C-ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (AXIS_CHAIN, AXIS_VARIANCE, AXIS_X, AXIS_Y, AXIS_Z,  # noqa: F401
+                   PRECISION_F32_CONFIRM, PRECISION_F64, SIZING_COUNT_FILL,
+                   SIZING_LEARNED, DeviceBuffer, Options, Stats)
+from .scenes import AABB_DTYPE  # noqa: F401
+
+PAIR_DTYPE = np.dtype([("first", "<i4"), ("second", "<i4")])
+
+
+class StqError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{message} (status {status})")
+        self.status = status
+
+
+def options(precision=PRECISION_F64, axis_policy=AXIS_X, sizing=SIZING_COUNT_FILL,
+            autotune=0, chunk=0, window=0, waves_per_cu=0) -> Options:
+    o = Options()
+    _lib.lib().stq_options_default(ctypes.byref(o))
+    o.precision, o.axis_policy, o.sizing = precision, axis_policy, sizing
+    o.autotune, o.chunk, o.window, o.waves_per_cu = autotune, chunk, window, waves_per_cu
+    return o
+
+
+class Context:
+    """One library context: a device, a stream, scratch and tuning state."""
+
+    def __init__(self, device=0, opts: Options | None = None, **kw):
+        L = _lib.lib()
+        self._L = L
+        self.ptr = ctypes.c_void_p()
+        o = opts if opts is not None else options(**kw)
+        rc = L.stq_create(ctypes.byref(self.ptr), device, ctypes.byref(o))
+        if rc != 0:
+            raise StqError(rc, "stq_create: " + L.stq_status_string(rc).decode())
+
+    def close(self):
+        if self.ptr is not None and self.ptr.value:
+            self._L.stq_destroy(self.ptr)
+        self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._L.stq_last_error(self.ptr).decode()
+            raise StqError(rc, f"{what}: {self._L.stq_status_string(rc).decode()}: {msg}")
+
+    def set_options(self, opts: Options | None = None, **kw):
+        o = opts if opts is not None else options(**kw)
+        self._check(self._L.stq_set_options(self.ptr, ctypes.byref(o)), "stq_set_options")
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._check(self._L.stq_get_stats(self.ptr, ctypes.byref(s)), "stq_get_stats")
+        return s.as_dict()
+
+    # -- device-resident form -------------------------------------------------
+    def find_pairs_device(self, d_boxes, n, d_out, capacity):
+        """Device boxes in, device pairs out (caller buffer). Returns the count."""
+        cnt = ctypes.c_int64(0)
+        rc = self._L.stq_find_pairs_device(self.ptr, d_boxes, n, d_out, capacity, ctypes.byref(cnt))
+        if rc == _lib.STQ_ERR_CAPACITY:
+            raise StqError(rc, f"capacity {capacity} < {cnt.value} pairs")
+        self._check(rc, "stq_find_pairs_device")
+        return cnt.value
+
+    def find_pairs(self, boxes: np.ndarray) -> np.ndarray:
+        """Host numpy boxes in; (P, 2) int32 canonical pairs out (device path,
+        library-allocated output, like the reference's d_overlaps)."""
+        b = np.ascontiguousarray(boxes)
+        assert b.dtype.itemsize == 72
+        db = DeviceBuffer.from_array(b)
+        out = ctypes.c_void_p()
+        cnt = ctypes.c_int64(0)
+        try:
+            self._check(self._L.stq_find_pairs_device_alloc(self.ptr, db.ptr, len(b), ctypes.byref(out),
+                                                            ctypes.byref(cnt)),
+                        "stq_find_pairs_device_alloc")
+            res = np.empty((cnt.value, 2), dtype=np.int32)
+            if cnt.value:
+                _lib._check_hip(_lib.hip().hipMemcpy(res.ctypes.data, out, res.nbytes, _lib.HIP_D2H),
+                                "hipMemcpy D2H")
+        finally:
+            if out.value:
+                self._L.stq_free_device(self.ptr, out)
+            db.close()
+        return res
+
+    def find_pairs_host(self, boxes: np.ndarray) -> np.ndarray:
+        b = np.ascontiguousarray(boxes)
+        out = ctypes.c_void_p()
+        cnt = ctypes.c_int64(0)
+        self._check(self._L.stq_find_pairs_host(self.ptr, b.ctypes.data, len(b), ctypes.byref(out),
+                                                ctypes.byref(cnt)), "stq_find_pairs_host")
+        try:
+            if cnt.value:
+                res = np.ctypeslib.as_array(ctypes.cast(out, ctypes.POINTER(ctypes.c_int32)),
+                                            shape=(cnt.value, 2)).copy()
+            else:
+                res = np.zeros((0, 2), dtype=np.int32)
+        finally:
+            self._L.stq_free_host(out)
+        return res
+
+    def find_pairs_batched(self, boxes: np.ndarray, scene_offsets: np.ndarray, capacity=None):
+        """Scenes back to back; returns (pairs (P,2), pair_offsets (S+1))."""
+        b = np.ascontiguousarray(boxes)
+        offs = np.ascontiguousarray(scene_offsets, dtype=np.int64)
+        ns = len(offs) - 1
+        poffs = np.zeros(ns + 1, dtype=np.int64)
+        db = DeviceBuffer.from_array(b)
+        cap = int(capacity) if capacity is not None else 1 << 20
+        try:
+            while True:
+                dout = DeviceBuffer(8 * max(cap, 1))
+                rc = self._L.stq_find_pairs_batched_device(self.ptr, db.ptr, offs.ctypes.data, ns, dout.ptr,
+                                                           cap, poffs.ctypes.data)
+                if rc == _lib.STQ_ERR_CAPACITY and capacity is None:
+                    cap = int(poffs[-1])
+                    dout.close()
+                    continue
+                self._check(rc, "stq_find_pairs_batched_device")
+                res = dout.to_array(np.int32, 2 * int(poffs[-1])).reshape(-1, 2)
+                dout.close()
+                return res, poffs
+        finally:
+            db.close()
+
+
+def canonical_sort(pairs: np.ndarray) -> np.ndarray:
+    """Lexicographic order of (first, second) pairs (numpy, for comparisons)."""
+    p = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
+    if len(p) == 0:
+        return p.copy()
+    key = (p[:, 0].astype(np.int64) << 32) | p[:, 1].astype(np.int64)
+    return p[np.argsort(key, kind="stable")]

@@ -1,0 +1,200 @@
+"""ctypes binding of libstq_amd.so (the C-ABI declared in include/stq.h) and a
+minimal HIP runtime binding for device buffers.
+
+The library is loaded from ``broadphase-gpu_amd/lib/libstq_amd.so``; there is
+no fallback: if it is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libstq_amd.so")
+
+STQ_OK = 0
+STQ_ERR_INVALID_ARGUMENT = 1
+STQ_ERR_INVALID_BOX = 2
+STQ_ERR_CAPACITY = 3
+STQ_ERR_OUT_OF_MEMORY = 4
+STQ_ERR_HIP = 5
+STQ_ERR_TOO_LARGE = 6
+STQ_ERR_INTERNAL = 7
+
+PRECISION_F64 = 0
+PRECISION_F32_CONFIRM = 1
+AXIS_X, AXIS_Y, AXIS_Z, AXIS_VARIANCE, AXIS_CHAIN = 0, 1, 2, 3, 4
+SIZING_COUNT_FILL = 0
+SIZING_LEARNED = 1
+
+# every entry point of include/stq.h
+EXPORTS = [
+    "stq_options_default", "stq_create", "stq_destroy", "stq_set_options",
+    "stq_find_pairs_device", "stq_find_pairs_device_alloc", "stq_find_pairs_host",
+    "stq_find_pairs_batched_device", "stq_free_device", "stq_free_host",
+    "stq_get_stats", "stq_last_error", "stq_status_string",
+    "stq_sizeof_aabb", "stq_sizeof_options", "stq_sizeof_stats",
+]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int32),
+        ("axis_policy", ctypes.c_int32),
+        ("sizing", ctypes.c_int32),
+        ("autotune", ctypes.c_int32),
+        ("chunk", ctypes.c_int32),
+        ("window", ctypes.c_int32),
+        ("waves_per_cu", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 9),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("n_boxes", ctypes.c_int64),
+        ("n_pairs", ctypes.c_int64),
+        ("n_candidates", ctypes.c_int64),
+        ("n_lane_steps", ctypes.c_int64),
+        ("n_items", ctypes.c_int64),
+        ("n_confirm", ctypes.c_int64),
+        ("n_tiles", ctypes.c_int64),
+        ("n_fallback_tiles", ctypes.c_int64),
+        ("n_scenes", ctypes.c_int64),
+        ("axis", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("sizing", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("ms_total", ctypes.c_double),
+        ("ms_keys", ctypes.c_double),
+        ("ms_sort", ctypes.c_double),
+        ("ms_gather", ctypes.c_double),
+        ("ms_plan", ctypes.c_double),
+        ("ms_sweep_count", ctypes.c_double),
+        ("ms_scan", ctypes.c_double),
+        ("ms_sweep_fill", ctypes.c_double),
+        ("ms_compact", ctypes.c_double),
+        ("ms_predict", ctypes.c_double),
+        ("sizing_mae", ctypes.c_double),
+        ("reserved", ctypes.c_double * 6),
+    ]
+
+    def as_dict(self):
+        return {k: (getattr(self, k)) for k, _ in self._fields_ if k not in ("reserved", "pad")}
+
+
+_LIB = None
+
+
+def lib():
+    """Load libstq_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I32, I64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    sig = {
+        "stq_options_default": (ctypes.c_int, [ctypes.POINTER(Options)]),
+        "stq_create": (ctypes.c_int, [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(Options)]),
+        "stq_destroy": (ctypes.c_int, [P]),
+        "stq_set_options": (ctypes.c_int, [P, ctypes.POINTER(Options)]),
+        "stq_find_pairs_device": (ctypes.c_int, [P, P, I64, P, I64, ctypes.POINTER(I64)]),
+        "stq_find_pairs_device_alloc": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
+        "stq_find_pairs_host": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
+        "stq_find_pairs_batched_device": (ctypes.c_int, [P, P, P, I32, P, I64, P]),
+        "stq_free_device": (ctypes.c_int, [P, P]),
+        "stq_free_host": (None, [P]),
+        "stq_get_stats": (ctypes.c_int, [P, ctypes.POINTER(Stats)]),
+        "stq_last_error": (ctypes.c_char_p, [P]),
+        "stq_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+        "stq_sizeof_aabb": (ctypes.c_int, []),
+        "stq_sizeof_options": (ctypes.c_int, []),
+        "stq_sizeof_stats": (ctypes.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    assert L.stq_sizeof_aabb() == 72
+    assert L.stq_sizeof_options() == ctypes.sizeof(Options)
+    assert L.stq_sizeof_stats() == ctypes.sizeof(Stats)
+    _LIB = L
+    return L
+
+
+# ---- HIP runtime (device buffers for callers that hold boxes on the GPU) ----
+_HIP = None
+HIP_H2D, HIP_D2H, HIP_D2D = 1, 2, 3
+
+
+def hip():
+    global _HIP
+    if _HIP is None:
+        H = ctypes.CDLL("libamdhip64.so")
+        P, S = ctypes.c_void_p, ctypes.c_size_t
+        H.hipMalloc.argtypes = [ctypes.POINTER(P), S]
+        H.hipMalloc.restype = ctypes.c_int
+        H.hipFree.argtypes = [P]
+        H.hipFree.restype = ctypes.c_int
+        H.hipMemcpy.argtypes = [P, P, S, ctypes.c_int]
+        H.hipMemcpy.restype = ctypes.c_int
+        H.hipDeviceSynchronize.restype = ctypes.c_int
+        H.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        H.hipGetDeviceCount.restype = ctypes.c_int
+        H.hipSetDevice.argtypes = [ctypes.c_int]
+        H.hipSetDevice.restype = ctypes.c_int
+        _HIP = H
+    return _HIP
+
+
+def _check_hip(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with hipError {rc}")
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    rc = hip().hipGetDeviceCount(ctypes.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class DeviceBuffer:
+    """A hipMalloc'd buffer; freed on close()/garbage collection."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        _check_hip(hip().hipMalloc(ctypes.byref(self.ptr), max(self.nbytes, 1)), "hipMalloc")
+
+    @classmethod
+    def from_array(cls, arr):
+        import numpy as np
+        a = np.ascontiguousarray(arr)
+        b = cls(a.nbytes)
+        if a.nbytes:
+            _check_hip(hip().hipMemcpy(b.ptr, a.ctypes.data, a.nbytes, HIP_H2D), "hipMemcpy H2D")
+        return b
+
+    def to_array(self, dtype, count):
+        import numpy as np
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            _check_hip(hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, HIP_D2H), "hipMemcpy D2H")
+        return out
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+    def close(self):
+        if self.ptr is not None and self.ptr.value:
+            hip().hipFree(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

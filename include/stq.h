@@ -1,0 +1,220 @@
+/*
+ * stq.h -- C-ABI of the MI355X (gfx950) sweep-and-tiniest-queue broad phase.
+ *
+ * Drop-in boundary for the reference GPU entry
+ *   stq::gpu::run_sweep_sharedqueue(const Aabb *boxes, MemHandler *memhandle,
+ *       int N, int nbox, std::vector<std::pair<int,int>> &finOverlaps,
+ *       int2 *&d_overlaps, int *&d_count, int &threads, int &tidstart,
+ *       int &devcount, const int memlimit)
+ *   (reference src/simulation.cuh:22-27, body src/simulation.cu:555-782)
+ * and the CPU entry it mirrors,
+ *   stq::cpu::sweep_cpu_single_batch (src/cpu/sweep.hpp:19-20).
+ *
+ * Plain C types only: no HIP, CUDA or torch types cross this boundary.  Device
+ * pointers are plain `void *`-compatible pointers into the current device's
+ * memory.  Every function returns an int status (enum stq_status); nothing
+ * calls exit() and no exception crosses the ABI (the reference exits on GPU
+ * errors, src/memory.cuh:11-19).  A context owns one device ordinal, one HIP
+ * stream, grow-only scratch, the autotune table and the sizing network; calls
+ * on one context must be serialised by the caller, distinct contexts may be
+ * used from distinct threads.  Every call is synchronous: it returns when its
+ * results are complete (the reference synchronises too, src/simulation.cu:694).
+ *
+ * Pair semantics (identical to the reference):
+ *   boxes a != b form a pair iff
+ *     (1) they overlap as closed intervals on all three axes (a.min <= b.max
+ *         and b.min <= a.max per axis; src/sweep.cu:245,269 and
+ *         src/collision.cu:38-42),
+ *     (2) is_valid_pair: (vertex,face), (face,vertex) or (edge,edge), where
+ *         face <=> v[2] >= 0, edge <=> v[2] < 0 && v[1] >= 0,
+ *         vertex <=> v[2] < 0 && v[1] < 0 (src/aabb.cu:12-40),
+ *     (3) they share no vertex id: none of the 9 equalities a.v[s] == b.v[t]
+ *         (src/collision.cu:44-48).
+ *   Ids are input positions, as the reference GPU path emits
+ *   (src/sweep.cu:160).  Output pairs are canonical: first < second.  The
+ *   order of pairs in the output is deterministic for a given input and
+ *   options; a caller that wants lexicographic order sorts them.
+ *   Boxes must have no NaN and min <= max on every axis, else the call
+ *   returns STQ_ERR_INVALID_BOX.
+ */
+#ifndef STQ_H
+#define STQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The reference FP64 box, byte for byte (stq::gpu::Aabb with CCD_USE_DOUBLE,
+   src/aabb.cuh:15-43): a std::vector<stq::gpu::Aabb> buffer of that build can
+   be passed as-is. */
+typedef struct stq_aabb_f64 {
+  int32_t id;            /* offset  0 (ignored: ids are positions)        */
+  int32_t pad0;          /* offset  4 (alignment padding)                 */
+  double min[3];         /* offset  8                                     */
+  double max[3];         /* offset 32                                     */
+  int32_t vertex_ids[3]; /* offset 56: vertex {v,-v-1,-v-1}, edge {a,b,-a-1},
+                            face {a,b,c}                                  */
+  int32_t ref_id;        /* offset 68 (ignored)                           */
+} stq_aabb_f64;          /* 72 bytes */
+
+/* One output pair; the reference's int2 (src/collision.cu:58). */
+typedef struct stq_pair {
+  int32_t first;
+  int32_t second;
+} stq_pair;
+
+enum stq_status {
+  STQ_OK = 0,
+  STQ_ERR_INVALID_ARGUMENT = 1, /* null pointer, negative size, bad option   */
+  STQ_ERR_INVALID_BOX = 2,      /* NaN, or min > max on some axis            */
+  STQ_ERR_CAPACITY = 3,         /* output buffer too small; *count = needed  */
+  STQ_ERR_OUT_OF_MEMORY = 4,    /* device or host allocation failed          */
+  STQ_ERR_HIP = 5,              /* HIP runtime error; see stq_last_error     */
+  STQ_ERR_TOO_LARGE = 6,        /* more than 2^31-1 boxes in one scene       */
+  STQ_ERR_INTERNAL = 7
+};
+
+/* Arithmetic of the sweep. */
+enum stq_precision {
+  /* Plain FP64: sort keys and every comparison in FP64 (the reference built
+     with CCD_USE_DOUBLE). */
+  STQ_PRECISION_F64 = 0,
+  /* FP32 sweep on boxes rounded outward (round to nearest, then one ulp
+     outward: the reference rule of src/aabb.cu:56-61), so its candidates are a
+     superset; every FP32 candidate is re-tested in FP64 against the original
+     boxes.  The pair set equals the plain FP64 set exactly. */
+  STQ_PRECISION_F32_CONFIRM = 1
+};
+
+/* Choice of the sweep axis.  The pair set never depends on it. */
+enum stq_axis_policy {
+  STQ_AXIS_X = 0, /* the reference's fixed choice (src/simulation.cu:637) */
+  STQ_AXIS_Y = 1,
+  STQ_AXIS_Z = 2,
+  /* largest variance of the box end points about their mean, the rule of the
+     reference's disabled calc_mean/calc_variance (src/sweep.cu:93-123,
+     src/simulation.cu:594-636), computed in FP64 with a fixed-order reduction */
+  STQ_AXIS_VARIANCE = 3,
+  /* smallest expected chain length: argmin over axes of mean extent divided by
+     the standard deviation of the box centres */
+  STQ_AXIS_CHAIN = 4
+};
+
+/* How the output is sized. */
+enum stq_sizing {
+  /* exact count pass, prefix sum, fill pass (the reference's count-then-refill
+     double pass, src/simulation.cu:451-489) */
+  STQ_SIZING_COUNT_FILL = 0,
+  /* one sweep pass into per-tile buffers sized by the learned per-tile pair
+     predictor; tiles that overflow are redone by the exact count pass */
+  STQ_SIZING_LEARNED = 1
+};
+
+typedef struct stq_options {
+  int32_t precision;     /* enum stq_precision                    */
+  int32_t axis_policy;   /* enum stq_axis_policy                  */
+  int32_t sizing;        /* enum stq_sizing                       */
+  int32_t autotune;      /* 1: pick launch parameters by timing    */
+  int32_t chunk;         /* q-steps per sweep work item; 0 = auto */
+  int32_t window;        /* rows regrouped per tile; 0 = auto     */
+  int32_t waves_per_cu;  /* sweep residency per CU; 0 = auto      */
+  int32_t reserved[9];
+} stq_options;
+
+typedef struct stq_stats {
+  int64_t n_boxes;          /* boxes in the last call                       */
+  int64_t n_pairs;          /* pairs in the last call                       */
+  int64_t n_candidates;     /* sweep-axis candidates (sum of chain lengths) */
+  int64_t n_lane_steps;     /* lane slots the sweep kernel executed         */
+  int64_t n_items;          /* sweep work items                             */
+  int64_t n_confirm;        /* FP32 candidates re-tested in FP64            */
+  int64_t n_tiles;          /* sizing tiles                                 */
+  int64_t n_fallback_tiles; /* tiles redone by the exact count pass         */
+  int64_t n_scenes;         /* scenes in the last call                      */
+  int32_t axis;             /* sweep axis used (last scene)                 */
+  int32_t precision;
+  int32_t sizing;
+  int32_t pad;
+  /* milliseconds, measured with HIP events on the context's stream */
+  double ms_total;          /* whole call on the stream                     */
+  double ms_keys;           /* validation, axis choice and key build        */
+  double ms_sort;           /* radix sort                                   */
+  double ms_gather;         /* sorted record build and chain ends           */
+  double ms_plan;           /* work planning                                */
+  double ms_sweep_count;    /* exact count sweep (0 if not run)             */
+  double ms_scan;           /* prefix sums                                  */
+  double ms_sweep_fill;     /* emitting sweep                               */
+  double ms_compact;        /* output compaction                            */
+  double ms_predict;        /* sizing network inference                     */
+  double sizing_mae;        /* mean |predicted - exact| pairs per tile      */
+  double reserved[6];
+} stq_stats;
+
+/* Fills *opt with the defaults: plain FP64, axis x, count-then-fill. */
+int stq_options_default(stq_options *opt);
+
+/* Creates a context on HIP device `device` (replaces the reference's
+   setup(), src/util.cu:9-55, and its MemHandler, src/memory.cuh:69-191).
+   `opt` may be NULL for the defaults. */
+int stq_create(void **ctx, int device, const stq_options *opt);
+int stq_destroy(void *ctx);
+int stq_set_options(void *ctx, const stq_options *opt);
+
+/* Device-resident broad phase: boxes already in device memory, pairs written
+   to the caller's device buffer `d_out` of `capacity` pairs.  On return *count
+   holds the number of pairs.  If it exceeds `capacity` the call returns
+   STQ_ERR_CAPACITY and the contents of d_out are unspecified.
+   Replaces run_sweep_sharedqueue's d_overlaps / d_count outputs
+   (src/simulation.cu:675-686, :752-753), without its capacity-driven batching
+   (src/memory.cuh:171-188): the whole scene is always processed. */
+int stq_find_pairs_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n,
+                          stq_pair *d_out, int64_t capacity, int64_t *count);
+
+/* As stq_find_pairs_device, but the library allocates *d_out with exactly
+   *count pairs (at least one), like the reference's cudaMalloc'd d_overlaps
+   that the caller owns (src/simulation.cu:686); release with stq_free_device. */
+int stq_find_pairs_device_alloc(void *ctx, const stq_aabb_f64 *d_boxes,
+                                int64_t n, stq_pair **d_out, int64_t *count);
+
+/* Host in, host out: copies the boxes in (src/simulation.cu:578-580), runs,
+   and copies the pairs back (the KEEP_CPU_OVERLAPS branch,
+   src/simulation.cu:763-780).  *h_out is malloc'd; release with
+   stq_free_host. */
+int stq_find_pairs_host(void *ctx, const stq_aabb_f64 *h_boxes, int64_t n,
+                        stq_pair **h_out, int64_t *count);
+
+/* Batched form: `nscenes` independent scenes stored back to back in d_boxes;
+   scene s holds boxes [h_scene_offsets[s], h_scene_offsets[s+1]) and its ids
+   are positions within the scene.  Pairs of scene s are written to
+   d_out[h_pair_offsets[s] .. h_pair_offsets[s+1]); h_pair_offsets has
+   nscenes+1 entries.  STQ_ERR_CAPACITY as above (h_pair_offsets[nscenes] then
+   holds the total needed).  No counterpart in the reference, which runs one
+   scene per call. */
+int stq_find_pairs_batched_device(void *ctx, const stq_aabb_f64 *d_boxes,
+                                  const int64_t *h_scene_offsets,
+                                  int32_t nscenes, stq_pair *d_out,
+                                  int64_t capacity, int64_t *h_pair_offsets);
+
+int stq_free_device(void *ctx, void *d_ptr);
+void stq_free_host(void *h_ptr);
+
+/* Statistics of the last call on this context. */
+int stq_get_stats(void *ctx, stq_stats *stats);
+
+/* Message of the last failing call on this context ("" if none). */
+const char *stq_last_error(void *ctx);
+const char *stq_status_string(int status);
+
+/* sizeof checks for foreign-function callers. */
+int stq_sizeof_aabb(void);
+int stq_sizeof_options(void);
+int stq_sizeof_stats(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STQ_H */

@@ -1,0 +1,139 @@
+"""ctypes front of the CPU oracle (liboracle.so built from stq_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, never by the product library.  Every function
+here restates the reference's CPU broad phase (src/cpu/sweep.cpp:17-158) or
+its FP32 rounding rule (src/aabb.cu:56-61); the C source cites each line.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+AABB_DTYPE = np.dtype(
+    [("id", "<i4"), ("pad0", "<i4"), ("min", "<f8", (3,)), ("max", "<f8", (3,)),
+     ("vids", "<i4", (3,)), ("ref_id", "<i4")])
+
+
+def build():
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        I64 = ctypes.c_int64
+        L.oracle_sweep_rows.restype = I64
+        L.oracle_sweep_rows.argtypes = [P, I64, ctypes.c_int, I64, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p),
+                                        ctypes.POINTER(ctypes.c_double)]
+        L.oracle_bruteforce.restype = I64
+        L.oracle_bruteforce.argtypes = [P, I64, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.oracle_candidates.restype = I64
+        L.oracle_candidates.argtypes = [P, I64, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(I64)]
+        L.oracle_canonicalize.restype = I64
+        L.oracle_canonicalize.argtypes = [P, I64]
+        L.oracle_round_outward.restype = None
+        L.oracle_round_outward.argtypes = [P, I64, P, P]
+        L.oracle_boxes_round_outward.restype = None
+        L.oracle_boxes_round_outward.argtypes = [P, I64, P]
+        L.oracle_pair_predicate.restype = ctypes.c_int
+        L.oracle_pair_predicate.argtypes = [P, P]
+        L.oracle_free.restype = None
+        L.oracle_free.argtypes = [P]
+        L.oracle_sizeof_aabb.restype = ctypes.c_int
+        L.oracle_max_threads.restype = ctypes.c_int
+        assert L.oracle_sizeof_aabb() == 72
+        _LIB = L
+    return _LIB
+
+
+def _as_boxes(boxes):
+    b = np.ascontiguousarray(boxes)
+    assert b.dtype.itemsize == 72
+    return b
+
+
+def _take(ptr, n):
+    if n == 0:
+        lib().oracle_free(ptr)
+        return np.zeros((0, 2), dtype=np.int32)
+    arr = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_int32)), shape=(n, 2)).copy()
+    lib().oracle_free(ptr)
+    return arr
+
+
+def default_threads():
+    return max(1, min(os.cpu_count() or 1, 64))
+
+
+def sweep(boxes, axis=0, nthreads=None, nrows=None, return_sort_time=False):
+    """Reference CPU sweep (sort by axis min, chain scan with break, filters).
+    Returns (earlier, later) position pairs in sorted-row order."""
+    b = _as_boxes(boxes)
+    n = len(b)
+    out = ctypes.c_void_p()
+    st = ctypes.c_double()
+    cnt = lib().oracle_sweep_rows(b.ctypes.data, n, axis, n if nrows is None else nrows,
+                                  nthreads or default_threads(), ctypes.byref(out), ctypes.byref(st))
+    res = _take(out, cnt)
+    return (res, st.value) if return_sort_time else res
+
+
+def bruteforce(boxes, nthreads=None):
+    b = _as_boxes(boxes)
+    out = ctypes.c_void_p()
+    cnt = lib().oracle_bruteforce(b.ctypes.data, len(b), nthreads or default_threads(), ctypes.byref(out))
+    return _take(out, cnt)
+
+
+def candidates(boxes, axis=0, nthreads=None):
+    """(total sweep candidates, longest chain) on an axis."""
+    b = _as_boxes(boxes)
+    mc = ctypes.c_int64()
+    tot = lib().oracle_candidates(b.ctypes.data, len(b), axis, nthreads or default_threads(), ctypes.byref(mc))
+    return int(tot), int(mc.value)
+
+
+def canonical(pairs):
+    """(min,max) orientation, lexicographic order, duplicates removed."""
+    p = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2)).copy()
+    m = lib().oracle_canonicalize(p.ctypes.data, len(p))
+    return p[:m].copy()
+
+
+def round_outward(values):
+    v = np.ascontiguousarray(np.asarray(values, dtype=np.float64))
+    lo = np.empty(v.shape, dtype=np.float32)
+    hi = np.empty(v.shape, dtype=np.float32)
+    lib().oracle_round_outward(v.ctypes.data, v.size, lo.ctypes.data, hi.ctypes.data)
+    return lo, hi
+
+
+def boxes_round_outward(boxes):
+    b = _as_boxes(boxes)
+    out = np.empty_like(b)
+    lib().oracle_boxes_round_outward(b.ctypes.data, len(b), out.ctypes.data)
+    return out
+
+
+def pair_predicate(boxes, i, j):
+    b = _as_boxes(boxes)
+    return bool(lib().oracle_pair_predicate(b[i:i + 1].ctypes.data, b[j:j + 1].ctypes.data))
+
+
+def reference_set(boxes, axis=0, nthreads=None):
+    """The canonical FP64 pair set of the reference sweep."""
+    return canonical(sweep(boxes, axis=axis, nthreads=nthreads))
