@@ -158,11 +158,10 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   const size_t ts = f32 ? 4 : 8;
   void *d_smin = smin.get((size_t)n * ts);
   void *d_smax = smax.get((size_t)n * ts);
-  void *d_geo = geo.get((size_t)n * (f32 ? sizeof(Geo32) : sizeof(Geo64)));
-  int4 *d_meta = meta.as<int4>((size_t)n);
+  void *d_rec = geo.get((size_t)n * (f32 ? sizeof(Rec32) : sizeof(Rec64)));
   Full64 *d_full = f32 ? full.as<Full64>((size_t)n) : nullptr;
   int *d_hi = hi.as<int>((size_t)n);
-  launch_gather(f32, boxes, order, n, axis, d_smin, d_smax, d_geo, d_meta, d_full, st);
+  launch_gather(f32, boxes, order, n, axis, d_smin, d_smax, d_rec, d_full, st);
   launch_chain_end(f32, d_smin, d_smax, n, d_hi, &ctr[1], st);
   STQ_HIP(hipEventRecord(ev[E_GATHER], st));
   unsigned long long hc[4];
@@ -199,8 +198,8 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
       1, std::min<int64_t>((nitems + 3) / 4, (int64_t)cus * std::max(1, wpc / 4)));
   uint32_t *d_counts = counts.as<uint32_t>((size_t)std::max<int64_t>(nitems, 1));
   int64_t *d_off = offsets.as<int64_t>((size_t)nitems + 1);
-  launch_sweep(f32, false, nblocks, d_items, nitems, d_rowlist, d_hi, d_geo, d_meta, d_full,
-               order, d_counts, nullptr, nullptr, &ctr[3], st);
+  launch_sweep(f32, false, nblocks, d_items, nitems, d_rowlist, d_hi, d_rec, d_full, order,
+               d_counts, nullptr, nullptr, &ctr[3], st);
   STQ_HIP(hipEventRecord(ev[E_COUNT], st));
   scan_u32_to_i64(d_counts, d_off, nitems, scan_tmp, st);
   int64_t total = 0;
@@ -222,9 +221,10 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     throw StatusError(STQ_ERR_CAPACITY, "output capacity too small");
   }
   if (total > 0)
-    launch_sweep(f32, true, nblocks, d_items, nitems, d_rowlist, d_hi, d_geo, d_meta, d_full,
-                 order, nullptr, d_off, dst, nullptr, st);
+    launch_sweep(f32, true, nblocks, d_items, nitems, d_rowlist, d_hi, d_rec, d_full, order,
+                 nullptr, d_off, dst, nullptr, st);
   STQ_HIP(hipEventRecord(ev[E_FILL], st));
+  launch_translate(dst, total, order, st);
   STQ_HIP(hipEventRecord(ev[E_END], st));
   STQ_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
   STQ_HIP(hipStreamSynchronize(st));
@@ -428,6 +428,39 @@ int stq_find_pairs_batched_device(void *ctx, const stq_aabb_f64 *d_boxes,
     c->stats.n_candidates = cand;
     c->stats.ms_total = ms;
     if (overflow) throw stqamd::StatusError(STQ_ERR_CAPACITY, "output capacity too small");
+  });
+}
+
+int stq_device_alloc(void *ctx, size_t bytes, void **d_ptr) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !d_ptr) return STQ_ERR_INVALID_ARGUMENT;
+  *d_ptr = nullptr;
+  return stqamd::guarded(c, [&] {
+    if (hipMalloc(d_ptr, bytes ? bytes : 1) != hipSuccess) {
+      (void)hipGetLastError();
+      *d_ptr = nullptr;
+      throw stqamd::StatusError(STQ_ERR_OUT_OF_MEMORY, "device allocation failed");
+    }
+  });
+}
+
+int stq_copy_to_device(void *ctx, void *d_dst, const void *h_src, size_t bytes) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || (bytes && (!d_dst || !h_src))) return STQ_ERR_INVALID_ARGUMENT;
+  return stqamd::guarded(c, [&] {
+    if (!bytes) return;
+    STQ_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, c->stream));
+    STQ_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int stq_copy_to_host(void *ctx, void *h_dst, const void *d_src, size_t bytes) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || (bytes && (!d_src || !h_dst))) return STQ_ERR_INVALID_ARGUMENT;
+  return stqamd::guarded(c, [&] {
+    if (!bytes) return;
+    STQ_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+    STQ_HIP(hipStreamSynchronize(c->stream));
   });
 }
 

@@ -39,7 +39,6 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__device__ inline uint64_t ballot64(bool p) { return __ballot(p); }
 __device__ inline unsigned mbcnt64(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
@@ -124,8 +123,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_f64(const stq_aabb_f64 *__r
                                                          int64_t n, int axis,
                                                          double *__restrict__ smin,
                                                          double *__restrict__ smax,
-                                                         Geo64 *__restrict__ geo,
-                                                         int4 *__restrict__ meta) {
+                                                         Rec64 *__restrict__ rec) {
   const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (s >= n) return;
   const stq_aabb_f64 &b = boxes[order[s]];
@@ -133,14 +131,16 @@ __global__ __launch_bounds__(kThreads) void k_gather_f64(const stq_aabb_f64 *__r
   other_axes(axis, a1, a2);
   smin[s] = b.min[axis];
   smax[s] = b.max[axis];
-  Geo64 g;
-  g.lo1 = b.min[a1];
-  g.hi1 = b.max[a1];
-  g.lo2 = b.min[a2];
-  g.hi2 = b.max[a2];
-  geo[s] = g;
-  const int v0 = b.vertex_ids[0], v1 = b.vertex_ids[1], v2 = b.vertex_ids[2];
-  meta[s] = make_int4(v0, v1, v2, type_bit(v1, v2));
+  Rec64 r;
+  r.lo1 = b.min[a1];
+  r.hi1 = b.max[a1];
+  r.lo2 = b.min[a2];
+  r.hi2 = b.max[a2];
+  r.v0 = b.vertex_ids[0];
+  r.v1 = b.vertex_ids[1];
+  r.v2 = b.vertex_ids[2];
+  r.type = type_bit(r.v1, r.v2);
+  rec[s] = r;
 }
 
 __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__restrict__ boxes,
@@ -148,8 +148,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__r
                                                          int64_t n, int axis,
                                                          float *__restrict__ smin,
                                                          float *__restrict__ smax,
-                                                         Geo32 *__restrict__ geo,
-                                                         int4 *__restrict__ meta,
+                                                         Rec32 *__restrict__ rec,
                                                          Full64 *__restrict__ full) {
   const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (s >= n) return;
@@ -158,12 +157,16 @@ __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__r
   other_axes(axis, a1, a2);
   smin[s] = round_down_f32(b.min[axis]);
   smax[s] = round_up_f32(b.max[axis]);
-  Geo32 g;
-  g.lo1 = round_down_f32(b.min[a1]);
-  g.hi1 = round_up_f32(b.max[a1]);
-  g.lo2 = round_down_f32(b.min[a2]);
-  g.hi2 = round_up_f32(b.max[a2]);
-  geo[s] = g;
+  Rec32 r;
+  r.lo1 = round_down_f32(b.min[a1]);
+  r.hi1 = round_up_f32(b.max[a1]);
+  r.lo2 = round_down_f32(b.min[a2]);
+  r.hi2 = round_up_f32(b.max[a2]);
+  r.v0 = b.vertex_ids[0];
+  r.v1 = b.vertex_ids[1];
+  r.v2 = b.vertex_ids[2];
+  r.type = type_bit(r.v1, r.v2);
+  rec[s] = r;
   Full64 f;
   f.lo0 = b.min[axis];
   f.hi0 = b.max[axis];
@@ -172,8 +175,6 @@ __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__r
   f.lo2 = b.min[a2];
   f.hi2 = b.max[a2];
   full[s] = f;
-  const int v0 = b.vertex_ids[0], v1 = b.vertex_ids[1], v2 = b.vertex_ids[2];
-  meta[s] = make_int4(v0, v1, v2, type_bit(v1, v2));
 }
 
 // ---- chain ends -----------------------------------------------------------------
@@ -294,143 +295,142 @@ __global__ __launch_bounds__(kThreads) void k_items(const int4 *__restrict__ gro
 }
 
 // ---- the sweep ----------------------------------------------------------------------
-// All tests are written with bitwise & and | so that they compile to v_cmp +
-// s_and on lane masks instead of short-circuit branches.
-__device__ inline bool covertex(int a0, int a1, int a2, const int4 &b) {
-  return (a0 == b.x) | (a0 == b.y) | (a0 == b.z) | (a1 == b.x) | (a1 == b.y) | (a1 == b.z) |
-         (a2 == b.x) | (a2 == b.y) | (a2 == b.z);
+// Every test is a v_cmp whose lane mask is combined with s_and: the ballot of
+// each comparison is taken separately, so no short-circuit branches appear.
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+template <class R>
+__device__ __forceinline__ uint64_t geo_mask(const R &m, const R &q) {
+  return ballot(m.hi1 >= q.lo1) & ballot(m.lo1 <= q.hi1) & ballot(m.hi2 >= q.lo2) &
+         ballot(m.lo2 <= q.hi2);
 }
 
-template <class G>
-__device__ inline bool overlap2(const G &m, const G &q) {
-  return (m.hi1 >= q.lo1) & (m.lo1 <= q.hi1) & (m.hi2 >= q.lo2) & (m.lo2 <= q.hi2);
-}
-
-__device__ inline bool confirm64(const Full64 &m, const Full64 &q) {
+__device__ __forceinline__ bool confirm64(const Full64 &m, const Full64 &q) {
   return (m.hi0 >= q.lo0) & (m.lo0 <= q.hi0) & (m.hi1 >= q.lo1) & (m.lo1 <= q.hi1) &
          (m.hi2 >= q.lo2) & (m.lo2 <= q.hi2);
 }
 
-__device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
-
-// Per-wave state of one work item: the 64 rows' records live in VGPRs.
-template <class G, bool CONFIRM, bool FILL>
-struct Rows {
-  int lane, p, myhi;
-  G mine;
-  int4 mv;
-  Full64 mf;
-  uint32_t op;
-  uint64_t act, accV, accE, accF;
-  unsigned cnt;
-  int64_t base;
-  unsigned long long nconf;
-
-  // Pairs of the rows in mask m (geometric test passed) with sorted q.
-  __device__ inline void hit(int q, uint64_t m, const int4 *__restrict__ meta,
-                             const Full64 *__restrict__ full, const uint32_t *__restrict__ orig,
-                             stq_pair *__restrict__ out) {
-    const int4 mq = meta[q];
-    m &= (mq.w == 1) ? accV : ((mq.w == 2) ? accE : accF);
-    if (m == 0) return;
-    bool h = (((m >> lane) & 1ull) != 0) & !covertex(mv.x, mv.y, mv.z, mq);
-    if (CONFIRM) {
-      const uint64_t hm0 = ballot(h);
-      if (hm0 == 0) return;
-      nconf += (unsigned long long)__popcll(hm0);
-      const Full64 fq = full[q];
-      h = h & confirm64(mf, fq);
-    }
-    const uint64_t hm = ballot(h);
-    if (hm == 0) return;
-    if (FILL) {
-      if (h) {
-        const uint32_t oq = orig[q];
-        stq_pair pr;
-        pr.first = (int)min(op, oq);
-        pr.second = (int)max(op, oq);
-        out[base + mbcnt64(hm)] = pr;
-      }
-      base += __popcll(hm);
-    } else {
-      cnt += (unsigned)__popcll(hm);
-    }
-  }
-
-  // q in [qa, qb]; CHECK adds the per-lane chain range test p < q <= hi_p.
-  template <bool CHECK>
-  __device__ inline void walk(int qa, int qb, const G *__restrict__ geo,
-                              const int4 *__restrict__ meta, const Full64 *__restrict__ full,
-                              const uint32_t *__restrict__ orig, stq_pair *__restrict__ out) {
-    int q = qa;
-    for (; q + 3 <= qb; q += 4) {
-      const G g0 = geo[q], g1 = geo[q + 1], g2 = geo[q + 2], g3 = geo[q + 3];
-      bool c0 = overlap2(mine, g0), c1 = overlap2(mine, g1), c2 = overlap2(mine, g2),
-           c3 = overlap2(mine, g3);
-      if (CHECK) {
-        c0 = c0 & (q > p) & (q <= myhi);
-        c1 = c1 & (q + 1 > p) & (q + 1 <= myhi);
-        c2 = c2 & (q + 2 > p) & (q + 2 <= myhi);
-        c3 = c3 & (q + 3 > p) & (q + 3 <= myhi);
-      }
-      const uint64_t m0 = ballot(c0) & act, m1 = ballot(c1) & act, m2 = ballot(c2) & act,
-                     m3 = ballot(c3) & act;
-      if ((m0 | m1 | m2 | m3) != 0) {
-        if (m0) hit(q, m0, meta, full, orig, out);
-        if (m1) hit(q + 1, m1, meta, full, orig, out);
-        if (m2) hit(q + 2, m2, meta, full, orig, out);
-        if (m3) hit(q + 3, m3, meta, full, orig, out);
-      }
-    }
-    for (; q <= qb; ++q) {
-      const G g = geo[q];
-      bool c = overlap2(mine, g);
-      if (CHECK) c = c & (q > p) & (q <= myhi);
-      const uint64_t m = ballot(c) & act;
-      if (m) hit(q, m, meta, full, orig, out);
-    }
-  }
-};
-
-// FILL = false: per-item pair counts; FILL = true: write pairs at offsets[it].
-// CONFIRM: G holds FP32 outward bounds, every FP32 hit is re-tested on Full64.
-template <class G, bool CONFIRM, bool FILL>
+// FILL = false: counts[it] = pairs of item it.  FILL = true: pairs of item
+// it go to out[offsets[it] ..) as (input id of the row, sorted q).
+// CONFIRM: R holds FP32 outward bounds; every FP32 hit is re-tested in FP64.
+// B: q records per scalar-load batch (the next batch is in flight while the
+// current one is tested).
+template <class R, bool CONFIRM, bool FILL, int B>
 __global__ __launch_bounds__(kThreads) void k_sweep(
     const Item *__restrict__ items, int64_t nitems, const int *__restrict__ rowlist,
-    const int *__restrict__ hi, const G *__restrict__ geo, const int4 *__restrict__ meta,
-    const Full64 *__restrict__ full, const uint32_t *__restrict__ orig,
-    uint32_t *__restrict__ counts, const int64_t *__restrict__ offsets,
-    stq_pair *__restrict__ out, unsigned long long *__restrict__ confirms) {
-  Rows<G, CONFIRM, FILL> r;
-  r.lane = threadIdx.x & 63;
+    const int *__restrict__ hi, const R *__restrict__ rec, const Full64 *__restrict__ full,
+    const uint32_t *__restrict__ orig, uint32_t *__restrict__ counts,
+    const int64_t *__restrict__ offsets, int2 *__restrict__ out,
+    unsigned long long *__restrict__ confirms, int zero) {
+  const int lane = threadIdx.x & 63;
   const int wave = uniform((int)(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)));
   const int nwaves = (int)gridDim.x * (kThreads / 64);
-  r.nconf = 0;
+  unsigned long long nconf = 0;
   for (int64_t it = wave; it < nitems; it += nwaves) {
     const Item item = items[it];
-    const bool act = r.lane < item.nrows;
-    r.p = act ? rowlist[item.rowoff + r.lane] : INT_MAX;
-    r.myhi = act ? hi[r.p] : INT_MIN;
-    r.mv = make_int4(0, 0, 0, 0);
-    r.op = 0;
+    const bool act = lane < item.nrows;
+    const int p = act ? rowlist[item.rowoff + lane] : INT_MAX;
+    const int myhi = act ? hi[p] : INT_MIN;
+    R mine;
+    mine.lo1 = mine.lo2 = (decltype(mine.lo1))INFINITY;  // empty box for idle lanes
+    mine.hi1 = mine.hi2 = -(decltype(mine.lo1))INFINITY;
+    mine.v0 = mine.v1 = mine.v2 = mine.type = 0;
+    Full64 mf{};
+    uint32_t op = 0;
     if (act) {
-      r.mine = geo[r.p];
-      r.mv = meta[r.p];
-      if (CONFIRM) r.mf = full[r.p];
-      if (FILL) r.op = orig[r.p];
-    } else {
-      r.mine = G{};
-      if (CONFIRM) r.mf = Full64{};
+      mine = rec[p];
+      if (CONFIRM) mf = full[p];
+      if (FILL) op = orig[p];
     }
-    const int acc = act ? accept_bits(r.mv.w) : 0;
-    r.act = ballot(act);
-    r.accV = ballot(acc & 1);
-    r.accE = ballot(acc & 2);
-    r.accF = ballot(acc & 4);
-    r.cnt = 0;
-    r.base = FILL ? offsets[it] : 0;
-    // rows with p < q <= hi_p for every active lane: q in (pmax, hmin]
-    int pmax = act ? r.p : INT_MIN, hmin = act ? r.myhi : INT_MAX;
+    const int acc = act ? accept_bits(mine.type) : 0;
+    unsigned cnt = 0;
+    int64_t base = FILL ? offsets[it] : 0;
+
+    // rows in m passed the geometric test against sorted q (record rq)
+    auto hit = [&](int q, const R &rq, uint64_t m) __attribute__((always_inline)) {
+      const bool h0 = (((m >> lane) & 1ull) != 0) & ((acc & rq.type) != 0) &
+                      (mine.v0 != rq.v0) & (mine.v0 != rq.v1) & (mine.v0 != rq.v2) &
+                      (mine.v1 != rq.v0) & (mine.v1 != rq.v1) & (mine.v1 != rq.v2) &
+                      (mine.v2 != rq.v0) & (mine.v2 != rq.v1) & (mine.v2 != rq.v2);
+      bool h = h0;
+      if (CONFIRM) {
+        const uint64_t hm0 = ballot(h0);
+        if (hm0 == 0) return;
+        nconf += (unsigned long long)__popcll(hm0);
+        // a uniform address the compiler cannot prove uniform: a vector load,
+        // so its wait does not drain the scalar prefetch
+        const Full64 fq = full[q + zero * lane];
+        h = h0 & confirm64(mf, fq);
+      }
+      const uint64_t hm = ballot(h);
+      if (hm == 0) return;
+      if (FILL) {
+        if (h) out[base + mbcnt64(hm)] = make_int2((int)op, q);
+        base += __popcll(hm);
+      } else {
+        cnt += (unsigned)__popcll(hm);
+      }
+    };
+
+    // Geometric masks of one batch of B records (inactive lanes can pass the
+    // geometric test only against unbounded boxes, and never the type test).
+    auto batch = [&](const R *b, int q, bool check) __attribute__((always_inline)) {
+      uint64_t m[B];
+      uint64_t any = 0;
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        m[j] = geo_mask(mine, b[j]);
+        if (check) m[j] &= ballot(q + j > p) & ballot(q + j <= myhi);
+        any |= m[j];
+      }
+      if (any) {
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+          if (m[j]) hit(q + j, b[j], m[j]);
+      }
+    };
+
+    // q in [qa, qb].  Scalar loads return out of order, so a wait for one
+    // batch waits for every load in flight; the next batch's address is made
+    // to depend on the current batch (times the runtime zero) so that its
+    // loads are issued after that wait, and it is in flight while the
+    // current batch is tested.  Two buffers alternate.
+    auto walk = [&](int qa, int qb, bool check) __attribute__((always_inline)) {
+      int q = qa;
+      int nb = (qb - qa + 1) / B;
+      if (nb > 0) {
+        R b0[B], b1[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) b0[j] = rec[q + j];
+        while (true) {
+          q += zero * b0[0].v0;  // waits for b0 on every path
+          if (nb > 1) {
+#pragma unroll
+            for (int j = 0; j < B; ++j) b1[j] = rec[q + B + j];
+          }
+          batch(b0, q, check);
+          q += B;
+          if (--nb == 0) break;
+          q += zero * b1[0].v0;
+          if (nb > 1) {
+#pragma unroll
+            for (int j = 0; j < B; ++j) b0[j] = rec[q + B + j];
+          }
+          batch(b1, q, check);
+          q += B;
+          if (--nb == 0) break;
+        }
+      }
+      for (; q <= qb; ++q) {
+        const R r = rec[q];
+        uint64_t m = geo_mask(mine, r);
+        if (check) m &= ballot(q > p) & ballot(q <= myhi);
+        if (m) hit(q, r, m);
+      }
+    };
+
+    // every active lane has p < q <= hi_p for q in (pmax, hmin]
+    int pmax = act ? p : INT_MIN, hmin = act ? myhi : INT_MAX;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       pmax = max(pmax, __shfl_xor(pmax, o, kWave));
@@ -441,15 +441,25 @@ __global__ __launch_bounds__(kThreads) void k_sweep(
     const int q0 = item.q0, q1 = item.q1;
     const int b0 = max(q0, pmax + 1), b1 = min(q1, hmin);
     if (b0 <= b1) {
-      if (q0 < b0) r.template walk<true>(q0, b0 - 1, geo, meta, full, orig, out);
-      r.template walk<false>(b0, b1, geo, meta, full, orig, out);
-      if (b1 < q1) r.template walk<true>(b1 + 1, q1, geo, meta, full, orig, out);
+      if (q0 < b0) walk(q0, b0 - 1, true);
+      walk(b0, b1, false);
+      if (b1 < q1) walk(b1 + 1, q1, true);
     } else {
-      r.template walk<true>(q0, q1, geo, meta, full, orig, out);
+      walk(q0, q1, true);
     }
-    if (!FILL && r.lane == 0) counts[it] = r.cnt;
+    if (!FILL && lane == 0) counts[it] = cnt;
   }
-  if (CONFIRM && !FILL && r.lane == 0 && r.nconf) atomicAdd(confirms, r.nconf);
+  if (CONFIRM && !FILL && lane == 0 && nconf) atomicAdd(confirms, nconf);
+}
+
+// (input id of the row, sorted q) -> canonical (min id, max id).
+__global__ __launch_bounds__(kThreads) void k_translate(int2 *__restrict__ pairs, int64_t count,
+                                                        const uint32_t *__restrict__ orig) {
+  const int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (k >= count) return;
+  const int2 e = pairs[k];
+  const int oq = (int)orig[e.y];
+  pairs[k] = make_int2(min(e.x, oq), max(e.x, oq));
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
@@ -476,15 +486,14 @@ void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, void 
 }
 
 void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, int64_t n,
-                   int axis, void *smin, void *smax, void *geo, int4 *meta, Full64 *full,
-                   hipStream_t st) {
+                   int axis, void *smin, void *smax, void *rec, Full64 *full, hipStream_t st) {
   if (n <= 0) return;
   if (f32)
     hipLaunchKernelGGL(k_gather_f32, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, order,
-                       n, axis, (float *)smin, (float *)smax, (Geo32 *)geo, meta, full);
+                       n, axis, (float *)smin, (float *)smax, (Rec32 *)rec, full);
   else
     hipLaunchKernelGGL(k_gather_f64, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, order,
-                       n, axis, (double *)smin, (double *)smax, (Geo64 *)geo, meta);
+                       n, axis, (double *)smin, (double *)smax, (Rec64 *)rec);
   STQ_HIP(hipGetLastError());
 }
 
@@ -518,30 +527,36 @@ void launch_items(const int4 *groups, const int64_t *ioff, int64_t ngroups, int 
 }
 
 void launch_sweep(bool f32, bool fill, int nblocks, const Item *items, int64_t nitems,
-                  const int *rowlist, const int *hi, const void *geo, const int4 *meta,
-                  const Full64 *full, const uint32_t *orig, uint32_t *counts,
-                  const int64_t *offsets, stq_pair *out, unsigned long long *confirms,
-                  hipStream_t st) {
+                  const int *rowlist, const int *hi, const void *rec, const Full64 *full,
+                  const uint32_t *orig, uint32_t *counts, const int64_t *offsets, stq_pair *out,
+                  unsigned long long *confirms, hipStream_t st) {
   if (nitems <= 0) return;
   const dim3 grid(nblocks), block(kThreads);
+  int2 *o = reinterpret_cast<int2 *>(out);
   if (f32) {
+    const Rec32 *r = (const Rec32 *)rec;
     if (fill)
-      hipLaunchKernelGGL((k_sweep<Geo32, true, true>), grid, block, 0, st, items, nitems, rowlist,
-                         hi, (const Geo32 *)geo, meta, full, orig, counts, offsets, out, confirms);
+      hipLaunchKernelGGL((k_sweep<Rec32, true, true, 3>), grid, block, 0, st, items, nitems,
+                         rowlist, hi, r, full, orig, counts, offsets, o, confirms, 0);
     else
-      hipLaunchKernelGGL((k_sweep<Geo32, true, false>), grid, block, 0, st, items, nitems,
-                         rowlist, hi, (const Geo32 *)geo, meta, full, orig, counts, offsets, out,
-                         confirms);
+      hipLaunchKernelGGL((k_sweep<Rec32, true, false, 3>), grid, block, 0, st, items, nitems,
+                         rowlist, hi, r, full, orig, counts, offsets, o, confirms, 0);
   } else {
+    const Rec64 *r = (const Rec64 *)rec;
     if (fill)
-      hipLaunchKernelGGL((k_sweep<Geo64, false, true>), grid, block, 0, st, items, nitems,
-                         rowlist, hi, (const Geo64 *)geo, meta, full, orig, counts, offsets, out,
-                         confirms);
+      hipLaunchKernelGGL((k_sweep<Rec64, false, true, 2>), grid, block, 0, st, items, nitems,
+                         rowlist, hi, r, full, orig, counts, offsets, o, confirms, 0);
     else
-      hipLaunchKernelGGL((k_sweep<Geo64, false, false>), grid, block, 0, st, items, nitems,
-                         rowlist, hi, (const Geo64 *)geo, meta, full, orig, counts, offsets, out,
-                         confirms);
+      hipLaunchKernelGGL((k_sweep<Rec64, false, false, 2>), grid, block, 0, st, items, nitems,
+                         rowlist, hi, r, full, orig, counts, offsets, o, confirms, 0);
   }
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_translate(stq_pair *pairs, int64_t count, const uint32_t *orig, hipStream_t st) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_translate, dim3(blocks_for(count)), dim3(kThreads), 0, st,
+                     reinterpret_cast<int2 *>(pairs), count, orig);
   STQ_HIP(hipGetLastError());
 }
 

@@ -31,7 +31,8 @@ SIZING_LEARNED = 1
 EXPORTS = [
     "stq_options_default", "stq_create", "stq_destroy", "stq_set_options",
     "stq_find_pairs_device", "stq_find_pairs_device_alloc", "stq_find_pairs_host",
-    "stq_find_pairs_batched_device", "stq_free_device", "stq_free_host",
+    "stq_find_pairs_batched_device", "stq_device_alloc", "stq_copy_to_device",
+    "stq_copy_to_host", "stq_free_device", "stq_free_host",
     "stq_get_stats", "stq_last_error", "stq_status_string",
     "stq_sizeof_aabb", "stq_sizeof_options", "stq_sizeof_stats",
 ]
@@ -104,6 +105,9 @@ def lib():
         "stq_find_pairs_device_alloc": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
         "stq_find_pairs_host": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
         "stq_find_pairs_batched_device": (ctypes.c_int, [P, P, P, I32, P, I64, P]),
+        "stq_device_alloc": (ctypes.c_int, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
+        "stq_copy_to_device": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
+        "stq_copy_to_host": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
         "stq_free_device": (ctypes.c_int, [P, P]),
         "stq_free_host": (None, [P]),
         "stq_get_stats": (ctypes.c_int, [P, ctypes.POINTER(Stats)]),

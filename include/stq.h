@@ -198,6 +198,13 @@ int stq_find_pairs_batched_device(void *ctx, const stq_aabb_f64 *d_boxes,
                                   int32_t nscenes, stq_pair *d_out,
                                   int64_t capacity, int64_t *h_pair_offsets);
 
+/* Device memory plumbing for callers that do not link the HIP runtime
+   themselves (foreign-function callers, the C++ drop-in of
+   include/stq/gpu/stq_dropin.hpp): allocate on the context's device, copy in
+   and out on the context's stream (synchronous). */
+int stq_device_alloc(void *ctx, size_t bytes, void **d_ptr);
+int stq_copy_to_device(void *ctx, void *d_dst, const void *h_src, size_t bytes);
+int stq_copy_to_host(void *ctx, void *h_dst, const void *d_src, size_t bytes);
 int stq_free_device(void *ctx, void *d_ptr);
 void stq_free_host(void *h_ptr);
 
