@@ -53,7 +53,8 @@ struct Context {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  int pick_axis(const stq_aabb_f64 *boxes, int64_t n);
+  int pick_axis(const stq_aabb_f64 *boxes, int64_t n, bool need_q, QParams *qp);
+  void finish_stats(hipStream_t st);
   void run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t capacity,
            stq_pair **alloc_out, int64_t *count);
 };
@@ -74,21 +75,41 @@ static bool options_valid(const stq_options &o) {
   return o.precision >= 0 && o.precision <= 1 && o.axis_policy >= 0 && o.axis_policy <= 4 &&
          o.sizing >= 0 && o.sizing <= 1 && o.chunk >= 0 && o.window >= 0 &&
          (o.window == 0 || (o.window >= 64 && o.window <= 2048 && (o.window & (o.window - 1)) == 0)) &&
-         o.waves_per_cu >= 0 && o.waves_per_cu <= 64;
+         o.waves_per_cu >= 0 && o.waves_per_cu <= 64 &&
+         (o.batch == 0 || o.batch == 2 || o.batch == 4 || o.batch == 8) && o.xcd_split >= 0 &&
+         o.xcd_split <= 2;
 }
 
-int Context::pick_axis(const stq_aabb_f64 *boxes, int64_t n) {
+// Axis choice and, for the reduced-precision form, the quantisation range:
+// one pass of per-block statistics, combined on the host in block order.
+int Context::pick_axis(const stq_aabb_f64 *boxes, int64_t n, bool need_q, QParams *qp) {
   const int pol = opt.axis_policy;
-  if (pol <= 2) return pol;
+  if (pol <= 2 && !need_q) return pol;
   const int nb = (int)std::min<int64_t>((n + 255) / 256, 1024);
-  double *part = axis_partial.as<double>((size_t)nb * 12);
+  double *part = axis_partial.as<double>((size_t)nb * kAxisStats);
   launch_axis_stats(boxes, n, part, nb, stream);
-  std::vector<double> h((size_t)nb * 12);
+  std::vector<double> h((size_t)nb * kAxisStats);
   STQ_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
   STQ_HIP(hipStreamSynchronize(stream));
-  double s[12] = {0};
+  double s[kAxisStats];
+  for (int k = 0; k < kAxisStats; ++k) s[k] = k < 12 ? 0.0 : (k < 15 ? INFINITY : -INFINITY);
   for (int b = 0; b < nb; ++b)
-    for (int k = 0; k < 12; ++k) s[k] += h[(size_t)b * 12 + k];
+    for (int k = 0; k < kAxisStats; ++k) {
+      const double v = h[(size_t)b * kAxisStats + k];
+      s[k] = k < 12 ? s[k] + v : (k < 15 ? std::min(s[k], v) : std::max(s[k], v));
+    }
+  if (qp) {
+    for (int a = 0; a < 3; ++a) {
+      const double lo = s[12 + a], up = s[15 + a];
+      const double range = up - lo;
+      qp->origin[a] = std::isfinite(lo) ? lo : 0.0;
+      // 32767 levels across the scene; a degenerate or unbounded range puts
+      // every box on the full level range (scale 0)
+      qp->scale[a] = (range > 0 && std::isfinite(range)) ? 32767.0 / range : 0.0;
+      if (!std::isfinite(qp->scale[a])) qp->scale[a] = 0.0;
+    }
+  }
+  if (pol <= 2) return pol;
   const double dn = (double)n;
   int best = 0;
   double bestv = 0;
@@ -142,7 +163,8 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   unsigned long long *ctr = counters.as<unsigned long long>(8);
   STQ_HIP(hipMemsetAsync(ctr, 0, 8 * sizeof(unsigned long long), st));
 
-  const int axis = pick_axis(boxes, n);
+  QParams qp{};
+  const int axis = pick_axis(boxes, n, f32, &qp);
   stats.axis = axis;
   uint32_t *order = vals.as<uint32_t>((size_t)n);
   void *kbuf = keys.get((size_t)n * (f32 ? 4 : 8));
@@ -158,10 +180,10 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   const size_t ts = f32 ? 4 : 8;
   void *d_smin = smin.get((size_t)n * ts);
   void *d_smax = smax.get((size_t)n * ts);
-  void *d_rec = geo.get((size_t)n * (f32 ? sizeof(Rec32) : sizeof(Rec64)));
-  Full64 *d_full = f32 ? full.as<Full64>((size_t)n) : nullptr;
+  void *d_rec = geo.get((size_t)n * (f32 ? sizeof(Exact64) : sizeof(Rec64)));
+  QRec *d_qrec = f32 ? meta.as<QRec>((size_t)n) : nullptr;
   int *d_hi = hi.as<int>((size_t)n);
-  launch_gather(f32, boxes, order, n, axis, d_smin, d_smax, d_rec, d_full, st);
+  launch_gather(f32, boxes, order, n, axis, qp, d_smin, d_smax, d_rec, d_qrec, st);
   launch_chain_end(f32, d_smin, d_smax, n, d_hi, &ctr[1], st);
   STQ_HIP(hipEventRecord(ev[E_GATHER], st));
   unsigned long long hc[4];
@@ -194,12 +216,24 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   STQ_HIP(hipEventRecord(ev[E_PLAN], st));
 
   const int wpc = opt.waves_per_cu > 0 ? opt.waves_per_cu : 32;
-  const int nblocks = (int)std::max<int64_t>(
-      1, std::min<int64_t>((nitems + 3) / 4, (int64_t)cus * std::max(1, wpc / 4)));
+  // a multiple of 8 blocks so every XCD gets the same number
+  const int nblocks = (int)(8 * std::max<int64_t>(
+      1, std::min<int64_t>((nitems + 31) / 32, (int64_t)cus * std::max(1, wpc / 4) / 8)));
   uint32_t *d_counts = counts.as<uint32_t>((size_t)std::max<int64_t>(nitems, 1));
   int64_t *d_off = offsets.as<int64_t>((size_t)nitems + 1);
-  launch_sweep(f32, false, nblocks, d_items, nitems, d_rowlist, d_hi, d_rec, d_full, order,
-               d_counts, nullptr, nullptr, &ctr[3], st);
+  SweepArgs sa;
+  sa.items = d_items;
+  sa.nitems = nitems;
+  sa.rowlist = d_rowlist;
+  sa.hi = d_hi;
+  sa.rec = d_rec;
+  sa.qrec = d_qrec;
+  sa.orig = order;
+  sa.counts = d_counts;
+  sa.confirms = &ctr[3];
+  sa.xcd_split = opt.xcd_split == 2 ? 0 : 1;
+  const int batch = opt.batch ? opt.batch : (f32 ? 8 : 2);
+  launch_sweep(f32, kCount, batch, nblocks, sa, st);
   STQ_HIP(hipEventRecord(ev[E_COUNT], st));
   scan_u32_to_i64(d_counts, d_off, nitems, scan_tmp, st);
   int64_t total = 0;
@@ -218,14 +252,26 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     dst = static_cast<stq_pair *>(p);
     *alloc_out = dst;
   } else if (total > capacity) {
+    for (int e = E_FILL; e < E_NUM; ++e) STQ_HIP(hipEventRecord(ev[e], st));
+    finish_stats(st);
     throw StatusError(STQ_ERR_CAPACITY, "output capacity too small");
   }
-  if (total > 0)
-    launch_sweep(f32, true, nblocks, d_items, nitems, d_rowlist, d_hi, d_rec, d_full, order,
-                 nullptr, d_off, dst, nullptr, st);
+  if (total > 0) {
+    sa.counts = nullptr;
+    sa.confirms = nullptr;
+    sa.offsets = d_off;
+    sa.out = reinterpret_cast<int2 *>(dst);
+    launch_sweep(f32, kFill, batch, nblocks, sa, st);
+  }
   STQ_HIP(hipEventRecord(ev[E_FILL], st));
   launch_translate(dst, total, order, st);
   STQ_HIP(hipEventRecord(ev[E_END], st));
+  finish_stats(st);
+}
+
+void Context::finish_stats(hipStream_t st) {
+  unsigned long long hc[4];
+  unsigned long long *ctr = counters.as<unsigned long long>(8);
   STQ_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
   STQ_HIP(hipStreamSynchronize(st));
   stats.n_lane_steps = (int64_t)hc[2];

@@ -38,6 +38,7 @@ namespace stqamd {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs of 32 CUs
 
 __device__ inline unsigned mbcnt64(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -47,34 +48,46 @@ __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v);
 __device__ inline bool bad(double a, double b) { return !(a <= b); }  // NaN or inverted
 
 // ---- axis statistics ----------------------------------------------------------
-// Per axis: sum(min+max), sum(min^2+max^2), sum(max-min), sum(((min+max)/2)^2).
+// Per axis: sum(min+max), sum(min^2+max^2), sum(max-min), sum(((min+max)/2)^2),
+// then min(min) and max(max) for the quantisation range.
 __global__ __launch_bounds__(kThreads) void k_axis_stats(const stq_aabb_f64 *__restrict__ boxes,
                                                          int64_t n, double *__restrict__ partial) {
-  double acc[12];
+  double acc[12], lo[3], up[3];
 #pragma unroll
   for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = INFINITY;
+    up[a] = -INFINITY;
+  }
   const int64_t stride = (int64_t)gridDim.x * kThreads;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
     const stq_aabb_f64 &b = boxes[i];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const double lo = b.min[a], up = b.max[a];
-      const double c = 0.5 * (lo + up);
-      acc[4 * a + 0] += lo + up;
-      acc[4 * a + 1] += lo * lo + up * up;
-      acc[4 * a + 2] += up - lo;
+      const double l = b.min[a], u = b.max[a];
+      const double c = 0.5 * (l + u);
+      acc[4 * a + 0] += l + u;
+      acc[4 * a + 1] += l * l + u * u;
+      acc[4 * a + 2] += u - l;
       acc[4 * a + 3] += c * c;
+      lo[a] = fmin(lo[a], l);
+      up[a] = fmax(up[a], u);
     }
   }
   __shared__ double red[kThreads];
-  for (int k = 0; k < 12; ++k) {
-    red[threadIdx.x] = acc[k];
+  for (int k = 0; k < kAxisStats; ++k) {
+    const int mode = k < 12 ? 0 : (k < 15 ? 1 : 2);  // sum, min, max
+    red[threadIdx.x] = k < 12 ? acc[k] : (k < 15 ? lo[k - 12] : up[k - 15]);
     __syncthreads();
     for (int s = kThreads / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      if ((int)threadIdx.x < s) {
+        const double x = red[threadIdx.x], y = red[threadIdx.x + s];
+        red[threadIdx.x] = mode == 0 ? x + y : (mode == 1 ? fmin(x, y) : fmax(x, y));
+      }
       __syncthreads();
     }
-    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * 12 + k] = red[0];
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * kAxisStats + k] = red[0];
     __syncthreads();
   }
 }
@@ -143,13 +156,27 @@ __global__ __launch_bounds__(kThreads) void k_gather_f64(const stq_aabb_f64 *__r
   rec[s] = r;
 }
 
+// Outward 15-bit levels: t = (x - origin) * scale is monotone in x under IEEE
+// rounding, so floor(t(lo)) <= ceil(t(hi')) whenever lo <= hi': a pair that
+// overlaps in FP64 always overlaps in levels (a superset test).
+__device__ inline uint32_t level_down(double x, double o, double sc) {
+  if (sc == 0.0) return 0u;
+  const double t = fmin(fmax(floor((x - o) * sc), 0.0), 32767.0);
+  return (uint32_t)t;
+}
+__device__ inline uint32_t level_up(double x, double o, double sc) {
+  if (sc == 0.0) return 32767u;
+  const double t = fmin(fmax(ceil((x - o) * sc), 0.0), 32767.0);
+  return (uint32_t)t;
+}
+
 __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__restrict__ boxes,
                                                          const uint32_t *__restrict__ order,
-                                                         int64_t n, int axis,
+                                                         int64_t n, int axis, QParams qp,
                                                          float *__restrict__ smin,
                                                          float *__restrict__ smax,
-                                                         Rec32 *__restrict__ rec,
-                                                         Full64 *__restrict__ full) {
+                                                         Exact64 *__restrict__ rec,
+                                                         QRec *__restrict__ qrec) {
   const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (s >= n) return;
   const stq_aabb_f64 &b = boxes[order[s]];
@@ -157,24 +184,24 @@ __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__r
   other_axes(axis, a1, a2);
   smin[s] = round_down_f32(b.min[axis]);
   smax[s] = round_up_f32(b.max[axis]);
-  Rec32 r;
-  r.lo1 = round_down_f32(b.min[a1]);
-  r.hi1 = round_up_f32(b.max[a1]);
-  r.lo2 = round_down_f32(b.min[a2]);
-  r.hi2 = round_up_f32(b.max[a2]);
+  Exact64 r;
+  r.lo0 = b.min[axis];
+  r.hi0 = b.max[axis];
+  r.lo1 = b.min[a1];
+  r.hi1 = b.max[a1];
+  r.lo2 = b.min[a2];
+  r.hi2 = b.max[a2];
   r.v0 = b.vertex_ids[0];
   r.v1 = b.vertex_ids[1];
   r.v2 = b.vertex_ids[2];
   r.type = type_bit(r.v1, r.v2);
   rec[s] = r;
-  Full64 f;
-  f.lo0 = b.min[axis];
-  f.hi0 = b.max[axis];
-  f.lo1 = b.min[a1];
-  f.hi1 = b.max[a1];
-  f.lo2 = b.min[a2];
-  f.hi2 = b.max[a2];
-  full[s] = f;
+  QRec q;
+  q.lo = level_down(r.lo1, qp.origin[a1], qp.scale[a1]) |
+         (level_down(r.lo2, qp.origin[a2], qp.scale[a2]) << 16);
+  q.hi = level_up(r.hi1, qp.origin[a1], qp.scale[a1]) |
+         (level_up(r.hi2, qp.origin[a2], qp.scale[a2]) << 16);
+  qrec[s] = q;
 }
 
 // ---- chain ends -----------------------------------------------------------------
@@ -299,82 +326,118 @@ __global__ __launch_bounds__(kThreads) void k_items(const int4 *__restrict__ gro
 // each comparison is taken separately, so no short-circuit branches appear.
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
+// Work split of the persistent grid.  Blocks are dispatched to the XCDs round
+// robin (block b on XCD b % kXcds); with xcd_split each XCD takes one
+// contiguous slice of the item list, so its L2 holds only that slice's q
+// records.
+struct WaveSlice {
+  int64_t kbeg, kend;
+  int wave, nwaves;
+};
+__device__ __forceinline__ WaveSlice wave_slice(int64_t nitems, int xcd_split) {
+  WaveSlice w;
+  if (xcd_split) {
+    const int xcd = (int)(blockIdx.x % kXcds);
+    const int per = (int)(gridDim.x / kXcds);
+    w.wave = uniform((int)((blockIdx.x / kXcds) * (kThreads / 64) + (threadIdx.x >> 6)));
+    w.nwaves = per * (kThreads / 64);
+    w.kbeg = nitems * xcd / kXcds;
+    w.kend = nitems * (xcd + 1) / kXcds;
+  } else {
+    w.wave = uniform((int)(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)));
+    w.nwaves = (int)gridDim.x * (kThreads / 64);
+    w.kbeg = 0;
+    w.kend = nitems;
+  }
+  return w;
+}
+
+// Bounds of the unchecked middle part: every active lane has p < q <= hi_p
+// for q in (pmax, hmin].
+__device__ __forceinline__ void row_span(bool act, int p, int myhi, int &pmax, int &hmin) {
+  pmax = act ? p : INT_MIN;
+  hmin = act ? myhi : INT_MAX;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    pmax = max(pmax, __shfl_xor(pmax, o, kWave));
+    hmin = min(hmin, __shfl_xor(hmin, o, kWave));
+  }
+  pmax = uniform(pmax);
+  hmin = uniform(hmin);
+}
+
+// Emission of the pairs of the rows in hm with sorted q (wave aggregated).
+template <int MODE>
+struct Emit {
+  unsigned cnt = 0;
+  int64_t base0 = 0, base = 0;
+  unsigned cap = 0;
+  __device__ __forceinline__ void put(bool h, uint64_t hm, uint32_t op, int q, int2 *out) {
+    if (MODE == kFill) {
+      if (h) out[base + mbcnt64(hm)] = make_int2((int)op, q);
+      base += __popcll(hm);
+    } else if (MODE == kLearned) {
+      const unsigned slot = cnt + mbcnt64(hm);
+      if (h && slot < cap) out[base0 + slot] = make_int2((int)op, q);
+      cnt += (unsigned)__popcll(hm);
+    } else {
+      cnt += (unsigned)__popcll(hm);
+    }
+  }
+};
+
+// ---- plain FP64 sweep ----
+// q records (Rec64, 48 B) arrive by scalar loads B at a time.  Scalar loads
+// return out of order, so a wait for one batch waits for every scalar load in
+// flight; the next batch's address is made to depend on the current batch
+// (times the runtime zero) so its loads are issued after that wait and stay in
+// flight while the current batch is tested.  Two buffers alternate.
 template <class R>
 __device__ __forceinline__ uint64_t geo_mask(const R &m, const R &q) {
   return ballot(m.hi1 >= q.lo1) & ballot(m.lo1 <= q.hi1) & ballot(m.hi2 >= q.lo2) &
          ballot(m.lo2 <= q.hi2);
 }
 
-__device__ __forceinline__ bool confirm64(const Full64 &m, const Full64 &q) {
-  return (m.hi0 >= q.lo0) & (m.lo0 <= q.hi0) & (m.hi1 >= q.lo1) & (m.lo1 <= q.hi1) &
-         (m.hi2 >= q.lo2) & (m.lo2 <= q.hi2);
-}
-
-// FILL = false: counts[it] = pairs of item it.  FILL = true: pairs of item
-// it go to out[offsets[it] ..) as (input id of the row, sorted q).
-// CONFIRM: R holds FP32 outward bounds; every FP32 hit is re-tested in FP64.
-// B: q records per scalar-load batch (the next batch is in flight while the
-// current one is tested).
-template <class R, bool CONFIRM, bool FILL, int B>
-__global__ __launch_bounds__(kThreads) void k_sweep(
-    const Item *__restrict__ items, int64_t nitems, const int *__restrict__ rowlist,
-    const int *__restrict__ hi, const R *__restrict__ rec, const Full64 *__restrict__ full,
-    const uint32_t *__restrict__ orig, uint32_t *__restrict__ counts,
-    const int64_t *__restrict__ offsets, int2 *__restrict__ out,
-    unsigned long long *__restrict__ confirms, int zero) {
+// The record and output pointers are separate restrict-qualified arguments:
+// that is what lets the compiler keep the uniform record loads on the scalar
+// path next to the pair stores.
+template <int MODE, int B>
+__global__ __launch_bounds__(kThreads) void k_sweep64(SweepArgs a, const Rec64 *__restrict__ rec,
+                                                      int2 *__restrict__ out) {
+  const int zero = a.zero;
   const int lane = threadIdx.x & 63;
-  const int wave = uniform((int)(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)));
-  const int nwaves = (int)gridDim.x * (kThreads / 64);
-  unsigned long long nconf = 0;
-  for (int64_t it = wave; it < nitems; it += nwaves) {
-    const Item item = items[it];
+  const WaveSlice ws = wave_slice(a.nitems, a.xcd_split);
+  for (int64_t k = ws.kbeg + ws.wave; k < ws.kend; k += ws.nwaves) {
+    const int64_t it = a.sel ? (int64_t)a.sel[k] : k;
+    const Item item = a.items[it];
     const bool act = lane < item.nrows;
-    const int p = act ? rowlist[item.rowoff + lane] : INT_MAX;
-    const int myhi = act ? hi[p] : INT_MIN;
-    R mine;
-    mine.lo1 = mine.lo2 = (decltype(mine.lo1))INFINITY;  // empty box for idle lanes
-    mine.hi1 = mine.hi2 = -(decltype(mine.lo1))INFINITY;
+    const int p = act ? a.rowlist[item.rowoff + lane] : INT_MAX;
+    const int myhi = act ? a.hi[p] : INT_MIN;
+    Rec64 mine;
+    mine.lo1 = mine.lo2 = INFINITY;  // empty box for idle lanes
+    mine.hi1 = mine.hi2 = -INFINITY;
     mine.v0 = mine.v1 = mine.v2 = mine.type = 0;
-    Full64 mf{};
     uint32_t op = 0;
     if (act) {
       mine = rec[p];
-      if (CONFIRM) mf = full[p];
-      if (FILL) op = orig[p];
+      if (MODE != kCount) op = a.orig[p];
     }
     const int acc = act ? accept_bits(mine.type) : 0;
-    unsigned cnt = 0;
-    int64_t base = FILL ? offsets[it] : 0;
+    Emit<MODE> em;
+    if (MODE != kCount) em.base0 = em.base = a.offsets[it];
+    if (MODE == kLearned) em.cap = a.caps[it];
 
-    // rows in m passed the geometric test against sorted q (record rq)
-    auto hit = [&](int q, const R &rq, uint64_t m) __attribute__((always_inline)) {
-      const bool h0 = (((m >> lane) & 1ull) != 0) & ((acc & rq.type) != 0) &
-                      (mine.v0 != rq.v0) & (mine.v0 != rq.v1) & (mine.v0 != rq.v2) &
-                      (mine.v1 != rq.v0) & (mine.v1 != rq.v1) & (mine.v1 != rq.v2) &
-                      (mine.v2 != rq.v0) & (mine.v2 != rq.v1) & (mine.v2 != rq.v2);
-      bool h = h0;
-      if (CONFIRM) {
-        const uint64_t hm0 = ballot(h0);
-        if (hm0 == 0) return;
-        nconf += (unsigned long long)__popcll(hm0);
-        // a uniform address the compiler cannot prove uniform: a vector load,
-        // so its wait does not drain the scalar prefetch
-        const Full64 fq = full[q + zero * lane];
-        h = h0 & confirm64(mf, fq);
-      }
+    // rows in m passed the geometric test against sorted q (record rq); idle
+    // lanes can pass it only against unbounded boxes and never the type test
+    auto hit = [&](int q, const Rec64 &rq, uint64_t m) __attribute__((always_inline)) {
+      const bool h = (((m >> lane) & 1ull) != 0) & ((acc & rq.type) != 0) &
+                     (mine.v0 != rq.v0) & (mine.v0 != rq.v1) & (mine.v0 != rq.v2) &
+                     (mine.v1 != rq.v0) & (mine.v1 != rq.v1) & (mine.v1 != rq.v2) &
+                     (mine.v2 != rq.v0) & (mine.v2 != rq.v1) & (mine.v2 != rq.v2);
       const uint64_t hm = ballot(h);
-      if (hm == 0) return;
-      if (FILL) {
-        if (h) out[base + mbcnt64(hm)] = make_int2((int)op, q);
-        base += __popcll(hm);
-      } else {
-        cnt += (unsigned)__popcll(hm);
-      }
+      if (hm) em.put(h, hm, op, q, out);
     };
-
-    // Geometric masks of one batch of B records (inactive lanes can pass the
-    // geometric test only against unbounded boxes, and never the type test).
-    auto batch = [&](const R *b, int q, bool check) __attribute__((always_inline)) {
+    auto batch = [&](const Rec64 *b, int q, bool check) __attribute__((always_inline)) {
       uint64_t m[B];
       uint64_t any = 0;
 #pragma unroll
@@ -389,17 +452,11 @@ __global__ __launch_bounds__(kThreads) void k_sweep(
           if (m[j]) hit(q + j, b[j], m[j]);
       }
     };
-
-    // q in [qa, qb].  Scalar loads return out of order, so a wait for one
-    // batch waits for every load in flight; the next batch's address is made
-    // to depend on the current batch (times the runtime zero) so that its
-    // loads are issued after that wait, and it is in flight while the
-    // current batch is tested.  Two buffers alternate.
     auto walk = [&](int qa, int qb, bool check) __attribute__((always_inline)) {
       int q = qa;
       int nb = (qb - qa + 1) / B;
       if (nb > 0) {
-        R b0[B], b1[B];
+        Rec64 b0[B], b1[B];
 #pragma unroll
         for (int j = 0; j < B; ++j) b0[j] = rec[q + j];
         while (true) {
@@ -422,22 +479,14 @@ __global__ __launch_bounds__(kThreads) void k_sweep(
         }
       }
       for (; q <= qb; ++q) {
-        const R r = rec[q];
+        const Rec64 r = rec[q];
         uint64_t m = geo_mask(mine, r);
         if (check) m &= ballot(q > p) & ballot(q <= myhi);
         if (m) hit(q, r, m);
       }
     };
-
-    // every active lane has p < q <= hi_p for q in (pmax, hmin]
-    int pmax = act ? p : INT_MIN, hmin = act ? myhi : INT_MAX;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      pmax = max(pmax, __shfl_xor(pmax, o, kWave));
-      hmin = min(hmin, __shfl_xor(hmin, o, kWave));
-    }
-    pmax = uniform(pmax);
-    hmin = uniform(hmin);
+    int pmax, hmin;
+    row_span(act, p, myhi, pmax, hmin);
     const int q0 = item.q0, q1 = item.q1;
     const int b0 = max(q0, pmax + 1), b1 = min(q1, hmin);
     if (b0 <= b1) {
@@ -447,9 +496,192 @@ __global__ __launch_bounds__(kThreads) void k_sweep(
     } else {
       walk(q0, q1, true);
     }
-    if (!FILL && lane == 0) counts[it] = cnt;
+    if (MODE != kFill && lane == 0) a.counts[it] = em.cnt;
   }
-  if (CONFIRM && !FILL && lane == 0 && nconf) atomicAdd(confirms, nconf);
+}
+
+// ---- reduced-precision sweep with FP64 confirmation ----
+// Hot loop: the 8-byte level records of q (QRec) arrive by scalar loads, B per
+// batch, one batch ahead as above; the overlap test of both non-sweep axes is
+// two packed 16-bit subtractions and one sign-bit test:
+//   (hi_p - lo_q) | (hi_q - lo_p) has no negative half  <=>  levels overlap.
+// Positions whose level test passes for some row are appended, with their row
+// mask, to a per-wave queue held in VGPR lanes (entry e in lane e).  When,
+// after a batch, the queue holds kDrainAt entries or more (or the range ends)
+// it is drained: every lane
+// fetches the exact FP64 record of its own entry with one vector load, and
+// each entry is confirmed in queue order with readlane broadcasts: exact FP64
+// overlap on all three axes, type filter, shared-vertex filter.  One load
+// latency is paid per kDrainAt or more candidates, and the emission order is
+// the order of q, as in the plain sweep.
+typedef short short2v __attribute__((ext_vector_type(2)));
+constexpr int kDrainAt = 16;  // a batch adds at most B more: kDrainAt + B <= 64 lanes
+
+__device__ __forceinline__ bool level_overlap(uint32_t phi, uint32_t plo, uint32_t qlo,
+                                              uint32_t qhi) {
+  const short2v d1 = __builtin_bit_cast(short2v, phi) - __builtin_bit_cast(short2v, qlo);
+  const short2v d2 = __builtin_bit_cast(short2v, qhi) - __builtin_bit_cast(short2v, plo);
+  return ((__builtin_bit_cast(uint32_t, d1) | __builtin_bit_cast(uint32_t, d2)) & 0x80008000u) == 0;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+template <int MODE, int B>
+__global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 *__restrict__ rec,
+                                                     const QRec *__restrict__ qrec,
+                                                     int2 *__restrict__ out) {
+  static_assert(kDrainAt + B <= kWave, "queue entries live one per lane");
+  __shared__ int2 slot_mem[kThreads / kWave][kWave];
+  const int zero = a.zero;
+  const int lane = threadIdx.x & 63;
+  int2 *slots = slot_mem[threadIdx.x >> 6];
+  unsigned long long nconf = 0;
+  const WaveSlice ws = wave_slice(a.nitems, a.xcd_split);
+  for (int64_t k = ws.kbeg + ws.wave; k < ws.kend; k += ws.nwaves) {
+    const int64_t it = a.sel ? (int64_t)a.sel[k] : k;
+    const Item item = a.items[it];
+    const bool act = lane < item.nrows;
+    const int p = act ? a.rowlist[item.rowoff + lane] : INT_MAX;
+    const int myhi = act ? a.hi[p] : INT_MIN;
+    QRec mq;
+    mq.lo = 0x7fff7fffu;  // idle lanes: levels above every upper bound
+    mq.hi = 0u;
+    if (act) mq = qrec[p];
+    Emit<MODE> em;
+    if (MODE != kCount) em.base0 = em.base = a.offsets[it];
+    if (MODE == kLearned) em.cap = a.caps[it];
+
+    // the candidate queue: entry e = (position eq, row mask emlo | emhi << 32)
+    int eq = 0;
+    uint32_t emlo = 0, emhi = 0;
+    int nq = 0;
+    // Draining expands the entries into (row, q) candidates, one per lane, in
+    // (entry, row lane) order through this wave's LDS slots, then tests all of
+    // them at once: both exact records are fetched per lane.
+    auto test_slots = [&](int T) __attribute__((always_inline)) {
+      bool h = false, hg = false;
+      int cp = 0, cq = 0;
+      if (lane < T) {
+        const int2 c = slots[lane];
+        cp = c.x;
+        cq = c.y;
+        const Exact64 xp = rec[cp], xq = rec[cq];
+        hg = ((accept_bits(xp.type) & xq.type) != 0) & (xp.hi0 >= xq.lo0) & (xp.lo0 <= xq.hi0) &
+             (xp.hi1 >= xq.lo1) & (xp.lo1 <= xq.hi1) & (xp.hi2 >= xq.lo2) & (xp.lo2 <= xq.hi2);
+        h = hg;
+        h = h & (xp.v0 != xq.v0) & (xp.v0 != xq.v1) & (xp.v0 != xq.v2) & (xp.v1 != xq.v0) &
+            (xp.v1 != xq.v1) & (xp.v1 != xq.v2) & (xp.v2 != xq.v0) & (xp.v2 != xq.v1) &
+            (xp.v2 != xq.v2);
+      }
+      nconf += (unsigned long long)__popcll(ballot(hg));
+      const uint64_t hm = ballot(h);
+      if (hm) {
+        uint32_t o = 0;
+        if (MODE != kCount && h) o = a.orig[cp];
+        em.put(h, hm, o, cq, out);
+      }
+    };
+    auto drain = [&]() __attribute__((always_inline)) {
+      int T = 0;
+      for (int e = 0; e < nq; ++e) {
+        const int q = __builtin_amdgcn_readlane(eq, e);
+        const uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)emhi, e) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)emlo, e);
+        const int c = __popcll(m);
+        if (T + c > kWave) {
+          __builtin_amdgcn_wave_barrier();
+          test_slots(T);
+          __builtin_amdgcn_wave_barrier();
+          T = 0;
+        }
+        if ((m >> lane) & 1ull) slots[T + mbcnt64(m)] = make_int2(p, q);
+        T += c;
+      }
+      __builtin_amdgcn_wave_barrier();
+      test_slots(T);
+      __builtin_amdgcn_wave_barrier();
+      nq = 0;
+    };
+    auto push = [&](int q, uint64_t m) __attribute__((always_inline)) {
+      if (lane == nq) {
+        eq = q;
+        emlo = (uint32_t)m;
+        emhi = (uint32_t)(m >> 32);
+      }
+      ++nq;
+    };
+    auto batch = [&](const QRec *b, int q, bool check) __attribute__((always_inline)) {
+      uint64_t m[B];
+      uint64_t any = 0;
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        m[j] = ballot(level_overlap(mq.hi, mq.lo, b[j].lo, b[j].hi));
+        if (check) m[j] &= ballot(q + j > p) & ballot(q + j <= myhi);
+        any |= m[j];
+      }
+      if (any) {
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+          if (m[j]) push(q + j, m[j]);
+        if (nq >= kDrainAt) drain();
+      }
+    };
+    auto walk = [&](int qa, int qb, bool check) __attribute__((always_inline)) {
+      int q = qa;
+      int nb = (qb - qa + 1) / B;
+      if (nb > 0) {
+        QRec b0[B], b1[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) b0[j] = qrec[q + j];
+        while (true) {
+          q += zero * (int)b0[0].lo;  // waits for b0 on every path
+          if (nb > 1) {
+#pragma unroll
+            for (int j = 0; j < B; ++j) b1[j] = qrec[q + B + j];
+          }
+          batch(b0, q, check);
+          q += B;
+          if (--nb == 0) break;
+          q += zero * (int)b1[0].lo;
+          if (nb > 1) {
+#pragma unroll
+            for (int j = 0; j < B; ++j) b0[j] = qrec[q + B + j];
+          }
+          batch(b1, q, check);
+          q += B;
+          if (--nb == 0) break;
+        }
+      }
+      for (; q <= qb; ++q) {
+        const QRec r = qrec[q];
+        uint64_t m = ballot(level_overlap(mq.hi, mq.lo, r.lo, r.hi));
+        if (check) m &= ballot(q > p) & ballot(q <= myhi);
+        if (m) {
+          push(q, m);
+          if (nq >= kDrainAt) drain();
+        }
+      }
+    };
+    int pmax, hmin;
+    row_span(act, p, myhi, pmax, hmin);
+    const int q0 = item.q0, q1 = item.q1;
+    const int b0 = max(q0, pmax + 1), b1 = min(q1, hmin);
+    if (b0 <= b1) {
+      if (q0 < b0) walk(q0, b0 - 1, true);
+      walk(b0, b1, false);
+      if (b1 < q1) walk(b1 + 1, q1, true);
+    } else {
+      walk(q0, q1, true);
+    }
+    if (nq) drain();
+    if (MODE != kFill && lane == 0) a.counts[it] = em.cnt;
+  }
+  if (MODE != kFill && lane == 0 && nconf) atomicAdd(a.confirms, nconf);
 }
 
 // (input id of the row, sorted q) -> canonical (min id, max id).
@@ -486,11 +718,12 @@ void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, void 
 }
 
 void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, int64_t n,
-                   int axis, void *smin, void *smax, void *rec, Full64 *full, hipStream_t st) {
+                   int axis, const QParams &qp, void *smin, void *smax, void *rec, QRec *qrec,
+                   hipStream_t st) {
   if (n <= 0) return;
   if (f32)
     hipLaunchKernelGGL(k_gather_f32, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, order,
-                       n, axis, (float *)smin, (float *)smax, (Rec32 *)rec, full);
+                       n, axis, qp, (float *)smin, (float *)smax, (Exact64 *)rec, qrec);
   else
     hipLaunchKernelGGL(k_gather_f64, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, order,
                        n, axis, (double *)smin, (double *)smax, (Rec64 *)rec);
@@ -526,30 +759,37 @@ void launch_items(const int4 *groups, const int64_t *ioff, int64_t ngroups, int 
   STQ_HIP(hipGetLastError());
 }
 
-void launch_sweep(bool f32, bool fill, int nblocks, const Item *items, int64_t nitems,
-                  const int *rowlist, const int *hi, const void *rec, const Full64 *full,
-                  const uint32_t *orig, uint32_t *counts, const int64_t *offsets, stq_pair *out,
-                  unsigned long long *confirms, hipStream_t st) {
-  if (nitems <= 0) return;
-  const dim3 grid(nblocks), block(kThreads);
-  int2 *o = reinterpret_cast<int2 *>(out);
+template <int B>
+static void launch_sweep_b(bool f32, int mode, const dim3 &grid, const SweepArgs &args,
+                           hipStream_t st) {
+  const dim3 block(kThreads);
   if (f32) {
-    const Rec32 *r = (const Rec32 *)rec;
-    if (fill)
-      hipLaunchKernelGGL((k_sweep<Rec32, true, true, 3>), grid, block, 0, st, items, nitems,
-                         rowlist, hi, r, full, orig, counts, offsets, o, confirms, 0);
+    const Exact64 *r = static_cast<const Exact64 *>(args.rec);
+    if (mode == kCount)
+      hipLaunchKernelGGL((k_qsweep<kCount, B>), grid, block, 0, st, args, r, args.qrec, args.out);
+    else if (mode == kFill)
+      hipLaunchKernelGGL((k_qsweep<kFill, B>), grid, block, 0, st, args, r, args.qrec, args.out);
     else
-      hipLaunchKernelGGL((k_sweep<Rec32, true, false, 3>), grid, block, 0, st, items, nitems,
-                         rowlist, hi, r, full, orig, counts, offsets, o, confirms, 0);
+      hipLaunchKernelGGL((k_qsweep<kLearned, B>), grid, block, 0, st, args, r, args.qrec, args.out);
   } else {
-    const Rec64 *r = (const Rec64 *)rec;
-    if (fill)
-      hipLaunchKernelGGL((k_sweep<Rec64, false, true, 2>), grid, block, 0, st, items, nitems,
-                         rowlist, hi, r, full, orig, counts, offsets, o, confirms, 0);
+    constexpr int B64 = B > 4 ? 4 : B;  // Rec64 batches above 4 exceed the SGPR file
+    const Rec64 *r = static_cast<const Rec64 *>(args.rec);
+    if (mode == kCount)
+      hipLaunchKernelGGL((k_sweep64<kCount, B64>), grid, block, 0, st, args, r, args.out);
+    else if (mode == kFill)
+      hipLaunchKernelGGL((k_sweep64<kFill, B64>), grid, block, 0, st, args, r, args.out);
     else
-      hipLaunchKernelGGL((k_sweep<Rec64, false, false, 2>), grid, block, 0, st, items, nitems,
-                         rowlist, hi, r, full, orig, counts, offsets, o, confirms, 0);
+      hipLaunchKernelGGL((k_sweep64<kLearned, B64>), grid, block, 0, st, args, r, args.out);
   }
+}
+
+void launch_sweep(bool f32, int mode, int batch, int nblocks, const SweepArgs &args,
+                  hipStream_t st) {
+  if (args.nitems <= 0) return;
+  const dim3 grid(nblocks);
+  if (batch >= 8) launch_sweep_b<8>(f32, mode, grid, args, st);
+  else if (batch >= 4) launch_sweep_b<4>(f32, mode, grid, args, st);
+  else launch_sweep_b<2>(f32, mode, grid, args, st);
   STQ_HIP(hipGetLastError());
 }
 

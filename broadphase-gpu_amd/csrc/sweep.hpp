@@ -13,34 +13,64 @@ struct alignas(16) Rec64 {
   double lo1, hi1, lo2, hi2;
   int v0, v1, v2, type;
 };
-struct alignas(16) Rec32 {
-  float lo1, hi1, lo2, hi2;
+// Reduced-precision form: the two non-sweep axes quantised outward to 15-bit
+// levels, packed (axis 1 in the low half, axis 2 in the high half) for the
+// lower and the upper bounds; and the exact FP64 box (sweep axis first) with
+// its ids for the confirmation of every candidate the levels let through.
+struct alignas(8) QRec {
+  uint32_t lo, hi;
+};
+struct alignas(16) Exact64 {
+  double lo0, hi0, lo1, hi1, lo2, hi2;
   int v0, v1, v2, type;
 };
-// The FP64 box of one sorted position, sweep axis first (FP32-confirm form).
-struct alignas(16) Full64 {
-  double lo0, hi0, lo1, hi1, lo2, hi2;
+// Quantisation of one scene: level(x) = (x - origin) * scale per axis;
+// scale 0 puts every box on the full level range.
+struct QParams {
+  double origin[3];
+  double scale[3];
 };
 
+// Per block: 12 sums (per axis: sum(min+max), sum(min^2+max^2), sum(max-min),
+// sum(centre^2)) then per axis min(min) and max(max): kAxisStats doubles.
+constexpr int kAxisStats = 18;
 void launch_axis_stats(const stq_aabb_f64 *boxes, int64_t n, double *partial, int nblocks,
                        hipStream_t st);
 void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, void *keys,
                  uint32_t *vals, unsigned *err, hipStream_t st);
 void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, int64_t n,
-                   int axis, void *smin, void *smax, void *rec, Full64 *full, hipStream_t st);
+                   int axis, const QParams &qp, void *smin, void *smax, void *rec, QRec *qrec,
+                   hipStream_t st);
 void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, int *hi,
                       unsigned long long *cand, hipStream_t st);
 void launch_plan(const int *hi, int64_t n, int W, int chunk, int *rowlist, int4 *groups,
                  uint32_t *nchunks, unsigned long long *lane_steps, hipStream_t st);
 void launch_items(const int4 *groups, const int64_t *ioff, int64_t ngroups, int chunk,
                   Item *items, hipStream_t st);
-// Count pass (fill = false): counts[it] = pairs of item it.  Fill pass: pairs
-// of item it go to out[offsets[it] ..) as (input id of the row, sorted q);
-// launch_translate then turns them into canonical input-id pairs.
-void launch_sweep(bool f32, bool fill, int nblocks, const Item *items, int64_t nitems,
-                  const int *rowlist, const int *hi, const void *rec, const Full64 *full,
-                  const uint32_t *orig, uint32_t *counts, const int64_t *offsets, stq_pair *out,
-                  unsigned long long *confirms, hipStream_t st);
+enum SweepMode { kCount = 0, kFill = 1, kLearned = 2 };
+
+struct SweepArgs {
+  const Item *items = nullptr;
+  int64_t nitems = 0;            // work items (or selected items when sel != null)
+  const int *sel = nullptr;      // optional item selection
+  const int *rowlist = nullptr;
+  const int *hi = nullptr;
+  const void *rec = nullptr;     // Rec64 (FP64 form) or Exact64 by sorted position
+  const QRec *qrec = nullptr;    // reduced-precision form only
+  const uint32_t *orig = nullptr;
+  uint32_t *counts = nullptr;    // kCount, kLearned: exact pairs per item
+  const int64_t *offsets = nullptr;  // kFill: output offset; kLearned: scratch offset
+  const uint32_t *caps = nullptr;    // kLearned: capacity per item
+  int2 *out = nullptr;           // kFill: output; kLearned: scratch
+  unsigned long long *confirms = nullptr;
+  int zero = 0;                  // always 0 (an opaque runtime zero)
+  int xcd_split = 1;             // 1: each XCD sweeps one contiguous slice of items
+};
+
+// Pairs come out as (input id of the row, sorted q); launch_translate turns
+// them into canonical input-id pairs.
+void launch_sweep(bool f32, int mode, int batch, int nblocks, const SweepArgs &args,
+                  hipStream_t st);
 void launch_translate(stq_pair *pairs, int64_t count, const uint32_t *orig, hipStream_t st);
 
 }  // namespace stqamd

@@ -27,11 +27,12 @@ class StqError(RuntimeError):
 
 
 def options(precision=PRECISION_F64, axis_policy=AXIS_X, sizing=SIZING_COUNT_FILL,
-            autotune=0, chunk=0, window=0, waves_per_cu=0) -> Options:
+            autotune=0, chunk=0, window=0, waves_per_cu=0, batch=0, xcd_split=0) -> Options:
     o = Options()
     _lib.lib().stq_options_default(ctypes.byref(o))
     o.precision, o.axis_policy, o.sizing = precision, axis_policy, sizing
     o.autotune, o.chunk, o.window, o.waves_per_cu = autotune, chunk, window, waves_per_cu
+    o.batch, o.xcd_split = batch, xcd_split
     return o
 
 

@@ -47,7 +47,9 @@ class Options(ctypes.Structure):
         ("chunk", ctypes.c_int32),
         ("window", ctypes.c_int32),
         ("waves_per_cu", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 9),
+        ("batch", ctypes.c_int32),
+        ("xcd_split", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
 

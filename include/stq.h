@@ -121,7 +121,11 @@ typedef struct stq_options {
   int32_t chunk;         /* q-steps per sweep work item; 0 = auto */
   int32_t window;        /* rows regrouped per tile; 0 = auto     */
   int32_t waves_per_cu;  /* sweep residency per CU; 0 = auto      */
-  int32_t reserved[9];
+  int32_t batch;         /* q records per scalar-load batch of the
+                            sweep: 0 = auto, or 2, 4, 8            */
+  int32_t xcd_split;     /* work split over XCDs: 0 = auto (on),
+                            1 = on, 2 = off                        */
+  int32_t reserved[7];
 } stq_options;
 
 typedef struct stq_stats {

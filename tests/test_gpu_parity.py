@@ -72,13 +72,14 @@ def test_scenes_vs_oracle(ctx, form, kw, name, maker):
         assert same_set(got, ref), (name, form, axis)
 
 
-@pytest.mark.parametrize("window,chunk", [(64, 1), (64, 7), (256, 64), (2048, 100000)])
-def test_planning_parameters_do_not_change_the_set(ctx, window, chunk):
+@pytest.mark.parametrize("window,chunk,batch,xcd", [(64, 1, 2, 1), (64, 7, 4, 2), (256, 64, 8, 1),
+                                                   (2048, 100000, 0, 2), (128, 33, 8, 2)])
+def test_planning_parameters_do_not_change_the_set(ctx, window, chunk, batch, xcd):
     b = scenes.uniform_scene(20000, seed=8, extent_scale=4.0)
     ref = oracle.reference_set(b)
     for kw in (dict(), dict(precision=stq_amd.PRECISION_F32_CONFIRM)):
-        got = run(ctx, b, window=window, chunk=chunk, **kw)
-        assert same_set(got, ref), (window, chunk, kw)
+        got = run(ctx, b, window=window, chunk=chunk, batch=batch, xcd_split=xcd, **kw)
+        assert same_set(got, ref), (window, chunk, batch, xcd, kw)
 
 
 def test_deterministic_output_order(ctx):
