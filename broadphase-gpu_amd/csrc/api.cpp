@@ -18,11 +18,15 @@
 #include <vector>
 
 #include "stq_internal.hpp"
+#include "sizer.hpp"
 #include "sweep.hpp"
 
 namespace stqamd {
 
-enum Ev { E_START, E_KEYS, E_SORT, E_GATHER, E_PLAN, E_COUNT, E_SCAN, E_FILL, E_END, E_NUM };
+enum Ev {
+  E_START, E_KEYS, E_SORT, E_GATHER, E_PLAN, E_PRED, E_COUNT, E_SCAN, E_FILL, E_COMPACT, E_END,
+  E_NUM
+};
 
 struct Context {
   int device = 0;
@@ -36,6 +40,13 @@ struct Context {
   DevBuf keys, vals, smin, smax, geo, meta, full, hi, rowlist, groups, nchunks, ioff, items,
       counts, offsets, scan_tmp, counters, axis_partial, h2d_boxes, d2h_pairs;
   SortScratch sort;
+  // learned sizing: parameters, Adam state, per-item buffers
+  DevBuf sz_params, sz_m, sz_v, sz_grad, sz_partial, sz_loss, feat, cand, pred, caps, cap_off,
+      targets, bfeat, btargets, scratch_pairs, ovf, mae_acc;
+  bool sz_ready = false;
+  int adam_t = 0;
+  int64_t scenes_seen = 0;
+  bool learned_pass = false;  // the last call used the single learned pass
 
   ~Context() {
     for (auto &e : ev)
@@ -50,11 +61,21 @@ struct Context {
     sort.counts.release();
     sort.hist.release();
     sort.misc.release();
+    DevBuf *szb[] = {&sz_params, &sz_m, &sz_v, &sz_grad, &sz_partial, &sz_loss, &feat, &cand,
+                     &pred, &caps, &cap_off, &targets, &bfeat, &btargets, &scratch_pairs, &ovf,
+                     &mae_acc};
+    for (DevBuf *b : szb) b->release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 
   int pick_axis(const stq_aabb_f64 *boxes, int64_t n, bool need_q, QParams *qp);
   void finish_stats(hipStream_t st);
+  void sizer_ensure();
+  void run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const float *d_feat,
+                   const uint32_t *d_cand, stq_pair *out, int64_t capacity, stq_pair **alloc_out,
+                   int64_t *count);
+  void sizer_train(const float *d_feat, const float *d_target, int64_t n, int steps, float lr);
+  float sizer_gradient(const float *d_feat, const float *d_target, int64_t n);
   void run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t capacity,
            stq_pair **alloc_out, int64_t *count);
 };
@@ -164,7 +185,8 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   STQ_HIP(hipMemsetAsync(ctr, 0, 8 * sizeof(unsigned long long), st));
 
   QParams qp{};
-  const int axis = pick_axis(boxes, n, f32, &qp);
+  const bool learned = opt.sizing == STQ_SIZING_LEARNED;
+  const int axis = pick_axis(boxes, n, f32 || learned, &qp);
   stats.axis = axis;
   uint32_t *order = vals.as<uint32_t>((size_t)n);
   void *kbuf = keys.get((size_t)n * (f32 ? 4 : 8));
@@ -233,6 +255,27 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   sa.confirms = &ctr[3];
   sa.xcd_split = opt.xcd_split == 2 ? 0 : 1;
   const int batch = opt.batch ? opt.batch : (f32 ? 8 : 2);
+
+  // learned sizing: features of every item; the network predicts, or (while
+  // training) learns from the exact counts of this call
+  float *d_feat = nullptr;
+  uint32_t *d_cand = nullptr;
+  const int train_scenes = opt.train_scenes > 0 ? opt.train_scenes : (opt.train_scenes < 0 ? 0 : 200);
+  const bool use_net = learned && nitems > 0;
+  learned_pass = use_net && scenes_seen >= train_scenes;
+  stats.n_tiles = use_net ? nitems : 0;
+  if (use_net) {
+    sizer_ensure();
+    d_feat = feat.as<float>((size_t)nitems * kFeat);
+    d_cand = cand.as<uint32_t>((size_t)nitems);
+    launch_features(f32, sa, nullptr, nullptr, qp, axis, d_feat, d_cand, st);
+    ++scenes_seen;
+  }
+  if (learned_pass) {
+    run_learned(sa, f32, batch, nblocks, d_feat, d_cand, out, capacity, alloc_out, count);
+    return;
+  }
+  STQ_HIP(hipEventRecord(ev[E_PRED], st));
   launch_sweep(f32, kCount, batch, nblocks, sa, st);
   STQ_HIP(hipEventRecord(ev[E_COUNT], st));
   scan_u32_to_i64(d_counts, d_off, nitems, scan_tmp, st);
@@ -242,6 +285,22 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   STQ_HIP(hipStreamSynchronize(st));
   *count = total;
   stats.n_pairs = total;
+  if (use_net) {
+    // prediction error before this scene's training, then the training steps
+    float *d_pred = pred.as<float>((size_t)nitems);
+    const float *w = sz_params.as<float>(kParams);
+    launch_mlp_forward(w, d_feat, nitems, d_pred, st);
+    double *acc = mae_acc.as<double>(1);
+    STQ_HIP(hipMemsetAsync(acc, 0, sizeof(double), st));
+    launch_abs_error(d_pred, d_counts, nitems, acc, st);
+    float *d_t = targets.as<float>((size_t)nitems);
+    launch_targets(d_counts, nitems, d_t, st);
+    sizer_train(d_feat, d_t, nitems, 16, 1e-3f);
+    double mae = 0;
+    STQ_HIP(hipMemcpyAsync(&mae, acc, sizeof(double), hipMemcpyDeviceToHost, st));
+    STQ_HIP(hipStreamSynchronize(st));
+    stats.sizing_mae = mae / (double)nitems;
+  }
   stq_pair *dst = out;
   if (alloc_out) {
     void *p = nullptr;
@@ -264,9 +323,137 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     launch_sweep(f32, kFill, batch, nblocks, sa, st);
   }
   STQ_HIP(hipEventRecord(ev[E_FILL], st));
+  STQ_HIP(hipEventRecord(ev[E_COMPACT], st));
   launch_translate(dst, total, order, st);
   STQ_HIP(hipEventRecord(ev[E_END], st));
   finish_stats(st);
+}
+
+// The single emitting pass sized by the network.  Every item gets
+// min(candidates, 1.25 * predicted + 16) slots of a scratch buffer; the pass
+// writes what fits and counts everything, so the exact counts (and the
+// output offsets) come out of the same pass.  Items whose count exceeded
+// their slots are redone by the exact fill pass straight into the output;
+// the rest are copied.  The output order is fixed by the exact counts alone,
+// so it is the same as the count-then-fill path's.
+void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const float *d_feat,
+                          const uint32_t *d_cand, stq_pair *out, int64_t capacity,
+                          stq_pair **alloc_out, int64_t *count) {
+  hipStream_t st = stream;
+  const int64_t nitems = sa.nitems;
+  const float *w = sz_params.as<float>(kParams);
+  float *d_pred = pred.as<float>((size_t)nitems);
+  uint32_t *d_caps = caps.as<uint32_t>((size_t)nitems);
+  int64_t *d_capoff = cap_off.as<int64_t>((size_t)nitems + 1);
+  launch_mlp_forward(w, d_feat, nitems, d_pred, st);
+  launch_caps(d_pred, d_cand, nitems, 1.25f, 16.f, d_caps, st);
+  scan_u32_to_i64(d_caps, d_capoff, nitems, scan_tmp, st);
+  int64_t total_cap = 0;
+  STQ_HIP(hipMemcpyAsync(&total_cap, d_capoff + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipEventRecord(ev[E_PRED], st));
+  STQ_HIP(hipStreamSynchronize(st));
+  int2 *d_scratch = scratch_pairs.as<int2>((size_t)std::max<int64_t>(total_cap, 1));
+  STQ_HIP(hipEventRecord(ev[E_COUNT], st));
+  uint32_t *d_counts = sa.counts;
+  int64_t *d_off = offsets.as<int64_t>((size_t)nitems + 1);
+  sa.offsets = d_capoff;
+  sa.caps = d_caps;
+  sa.out = d_scratch;
+  launch_sweep(f32, kLearned, batch, nblocks, sa, st);
+  STQ_HIP(hipEventRecord(ev[E_FILL], st));
+  scan_u32_to_i64(d_counts, d_off, nitems, scan_tmp, st);
+  int64_t total = 0;
+  STQ_HIP(hipMemcpyAsync(&total, d_off + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  double *acc = mae_acc.as<double>(1);
+  STQ_HIP(hipMemsetAsync(acc, 0, sizeof(double), st));
+  launch_abs_error(d_pred, d_counts, nitems, acc, st);
+  STQ_HIP(hipEventRecord(ev[E_SCAN], st));
+  STQ_HIP(hipStreamSynchronize(st));
+  *count = total;
+  stats.n_pairs = total;
+  stq_pair *dst = out;
+  if (alloc_out) {
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(stq_pair) * (size_t)std::max<int64_t>(total, 1)) != hipSuccess) {
+      (void)hipGetLastError();
+      throw StatusError(STQ_ERR_OUT_OF_MEMORY, "output allocation failed");
+    }
+    dst = static_cast<stq_pair *>(p);
+    *alloc_out = dst;
+  } else if (total > capacity) {
+    for (int e = E_COMPACT; e < E_NUM; ++e) STQ_HIP(hipEventRecord(ev[e], st));
+    finish_stats(st);
+    throw StatusError(STQ_ERR_CAPACITY, "output capacity too small");
+  }
+  int *d_ovf = ovf.as<int>((size_t)nitems);
+  unsigned *d_novf = reinterpret_cast<unsigned *>(counters.as<unsigned long long>(8) + 4);
+  STQ_HIP(hipMemsetAsync(d_novf, 0, sizeof(unsigned), st));
+  launch_compact(d_counts, d_caps, d_capoff, d_off, nitems, d_scratch,
+                 reinterpret_cast<int2 *>(dst), d_ovf, d_novf, st);
+  unsigned novf = 0;
+  STQ_HIP(hipMemcpyAsync(&novf, d_novf, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipStreamSynchronize(st));
+  stats.n_fallback_tiles = novf;
+  if (novf > 0) {
+    SweepArgs fa = sa;
+    fa.sel = d_ovf;
+    fa.nitems = novf;
+    fa.counts = nullptr;
+    fa.confirms = nullptr;
+    fa.caps = nullptr;
+    fa.offsets = d_off;
+    fa.out = reinterpret_cast<int2 *>(dst);
+    launch_sweep(f32, kFill, batch, nblocks, fa, st);
+  }
+  STQ_HIP(hipEventRecord(ev[E_COMPACT], st));
+  launch_translate(dst, total, sa.orig, st);
+  STQ_HIP(hipEventRecord(ev[E_END], st));
+  double mae = 0;
+  STQ_HIP(hipMemcpyAsync(&mae, acc, sizeof(double), hipMemcpyDeviceToHost, st));
+  finish_stats(st);
+  stats.sizing_mae = mae / (double)std::max<int64_t>(nitems, 1);
+}
+
+void Context::sizer_ensure() {
+  if (sz_ready) return;
+  std::vector<float> h(kParams);
+  sizer_init_params(h.data(), 0x5717u);
+  float *w = sz_params.as<float>(kParams);
+  STQ_HIP(hipMemcpyAsync(w, h.data(), sizeof(float) * kParams, hipMemcpyHostToDevice, stream));
+  STQ_HIP(hipMemsetAsync(sz_m.as<float>(kParams), 0, sizeof(float) * kParams, stream));
+  STQ_HIP(hipMemsetAsync(sz_v.as<float>(kParams), 0, sizeof(float) * kParams, stream));
+  STQ_HIP(hipStreamSynchronize(stream));
+  adam_t = 0;
+  sz_ready = true;
+}
+
+float Context::sizer_gradient(const float *d_feat, const float *d_target, int64_t n) {
+  const int64_t nb = (n + 31) / 32;
+  float *part = sz_partial.as<float>((size_t)nb * (kParams + 1));
+  float *g = sz_grad.as<float>(kParams);
+  float *loss = sz_loss.as<float>(1);
+  launch_mlp_gradient(sz_params.as<float>(kParams), d_feat, d_target, n, part, g, loss, stream);
+  return 0.f;
+}
+
+// `steps` Adam steps on minibatches of up to 8192 items taken with a
+// stride co-prime to n (a fixed pseudo-random order, restarted each call).
+void Context::sizer_train(const float *d_feat, const float *d_target, int64_t n, int steps,
+                          float lr) {
+  if (n <= 0) return;
+  const int64_t bs = std::min<int64_t>(n, 8192);
+  float *bf = bfeat.as<float>((size_t)bs * kFeat);
+  float *bt = btargets.as<float>((size_t)bs);
+  int64_t stride = 7919;
+  while (std::__gcd(stride, n) != 1) ++stride;
+  for (int s = 0; s < steps; ++s) {
+    const int64_t start = (int64_t)s * bs;
+    launch_gather_batch(d_feat, d_target, n, start, stride, bs, bf, bt, stream);
+    sizer_gradient(bf, bt, bs);
+    ++adam_t;
+    launch_adam(sz_params.as<float>(kParams), sz_grad.as<float>(kParams), sz_m.as<float>(kParams),
+                sz_v.as<float>(kParams), lr, adam_t, stream);
+  }
 }
 
 void Context::finish_stats(hipStream_t st) {
@@ -281,9 +468,19 @@ void Context::finish_stats(hipStream_t st) {
   stats.ms_sort = ev_ms(ev[E_KEYS], ev[E_SORT]);
   stats.ms_gather = ev_ms(ev[E_SORT], ev[E_GATHER]);
   stats.ms_plan = ev_ms(ev[E_GATHER], ev[E_PLAN]);
-  stats.ms_sweep_count = ev_ms(ev[E_PLAN], ev[E_COUNT]);
-  stats.ms_scan = ev_ms(ev[E_COUNT], ev[E_SCAN]);
-  stats.ms_sweep_fill = ev_ms(ev[E_SCAN], ev[E_FILL]);
+  stats.ms_predict = ev_ms(ev[E_PLAN], ev[E_PRED]);
+  if (learned_pass) {
+    // predict | (scratch) | learned pass | scan | compact + refill | translate
+    stats.ms_sweep_count = 0;
+    stats.ms_sweep_fill = ev_ms(ev[E_COUNT], ev[E_FILL]);
+    stats.ms_scan = ev_ms(ev[E_FILL], ev[E_SCAN]);
+    stats.ms_compact = ev_ms(ev[E_SCAN], ev[E_COMPACT]);
+  } else {
+    stats.ms_sweep_count = ev_ms(ev[E_PRED], ev[E_COUNT]);
+    stats.ms_scan = ev_ms(ev[E_COUNT], ev[E_SCAN]);
+    stats.ms_sweep_fill = ev_ms(ev[E_SCAN], ev[E_FILL]);
+    stats.ms_compact = 0;
+  }
 }
 
 template <class F>
@@ -518,6 +715,129 @@ int stq_free_device(void *ctx, void *d_ptr) {
 }
 
 void stq_free_host(void *h_ptr) { std::free(h_ptr); }
+
+int stq_sizer_num_params(void) { return stqamd::kParams; }
+int stq_sizer_num_features(void) { return stqamd::kFeat; }
+
+int stq_sizer_get_params(void *ctx, float *h_params) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !h_params) return STQ_ERR_INVALID_ARGUMENT;
+  return stqamd::guarded(c, [&] {
+    c->sizer_ensure();
+    STQ_HIP(hipMemcpyAsync(h_params, c->sz_params.p, sizeof(float) * stqamd::kParams,
+                           hipMemcpyDeviceToHost, c->stream));
+    STQ_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int stq_sizer_set_params(void *ctx, const float *h_params) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !h_params) return STQ_ERR_INVALID_ARGUMENT;
+  return stqamd::guarded(c, [&] {
+    c->sizer_ensure();
+    STQ_HIP(hipMemcpyAsync(c->sz_params.p, h_params, sizeof(float) * stqamd::kParams,
+                           hipMemcpyHostToDevice, c->stream));
+    STQ_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int stq_sizer_reset(void *ctx, uint64_t seed) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c) return STQ_ERR_INVALID_ARGUMENT;
+  return stqamd::guarded(c, [&] {
+    c->sizer_ensure();
+    std::vector<float> h(stqamd::kParams);
+    stqamd::sizer_init_params(h.data(), seed);
+    STQ_HIP(hipMemcpyAsync(c->sz_params.p, h.data(), sizeof(float) * stqamd::kParams,
+                           hipMemcpyHostToDevice, c->stream));
+    STQ_HIP(hipMemsetAsync(c->sz_m.p, 0, sizeof(float) * stqamd::kParams, c->stream));
+    STQ_HIP(hipMemsetAsync(c->sz_v.p, 0, sizeof(float) * stqamd::kParams, c->stream));
+    STQ_HIP(hipStreamSynchronize(c->stream));
+    c->adam_t = 0;
+    c->scenes_seen = 0;
+  });
+}
+
+// Host rows to device scratch (features and optional targets).
+static void upload_rows(Context *c, const float *h_feat, const float *h_t, int64_t n, float **df,
+                        float **dt) {
+  *df = c->bfeat.as<float>((size_t)std::max<int64_t>(n, 1) * stqamd::kFeat);
+  STQ_HIP(hipMemcpyAsync(*df, h_feat, sizeof(float) * stqamd::kFeat * n, hipMemcpyHostToDevice,
+                         c->stream));
+  if (dt) {
+    *dt = c->btargets.as<float>((size_t)std::max<int64_t>(n, 1));
+    STQ_HIP(hipMemcpyAsync(*dt, h_t, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+  }
+}
+
+int stq_sizer_forward(void *ctx, const float *h_features, int64_t n, float *h_out) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || n < 0 || (n > 0 && (!h_features || !h_out))) return STQ_ERR_INVALID_ARGUMENT;
+  return stqamd::guarded(c, [&] {
+    if (n == 0) return;
+    c->sizer_ensure();
+    float *df = nullptr;
+    upload_rows(c, h_features, nullptr, n, &df, nullptr);
+    float *dp = c->pred.as<float>((size_t)n);
+    stqamd::launch_mlp_forward(c->sz_params.as<float>(stqamd::kParams), df, n, dp, c->stream);
+    STQ_HIP(hipMemcpyAsync(h_out, dp, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    STQ_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int stq_sizer_gradient(void *ctx, const float *h_features, const float *h_targets, int64_t n,
+                       float *h_grad, float *h_loss) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || n <= 0 || !h_features || !h_targets || !h_grad || !h_loss)
+    return STQ_ERR_INVALID_ARGUMENT;
+  return stqamd::guarded(c, [&] {
+    c->sizer_ensure();
+    float *df = nullptr, *dt = nullptr;
+    upload_rows(c, h_features, h_targets, n, &df, &dt);
+    c->sizer_gradient(df, dt, n);
+    STQ_HIP(hipMemcpyAsync(h_grad, c->sz_grad.p, sizeof(float) * stqamd::kParams,
+                           hipMemcpyDeviceToHost, c->stream));
+    STQ_HIP(hipMemcpyAsync(h_loss, c->sz_loss.p, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    STQ_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int stq_sizer_train(void *ctx, const float *h_features, const float *h_targets, int64_t n,
+                    int32_t steps, float lr) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || n <= 0 || steps < 0 || !h_features || !h_targets || !(lr > 0))
+    return STQ_ERR_INVALID_ARGUMENT;
+  return stqamd::guarded(c, [&] {
+    c->sizer_ensure();
+    // rows go to the per-item buffers; training draws minibatches from them
+    float *df = c->feat.as<float>((size_t)n * stqamd::kFeat);
+    float *dt = c->targets.as<float>((size_t)n);
+    STQ_HIP(hipMemcpyAsync(df, h_features, sizeof(float) * stqamd::kFeat * n, hipMemcpyHostToDevice,
+                           c->stream));
+    STQ_HIP(hipMemcpyAsync(dt, h_targets, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    c->sizer_train(df, dt, n, steps, lr);
+    STQ_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int stq_sizer_last_items(void *ctx, float *h_features, uint32_t *h_counts, int64_t capacity,
+                         int64_t *n) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !n || capacity < 0) return STQ_ERR_INVALID_ARGUMENT;
+  return stqamd::guarded(c, [&] {
+    const int64_t m = c->stats.n_tiles;
+    *n = m;
+    const int64_t k = std::min(m, capacity);
+    if (k <= 0) return;
+    if (h_features)
+      STQ_HIP(hipMemcpyAsync(h_features, c->feat.p, sizeof(float) * stqamd::kFeat * k,
+                             hipMemcpyDeviceToHost, c->stream));
+    if (h_counts)
+      STQ_HIP(hipMemcpyAsync(h_counts, c->counts.p, sizeof(uint32_t) * k, hipMemcpyDeviceToHost,
+                             c->stream));
+    STQ_HIP(hipStreamSynchronize(c->stream));
+  });
+}
 
 int stq_get_stats(void *ctx, stq_stats *stats) {
   Context *c = static_cast<Context *>(ctx);

@@ -1,0 +1,426 @@
+// sizer.hip -- features, MLP inference/training and the compaction of the
+// learned single-pass sweep (see sizer.hpp).  Small kernels: the network has
+// 6,337 parameters and a scene has at most a few hundred thousand items.
+#include <climits>
+#include <cmath>
+
+#include "sizer.hpp"
+
+namespace stqamd {
+
+namespace {
+
+constexpr int kT = 256;
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// ---- features ----------------------------------------------------------------
+// One wave per item.  Rows are the item's lanes; the q range is sampled at 64
+// evenly spaced positions.  Extents enter as log(extent)/10, centres as the
+// position inside the scene's quantisation range (0..1).
+template <class R>
+__global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items, int64_t nitems,
+                                                 const int *__restrict__ rowlist,
+                                                 const int *__restrict__ hi,
+                                                 const R *__restrict__ rec, QParams qp, int a1,
+                                                 int a2, float *__restrict__ feat,
+                                                 uint32_t *__restrict__ cand) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kT / 64) + (threadIdx.x >> 6)));
+  const int nwaves = (int)gridDim.x * (kT / 64);
+  const double n1 = qp.scale[a1] / 32767.0, n2 = qp.scale[a2] / 32767.0;
+  const double o1 = qp.origin[a1], o2 = qp.origin[a2];
+  for (int64_t it = wave; it < nitems; it += nwaves) {
+    const Item item = items[it];
+    const bool act = lane < item.nrows;
+    const int q0 = item.q0, q1 = item.q1;
+    float fr = act ? 1.f : 0.f;
+    float le1 = 0, le2 = 0, c1 = 0, c2 = 0, tv = 0, te = 0, tf = 0, chain = 0;
+    long long cnd = 0;
+    if (act) {
+      const int p = rowlist[item.rowoff + lane];
+      const int h = hi[p];
+      const R r = rec[p];
+      le1 = (float)(log(fmax(r.hi1 - r.lo1, 1e-30)) * 0.1);
+      le2 = (float)(log(fmax(r.hi2 - r.lo2, 1e-30)) * 0.1);
+      c1 = (float)((0.5 * (r.lo1 + r.hi1) - o1) * n1);
+      c2 = (float)((0.5 * (r.lo2 + r.hi2) - o2) * n2);
+      tv = r.type == 1 ? 1.f : 0.f;
+      te = r.type == 2 ? 1.f : 0.f;
+      tf = r.type == 4 ? 1.f : 0.f;
+      const int lo = max(q0, p + 1), up = min(q1, h);
+      cnd = up >= lo ? (long long)(up - lo + 1) : 0;
+      chain = (float)(h - p);
+    }
+    const int span = q1 - q0;
+    const int qs = q0 + (span > 0 ? (int)(((long long)lane * span) / 63) : 0);
+    const R s = rec[qs];
+    const float se1 = (float)(log(fmax(s.hi1 - s.lo1, 1e-30)) * 0.1);
+    const float se2 = (float)(log(fmax(s.hi2 - s.lo2, 1e-30)) * 0.1);
+    const float sc1 = (float)((0.5 * (s.lo1 + s.hi1) - o1) * n1);
+    const float sc2 = (float)((0.5 * (s.lo2 + s.hi2) - o2) * n2);
+    const float sv = s.type == 1 ? 1.f : 0.f, se = s.type == 2 ? 1.f : 0.f,
+                sf = s.type == 4 ? 1.f : 0.f;
+    const float nr = fmaxf(wave_sum(fr), 1.f);
+    const float m_le1 = wave_sum(le1) / nr, m_le2 = wave_sum(le2) / nr;
+    const float v_le1 = wave_sum(le1 * le1) / nr - m_le1 * m_le1;
+    const float v_le2 = wave_sum(le2 * le2) / nr - m_le2 * m_le2;
+    const float m_c1 = wave_sum(c1) / nr, m_c2 = wave_sum(c2) / nr;
+    const float v_c1 = wave_sum(c1 * c1) / nr - m_c1 * m_c1;
+    const float v_c2 = wave_sum(c2 * c2) / nr - m_c2 * m_c2;
+    const float f_v = wave_sum(tv) / nr, f_e = wave_sum(te) / nr, f_f = wave_sum(tf) / nr;
+    const float m_chain = wave_sum(chain) / nr;
+    const long long cand_it = wave_sum(cnd);
+    const float m_se1 = wave_sum(se1) / 64.f, m_se2 = wave_sum(se2) / 64.f;
+    const float v_se1 = wave_sum(se1 * se1) / 64.f - m_se1 * m_se1;
+    const float v_se2 = wave_sum(se2 * se2) / 64.f - m_se2 * m_se2;
+    const float m_sc1 = wave_sum(sc1) / 64.f, m_sc2 = wave_sum(sc2) / 64.f;
+    const float v_sc1 = wave_sum(sc1 * sc1) / 64.f - m_sc1 * m_sc1;
+    const float v_sc2 = wave_sum(sc2 * sc2) / 64.f - m_sc2 * m_sc2;
+    const float g_v = wave_sum(sv) / 64.f, g_e = wave_sum(se) / 64.f, g_f = wave_sum(sf) / 64.f;
+    if (lane == 0) {
+      const float steps = 64.f * (float)(span + 1);
+      const float e1r = expf(10.f * m_le1) * n1, e1q = expf(10.f * m_se1) * n1;
+      const float e2r = expf(10.f * m_le2) * n2, e2q = expf(10.f * m_se2) * n2;
+      const float sd_c1 = sqrtf(fmaxf(v_c1, 0.f)) + sqrtf(fmaxf(v_sc1, 0.f));
+      const float sd_c2 = sqrtf(fmaxf(v_c2, 0.f)) + sqrtf(fmaxf(v_sc2, 0.f));
+      const float py = fminf(1.f, (e1r + e1q) / (sd_c1 + fabsf(m_c1 - m_sc1) + 1e-12f));
+      const float pz = fminf(1.f, (e2r + e2q) / (sd_c2 + fabsf(m_c2 - m_sc2) + 1e-12f));
+      const float tc = f_v * g_f + f_f * g_v + f_e * g_e;
+      float *f = feat + it * kFeat;
+      f[0] = nr / 64.f;
+      f[1] = log1pf((float)cand_it) * 0.1f;
+      f[2] = log1pf(steps) * 0.1f;
+      f[3] = (float)cand_it / steps;
+      f[4] = m_le1;
+      f[5] = sqrtf(fmaxf(v_le1, 0.f));
+      f[6] = m_le2;
+      f[7] = sqrtf(fmaxf(v_le2, 0.f));
+      f[8] = m_c1;
+      f[9] = sqrtf(fmaxf(v_c1, 0.f));
+      f[10] = m_c2;
+      f[11] = sqrtf(fmaxf(v_c2, 0.f));
+      f[12] = f_v;
+      f[13] = f_e;
+      f[14] = f_f;
+      f[15] = m_se1;
+      f[16] = sqrtf(fmaxf(v_se1, 0.f));
+      f[17] = m_se2;
+      f[18] = sqrtf(fmaxf(v_se2, 0.f));
+      f[19] = m_sc1;
+      f[20] = sqrtf(fmaxf(v_sc1, 0.f));
+      f[21] = m_sc2;
+      f[22] = sqrtf(fmaxf(v_sc2, 0.f));
+      f[23] = g_v;
+      f[24] = g_e;
+      f[25] = g_f;
+      f[26] = logf(py + 1e-9f) * 0.1f;
+      f[27] = logf(pz + 1e-9f) * 0.1f;
+      f[28] = tc;
+      f[29] = log1pf((float)cand_it * py * pz * tc) * 0.1f;
+      f[30] = log1pf(m_chain) * 0.1f;
+      f[31] = 1.f;
+      cand[it] = (uint32_t)min(cand_it, (long long)UINT_MAX);
+    }
+  }
+}
+
+// ---- MLP forward -----------------------------------------------------------------
+__global__ __launch_bounds__(kT) void k_mlp_forward(const float *__restrict__ params,
+                                                    const float *__restrict__ feat, int64_t n,
+                                                    float *__restrict__ pred) {
+  __shared__ float w[kParams];
+  for (int i = threadIdx.x; i < kParams; i += kT) w[i] = params[i];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  float x[kFeat];
+#pragma unroll
+  for (int k = 0; k < kFeat; ++k) x[k] = feat[i * kFeat + k];
+  float h1[kHid];
+  for (int j = 0; j < kHid; ++j) {
+    float s = w[kOffB1 + j];
+#pragma unroll
+    for (int k = 0; k < kFeat; ++k) s += w[kOffW1 + j * kFeat + k] * x[k];
+    h1[j] = fmaxf(s, 0.f);
+  }
+  float out = w[kOffB3];
+  for (int j = 0; j < kHid; ++j) {
+    float s = w[kOffB2 + j];
+#pragma unroll
+    for (int k = 0; k < kHid; ++k) s += w[kOffW2 + j * kHid + k] * h1[k];
+    out += w[kOffW3 + j] * fmaxf(s, 0.f);
+  }
+  pred[i] = out;
+}
+
+// ---- MLP gradient --------------------------------------------------------------------
+// One wave per block, 32 items per block.  Phase 1: lanes 0..31 run forward
+// and backward for one item each and stage the per-item vectors in LDS.
+// Phase 2: every lane sums its share of the parameter gradients over the
+// block's items.  partial[block] holds kParams gradients and the loss.
+constexpr int kGradItems = 32;
+
+__global__ __launch_bounds__(64) void k_mlp_grad(const float *__restrict__ params,
+                                                 const float *__restrict__ feat,
+                                                 const float *__restrict__ target, int64_t n,
+                                                 float *__restrict__ partial) {
+  __shared__ float xs[kGradItems][kFeat + 1];
+  __shared__ float h1s[kGradItems][kHid + 1];
+  __shared__ float h2s[kGradItems][kHid + 1];
+  __shared__ float d1s[kGradItems][kHid + 1];
+  __shared__ float d2s[kGradItems][kHid + 1];
+  __shared__ float douts[kGradItems];
+  __shared__ float losses[kGradItems];
+  const int t = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * kGradItems;
+  const float invn = 1.f / (float)n;
+  if (t < kGradItems) {
+    const int64_t i = base + t;
+    const bool valid = i < n;
+    for (int k = 0; k < kFeat; ++k) xs[t][k] = valid ? feat[i * kFeat + k] : 0.f;
+    for (int j = 0; j < kHid; ++j) {
+      float s = params[kOffB1 + j];
+      for (int k = 0; k < kFeat; ++k) s += params[kOffW1 + j * kFeat + k] * xs[t][k];
+      h1s[t][j] = fmaxf(s, 0.f);
+    }
+    float out = params[kOffB3];
+    for (int j = 0; j < kHid; ++j) {
+      float s = params[kOffB2 + j];
+      for (int k = 0; k < kHid; ++k) s += params[kOffW2 + j * kHid + k] * h1s[t][k];
+      const float h2 = fmaxf(s, 0.f);
+      h2s[t][j] = h2;
+      out += params[kOffW3 + j] * h2;
+    }
+    const float err = valid ? out - target[i] : 0.f;
+    const float dout = 2.f * err * invn;
+    douts[t] = dout;
+    losses[t] = err * err * invn;
+    for (int j = 0; j < kHid; ++j)
+      d2s[t][j] = h2s[t][j] > 0.f ? dout * params[kOffW3 + j] : 0.f;
+    for (int k = 0; k < kHid; ++k) {
+      float s = 0.f;
+      for (int j = 0; j < kHid; ++j) s += d2s[t][j] * params[kOffW2 + j * kHid + k];
+      d1s[t][k] = h1s[t][k] > 0.f ? s : 0.f;
+    }
+  }
+  __syncthreads();
+  float *out = partial + (int64_t)blockIdx.x * (kParams + 1);
+  for (int pidx = t; pidx <= kParams; pidx += 64) {
+    float g = 0.f;
+    if (pidx < kOffB1) {
+      const int j = pidx / kFeat, k = pidx % kFeat;
+      for (int s = 0; s < kGradItems; ++s) g += d1s[s][j] * xs[s][k];
+    } else if (pidx < kOffW2) {
+      const int j = pidx - kOffB1;
+      for (int s = 0; s < kGradItems; ++s) g += d1s[s][j];
+    } else if (pidx < kOffB2) {
+      const int j = (pidx - kOffW2) / kHid, k = (pidx - kOffW2) % kHid;
+      for (int s = 0; s < kGradItems; ++s) g += d2s[s][j] * h1s[s][k];
+    } else if (pidx < kOffW3) {
+      const int j = pidx - kOffB2;
+      for (int s = 0; s < kGradItems; ++s) g += d2s[s][j];
+    } else if (pidx < kOffB3) {
+      const int j = pidx - kOffW3;
+      for (int s = 0; s < kGradItems; ++s) g += douts[s] * h2s[s][j];
+    } else if (pidx == kOffB3) {
+      for (int s = 0; s < kGradItems; ++s) g += douts[s];
+    } else {
+      for (int s = 0; s < kGradItems; ++s) g += losses[s];
+    }
+    out[pidx] = g;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_grad_reduce(const float *__restrict__ partial, int nblocks,
+                                                    float *__restrict__ grad,
+                                                    float *__restrict__ loss) {
+  const int pidx = blockIdx.x * kT + threadIdx.x;
+  if (pidx > kParams) return;
+  float g = 0.f;
+  for (int b = 0; b < nblocks; ++b) g += partial[(int64_t)b * (kParams + 1) + pidx];
+  if (pidx < kParams) grad[pidx] = g;
+  else *loss = g;
+}
+
+__global__ __launch_bounds__(kT) void k_adam(float *__restrict__ w, const float *__restrict__ g,
+                                             float *__restrict__ m, float *__restrict__ v, float lr,
+                                             float c1, float c2) {
+  const int i = blockIdx.x * kT + threadIdx.x;
+  if (i >= kParams) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ __launch_bounds__(kT) void k_caps(const float *__restrict__ pred,
+                                             const uint32_t *__restrict__ cand, int64_t n,
+                                             float alpha, float beta,
+                                             uint32_t *__restrict__ caps) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const float pr = fminf(fmaxf(pred[i], 0.f), 30.f);
+  const float want = ceilf(alpha * expm1f(pr) + beta);
+  const uint32_t c = cand[i];
+  caps[i] = want >= (float)c ? c : (uint32_t)want;
+}
+
+__global__ __launch_bounds__(kT) void k_targets(const uint32_t *__restrict__ counts, int64_t n,
+                                                float *__restrict__ target) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) target[i] = log1pf((float)counts[i]);
+}
+
+__global__ __launch_bounds__(kT) void k_abs_error(const float *__restrict__ pred,
+                                                  const uint32_t *__restrict__ counts, int64_t n,
+                                                  double *__restrict__ acc) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  double e = 0.0;
+  if (i < n) e = fabs((double)expm1f(fminf(fmaxf(pred[i], 0.f), 30.f)) - (double)counts[i]);
+  e = wave_sum(e);
+  if ((threadIdx.x & 63) == 0 && e != 0.0) atomicAdd(acc, e);
+}
+
+__global__ __launch_bounds__(kT) void k_compact(const uint32_t *__restrict__ counts,
+                                                const uint32_t *__restrict__ caps,
+                                                const int64_t *__restrict__ cap_off,
+                                                const int64_t *__restrict__ off, int64_t n,
+                                                const int2 *__restrict__ scratch,
+                                                int2 *__restrict__ out, int *__restrict__ ovf,
+                                                unsigned *__restrict__ novf) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kT / 64) + (threadIdx.x >> 6)));
+  const int nwaves = (int)gridDim.x * (kT / 64);
+  for (int64_t it = wave; it < n; it += nwaves) {
+    const uint32_t c = counts[it];
+    if (c <= caps[it]) {
+      const int2 *src = scratch + cap_off[it];
+      int2 *dst = out + off[it];
+      for (uint32_t k = lane; k < c; k += 64) dst[k] = src[k];
+    } else if (lane == 0) {
+      ovf[atomicAdd(novf, 1u)] = (int)it;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_gather_batch(const float *__restrict__ feat,
+                                                     const float *__restrict__ target, int64_t n,
+                                                     int64_t start, int64_t stride, int64_t bs,
+                                                     float *__restrict__ bfeat,
+                                                     float *__restrict__ btarget) {
+  const int64_t k = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (k >= bs) return;
+  const int64_t i = (int64_t)(((unsigned __int128)(start + k) * (unsigned __int128)stride) % (unsigned __int128)n);
+  for (int f = 0; f < kFeat; ++f) bfeat[k * kFeat + f] = feat[i * kFeat + f];
+  btarget[k] = target[i];
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
+
+}  // namespace
+
+void launch_features(bool f32, const SweepArgs &sa, const void *, const void *, const QParams &qp,
+                     int axis, float *feat, uint32_t *cand, hipStream_t st) {
+  if (sa.nitems <= 0) return;
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  const unsigned g = (unsigned)std::min<int64_t>((sa.nitems + 3) / 4, 4096);
+  if (f32)
+    hipLaunchKernelGGL((k_features<Exact64>), dim3(g), dim3(kT), 0, st, sa.items, sa.nitems,
+                       sa.rowlist, sa.hi, (const Exact64 *)sa.rec, qp, a1, a2, feat, cand);
+  else
+    hipLaunchKernelGGL((k_features<Rec64>), dim3(g), dim3(kT), 0, st, sa.items, sa.nitems,
+                       sa.rowlist, sa.hi, (const Rec64 *)sa.rec, qp, a1, a2, feat, cand);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_mlp_forward(const float *params, const float *feat, int64_t n, float *pred,
+                        hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mlp_forward, dim3(nblk(n)), dim3(kT), 0, st, params, feat, n, pred);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_caps(const float *pred, const uint32_t *cand, int64_t n, float alpha, float beta,
+                 uint32_t *caps, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_caps, dim3(nblk(n)), dim3(kT), 0, st, pred, cand, n, alpha, beta, caps);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_targets(const uint32_t *counts, int64_t n, float *targets, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_targets, dim3(nblk(n)), dim3(kT), 0, st, counts, n, targets);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_mlp_gradient(const float *params, const float *feat, const float *targets, int64_t n,
+                         float *partial, float *grad, float *loss, hipStream_t st) {
+  if (n <= 0) return;
+  const int nb = (int)((n + kGradItems - 1) / kGradItems);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(64), 0, st, params, feat, targets, n, partial);
+  hipLaunchKernelGGL(k_grad_reduce, dim3((kParams + 1 + kT - 1) / kT), dim3(kT), 0, st,
+                     (const float *)partial, nb, grad, loss);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_adam(float *params, const float *grad, float *m, float *v, float lr, int t,
+                 hipStream_t st) {
+  const float c1 = 1.f - powf(0.9f, (float)t), c2 = 1.f - powf(0.999f, (float)t);
+  hipLaunchKernelGGL(k_adam, dim3((kParams + kT - 1) / kT), dim3(kT), 0, st, params, grad, m, v,
+                     lr, c1, c2);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_abs_error(const float *pred, const uint32_t *counts, int64_t n, double *acc,
+                      hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_abs_error, dim3(nblk(n)), dim3(kT), 0, st, pred, counts, n, acc);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_compact(const uint32_t *counts, const uint32_t *caps, const int64_t *cap_off,
+                    const int64_t *off, int64_t n, const int2 *scratch, int2 *out, int *ovf,
+                    unsigned *novf, hipStream_t st) {
+  if (n <= 0) return;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 3) / 4, 8192);
+  hipLaunchKernelGGL(k_compact, dim3(g), dim3(kT), 0, st, counts, caps, cap_off, off, n, scratch,
+                     out, ovf, novf);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_gather_batch(const float *feat, const float *target, int64_t n, int64_t start,
+                         int64_t stride, int64_t bs, float *bfeat, float *btarget, hipStream_t st) {
+  if (bs <= 0) return;
+  hipLaunchKernelGGL(k_gather_batch, dim3(nblk(bs)), dim3(kT), 0, st, feat, target, n, start,
+                     stride, bs, bfeat, btarget);
+  STQ_HIP(hipGetLastError());
+}
+
+// splitmix64 of (seed, index) -> uniform in [-1, 1)
+static inline float hash_uniform(uint64_t seed, uint64_t i) {
+  uint64_t z = seed + 0x9e3779b97f4a7c15ull * (i + 1);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return (float)((double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0);
+}
+
+void sizer_init_params(float *w, uint64_t seed) {
+  const float a1 = std::sqrt(6.f / kFeat), a2 = std::sqrt(6.f / kHid), a3 = std::sqrt(1.f / kHid);
+  for (int i = 0; i < kParams; ++i) w[i] = 0.f;
+  for (int i = 0; i < kHid * kFeat; ++i) w[kOffW1 + i] = a1 * hash_uniform(seed, kOffW1 + i);
+  for (int i = 0; i < kHid * kHid; ++i) w[kOffW2 + i] = a2 * hash_uniform(seed, kOffW2 + i);
+  for (int i = 0; i < kHid; ++i) w[kOffW3 + i] = a3 * hash_uniform(seed, kOffW3 + i);
+  w[kOffB3] = 2.f;
+}
+
+}  // namespace stqamd

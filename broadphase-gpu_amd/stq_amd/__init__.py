@@ -27,12 +27,13 @@ class StqError(RuntimeError):
 
 
 def options(precision=PRECISION_F64, axis_policy=AXIS_X, sizing=SIZING_COUNT_FILL,
-            autotune=0, chunk=0, window=0, waves_per_cu=0, batch=0, xcd_split=0) -> Options:
+            autotune=0, chunk=0, window=0, waves_per_cu=0, batch=0, xcd_split=0,
+            train_scenes=0) -> Options:
     o = Options()
     _lib.lib().stq_options_default(ctypes.byref(o))
     o.precision, o.axis_policy, o.sizing = precision, axis_policy, sizing
     o.autotune, o.chunk, o.window, o.waves_per_cu = autotune, chunk, window, waves_per_cu
-    o.batch, o.xcd_split = batch, xcd_split
+    o.batch, o.xcd_split, o.train_scenes = batch, xcd_split, train_scenes
     return o
 
 
@@ -110,6 +111,53 @@ class Context:
                 self._L.stq_free_device(self.ptr, out)
             db.close()
         return res
+
+    # -- the learned sizing network ---------------------------------------------
+    def sizer_params(self) -> np.ndarray:
+        w = np.empty(self._L.stq_sizer_num_params(), dtype=np.float32)
+        self._check(self._L.stq_sizer_get_params(self.ptr, w.ctypes.data), "stq_sizer_get_params")
+        return w
+
+    def sizer_set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self._L.stq_sizer_num_params()
+        self._check(self._L.stq_sizer_set_params(self.ptr, w.ctypes.data), "stq_sizer_set_params")
+
+    def sizer_reset(self, seed=0x5717):
+        self._check(self._L.stq_sizer_reset(self.ptr, seed), "stq_sizer_reset")
+
+    def sizer_forward(self, feat) -> np.ndarray:
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        out = np.empty(len(f), dtype=np.float32)
+        self._check(self._L.stq_sizer_forward(self.ptr, f.ctypes.data, len(f), out.ctypes.data),
+                    "stq_sizer_forward")
+        return out
+
+    def sizer_gradient(self, feat, targets):
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        g = np.empty(self._L.stq_sizer_num_params(), dtype=np.float32)
+        loss = ctypes.c_float()
+        self._check(self._L.stq_sizer_gradient(self.ptr, f.ctypes.data, t.ctypes.data, len(f),
+                                               g.ctypes.data, ctypes.byref(loss)), "stq_sizer_gradient")
+        return float(loss.value), g
+
+    def sizer_train(self, feat, targets, steps, lr=1e-3):
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        self._check(self._L.stq_sizer_train(self.ptr, f.ctypes.data, t.ctypes.data, len(f), steps, lr),
+                    "stq_sizer_train")
+
+    def sizer_last_items(self):
+        """(features (T,32), exact pair counts (T,)) of the last learned-sizing call."""
+        n = ctypes.c_int64()
+        self._check(self._L.stq_sizer_last_items(self.ptr, None, None, 0, ctypes.byref(n)),
+                    "stq_sizer_last_items")
+        f = np.empty((n.value, self._L.stq_sizer_num_features()), dtype=np.float32)
+        c = np.empty(n.value, dtype=np.uint32)
+        self._check(self._L.stq_sizer_last_items(self.ptr, f.ctypes.data, c.ctypes.data, n.value,
+                                                 ctypes.byref(n)), "stq_sizer_last_items")
+        return f, c
 
     def find_pairs_host(self, boxes: np.ndarray) -> np.ndarray:
         b = np.ascontiguousarray(boxes)

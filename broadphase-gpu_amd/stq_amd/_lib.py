@@ -33,6 +33,9 @@ EXPORTS = [
     "stq_find_pairs_device", "stq_find_pairs_device_alloc", "stq_find_pairs_host",
     "stq_find_pairs_batched_device", "stq_device_alloc", "stq_copy_to_device",
     "stq_copy_to_host", "stq_free_device", "stq_free_host",
+    "stq_sizer_num_params", "stq_sizer_num_features", "stq_sizer_get_params",
+    "stq_sizer_set_params", "stq_sizer_reset", "stq_sizer_forward", "stq_sizer_gradient",
+    "stq_sizer_train", "stq_sizer_last_items",
     "stq_get_stats", "stq_last_error", "stq_status_string",
     "stq_sizeof_aabb", "stq_sizeof_options", "stq_sizeof_stats",
 ]
@@ -49,7 +52,8 @@ class Options(ctypes.Structure):
         ("waves_per_cu", ctypes.c_int32),
         ("batch", ctypes.c_int32),
         ("xcd_split", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("train_scenes", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -112,6 +116,15 @@ def lib():
         "stq_copy_to_host": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
         "stq_free_device": (ctypes.c_int, [P, P]),
         "stq_free_host": (None, [P]),
+        "stq_sizer_num_params": (ctypes.c_int, []),
+        "stq_sizer_num_features": (ctypes.c_int, []),
+        "stq_sizer_get_params": (ctypes.c_int, [P, P]),
+        "stq_sizer_set_params": (ctypes.c_int, [P, P]),
+        "stq_sizer_reset": (ctypes.c_int, [P, ctypes.c_uint64]),
+        "stq_sizer_forward": (ctypes.c_int, [P, P, I64, P]),
+        "stq_sizer_gradient": (ctypes.c_int, [P, P, P, I64, P, P]),
+        "stq_sizer_train": (ctypes.c_int, [P, P, P, I64, I32, ctypes.c_float]),
+        "stq_sizer_last_items": (ctypes.c_int, [P, P, P, I64, ctypes.POINTER(I64)]),
         "stq_get_stats": (ctypes.c_int, [P, ctypes.POINTER(Stats)]),
         "stq_last_error": (ctypes.c_char_p, [P]),
         "stq_status_string": (ctypes.c_char_p, [ctypes.c_int]),

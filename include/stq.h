@@ -125,7 +125,11 @@ typedef struct stq_options {
                             sweep: 0 = auto, or 2, 4, 8            */
   int32_t xcd_split;     /* work split over XCDs: 0 = auto (on),
                             1 = on, 2 = off                        */
-  int32_t reserved[7];
+  int32_t train_scenes;  /* learned sizing: scenes run count-then-fill
+                            while the network trains on their exact
+                            counts before the single learned pass is
+                            used; 0 = 200, negative = none        */
+  int32_t reserved[6];
 } stq_options;
 
 typedef struct stq_stats {
@@ -211,6 +215,32 @@ int stq_copy_to_device(void *ctx, void *d_dst, const void *h_src, size_t bytes);
 int stq_copy_to_host(void *ctx, void *h_dst, const void *d_src, size_t bytes);
 int stq_free_device(void *ctx, void *d_ptr);
 void stq_free_host(void *h_ptr);
+
+/* ---- the learned sizing network (no counterpart in the reference) ----
+   An MLP (32 features -> 64 -> 64 -> 1, ReLU, FP32 weights) predicts the
+   log1p of each sweep work item's pair count.  Its parameters live on the
+   context's device; these entry points expose it for inspection, for
+   gradient checks and for offline training.  Feature rows are 32 floats;
+   the parameter vector is W1[64][32], b1[64], W2[64][64], b2[64], W3[64], b3. */
+int stq_sizer_num_params(void);
+int stq_sizer_num_features(void);
+int stq_sizer_get_params(void *ctx, float *h_params);
+int stq_sizer_set_params(void *ctx, const float *h_params);
+/* Re-initialises the parameters (deterministic in seed), the Adam state and
+   the count of scenes seen. */
+int stq_sizer_reset(void *ctx, uint64_t seed);
+int stq_sizer_forward(void *ctx, const float *h_features, int64_t n, float *h_out);
+/* Mean squared error over n rows and its gradient (num_params floats). */
+int stq_sizer_gradient(void *ctx, const float *h_features, const float *h_targets, int64_t n,
+                       float *h_grad, float *h_loss);
+/* `steps` Adam steps with learning rate lr on the given rows. */
+int stq_sizer_train(void *ctx, const float *h_features, const float *h_targets, int64_t n,
+                    int32_t steps, float lr);
+/* Features and exact pair counts of the work items of the last call made
+   with learned sizing (the training data the network sees).  *n receives
+   the number of items; at most `capacity` rows are copied. */
+int stq_sizer_last_items(void *ctx, float *h_features, uint32_t *h_counts, int64_t capacity,
+                         int64_t *n);
 
 /* Statistics of the last call on this context. */
 int stq_get_stats(void *ctx, stq_stats *stats);
