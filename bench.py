@@ -6,8 +6,11 @@ device buffer.  Two forms run on the same scene:
 
   plain  : FP64 sort keys and comparisons, sweep axis x, exact count pass +
            fill pass (the reference method, src/simulation.cu:555-782)
-  chosen : FP32 outward-rounded sweep + FP64 confirmation of every candidate,
-           chain-length axis policy (and the learned sizing when enabled)
+  chosen : FP32 outward-rounded keys and chain ends, 15-bit outward levels in
+           the sweep's hot loop, FP64 confirmation of every candidate,
+           chain-length axis policy, and learned sizing: one emitting pass
+           into per-item slots predicted by the network (trained online on
+           the exact counts of a stream of other scenes before timing)
 
 The JSON line reports the chosen form's boxes/s as `value`, its pairs/s, the
 plain form's numbers and the speedup, whether both pair sets are identical,
@@ -42,6 +45,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=float, default=0.25,
                     help="fraction of sorted rows the CPU baseline sweeps (0 = skip)")
     ap.add_argument("--no-check", action="store_true", help="skip the pair-set comparison")
+    ap.add_argument("--train-scenes", type=int, default=80,
+                    help="scenes of the online training stream run before timing the chosen form")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (traffic calibration)")
     return ap.parse_args()
@@ -84,15 +89,35 @@ def main():
 
     forms = {
         "chosen": dict(precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN,
-                       sizing=stq_amd.SIZING_COUNT_FILL),
+                       sizing=stq_amd.SIZING_LEARNED, train_scenes=max(args.train_scenes, 1)),
         "plain": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X,
                       sizing=stq_amd.SIZING_COUNT_FILL),
     }
     want = [f for f in args.forms.split(",") if f]
     res = {}
     outputs = {}
+    training = None
     for name in want:
         ctx = stq_amd.Context(0, **forms[name])
+        if forms[name]["sizing"] == stq_amd.SIZING_LEARNED:
+            # the online training phase: exact counts of a stream of other
+            # scenes (cloth-like and uniform-like, mixed sizes and densities)
+            rng = np.random.default_rng(1234)
+            maes, t0 = [], time.perf_counter()
+            for i in range(max(args.train_scenes, 1)):
+                nb = int(10 ** rng.uniform(np.log10(5e4), np.log10(6e5)))
+                rho = float(10 ** rng.uniform(np.log10(0.25), np.log10(8.0)))
+                sc = scenes.density_scene(i % 2, nb, rho, seed=50_000 + i)
+                dsc = stq_amd.DeviceBuffer.from_array(sc)
+                try:
+                    ctx.find_pairs_device(dsc.ptr, len(sc), None, 0)
+                except stq_amd.StqError as e:
+                    if e.status != _lib.STQ_ERR_CAPACITY:
+                        raise
+                maes.append(ctx.stats()["sizing_mae"])
+                dsc.close()
+            training = {"scenes": len(maes), "seconds": time.perf_counter() - t0,
+                        "mae_first": maes[0], "mae_last": maes[-1]}
         # size the output once (the count is part of every step anyway)
         try:
             ctx.find_pairs_device(d_boxes.ptr, n, None, 0)
@@ -104,19 +129,21 @@ def main():
         for _ in range(args.warmup):
             ctx.find_pairs_device(d_boxes.ptr, n, d_out.ptr, cap)
         H.hipDeviceSynchronize()
-        sweeps, totals = [], []
+        sweeps, totals, fallback = [], [], []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             cnt = ctx.find_pairs_device(d_boxes.ptr, n, d_out.ptr, cap)
             st = ctx.stats()
             sweeps.append(st["ms_sweep_fill"])
             totals.append(st["ms_total"])
+            fallback.append(st["n_fallback_tiles"])
         H.hipDeviceSynchronize()
         t1 = time.perf_counter()
         ms = (t1 - t0) * 1e3 / max(args.steps, 1)
         res[name] = dict(ms_per_step=ms, boxes_per_s=n / (ms * 1e-3), pairs_per_s=cnt / (ms * 1e-3),
                          pairs=cnt, stream_ms=float(np.median(totals)),
-                         fill_sweep_ms=float(np.mean(sweeps)), stats=st, form=forms[name])
+                         fill_sweep_ms=float(np.mean(sweeps)), stats=st, form=forms[name],
+                         fallback_items=int(np.max(fallback)) if fallback else 0)
         if not args.no_check:
             outputs[name] = d_out.to_array(np.int32, 2 * cnt).reshape(-1, 2)
         d_out.close()
@@ -188,9 +215,12 @@ def main():
                   for k, v in res.items()},
         "speedup_vs_plain_fp64": (res["chosen"]["boxes_per_s"] / res["plain"]["boxes_per_s"]
                                   if "chosen" in res and "plain" in res else None),
+        "sizing": ({"training": training, "items": r["stats"]["n_tiles"],
+                    "fallback_items": r["fallback_items"], "mae_pairs_per_item": r["stats"]["sizing_mae"]}
+                   if training else None),
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "k_sweep (emitting pass)",
+                     "kernel": "emitting sweep (k_qsweep learned pass for the chosen form)",
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "kernel_ms": r["fill_sweep_ms"]},
         "cpu_baseline": cpu,

@@ -23,6 +23,17 @@
 
 namespace stqamd {
 
+// Learned sizing: Adam steps (minibatches of up to 8192 items) and learning
+// rate per training scene; slots per item = kCapScale * prediction + kCapSlack.
+constexpr int kTrainSteps = 64;
+constexpr float kTrainRate = 2e-3f;
+constexpr float kCapScale = 2.0f;
+constexpr float kCapSlack = 32.f;
+// Replay ring of training rows: each training scene adds a strided sample of
+// up to kReplayPerScene of its items.
+constexpr int64_t kReplay = 262144;
+constexpr int64_t kReplayPerScene = 8192;
+
 enum Ev {
   E_START, E_KEYS, E_SORT, E_GATHER, E_PLAN, E_PRED, E_COUNT, E_SCAN, E_FILL, E_COMPACT, E_END,
   E_NUM
@@ -42,9 +53,10 @@ struct Context {
   SortScratch sort;
   // learned sizing: parameters, Adam state, per-item buffers
   DevBuf sz_params, sz_m, sz_v, sz_grad, sz_partial, sz_loss, feat, cand, pred, caps, cap_off,
-      targets, bfeat, btargets, scratch_pairs, ovf, mae_acc;
+      targets, bfeat, btargets, scratch_pairs, ovf, mae_acc, replay_feat, replay_t;
   bool sz_ready = false;
   int adam_t = 0;
+  int64_t replay_n = 0, replay_pos = 0, train_calls = 0;
   int64_t scenes_seen = 0;
   bool learned_pass = false;  // the last call used the single learned pass
 
@@ -63,7 +75,7 @@ struct Context {
     sort.misc.release();
     DevBuf *szb[] = {&sz_params, &sz_m, &sz_v, &sz_grad, &sz_partial, &sz_loss, &feat, &cand,
                      &pred, &caps, &cap_off, &targets, &bfeat, &btargets, &scratch_pairs, &ovf,
-                     &mae_acc};
+                     &mae_acc, &replay_feat, &replay_t};
     for (DevBuf *b : szb) b->release();
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -75,6 +87,7 @@ struct Context {
                    const uint32_t *d_cand, stq_pair *out, int64_t capacity, stq_pair **alloc_out,
                    int64_t *count);
   void sizer_train(const float *d_feat, const float *d_target, int64_t n, int steps, float lr);
+  void sizer_remember(const float *d_feat, const float *d_target, int64_t n);
   float sizer_gradient(const float *d_feat, const float *d_target, int64_t n);
   void run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t capacity,
            stq_pair **alloc_out, int64_t *count);
@@ -295,7 +308,11 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     launch_abs_error(d_pred, d_counts, nitems, acc, st);
     float *d_t = targets.as<float>((size_t)nitems);
     launch_targets(d_counts, nitems, d_t, st);
-    sizer_train(d_feat, d_t, nitems, 16, 1e-3f);
+    // a sample of this scene joins the replay ring; training draws from the
+    // whole ring, so earlier scenes are not forgotten
+    sizer_remember(d_feat, d_t, nitems);
+    sizer_train(replay_feat.as<float>((size_t)kReplay * kFeat), replay_t.as<float>(kReplay), replay_n,
+                kTrainSteps, kTrainRate);
     double mae = 0;
     STQ_HIP(hipMemcpyAsync(&mae, acc, sizeof(double), hipMemcpyDeviceToHost, st));
     STQ_HIP(hipStreamSynchronize(st));
@@ -330,7 +347,8 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
 }
 
 // The single emitting pass sized by the network.  Every item gets
-// min(candidates, 1.25 * predicted + 16) slots of a scratch buffer; the pass
+// min(candidates, kCapScale * predicted + kCapSlack) slots of a scratch
+// buffer; the pass
 // writes what fits and counts everything, so the exact counts (and the
 // output offsets) come out of the same pass.  Items whose count exceeded
 // their slots are redone by the exact fill pass straight into the output;
@@ -346,7 +364,7 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
   uint32_t *d_caps = caps.as<uint32_t>((size_t)nitems);
   int64_t *d_capoff = cap_off.as<int64_t>((size_t)nitems + 1);
   launch_mlp_forward(w, d_feat, nitems, d_pred, st);
-  launch_caps(d_pred, d_cand, nitems, 1.25f, 16.f, d_caps, st);
+  launch_caps(d_pred, d_cand, nitems, kCapScale, kCapSlack, d_caps, st);
   scan_u32_to_i64(d_caps, d_capoff, nitems, scan_tmp, st);
   int64_t total_cap = 0;
   STQ_HIP(hipMemcpyAsync(&total_cap, d_capoff + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -436,8 +454,24 @@ float Context::sizer_gradient(const float *d_feat, const float *d_target, int64_
   return 0.f;
 }
 
-// `steps` Adam steps on minibatches of up to 8192 items taken with a
-// stride co-prime to n (a fixed pseudo-random order, restarted each call).
+void Context::sizer_remember(const float *d_feat, const float *d_target, int64_t n) {
+  if (n <= 0) return;
+  float *rf = replay_feat.as<float>((size_t)kReplay * kFeat);
+  float *rt = replay_t.as<float>((size_t)kReplay);
+  const int64_t take = std::min(n, kReplayPerScene);
+  int64_t stride = 104729;
+  while (std::__gcd(stride, n) != 1) ++stride;
+  const int64_t first = std::min(take, kReplay - replay_pos);
+  launch_gather_batch(d_feat, d_target, n, 0, stride, first, rf + replay_pos * kFeat,
+                      rt + replay_pos, stream);
+  if (take > first)
+    launch_gather_batch(d_feat, d_target, n, first, stride, take - first, rf, rt, stream);
+  replay_pos = (replay_pos + take) % kReplay;
+  replay_n = std::min(kReplay, replay_n + take);
+}
+
+// `steps` Adam steps on minibatches of up to 8192 rows taken with a stride
+// co-prime to n (a fixed pseudo-random order continued across calls).
 void Context::sizer_train(const float *d_feat, const float *d_target, int64_t n, int steps,
                           float lr) {
   if (n <= 0) return;
@@ -446,8 +480,9 @@ void Context::sizer_train(const float *d_feat, const float *d_target, int64_t n,
   float *bt = btargets.as<float>((size_t)bs);
   int64_t stride = 7919;
   while (std::__gcd(stride, n) != 1) ++stride;
+  const int64_t call = train_calls++;
   for (int s = 0; s < steps; ++s) {
-    const int64_t start = (int64_t)s * bs;
+    const int64_t start = ((call * steps + s) * bs) % n;
     launch_gather_batch(d_feat, d_target, n, start, stride, bs, bf, bt, stream);
     sizer_gradient(bf, bt, bs);
     ++adam_t;
@@ -755,6 +790,9 @@ int stq_sizer_reset(void *ctx, uint64_t seed) {
     STQ_HIP(hipStreamSynchronize(c->stream));
     c->adam_t = 0;
     c->scenes_seen = 0;
+    c->replay_n = 0;
+    c->replay_pos = 0;
+    c->train_calls = 0;
   });
 }
 
