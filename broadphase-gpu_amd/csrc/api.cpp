@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "stq_internal.hpp"
 #include "sizer.hpp"
 #include "sweep.hpp"
@@ -33,6 +34,55 @@ constexpr float kCapSlack = 32.f;
 // up to kReplayPerScene of its items.
 constexpr int64_t kReplay = 262144;
 constexpr int64_t kReplayPerScene = 8192;
+
+// Batched scenes: planning window when none is set.
+constexpr int kBatchWindow = 256;
+
+// One batch shape (scene offsets, window, chunk): its host and device tables
+// and the graph captured for it.
+struct BatchPlan {
+  std::vector<int64_t> offsets;
+  int W = 0, chunk = 0;
+  BatchTables t;
+  BatchDev dev;
+  int64_t item_bound = 0;  // no more items than this, whatever the boxes
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t graph_sig = 0;  // launch signature the graph was captured with
+  uint64_t eager_sig = 0;  // signature of the last eager run
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    graph_sig = 0;
+  }
+};
+
+// Grow-only pinned host buffer (results read back after a batched call).
+struct PinnedBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void *get(size_t need) {
+    if (need > bytes) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      bytes = 0;
+      if (hipHostMalloc(&p, need, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        throw StatusError(STQ_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
+      }
+      bytes = need;
+    }
+    return p;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
 
 enum Ev {
   E_START, E_KEYS, E_SORT, E_GATHER, E_PLAN, E_PRED, E_COUNT, E_SCAN, E_FILL, E_COMPACT, E_END,
@@ -59,8 +109,16 @@ struct Context {
   int64_t replay_n = 0, replay_pos = 0, train_calls = 0;
   int64_t scenes_seen = 0;
   bool learned_pass = false;  // the last call used the single learned pass
+  // batched pipeline
+  DevBuf b_tables, b_skey, b_orig, b_gchunk, b_pairoff;
+  PinnedBuf b_host;
+  BatchPlan bplan;
 
   ~Context() {
+    bplan.drop_graph();
+    b_host.release();
+    DevBuf *bb[] = {&b_tables, &b_skey, &b_orig, &b_gchunk, &b_pairoff};
+    for (DevBuf *b : bb) b->release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
     DevBuf *bufs[] = {&keys,  &vals,    &smin,    &smax,     &geo,          &meta,
@@ -91,6 +149,9 @@ struct Context {
   float sizer_gradient(const float *d_feat, const float *d_target, int64_t n);
   void run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t capacity,
            stq_pair **alloc_out, int64_t *count);
+  void batch_plan(const int64_t *off, int nscenes, int W, int chunk);
+  void run_batched(const stq_aabb_f64 *boxes, const int64_t *off, int nscenes, stq_pair *out,
+                   int64_t capacity, int64_t *pair_off);
 };
 
 struct DeviceGuard {
@@ -518,6 +579,273 @@ void Context::finish_stats(hipStream_t st) {
   }
 }
 
+// ---- batched scenes ----------------------------------------------------------
+// Host tables of a batch shape, uploaded once; the graph of the previous
+// shape is dropped when the shape changes.
+void Context::batch_plan(const int64_t *off, int nscenes, int W, int chunk) {
+  BatchPlan &bp = bplan;
+  if (bp.W == W && bp.chunk == chunk && (int)bp.offsets.size() == nscenes + 1 &&
+      std::equal(bp.offsets.begin(), bp.offsets.end(), off))
+    return;
+  bp.drop_graph();
+  bp.eager_sig = 0;
+  bp.offsets.clear();
+  build_batch_tables(off, nscenes, W, bp.t);
+  const BatchTables &t = bp.t;
+  int64_t bound = 0;
+  for (int s = 0; s < nscenes; ++s) {
+    const int64_t m = off[s + 1] - off[s];
+    if (m < 2) continue;
+    const int64_t g = t.scene_first_group[s + 1] - t.scene_first_group[s];
+    bound += g * std::min<int64_t>(kMaxChunks, (m - 1 + chunk - 1) / chunk);
+  }
+  bp.item_bound = bound;
+
+  const size_t nc = t.chunk_start.size(), nw = t.win_start.size(), ns1 = (size_t)nscenes + 1;
+  size_t at = 0;
+  auto place = [&](size_t bytes) {
+    const size_t o = at;
+    at += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t o_seg = place(8 * ns1), o_cs = place(8 * nc), o_ce = place(8 * nc),
+               o_scb = place(4 * ns1), o_ws = place(8 * nw), o_wc = place(4 * nw),
+               o_wg = place(8 * nw), o_sfg = place(8 * ns1), upload = at;
+  const size_t o_par = place(sizeof(SceneParams) * (size_t)nscenes),
+               o_part = place(sizeof(double) * kAxisStats * std::max<size_t>(nc, 1));
+  std::vector<char> h(upload);
+  auto put = [&](size_t o, const void *src, size_t bytes) {
+    if (bytes) std::memcpy(h.data() + o, src, bytes);
+  };
+  put(o_seg, t.seg_off.data(), 8 * ns1);
+  put(o_cs, t.chunk_start.data(), 8 * nc);
+  put(o_ce, t.chunk_end.data(), 8 * nc);
+  put(o_scb, t.scene_chunk_begin.data(), 4 * ns1);
+  put(o_ws, t.win_start.data(), 8 * nw);
+  put(o_wc, t.win_cnt.data(), 4 * nw);
+  put(o_wg, t.win_gbase.data(), 8 * nw);
+  put(o_sfg, t.scene_first_group.data(), 8 * ns1);
+  char *d = static_cast<char *>(b_tables.get(at));
+  STQ_HIP(hipMemcpyAsync(d, h.data(), upload, hipMemcpyHostToDevice, stream));
+  STQ_HIP(hipStreamSynchronize(stream));
+  BatchDev &dv = bp.dev;
+  dv.seg_off = reinterpret_cast<int64_t *>(d + o_seg);
+  dv.chunk_start = reinterpret_cast<int64_t *>(d + o_cs);
+  dv.chunk_end = reinterpret_cast<int64_t *>(d + o_ce);
+  dv.scene_chunk_begin = reinterpret_cast<int *>(d + o_scb);
+  dv.win_start = reinterpret_cast<int64_t *>(d + o_ws);
+  dv.win_cnt = reinterpret_cast<int *>(d + o_wc);
+  dv.win_gbase = reinterpret_cast<int64_t *>(d + o_wg);
+  dv.scene_first_group = reinterpret_cast<int64_t *>(d + o_sfg);
+  dv.params = reinterpret_cast<SceneParams *>(d + o_par);
+  dv.partial = reinterpret_cast<double *>(d + o_part);
+  bp.offsets.assign(off, off + nscenes + 1);
+  bp.W = W;
+  bp.chunk = chunk;
+}
+
+static uint64_t fnv1a(const void *p, size_t n, uint64_t h = 1469598103934665603ull) {
+  const unsigned char *b = static_cast<const unsigned char *>(p);
+  for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  return h;
+}
+
+// All scenes in one sequence of launches with no host synchronisation until
+// the end: per-scene statistics, segmented keys and sort, scene-bounded chain
+// ends and planning, one count sweep and one fill sweep over every scene's
+// items (the item and pair totals stay on the device), per-scene pair
+// offsets.  The sequence is captured in a hipGraph and replayed while the
+// batch shape, the buffers and the options are unchanged.
+void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int nscenes,
+                          stq_pair *out, int64_t capacity, int64_t *pair_off) {
+  stats = stq_stats{};
+  stats.n_scenes = nscenes;
+  stats.precision = opt.precision;
+  stats.sizing = STQ_SIZING_COUNT_FILL;
+  stats.axis = opt.axis_policy <= 2 ? opt.axis_policy : -1;
+  const bool f32 = opt.precision == STQ_PRECISION_F32_CONFIRM;
+  hipStream_t st = stream;
+  std::vector<int64_t> off((size_t)nscenes + 1);
+  for (int s = 0; s <= nscenes; ++s) off[s] = off_in[s] - off_in[0];
+  boxes += off_in[0];
+  const int64_t n = off[nscenes];
+  if (n > INT32_MAX) throw StatusError(STQ_ERR_TOO_LARGE, "more than 2^31-1 boxes in a batch");
+  stats.n_boxes = n;
+  const int W = opt.window ? opt.window : kBatchWindow;
+  const int chunk = opt.chunk > 0 ? opt.chunk : 2048;
+  batch_plan(off.data(), nscenes, W, chunk);
+  BatchPlan &bp = bplan;
+  const BatchTables &t = bp.t;
+  const int64_t ngroups = t.ngroups, bound = bp.item_bound;
+  if (bound == 0) {  // no scene holds two boxes
+    for (int s = 0; s <= nscenes; ++s) pair_off[s] = 0;
+    STQ_HIP(hipEventRecord(ev[E_START], st));
+    STQ_HIP(hipEventRecord(ev[E_END], st));
+    STQ_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  int sbits = 0;
+  while ((1ll << sbits) < nscenes) ++sbits;
+
+  // every buffer is reserved here: the launches below allocate nothing
+  const size_t ts = f32 ? 4 : 8;
+  uint64_t *d_keys = keys.as<uint64_t>((size_t)n);
+  uint32_t *d_vals = vals.as<uint32_t>((size_t)n);
+  uint32_t *d_skey = f32 ? nullptr : b_skey.as<uint32_t>((size_t)n);
+  radix_sort_reserve(n, 8, sort);
+  void *d_smin = smin.get((size_t)n * ts);
+  void *d_smax = smax.get((size_t)n * ts);
+  void *d_rec = geo.get((size_t)n * (f32 ? sizeof(Exact64) : sizeof(Rec64)));
+  QRec *d_qrec = f32 ? meta.as<QRec>((size_t)n) : nullptr;
+  uint32_t *d_orig = b_orig.as<uint32_t>((size_t)n);
+  int *d_hi = hi.as<int>((size_t)n);
+  int *d_rowlist = rowlist.as<int>((size_t)n);
+  int4 *d_groups = groups.as<int4>((size_t)ngroups);
+  int *d_gchunk = b_gchunk.as<int>((size_t)ngroups);
+  uint32_t *d_nch = nchunks.as<uint32_t>((size_t)ngroups);
+  int64_t *d_ioff = ioff.as<int64_t>((size_t)ngroups + 1);
+  Item *d_items = items.as<Item>((size_t)bound);
+  uint32_t *d_counts = counts.as<uint32_t>((size_t)bound);
+  int64_t *d_off = offsets.as<int64_t>((size_t)bound + 1);
+  scan_reserve_i64(std::max(ngroups, bound), scan_tmp);
+  int64_t *d_pairoff = b_pairoff.as<int64_t>((size_t)nscenes + 1);
+  unsigned long long *ctr = counters.as<unsigned long long>(8);
+  int64_t *hbuf = static_cast<int64_t *>(b_host.get(sizeof(int64_t) * ((size_t)nscenes + 6)));
+
+  const int wpc = opt.waves_per_cu > 0 ? opt.waves_per_cu : 32;
+  const int nblocks = (int)(8 * std::max<int64_t>(
+      1, std::min<int64_t>((bound + 31) / 32, (int64_t)cus * std::max(1, wpc / 4) / 8)));
+  const int batch = opt.batch ? opt.batch : (f32 ? 8 : 2);
+
+  auto enqueue = [&]() {
+    launch_fill_u64(ctr, 0ull, 8, st);
+    launch_seg_stats(boxes, t, bp.dev, opt.axis_policy, opt.axis_policy <= 2 ? opt.axis_policy : 0,
+                     f32, st);
+    launch_keys_seg(f32, boxes, t, bp.dev, d_keys, d_vals, reinterpret_cast<unsigned *>(&ctr[0]),
+                    st);
+    if (f32) {
+      radix_sort_u64_sync_free(d_keys, d_vals, n, 32 + sbits, sort, st);
+    } else {
+      radix_sort_u64_sync_free(d_keys, d_vals, n, 64, sort, st);
+      if (sbits > 0) {
+        launch_scene_keys(d_vals, t, bp.dev, d_skey, st);
+        radix_sort_u32_sync_free(d_skey, d_vals, n, sbits, sort, st);
+      }
+    }
+    launch_gather_seg(f32, boxes, d_vals, t, bp.dev, d_smin, d_smax, d_rec, d_qrec, d_orig, st);
+    launch_chain_end_seg(f32, d_smin, d_smax, t, bp.dev, d_hi, &ctr[1], st);
+    launch_plan_seg(d_hi, t, bp.dev, W, chunk, d_rowlist, d_groups, d_gchunk, d_nch, &ctr[2], st);
+    scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
+    launch_items_seg(d_groups, d_gchunk, d_ioff, ngroups, d_items, st);
+    launch_fill_u32(d_counts, 0u, bound, st);
+    SweepArgs sa;
+    sa.items = d_items;
+    sa.nitems = bound;
+    sa.nitems_dev = d_ioff + ngroups;
+    sa.rowlist = d_rowlist;
+    sa.hi = d_hi;
+    sa.rec = d_rec;
+    sa.qrec = d_qrec;
+    sa.orig = d_orig;
+    sa.counts = d_counts;
+    sa.confirms = &ctr[3];
+    sa.xcd_split = opt.xcd_split == 2 ? 0 : 1;
+    launch_sweep(f32, kCount, batch, nblocks, sa, st);
+    scan_u32_to_i64(d_counts, d_off, bound, scan_tmp, st);
+    sa.counts = nullptr;
+    sa.confirms = nullptr;
+    sa.offsets = d_off;
+    sa.out = reinterpret_cast<int2 *>(out);
+    sa.total_dev = d_off + bound;
+    sa.capacity = capacity;
+    launch_sweep(f32, kFill, batch, nblocks, sa, st);
+    launch_translate_dev(out, d_off + bound, capacity, d_orig, st);
+    launch_scene_pairs(d_off, d_ioff, t, bp.dev, d_pairoff, st);
+    STQ_HIP(hipMemcpyAsync(hbuf, d_pairoff, sizeof(int64_t) * ((size_t)nscenes + 1),
+                           hipMemcpyDeviceToHost, st));
+    STQ_HIP(hipMemcpyAsync(hbuf + nscenes + 1, ctr, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    STQ_HIP(hipMemcpyAsync(hbuf + nscenes + 5, d_ioff + ngroups, sizeof(int64_t),
+                           hipMemcpyDeviceToHost, st));
+  };
+
+  struct {
+    const void *boxes, *out, *host;
+    int64_t capacity;
+    uint64_t generation;
+    int32_t precision, axis_policy, batch, nblocks, xcd_split, pad;
+  } key;
+  std::memset(&key, 0, sizeof(key));
+  key.boxes = boxes;
+  key.out = out;
+  key.host = hbuf;
+  key.capacity = capacity;
+  key.generation = g_devbuf_generation.load();
+  key.precision = opt.precision;
+  key.axis_policy = opt.axis_policy;
+  key.batch = batch;
+  key.nblocks = nblocks;
+  key.xcd_split = opt.xcd_split;
+  const uint64_t sig = fnv1a(&key, sizeof(key)) | 1ull;
+
+  bool done = false;
+  if (opt.graph != 2) {
+    if (bp.exec && bp.graph_sig == sig) {
+      STQ_HIP(hipEventRecord(ev[E_START], st));
+      STQ_HIP(hipGraphLaunch(bp.exec, st));
+      stats.graph = 2;
+      done = true;
+    } else if (opt.graph == 1 || bp.eager_sig == sig) {
+      bp.drop_graph();
+      hipGraph_t g = nullptr;
+      STQ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue();
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      STQ_HIP(hipStreamEndCapture(st, &g));
+      hipGraphExec_t ex = nullptr;
+      const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        throw HipError(e, "hipGraphInstantiate", __FILE__, __LINE__);
+      }
+      if (g_devbuf_generation.load() != key.generation) {  // cannot happen: all reserved above
+        (void)hipGraphExecDestroy(ex);
+        (void)hipGraphDestroy(g);
+        throw StatusError(STQ_ERR_INTERNAL, "buffer allocated during graph capture");
+      }
+      bp.graph = g;
+      bp.exec = ex;
+      bp.graph_sig = sig;
+      STQ_HIP(hipEventRecord(ev[E_START], st));
+      STQ_HIP(hipGraphLaunch(ex, st));
+      stats.graph = 1;
+      done = true;
+    }
+  }
+  if (!done) {
+    STQ_HIP(hipEventRecord(ev[E_START], st));
+    enqueue();
+    bp.eager_sig = sig;
+  }
+  STQ_HIP(hipEventRecord(ev[E_END], st));
+  STQ_HIP(hipStreamSynchronize(st));
+  stats.ms_total = ev_ms(ev[E_START], ev[E_END]);
+  const uint64_t *hc = reinterpret_cast<const uint64_t *>(hbuf + nscenes + 1);
+  for (int s = 0; s <= nscenes; ++s) pair_off[s] = hbuf[s];
+  stats.n_candidates = (int64_t)hc[1];
+  stats.n_lane_steps = (int64_t)hc[2];
+  stats.n_confirm = (int64_t)hc[3];
+  stats.n_items = hbuf[nscenes + 5];
+  stats.n_pairs = pair_off[nscenes];
+  if (hc[0]) throw StatusError(STQ_ERR_INVALID_BOX, "input box with NaN or min > max");
+  if (pair_off[nscenes] > capacity)
+    throw StatusError(STQ_ERR_CAPACITY, "output capacity too small");
+}
+
 template <class F>
 static int guarded(Context *c, F &&f) {
   if (!c) return STQ_ERR_INVALID_ARGUMENT;
@@ -679,33 +1007,7 @@ int stq_find_pairs_batched_device(void *ctx, const stq_aabb_f64 *d_boxes,
     if (m > 0 && !d_boxes) return STQ_ERR_INVALID_ARGUMENT;
   }
   return stqamd::guarded(c, [&] {
-    h_pair_offsets[0] = 0;
-    bool overflow = false;
-    double ms = 0;
-    int64_t nb = 0, cand = 0;
-    for (int s = 0; s < nscenes; ++s) {
-      const int64_t m = h_scene_offsets[s + 1] - h_scene_offsets[s];
-      const int64_t at = h_pair_offsets[s];
-      int64_t cnt = 0;
-      const int64_t room = overflow ? 0 : std::max<int64_t>(capacity - at, 0);
-      try {
-        c->run(d_boxes + h_scene_offsets[s], m, room > 0 ? d_out + at : nullptr, room, nullptr, &cnt);
-      } catch (const stqamd::StatusError &e) {
-        if (e.status != STQ_ERR_CAPACITY) throw;
-        overflow = true;
-        cnt = c->stats.n_pairs;
-      }
-      h_pair_offsets[s + 1] = at + cnt;
-      ms += c->stats.ms_total;
-      nb += m;
-      cand += c->stats.n_candidates;
-    }
-    c->stats.n_scenes = nscenes;
-    c->stats.n_boxes = nb;
-    c->stats.n_pairs = h_pair_offsets[nscenes];
-    c->stats.n_candidates = cand;
-    c->stats.ms_total = ms;
-    if (overflow) throw stqamd::StatusError(STQ_ERR_CAPACITY, "output capacity too small");
+    c->run_batched(d_boxes, h_scene_offsets, nscenes, d_out, capacity, h_pair_offsets);
   });
 }
 

@@ -1,0 +1,491 @@
+// batch.hip -- kernels of the batched pipeline (see batch.hpp).
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+#include "batch.hpp"
+
+namespace stqamd {
+
+namespace {
+
+constexpr int kT = 256;
+inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
+
+// scene of a position: the last s with seg_off[s] <= pos (empty scenes skip)
+__device__ __forceinline__ int scene_of(const int64_t *__restrict__ seg_off, int nscenes,
+                                        int64_t pos) {
+  int lo = 0, hi = nscenes;  // seg_off[lo] <= pos < seg_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (seg_off[mid] <= pos) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline void other_axes_b(int axis, int &a1, int &a2) {
+  a1 = axis == 0 ? 1 : 0;
+  a2 = axis == 2 ? 1 : 2;
+}
+
+// ---- per-scene statistics ---------------------------------------------------
+__global__ __launch_bounds__(kT) void k_seg_stats(const stq_aabb_f64 *__restrict__ boxes,
+                                                  const int64_t *__restrict__ cstart,
+                                                  const int64_t *__restrict__ cend,
+                                                  double *__restrict__ partial) {
+  const int64_t b0 = cstart[blockIdx.x], b1 = cend[blockIdx.x];
+  double acc[12], lo[3], up[3];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = INFINITY;
+    up[a] = -INFINITY;
+  }
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += kT) {
+    const stq_aabb_f64 &b = boxes[i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double l = b.min[a], u = b.max[a];
+      const double c = 0.5 * (l + u);
+      acc[4 * a + 0] += l + u;
+      acc[4 * a + 1] += l * l + u * u;
+      acc[4 * a + 2] += u - l;
+      acc[4 * a + 3] += c * c;
+      lo[a] = fmin(lo[a], l);
+      up[a] = fmax(up[a], u);
+    }
+  }
+  __shared__ double red[kT];
+  for (int k = 0; k < kAxisStats; ++k) {
+    const int mode = k < 12 ? 0 : (k < 15 ? 1 : 2);
+    red[threadIdx.x] = k < 12 ? acc[k] : (k < 15 ? lo[k - 12] : up[k - 15]);
+    __syncthreads();
+    for (int s = kT / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) {
+        const double x = red[threadIdx.x], y = red[threadIdx.x + s];
+        red[threadIdx.x] = mode == 0 ? x + y : (mode == 1 ? fmin(x, y) : fmax(x, y));
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * kAxisStats + k] = red[0];
+    __syncthreads();
+  }
+}
+
+// One thread per scene: its chunks are combined in order, then the axis is
+// chosen (policy as in the single-scene path) and the level range set.
+__global__ __launch_bounds__(kT) void k_seg_finalize(const double *__restrict__ partial,
+                                                     const int *__restrict__ scb,
+                                                     const int64_t *__restrict__ seg_off,
+                                                     int nscenes, int policy, int fixed_axis,
+                                                     SceneParams *__restrict__ out) {
+  const int s = blockIdx.x * kT + threadIdx.x;
+  if (s >= nscenes) return;
+  double v[kAxisStats];
+  for (int k = 0; k < kAxisStats; ++k) v[k] = k < 12 ? 0.0 : (k < 15 ? INFINITY : -INFINITY);
+  for (int c = scb[s]; c < scb[s + 1]; ++c)
+    for (int k = 0; k < kAxisStats; ++k) {
+      const double x = partial[(int64_t)c * kAxisStats + k];
+      v[k] = k < 12 ? v[k] + x : (k < 15 ? fmin(v[k], x) : fmax(v[k], x));
+    }
+  SceneParams p;
+  p.pad = 0;
+  for (int a = 0; a < 3; ++a) {
+    const double lo = v[12 + a], up = v[15 + a], range = up - lo;
+    p.qp.origin[a] = isfinite(lo) ? lo : 0.0;
+    double sc = (range > 0 && isfinite(range)) ? 32767.0 / range : 0.0;
+    p.qp.scale[a] = isfinite(sc) ? sc : 0.0;
+  }
+  const double dn = (double)(seg_off[s + 1] - seg_off[s]);
+  int best = fixed_axis;
+  if (policy >= 3 && dn > 0) {
+    double bestv = 0;
+    for (int a = 0; a < 3; ++a) {
+      const double s1 = v[4 * a], s2 = v[4 * a + 1], ext = v[4 * a + 2], cc = v[4 * a + 3];
+      const double mu = s1 / (2.0 * dn);
+      if (policy == 3) {
+        const double var = s2 - 2.0 * mu * s1 + 2.0 * dn * mu * mu;
+        if (a == 0 || var > bestv) { best = a; bestv = var; }
+      } else {
+        const double sd = sqrt(fmax(cc / dn - mu * mu, 0.0));
+        const double score = sd > 0 ? (ext / dn) / sd : INFINITY;
+        if (a == 0 || score < bestv) { best = a; bestv = score; }
+      }
+    }
+  }
+  p.axis = best;
+  out[s] = p;
+}
+
+// ---- keys -------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void k_keys_seg(const stq_aabb_f64 *__restrict__ boxes, int64_t n,
+                                                 const int64_t *__restrict__ seg_off, int nscenes,
+                                                 const SceneParams *__restrict__ sp, int f32,
+                                                 uint64_t *__restrict__ keys,
+                                                 uint32_t *__restrict__ vals,
+                                                 unsigned *__restrict__ err) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const int s = scene_of(seg_off, nscenes, i);
+  const int axis = sp[s].axis;
+  const stq_aabb_f64 &b = boxes[i];
+  const double x0 = b.min[0], x1 = b.min[1], x2 = b.min[2];
+  const double y0 = b.max[0], y1 = b.max[1], y2 = b.max[2];
+  if (!(x0 <= y0) || !(x1 <= y1) || !(x2 <= y2)) atomicOr(err, kErrInverted);
+  const double m = axis == 0 ? x0 : (axis == 1 ? x1 : x2);
+  keys[i] = f32 ? (((uint64_t)(uint32_t)s << 32) | key_of(round_down_f32(m))) : key_of(m);
+  vals[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(kT) void k_scene_keys(const uint32_t *__restrict__ vals, int64_t n,
+                                                   const int64_t *__restrict__ seg_off,
+                                                   int nscenes, uint32_t *__restrict__ sk) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) sk[i] = (uint32_t)scene_of(seg_off, nscenes, (int64_t)vals[i]);
+}
+
+// ---- gather --------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void k_gather_seg(const stq_aabb_f64 *__restrict__ boxes,
+                                                   const uint32_t *__restrict__ order, int64_t n,
+                                                   const int64_t *__restrict__ seg_off,
+                                                   int nscenes,
+                                                   const SceneParams *__restrict__ sp, int f32,
+                                                   void *__restrict__ smin, void *__restrict__ smax,
+                                                   void *__restrict__ rec, QRec *__restrict__ qrec,
+                                                   uint32_t *__restrict__ orig_local) {
+  const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (s >= n) return;
+  const int sc = scene_of(seg_off, nscenes, s);
+  const uint32_t o = order[s];
+  orig_local[s] = (uint32_t)((int64_t)o - seg_off[sc]);
+  const stq_aabb_f64 &b = boxes[o];
+  const SceneParams p = sp[sc];
+  const int axis = p.axis;
+  int a1, a2;
+  other_axes_b(axis, a1, a2);
+  if (f32) {
+    static_cast<float *>(smin)[s] = round_down_f32(b.min[axis]);
+    static_cast<float *>(smax)[s] = round_up_f32(b.max[axis]);
+    Exact64 r;
+    r.lo0 = b.min[axis];
+    r.hi0 = b.max[axis];
+    r.lo1 = b.min[a1];
+    r.hi1 = b.max[a1];
+    r.lo2 = b.min[a2];
+    r.hi2 = b.max[a2];
+    r.v0 = b.vertex_ids[0];
+    r.v1 = b.vertex_ids[1];
+    r.v2 = b.vertex_ids[2];
+    r.type = type_bit(r.v1, r.v2);
+    static_cast<Exact64 *>(rec)[s] = r;
+    QRec q;
+    q.lo = level_down(r.lo1, p.qp.origin[a1], p.qp.scale[a1]) |
+           (level_down(r.lo2, p.qp.origin[a2], p.qp.scale[a2]) << 16);
+    q.hi = level_up(r.hi1, p.qp.origin[a1], p.qp.scale[a1]) |
+           (level_up(r.hi2, p.qp.origin[a2], p.qp.scale[a2]) << 16);
+    qrec[s] = q;
+  } else {
+    static_cast<double *>(smin)[s] = b.min[axis];
+    static_cast<double *>(smax)[s] = b.max[axis];
+    Rec64 r;
+    r.lo1 = b.min[a1];
+    r.hi1 = b.max[a1];
+    r.lo2 = b.min[a2];
+    r.hi2 = b.max[a2];
+    r.v0 = b.vertex_ids[0];
+    r.v1 = b.vertex_ids[1];
+    r.v2 = b.vertex_ids[2];
+    r.type = type_bit(r.v1, r.v2);
+    static_cast<Rec64 *>(rec)[s] = r;
+  }
+}
+
+// ---- chain ends, bounded by the scene ---------------------------------------
+template <class T>
+__global__ __launch_bounds__(kT) void k_chain_end_seg(const T *__restrict__ smin,
+                                                      const T *__restrict__ smax, int64_t n,
+                                                      const int64_t *__restrict__ seg_off,
+                                                      int nscenes, int *__restrict__ hi,
+                                                      unsigned long long *__restrict__ cand) {
+  const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
+  unsigned long long len = 0;
+  if (s < n) {
+    const int64_t end = seg_off[scene_of(seg_off, nscenes, s) + 1];
+    const T x = smax[s];
+    int64_t lo = s, step = 1, up;
+    while (true) {
+      const int64_t probe = lo + step;
+      if (probe >= end) { up = end; break; }
+      if (smin[probe] <= x) { lo = probe; step <<= 1; }
+      else { up = probe; break; }
+    }
+    while (up - lo > 1) {
+      const int64_t mid = lo + ((up - lo) >> 1);
+      if (smin[mid] <= x) lo = mid; else up = mid;
+    }
+    hi[s] = (int)lo;
+    len = (unsigned long long)(lo - s);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) len += __shfl_xor(len, o, kWave);
+  if ((threadIdx.x & 63) == 0 && len) atomicAdd(cand, len);
+}
+
+// ---- planning -------------------------------------------------------------------
+constexpr int kMaxWin = 2048;
+
+__global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
+                                                 const int64_t *__restrict__ win_start,
+                                                 const int *__restrict__ win_cnt,
+                                                 const int64_t *__restrict__ win_gbase, int W,
+                                                 int chunk, int *__restrict__ rowlist,
+                                                 int4 *__restrict__ groups,
+                                                 int *__restrict__ gchunk,
+                                                 uint32_t *__restrict__ nchunks,
+                                                 unsigned long long *__restrict__ lane_steps) {
+  __shared__ uint64_t key[kMaxWin];
+  const int64_t wbase = win_start[blockIdx.x];
+  const int cnt = win_cnt[blockIdx.x];
+  const int64_t gbase = win_gbase[blockIdx.x];
+  for (int j = threadIdx.x; j < W; j += kT) {
+    uint64_t k = ~0ull;
+    if (j < cnt) k = ((uint64_t)(uint32_t)hi[wbase + j] << 32) | (uint32_t)j;
+    key[j] = k;
+  }
+  __syncthreads();
+  for (int size = 2; size <= W; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int j = threadIdx.x; j < W; j += kT) {
+        const int pj = j ^ stride;
+        if (pj > j) {
+          const bool upd = (j & size) == 0;
+          const uint64_t a = key[j], b = key[pj];
+          if ((a > b) == upd) {
+            key[j] = b;
+            key[pj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int j = threadIdx.x; j < cnt; j += kT)
+    rowlist[wbase + j] = (int)(wbase + (int64_t)(key[j] & 0xffffffffu));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ng = (cnt + 63) / 64;
+  unsigned long long steps = 0;
+  for (int k = w; k < ng; k += kT / 64) {
+    const int j = 64 * k + lane;
+    const bool act = j < cnt;
+    int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
+    int h = act ? (int)(key[j] >> 32) : INT_MIN;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      p = min(p, __shfl_xor(p, o, kWave));
+      h = max(h, __shfl_xor(h, o, kWave));
+    }
+    const int nrows = min(64, cnt - 64 * k);
+    const int q0 = p + 1, q1 = h;
+    const int64_t g = gbase + k;
+    if (lane == 0) {
+      groups[g] = make_int4((int)(wbase + 64 * k), nrows, q0, q1);
+      const int64_t span = (int64_t)q1 - q0 + 1;
+      int ch = chunk;
+      if (span > 0 && (span + ch - 1) / ch > kMaxChunks) ch = (int)((span + kMaxChunks - 1) / kMaxChunks);
+      gchunk[g] = ch;
+      nchunks[g] = span > 0 ? (uint32_t)((span + ch - 1) / ch) : 0u;
+      if (span > 0) steps += (unsigned long long)span * 64ull;
+    }
+  }
+  if (lane == 0 && steps) atomicAdd(lane_steps, steps);
+}
+
+__global__ __launch_bounds__(kT) void k_items_seg(const int4 *__restrict__ groups,
+                                                  const int *__restrict__ gchunk,
+                                                  const int64_t *__restrict__ ioff,
+                                                  int64_t ngroups, Item *__restrict__ items) {
+  const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (g >= ngroups) return;
+  const int4 gr = groups[g];
+  const int ch = gchunk[g];
+  const int64_t o = ioff[g], e = ioff[g + 1];
+  for (int64_t c = 0; c < e - o; ++c) {
+    Item it;
+    it.rowoff = gr.x;
+    it.nrows = gr.y;
+    it.q0 = gr.z + (int)(c * ch);
+    it.q1 = min(it.q0 + ch - 1, gr.w);
+    items[o + c] = it;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_scene_pairs(const int64_t *__restrict__ off,
+                                                    const int64_t *__restrict__ ioff,
+                                                    const int64_t *__restrict__ sfg, int nscenes,
+                                                    int64_t *__restrict__ pair_off) {
+  const int s = blockIdx.x * kT + threadIdx.x;
+  if (s <= nscenes) pair_off[s] = off[ioff[sfg[s]]];
+}
+
+__global__ __launch_bounds__(kT) void k_translate_dev(int2 *__restrict__ pairs,
+                                                      const int64_t *__restrict__ total,
+                                                      int64_t capacity,
+                                                      const uint32_t *__restrict__ orig) {
+  const int64_t cnt = *total;
+  if (cnt > capacity) return;  // nothing was written
+  const int64_t stride = (int64_t)gridDim.x * kT;
+  for (int64_t k = (int64_t)blockIdx.x * kT + threadIdx.x; k < cnt; k += stride) {
+    const int2 e = pairs[k];
+    const int oq = (int)orig[e.y];
+    pairs[k] = make_int2(min(e.x, oq), max(e.x, oq));
+  }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+void build_batch_tables(const int64_t *off, int nscenes, int W, BatchTables &t) {
+  t.nscenes = nscenes;
+  t.seg_off.assign(off, off + nscenes + 1);
+  t.n = off[nscenes];
+  t.chunk_scene.clear();
+  t.chunk_start.clear();
+  t.chunk_end.clear();
+  t.scene_chunk_begin.assign(nscenes + 1, 0);
+  t.win_start.clear();
+  t.win_cnt.clear();
+  t.win_gbase.clear();
+  t.scene_first_group.assign(nscenes + 1, 0);
+  int64_t g = 0;
+  const int64_t kStatChunk = 65536;
+  for (int s = 0; s < nscenes; ++s) {
+    const int64_t a = off[s], b = off[s + 1];
+    t.scene_chunk_begin[s] = (int)t.chunk_scene.size();
+    for (int64_t c = a; c < b; c += kStatChunk) {
+      t.chunk_scene.push_back(s);
+      t.chunk_start.push_back(c);
+      t.chunk_end.push_back(std::min(b, c + kStatChunk));
+    }
+    t.scene_first_group[s] = g;
+    if (b - a > 1) {
+      for (int64_t w = a; w < b; w += W) {
+        const int cnt = (int)std::min<int64_t>(W, b - w);
+        t.win_start.push_back(w);
+        t.win_cnt.push_back(cnt);
+        t.win_gbase.push_back(g);
+        g += (cnt + 63) / 64;
+      }
+    }
+  }
+  t.scene_chunk_begin[nscenes] = (int)t.chunk_scene.size();
+  t.scene_first_group[nscenes] = g;
+  t.ngroups = g;
+}
+
+void launch_seg_stats(const stq_aabb_f64 *boxes, const BatchTables &t, const BatchDev &d,
+                      int policy, int fixed_axis, bool, hipStream_t st) {
+  const int nc = (int)t.chunk_scene.size();
+  if (nc > 0)
+    hipLaunchKernelGGL(k_seg_stats, dim3(nc), dim3(kT), 0, st, boxes, d.chunk_start, d.chunk_end,
+                       d.partial);
+  if (t.nscenes > 0)
+    hipLaunchKernelGGL(k_seg_finalize, dim3(nblk(t.nscenes)), dim3(kT), 0, st,
+                       (const double *)d.partial, d.scene_chunk_begin, d.seg_off, t.nscenes, policy,
+                       fixed_axis, d.params);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_keys_seg(bool f32, const stq_aabb_f64 *boxes, const BatchTables &t, const BatchDev &d,
+                     uint64_t *keys, uint32_t *vals, unsigned *err, hipStream_t st) {
+  if (t.n <= 0) return;
+  hipLaunchKernelGGL(k_keys_seg, dim3(nblk(t.n)), dim3(kT), 0, st, boxes, t.n, d.seg_off, t.nscenes,
+                     (const SceneParams *)d.params, f32 ? 1 : 0, keys, vals, err);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_scene_keys(const uint32_t *vals, const BatchTables &t, const BatchDev &d,
+                       uint32_t *scene_key, hipStream_t st) {
+  if (t.n <= 0) return;
+  hipLaunchKernelGGL(k_scene_keys, dim3(nblk(t.n)), dim3(kT), 0, st, vals, t.n, d.seg_off,
+                     t.nscenes, scene_key);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_gather_seg(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order,
+                       const BatchTables &t, const BatchDev &d, void *smin, void *smax, void *rec,
+                       QRec *qrec, uint32_t *orig_local, hipStream_t st) {
+  if (t.n <= 0) return;
+  hipLaunchKernelGGL(k_gather_seg, dim3(nblk(t.n)), dim3(kT), 0, st, boxes, order, t.n, d.seg_off,
+                     t.nscenes, (const SceneParams *)d.params, f32 ? 1 : 0, smin, smax, rec, qrec,
+                     orig_local);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_chain_end_seg(bool f32, const void *smin, const void *smax, const BatchTables &t,
+                          const BatchDev &d, int *hi, unsigned long long *cand, hipStream_t st) {
+  if (t.n <= 0) return;
+  if (f32)
+    hipLaunchKernelGGL(k_chain_end_seg<float>, dim3(nblk(t.n)), dim3(kT), 0, st,
+                       (const float *)smin, (const float *)smax, t.n, d.seg_off, t.nscenes, hi, cand);
+  else
+    hipLaunchKernelGGL(k_chain_end_seg<double>, dim3(nblk(t.n)), dim3(kT), 0, st,
+                       (const double *)smin, (const double *)smax, t.n, d.seg_off, t.nscenes, hi,
+                       cand);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_plan_seg(const int *hi, const BatchTables &t, const BatchDev &d, int W, int chunk,
+                     int *rowlist, int4 *groups, int *gchunk, uint32_t *nchunks,
+                     unsigned long long *lane_steps, hipStream_t st) {
+  const int nw = (int)t.win_start.size();
+  if (nw <= 0) return;
+  hipLaunchKernelGGL(k_plan_seg, dim3(nw), dim3(kT), 0, st, hi, d.win_start, d.win_cnt, d.win_gbase,
+                     W, chunk, rowlist, groups, gchunk, nchunks, lane_steps);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_items_seg(const int4 *groups, const int *gchunk, const int64_t *ioff, int64_t ngroups,
+                      Item *items, hipStream_t st) {
+  if (ngroups <= 0) return;
+  hipLaunchKernelGGL(k_items_seg, dim3(nblk(ngroups)), dim3(kT), 0, st, groups, gchunk, ioff,
+                     ngroups, items);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_scene_pairs(const int64_t *off, const int64_t *ioff, const BatchTables &t,
+                        const BatchDev &d, int64_t *pair_off, hipStream_t st) {
+  hipLaunchKernelGGL(k_scene_pairs, dim3(nblk(t.nscenes + 1)), dim3(kT), 0, st, off, ioff,
+                     d.scene_first_group, t.nscenes, pair_off);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_translate_dev(stq_pair *pairs, const int64_t *total, int64_t bound,
+                          const uint32_t *orig, hipStream_t st) {
+  if (bound <= 0) return;
+  const unsigned g = (unsigned)std::min<int64_t>(nblk(bound), 16384);
+  hipLaunchKernelGGL(k_translate_dev, dim3(g), dim3(kT), 0, st, reinterpret_cast<int2 *>(pairs),
+                     total, bound, orig);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(nblk(n)), dim3(kT), 0, st, p, v, n);
+  STQ_HIP(hipGetLastError());
+}
+void launch_fill_u64(unsigned long long *p, unsigned long long v, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_u64, dim3(nblk(n)), dim3(kT), 0, st, p, v, n);
+  STQ_HIP(hipGetLastError());
+}
+
+}  // namespace stqamd

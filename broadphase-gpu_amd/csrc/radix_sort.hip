@@ -55,7 +55,9 @@ __global__ __launch_bounds__(kSortThreads) void k_hist_all(const K *__restrict__
 template <class K>
 __global__ __launch_bounds__(kSortThreads) void k_count(const K *__restrict__ keys, int64_t n,
                                                          int shift, int ntiles,
-                                                         unsigned *__restrict__ counts) {
+                                                         unsigned *__restrict__ counts,
+                                                         const unsigned *__restrict__ skip) {
+  if (skip && skip[shift >> 3]) return;
   __shared__ unsigned h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -76,7 +78,18 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const K *__restrict__ 
                                                            K *__restrict__ kout,
                                                            uint32_t *__restrict__ vout, int64_t n,
                                                            int shift, int ntiles,
-                                                           const unsigned *__restrict__ offsets) {
+                                                           const unsigned *__restrict__ offsets,
+                                                           const unsigned *__restrict__ skip) {
+  if (skip && skip[shift >> 3]) {
+    // the digit is equal for every key: the stable order is the identity
+    const int64_t b = (int64_t)blockIdx.x * kTile;
+    for (int j = threadIdx.x; j < kTile; j += kSortThreads)
+      if (b + j < n) {
+        kout[b + j] = kin[b + j];
+        vout[b + j] = vin[b + j];
+      }
+    return;
+  }
   __shared__ K sk[kTile];
   __shared__ uint32_t sv[kTile];
   __shared__ unsigned wh[kWavesPerBlock][256];  // per-wave running digit counts
@@ -245,9 +258,15 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *__re
 }
 
 template <class T>
+T *scan_scratch_reserve(int64_t n, DevBuf &tmp) {
+  const int64_t nb = n > 0 ? (n + kScanTile - 1) / kScanTile : 1;
+  return tmp.as<T>((size_t)nb + 1);
+}
+
+template <class T>
 void scan_impl(const uint32_t *in, T *out, int64_t n, DevBuf &tmp, hipStream_t st) {
   const int64_t nb = n > 0 ? (n + kScanTile - 1) / kScanTile : 1;
-  T *partial = tmp.as<T>((size_t)nb + 1);
+  T *partial = scan_scratch_reserve<T>(n, tmp);
   hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n, partial);
   hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(kScanThreads), 0, st, partial, nb);
   hipLaunchKernelGGL(k_scan_down<T>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n,
@@ -255,41 +274,70 @@ void scan_impl(const uint32_t *in, T *out, int64_t n, DevBuf &tmp, hipStream_t s
   STQ_HIP(hipGetLastError());
 }
 
+// Digit d is skipped when one bucket of its histogram holds all n keys.
+__global__ __launch_bounds__(kSortThreads) void k_skip_flags(const unsigned *__restrict__ hist,
+                                                              int ndigits, int64_t n,
+                                                              unsigned *__restrict__ skip) {
+  if (threadIdx.x < 8) skip[threadIdx.x] = 0;
+  __syncthreads();
+  for (int d = 0; d < ndigits; ++d)
+    if ((int64_t)hist[d * 256 + threadIdx.x] == n) skip[d] = 1;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_zero_u32(unsigned *__restrict__ p, int n) {
+  const int i = blockIdx.x * kSortThreads + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// With sync_free, the skip decision is taken on the device (a skipped digit
+// costs one copy pass) and nothing is read back to the host, so the sequence
+// can be captured in a graph; otherwise the histograms are read back and
+// skipped digits cost nothing.
 template <class K>
 void radix_sort_impl(K *keys, uint32_t *vals, int64_t n, int bits, SortScratch &s,
-                     hipStream_t st, int *passes_run) {
+                     hipStream_t st, int *passes_run, bool sync_free) {
   if (passes_run) *passes_run = 0;
   if (n <= 1) return;
   const int ndigits = (bits + 7) / 8;
-  unsigned *hist = s.hist.as<unsigned>(8 * 256);
-  STQ_HIP(hipMemsetAsync(hist, 0, 8 * 256 * sizeof(unsigned), st));
+  unsigned *hist = s.hist.as<unsigned>(8 * 256 + 8);
+  unsigned *skip = hist + 8 * 256;
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  K *ak = s.alt_keys.as<K>((size_t)n);
+  uint32_t *av = s.alt_vals.as<uint32_t>((size_t)n);
+  unsigned *counts = s.counts.as<unsigned>((size_t)(256 * ntiles + 1));
+  scan_scratch_reserve<unsigned>(256 * ntiles, s.misc);
+  hipLaunchKernelGGL(k_zero_u32, dim3(8), dim3(kSortThreads), 0, st, hist, 8 * 256);
   const int hb = (int)std::min<int64_t>((n + kSortThreads - 1) / kSortThreads, 2048);
   hipLaunchKernelGGL(k_hist_all<K>, dim3(hb), dim3(kSortThreads), 0, st, (const K *)keys, n,
                      ndigits, hist);
   STQ_HIP(hipGetLastError());
   unsigned hh[8 * 256];
-  STQ_HIP(hipMemcpyAsync(hh, hist, sizeof(unsigned) * ndigits * 256, hipMemcpyDeviceToHost, st));
-  STQ_HIP(hipStreamSynchronize(st));
+  if (sync_free) {
+    hipLaunchKernelGGL(k_skip_flags, dim3(1), dim3(kSortThreads), 0, st, (const unsigned *)hist,
+                       ndigits, n, skip);
+  } else {
+    STQ_HIP(hipMemcpyAsync(hh, hist, sizeof(unsigned) * ndigits * 256, hipMemcpyDeviceToHost, st));
+    STQ_HIP(hipStreamSynchronize(st));
+  }
 
-  const int64_t ntiles = (n + kTile - 1) / kTile;
-  K *ak = s.alt_keys.as<K>((size_t)n);
-  uint32_t *av = s.alt_vals.as<uint32_t>((size_t)n);
-  unsigned *counts = s.counts.as<unsigned>((size_t)(256 * ntiles + 1));
   K *src_k = keys, *dst_k = ak;
   uint32_t *src_v = vals, *dst_v = av;
   int run = 0;
   for (int d = 0; d < ndigits; ++d) {
-    bool trivial = false;
-    for (int b = 0; b < 256; ++b)
-      if ((int64_t)hh[d * 256 + b] == n) trivial = true;
-    if (trivial) continue;
+    if (!sync_free) {
+      bool trivial = false;
+      for (int b = 0; b < 256; ++b)
+        if ((int64_t)hh[d * 256 + b] == n) trivial = true;
+      if (trivial) continue;
+    }
     const int shift = 8 * d;
+    const unsigned *sk = sync_free ? skip : nullptr;
     hipLaunchKernelGGL(k_count<K>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, st,
-                       (const K *)src_k, n, shift, (int)ntiles, counts);
+                       (const K *)src_k, n, shift, (int)ntiles, counts, sk);
     scan_impl<unsigned>(counts, counts, 256 * ntiles, s.misc, st);
     hipLaunchKernelGGL(k_scatter<K>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, st,
                        (const K *)src_k, (const uint32_t *)src_v, dst_k, dst_v, n, shift,
-                       (int)ntiles, (const unsigned *)counts);
+                       (int)ntiles, (const unsigned *)counts, sk);
     STQ_HIP(hipGetLastError());
     std::swap(src_k, dst_k);
     std::swap(src_v, dst_v);
@@ -306,12 +354,30 @@ void radix_sort_impl(K *keys, uint32_t *vals, int64_t n, int bits, SortScratch &
 
 void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bits, SortScratch &s,
                     hipStream_t st, int *passes_run) {
-  radix_sort_impl<uint64_t>(keys, vals, n, bits, s, st, passes_run);
+  radix_sort_impl<uint64_t>(keys, vals, n, bits, s, st, passes_run, false);
 }
 void radix_sort_u32(uint32_t *keys, uint32_t *vals, int64_t n, int bits, SortScratch &s,
                     hipStream_t st, int *passes_run) {
-  radix_sort_impl<uint32_t>(keys, vals, n, bits, s, st, passes_run);
+  radix_sort_impl<uint32_t>(keys, vals, n, bits, s, st, passes_run, false);
 }
+void radix_sort_u64_sync_free(uint64_t *keys, uint32_t *vals, int64_t n, int bits,
+                              SortScratch &s, hipStream_t st) {
+  radix_sort_impl<uint64_t>(keys, vals, n, bits, s, st, nullptr, true);
+}
+void radix_sort_u32_sync_free(uint32_t *keys, uint32_t *vals, int64_t n, int bits,
+                              SortScratch &s, hipStream_t st) {
+  radix_sort_impl<uint32_t>(keys, vals, n, bits, s, st, nullptr, true);
+}
+void radix_sort_reserve(int64_t n, int key_bytes, SortScratch &s) {
+  if (n <= 1) return;
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  s.hist.as<unsigned>(8 * 256 + 8);
+  s.alt_keys.get((size_t)n * key_bytes);
+  s.alt_vals.as<uint32_t>((size_t)n);
+  s.counts.as<unsigned>((size_t)(256 * ntiles + 1));
+  scan_scratch_reserve<unsigned>(256 * ntiles, s.misc);
+}
+void scan_reserve_i64(int64_t n, DevBuf &tmp) { scan_scratch_reserve<int64_t>(n, tmp); }
 void scan_u32_to_i64(const uint32_t *in, int64_t *out, int64_t n, DevBuf &tmp, hipStream_t st) {
   scan_impl<int64_t>(in, out, n, tmp, st);
 }

@@ -15,6 +15,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -110,6 +111,10 @@ struct StatusError {
   StatusError(int s, std::string w) : status(s), what(std::move(w)) {}
 };
 
+// Incremented whenever any DevBuf is (re)allocated or released: a captured
+// graph that holds buffer addresses is valid only while it is unchanged.
+inline std::atomic<uint64_t> g_devbuf_generation{0};
+
 // Grow-only device buffer.
 struct DevBuf {
   void *p = nullptr;
@@ -117,6 +122,7 @@ struct DevBuf {
   void *get(size_t need) {
     if (need == 0) need = 256;
     if (need > bytes) {
+      g_devbuf_generation.fetch_add(1);
       if (p) (void)hipFree(p);
       p = nullptr;
       bytes = 0;
@@ -136,6 +142,7 @@ struct DevBuf {
   }
   template <class T> T *as(size_t count) { return static_cast<T *>(get(count * sizeof(T))); }
   void release() {
+    if (p) g_devbuf_generation.fetch_add(1);
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
@@ -153,11 +160,22 @@ void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bits,
                     SortScratch &s, hipStream_t st, int *passes_run);
 void radix_sort_u32(uint32_t *keys, uint32_t *vals, int64_t n, int bits,
                     SortScratch &s, hipStream_t st, int *passes_run);
+// The same sort with no host synchronisation (capturable in a graph): a digit
+// that is equal for all keys costs a copy pass instead of nothing.
+void radix_sort_u64_sync_free(uint64_t *keys, uint32_t *vals, int64_t n, int bits,
+                              SortScratch &s, hipStream_t st);
+void radix_sort_u32_sync_free(uint32_t *keys, uint32_t *vals, int64_t n, int bits,
+                              SortScratch &s, hipStream_t st);
+// Allocates the scratch a sort of n keys of key_bytes needs (nothing is
+// allocated by the sort itself afterwards).
+void radix_sort_reserve(int64_t n, int key_bytes, SortScratch &s);
 
 // Exclusive prefix sum of n uint32 counts into n+1 int64 offsets (out[n] is
 // the total).  Scratch is taken from `tmp`.
 void scan_u32_to_i64(const uint32_t *in, int64_t *out, int64_t n, DevBuf &tmp,
                      hipStream_t st);
+// Allocates the scratch of scan_u32_to_i64 over n counts.
+void scan_reserve_i64(int64_t n, DevBuf &tmp);
 // Exclusive prefix sum of uint32 into uint32 (n+1 outputs), for sort counts.
 void scan_u32(const uint32_t *in, uint32_t *out, int64_t n, DevBuf &tmp,
               hipStream_t st);

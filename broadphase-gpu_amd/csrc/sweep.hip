@@ -156,20 +156,6 @@ __global__ __launch_bounds__(kThreads) void k_gather_f64(const stq_aabb_f64 *__r
   rec[s] = r;
 }
 
-// Outward 15-bit levels: t = (x - origin) * scale is monotone in x under IEEE
-// rounding, so floor(t(lo)) <= ceil(t(hi')) whenever lo <= hi': a pair that
-// overlaps in FP64 always overlaps in levels (a superset test).
-__device__ inline uint32_t level_down(double x, double o, double sc) {
-  if (sc == 0.0) return 0u;
-  const double t = fmin(fmax(floor((x - o) * sc), 0.0), 32767.0);
-  return (uint32_t)t;
-}
-__device__ inline uint32_t level_up(double x, double o, double sc) {
-  if (sc == 0.0) return 32767u;
-  const double t = fmin(fmax(ceil((x - o) * sc), 0.0), 32767.0);
-  return (uint32_t)t;
-}
-
 __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__restrict__ boxes,
                                                          const uint32_t *__restrict__ order,
                                                          int64_t n, int axis, QParams qp,
@@ -406,7 +392,8 @@ __global__ __launch_bounds__(kThreads) void k_sweep64(SweepArgs a, const Rec64 *
                                                       int2 *__restrict__ out) {
   const int zero = a.zero;
   const int lane = threadIdx.x & 63;
-  const WaveSlice ws = wave_slice(a.nitems, a.xcd_split);
+  if (MODE == kFill && a.total_dev && *a.total_dev > a.capacity) return;
+  const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split);
   for (int64_t k = ws.kbeg + ws.wave; k < ws.kend; k += ws.nwaves) {
     const int64_t it = a.sel ? (int64_t)a.sel[k] : k;
     const Item item = a.items[it];
@@ -541,7 +528,8 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
   const int lane = threadIdx.x & 63;
   int2 *slots = slot_mem[threadIdx.x >> 6];
   unsigned long long nconf = 0;
-  const WaveSlice ws = wave_slice(a.nitems, a.xcd_split);
+  if (MODE == kFill && a.total_dev && *a.total_dev > a.capacity) return;
+  const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split);
   for (int64_t k = ws.kbeg + ws.wave; k < ws.kend; k += ws.nwaves) {
     const int64_t it = a.sel ? (int64_t)a.sel[k] : k;
     const Item item = a.items[it];

@@ -34,6 +34,20 @@ struct QParams {
 // Per block: 12 sums (per axis: sum(min+max), sum(min^2+max^2), sum(max-min),
 // sum(centre^2)) then per axis min(min) and max(max): kAxisStats doubles.
 constexpr int kAxisStats = 18;
+// Outward 15-bit levels: t = (x - origin) * scale is monotone in x under IEEE
+// rounding, so floor(t(lo)) <= ceil(t(hi')) whenever lo <= hi': a pair that
+// overlaps in FP64 always overlaps in levels (a superset test).
+__device__ inline uint32_t level_down(double x, double o, double sc) {
+  if (sc == 0.0) return 0u;
+  const double t = fmin(fmax(floor((x - o) * sc), 0.0), 32767.0);
+  return (uint32_t)t;
+}
+__device__ inline uint32_t level_up(double x, double o, double sc) {
+  if (sc == 0.0) return 32767u;
+  const double t = fmin(fmax(ceil((x - o) * sc), 0.0), 32767.0);
+  return (uint32_t)t;
+}
+
 void launch_axis_stats(const stq_aabb_f64 *boxes, int64_t n, double *partial, int nblocks,
                        hipStream_t st);
 void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, void *keys,
@@ -65,6 +79,12 @@ struct SweepArgs {
   unsigned long long *confirms = nullptr;
   int zero = 0;                  // always 0 (an opaque runtime zero)
   int xcd_split = 1;             // 1: each XCD sweeps one contiguous slice of items
+  // device-resident sizes (batched pipeline): the item count, and for kFill
+  // the pair total with the output capacity (nothing is written when the
+  // total exceeds it)
+  const int64_t *nitems_dev = nullptr;
+  const int64_t *total_dev = nullptr;
+  int64_t capacity = 0;
 };
 
 // Pairs come out as (input id of the row, sorted q); launch_translate turns

@@ -28,12 +28,13 @@ class StqError(RuntimeError):
 
 def options(precision=PRECISION_F64, axis_policy=AXIS_X, sizing=SIZING_COUNT_FILL,
             autotune=0, chunk=0, window=0, waves_per_cu=0, batch=0, xcd_split=0,
-            train_scenes=0) -> Options:
+            train_scenes=0, graph=0) -> Options:
     o = Options()
     _lib.lib().stq_options_default(ctypes.byref(o))
     o.precision, o.axis_policy, o.sizing = precision, axis_policy, sizing
     o.autotune, o.chunk, o.window, o.waves_per_cu = autotune, chunk, window, waves_per_cu
     o.batch, o.xcd_split, o.train_scenes = batch, xcd_split, train_scenes
+    o.graph = graph
     return o
 
 
@@ -174,6 +175,19 @@ class Context:
         finally:
             self._L.stq_free_host(out)
         return res
+
+    def find_pairs_batched_device(self, d_boxes, scene_offsets, d_out, capacity):
+        """Device-resident batched call.  Returns (status, pair_offsets); the
+        status is STQ_OK or STQ_ERR_CAPACITY (pair_offsets[-1] then holds the
+        number needed), anything else raises."""
+        offs = np.ascontiguousarray(scene_offsets, dtype=np.int64)
+        ns = len(offs) - 1
+        poffs = np.zeros(ns + 1, dtype=np.int64)
+        rc = self._L.stq_find_pairs_batched_device(self.ptr, d_boxes, offs.ctypes.data, ns, d_out,
+                                                   capacity, poffs.ctypes.data)
+        if rc != _lib.STQ_ERR_CAPACITY:
+            self._check(rc, "stq_find_pairs_batched_device")
+        return rc, poffs
 
     def find_pairs_batched(self, boxes: np.ndarray, scene_offsets: np.ndarray, capacity=None):
         """Scenes back to back; returns (pairs (P,2), pair_offsets (S+1))."""

@@ -53,7 +53,8 @@ class Options(ctypes.Structure):
         ("batch", ctypes.c_int32),
         ("xcd_split", ctypes.c_int32),
         ("train_scenes", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("graph", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -71,7 +72,7 @@ class Stats(ctypes.Structure):
         ("axis", ctypes.c_int32),
         ("precision", ctypes.c_int32),
         ("sizing", ctypes.c_int32),
-        ("pad", ctypes.c_int32),
+        ("graph", ctypes.c_int32),
         ("ms_total", ctypes.c_double),
         ("ms_keys", ctypes.c_double),
         ("ms_sort", ctypes.c_double),
@@ -195,6 +196,15 @@ class DeviceBuffer:
         if a.nbytes:
             _check_hip(hip().hipMemcpy(b.ptr, a.ctypes.data, a.nbytes, HIP_H2D), "hipMemcpy H2D")
         return b
+
+    def write(self, arr, offset=0):
+        """Copies a host array into the buffer at a byte offset."""
+        import numpy as np
+        a = np.ascontiguousarray(arr)
+        assert offset + a.nbytes <= self.nbytes
+        if a.nbytes:
+            dst = ctypes.c_void_p(self.ptr.value + offset)
+            _check_hip(hip().hipMemcpy(dst, a.ctypes.data, a.nbytes, HIP_H2D), "hipMemcpy H2D")
 
     def to_array(self, dtype, count):
         import numpy as np

@@ -129,7 +129,11 @@ typedef struct stq_options {
                             while the network trains on their exact
                             counts before the single learned pass is
                             used; 0 = 200, negative = none        */
-  int32_t reserved[6];
+  int32_t graph;         /* batched calls: 0 = auto (the second call
+                            of an unchanged batch is captured in a
+                            hipGraph, later ones replay it), 1 = capture
+                            from the first call, 2 = never          */
+  int32_t reserved[5];
 } stq_options;
 
 typedef struct stq_stats {
@@ -145,7 +149,8 @@ typedef struct stq_stats {
   int32_t axis;             /* sweep axis used (last scene)                 */
   int32_t precision;
   int32_t sizing;
-  int32_t pad;
+  int32_t graph;            /* batched call: 0 eager launches, 1 captured
+                               into a graph and launched, 2 graph replay  */
   /* milliseconds, measured with HIP events on the context's stream */
   double ms_total;          /* whole call on the stream                     */
   double ms_keys;           /* validation, axis choice and key build        */
