@@ -113,6 +113,9 @@ struct Context {
   DevBuf b_tables, b_skey, b_orig, b_gchunk, b_pairoff;
   PinnedBuf b_host;
   BatchPlan bplan;
+  int64_t b_budget = 0;      // scratch pairs of the learned batched pass
+  int64_t b_last_pairs = 0;  // pairs and items of the last batched call
+  int64_t b_last_items = 0;
 
   ~Context() {
     bplan.drop_graph();
@@ -145,7 +148,7 @@ struct Context {
                    const uint32_t *d_cand, stq_pair *out, int64_t capacity, stq_pair **alloc_out,
                    int64_t *count);
   void sizer_train(const float *d_feat, const float *d_target, int64_t n, int steps, float lr);
-  void sizer_remember(const float *d_feat, const float *d_target, int64_t n);
+  void sizer_remember(const float *d_feat, const float *d_target, int64_t n, int64_t start = 0);
   float sizer_gradient(const float *d_feat, const float *d_target, int64_t n);
   void run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t capacity,
            stq_pair **alloc_out, int64_t *count);
@@ -342,7 +345,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     sizer_ensure();
     d_feat = feat.as<float>((size_t)nitems * kFeat);
     d_cand = cand.as<uint32_t>((size_t)nitems);
-    launch_features(f32, sa, nullptr, nullptr, qp, axis, d_feat, d_cand, st);
+    launch_features(f32, sa, qp, axis, nullptr, nullptr, 0, d_feat, d_cand, st);
     ++scenes_seen;
   }
   if (learned_pass) {
@@ -465,12 +468,12 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
     throw StatusError(STQ_ERR_CAPACITY, "output capacity too small");
   }
   int *d_ovf = ovf.as<int>((size_t)nitems);
-  unsigned *d_novf = reinterpret_cast<unsigned *>(counters.as<unsigned long long>(8) + 4);
-  STQ_HIP(hipMemsetAsync(d_novf, 0, sizeof(unsigned), st));
+  unsigned long long *d_novf = counters.as<unsigned long long>(8) + 4;
+  STQ_HIP(hipMemsetAsync(d_novf, 0, sizeof(unsigned long long), st));
   launch_compact(d_counts, d_caps, d_capoff, d_off, nitems, d_scratch,
                  reinterpret_cast<int2 *>(dst), d_ovf, d_novf, st);
-  unsigned novf = 0;
-  STQ_HIP(hipMemcpyAsync(&novf, d_novf, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  unsigned long long novf = 0;
+  STQ_HIP(hipMemcpyAsync(&novf, d_novf, sizeof(novf), hipMemcpyDeviceToHost, st));
   STQ_HIP(hipStreamSynchronize(st));
   stats.n_fallback_tiles = novf;
   if (novf > 0) {
@@ -515,7 +518,9 @@ float Context::sizer_gradient(const float *d_feat, const float *d_target, int64_
   return 0.f;
 }
 
-void Context::sizer_remember(const float *d_feat, const float *d_target, int64_t n) {
+// Adds rows (start + k) * stride mod n, k < kReplayPerScene, to the ring.
+void Context::sizer_remember(const float *d_feat, const float *d_target, int64_t n,
+                             int64_t start) {
   if (n <= 0) return;
   float *rf = replay_feat.as<float>((size_t)kReplay * kFeat);
   float *rt = replay_t.as<float>((size_t)kReplay);
@@ -523,10 +528,10 @@ void Context::sizer_remember(const float *d_feat, const float *d_target, int64_t
   int64_t stride = 104729;
   while (std::__gcd(stride, n) != 1) ++stride;
   const int64_t first = std::min(take, kReplay - replay_pos);
-  launch_gather_batch(d_feat, d_target, n, 0, stride, first, rf + replay_pos * kFeat,
+  launch_gather_batch(d_feat, d_target, n, start, stride, first, rf + replay_pos * kFeat,
                       rt + replay_pos, stream);
   if (take > first)
-    launch_gather_batch(d_feat, d_target, n, first, stride, take - first, rf, rt, stream);
+    launch_gather_batch(d_feat, d_target, n, start + first, stride, take - first, rf, rt, stream);
   replay_pos = (replay_pos + take) % kReplay;
   replay_n = std::min(kReplay, replay_n + take);
 }
@@ -652,16 +657,18 @@ static uint64_t fnv1a(const void *p, size_t n, uint64_t h = 1469598103934665603u
 
 // All scenes in one sequence of launches with no host synchronisation until
 // the end: per-scene statistics, segmented keys and sort, scene-bounded chain
-// ends and planning, one count sweep and one fill sweep over every scene's
-// items (the item and pair totals stay on the device), per-scene pair
-// offsets.  The sequence is captured in a hipGraph and replayed while the
-// batch shape, the buffers and the options are unchanged.
+// ends and planning, then either one count sweep and one fill sweep over
+// every scene's items, or (learned sizing) one emitting sweep into predicted
+// slots, compaction and the exact refill of overflowed items.  Item and pair
+// totals stay on the device; the output is written only when it fits.  The
+// sequence is captured in a hipGraph and replayed while the batch shape, the
+// buffers and the options are unchanged.
 void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int nscenes,
                           stq_pair *out, int64_t capacity, int64_t *pair_off) {
   stats = stq_stats{};
   stats.n_scenes = nscenes;
   stats.precision = opt.precision;
-  stats.sizing = STQ_SIZING_COUNT_FILL;
+  stats.sizing = opt.sizing;
   stats.axis = opt.axis_policy <= 2 ? opt.axis_policy : -1;
   const bool f32 = opt.precision == STQ_PRECISION_F32_CONFIRM;
   hipStream_t st = stream;
@@ -687,6 +694,17 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   int sbits = 0;
   while ((1ll << sbits) < nscenes) ++sbits;
 
+  // sizing: count-then-fill, count-then-fill while the network trains on the
+  // exact counts, or the single learned pass
+  const bool learned = opt.sizing == STQ_SIZING_LEARNED;
+  const int train_scenes = opt.train_scenes > 0 ? opt.train_scenes : (opt.train_scenes < 0 ? 0 : 200);
+  const bool training = learned && scenes_seen < train_scenes;
+  const bool single_pass = learned && !training;
+  if (learned) sizer_ensure();
+  if (single_pass && b_budget == 0)
+    b_budget = std::max<int64_t>(int64_t(1) << 22,
+                                 (int64_t)kCapScale * b_last_pairs + (int64_t)kCapSlack * b_last_items);
+
   // every buffer is reserved here: the launches below allocate nothing
   const size_t ts = f32 ? 4 : 8;
   uint64_t *d_keys = keys.as<uint64_t>((size_t)n);
@@ -710,7 +728,19 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   scan_reserve_i64(std::max(ngroups, bound), scan_tmp);
   int64_t *d_pairoff = b_pairoff.as<int64_t>((size_t)nscenes + 1);
   unsigned long long *ctr = counters.as<unsigned long long>(8);
-  int64_t *hbuf = static_cast<int64_t *>(b_host.get(sizeof(int64_t) * ((size_t)nscenes + 6)));
+  float *d_feat = learned ? feat.as<float>((size_t)bound * kFeat) : nullptr;
+  uint32_t *d_cand = learned ? cand.as<uint32_t>((size_t)bound) : nullptr;
+  float *d_pred = learned ? pred.as<float>((size_t)bound) : nullptr;
+  uint32_t *d_caps = single_pass ? caps.as<uint32_t>((size_t)bound) : nullptr;
+  int64_t *d_capoff = single_pass ? cap_off.as<int64_t>((size_t)bound + 1) : nullptr;
+  int *d_ovf = single_pass ? ovf.as<int>((size_t)bound) : nullptr;
+  int2 *d_scratch = single_pass ? scratch_pairs.as<int2>((size_t)b_budget) : nullptr;
+  double *d_acc = single_pass ? mae_acc.as<double>(1) : nullptr;
+  const float *d_w = learned ? sz_params.as<float>(kParams) : nullptr;
+  // pinned results: pair offsets, 5 counters, items, total slots, error sum
+  const int64_t nh = (int64_t)nscenes + 1 + 8;
+  int64_t *hbuf = static_cast<int64_t *>(b_host.get(sizeof(int64_t) * (size_t)nh));
+  int64_t *h_ctr = hbuf + nscenes + 1;
 
   const int wpc = opt.waves_per_cu > 0 ? opt.waves_per_cu : 32;
   const int nblocks = (int)(8 * std::max<int64_t>(
@@ -750,45 +780,82 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     sa.counts = d_counts;
     sa.confirms = &ctr[3];
     sa.xcd_split = opt.xcd_split == 2 ? 0 : 1;
-    launch_sweep(f32, kCount, batch, nblocks, sa, st);
-    scan_u32_to_i64(d_counts, d_off, bound, scan_tmp, st);
-    sa.counts = nullptr;
-    sa.confirms = nullptr;
-    sa.offsets = d_off;
-    sa.out = reinterpret_cast<int2 *>(out);
-    sa.total_dev = d_off + bound;
-    sa.capacity = capacity;
-    launch_sweep(f32, kFill, batch, nblocks, sa, st);
-    launch_translate_dev(out, d_off + bound, capacity, d_orig, st);
+    if (learned)
+      launch_features(f32, sa, QParams{}, 0, bp.dev.params, bp.dev.seg_off, nscenes, d_feat, d_cand,
+                      st);
+    const int64_t *total = d_off + bound;
+    if (!single_pass) {
+      launch_sweep(f32, kCount, batch, nblocks, sa, st);
+      scan_u32_to_i64(d_counts, d_off, bound, scan_tmp, st);
+      sa.counts = nullptr;
+      sa.confirms = nullptr;
+      sa.offsets = d_off;
+      sa.out = reinterpret_cast<int2 *>(out);
+      sa.total_dev = total;
+      sa.capacity = capacity;
+      launch_sweep(f32, kFill, batch, nblocks, sa, st);
+    } else {
+      launch_mlp_forward(d_w, d_feat, bound, d_pred, st, sa.nitems_dev);
+      launch_caps(d_pred, d_cand, bound, kCapScale, kCapSlack, d_caps, st, sa.nitems_dev);
+      scan_u32_to_i64(d_caps, d_capoff, bound, scan_tmp, st);
+      launch_cap_budget(d_caps, d_capoff, bound, b_budget, st);
+      sa.offsets = d_capoff;
+      sa.caps = d_caps;
+      sa.out = d_scratch;
+      launch_sweep(f32, kLearned, batch, nblocks, sa, st);
+      scan_u32_to_i64(d_counts, d_off, bound, scan_tmp, st);
+      launch_fill_u64(reinterpret_cast<unsigned long long *>(d_acc), 0ull, 1, st);
+      launch_abs_error(d_pred, d_counts, bound, d_acc, st, sa.nitems_dev);
+      launch_compact(d_counts, d_caps, d_capoff, d_off, bound, d_scratch,
+                     reinterpret_cast<int2 *>(out), d_ovf, &ctr[4], st, sa.nitems_dev, total,
+                     capacity);
+      SweepArgs fa = sa;  // the overflowed items, exactly, straight into the output
+      fa.sel = d_ovf;
+      fa.nitems_dev = reinterpret_cast<const int64_t *>(&ctr[4]);
+      fa.counts = nullptr;
+      fa.confirms = nullptr;
+      fa.caps = nullptr;
+      fa.offsets = d_off;
+      fa.out = reinterpret_cast<int2 *>(out);
+      fa.total_dev = total;
+      fa.capacity = capacity;
+      launch_sweep(f32, kFill, batch, nblocks, fa, st);
+    }
+    launch_translate_dev(out, total, capacity, d_orig, st);
     launch_scene_pairs(d_off, d_ioff, t, bp.dev, d_pairoff, st);
     STQ_HIP(hipMemcpyAsync(hbuf, d_pairoff, sizeof(int64_t) * ((size_t)nscenes + 1),
                            hipMemcpyDeviceToHost, st));
-    STQ_HIP(hipMemcpyAsync(hbuf + nscenes + 1, ctr, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    STQ_HIP(hipMemcpyAsync(hbuf + nscenes + 5, d_ioff + ngroups, sizeof(int64_t),
-                           hipMemcpyDeviceToHost, st));
+    STQ_HIP(hipMemcpyAsync(h_ctr, ctr, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    STQ_HIP(hipMemcpyAsync(h_ctr + 5, d_ioff + ngroups, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    if (single_pass) {
+      STQ_HIP(hipMemcpyAsync(h_ctr + 6, d_capoff + bound, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+      STQ_HIP(hipMemcpyAsync(h_ctr + 7, d_acc, sizeof(double), hipMemcpyDeviceToHost, st));
+    }
   };
 
   struct {
     const void *boxes, *out, *host;
-    int64_t capacity;
+    int64_t capacity, budget;
     uint64_t generation;
-    int32_t precision, axis_policy, batch, nblocks, xcd_split, pad;
+    int32_t precision, axis_policy, batch, nblocks, xcd_split, sizing;
   } key;
   std::memset(&key, 0, sizeof(key));
   key.boxes = boxes;
   key.out = out;
   key.host = hbuf;
   key.capacity = capacity;
+  key.budget = single_pass ? b_budget : 0;
   key.generation = g_devbuf_generation.load();
   key.precision = opt.precision;
   key.axis_policy = opt.axis_policy;
   key.batch = batch;
   key.nblocks = nblocks;
   key.xcd_split = opt.xcd_split;
+  key.sizing = single_pass ? 1 : 0;
   const uint64_t sig = fnv1a(&key, sizeof(key)) | 1ull;
 
   bool done = false;
-  if (opt.graph != 2) {
+  if (opt.graph != 2 && !training) {
     if (bp.exec && bp.graph_sig == sig) {
       STQ_HIP(hipEventRecord(ev[E_START], st));
       STQ_HIP(hipGraphLaunch(bp.exec, st));
@@ -834,16 +901,52 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   STQ_HIP(hipEventRecord(ev[E_END], st));
   STQ_HIP(hipStreamSynchronize(st));
   stats.ms_total = ev_ms(ev[E_START], ev[E_END]);
-  const uint64_t *hc = reinterpret_cast<const uint64_t *>(hbuf + nscenes + 1);
+  const uint64_t *hc = reinterpret_cast<const uint64_t *>(h_ctr);
   for (int s = 0; s <= nscenes; ++s) pair_off[s] = hbuf[s];
+  const int64_t nitems = h_ctr[5], total = pair_off[nscenes];
   stats.n_candidates = (int64_t)hc[1];
   stats.n_lane_steps = (int64_t)hc[2];
   stats.n_confirm = (int64_t)hc[3];
-  stats.n_items = hbuf[nscenes + 5];
-  stats.n_pairs = pair_off[nscenes];
+  stats.n_items = nitems;
+  stats.n_pairs = total;
   if (hc[0]) throw StatusError(STQ_ERR_INVALID_BOX, "input box with NaN or min > max");
-  if (pair_off[nscenes] > capacity)
-    throw StatusError(STQ_ERR_CAPACITY, "output capacity too small");
+  b_last_pairs = total;
+  b_last_items = nitems;
+  if (learned) {
+    stats.n_tiles = nitems;
+    scenes_seen += nscenes;
+  }
+  if (single_pass) {
+    stats.n_fallback_tiles = (int64_t)hc[4];
+    double mae;
+    std::memcpy(&mae, &h_ctr[7], sizeof(double));
+    stats.sizing_mae = mae / (double)std::max<int64_t>(nitems, 1);
+    // predicted slots beyond the scratch were redone exactly; the next call
+    // gets room for all of them (a new buffer, hence a new graph)
+    const int64_t want = h_ctr[6];
+    if (want > b_budget) b_budget = want + want / 8;
+  }
+  if (training && nitems > 0) {
+    // error of the prediction before training, then a few training rounds
+    // on samples of this batch's items
+    launch_mlp_forward(d_w, d_feat, nitems, d_pred, st);
+    double *acc = mae_acc.as<double>(1);
+    STQ_HIP(hipMemsetAsync(acc, 0, sizeof(double), st));
+    launch_abs_error(d_pred, d_counts, nitems, acc, st);
+    float *d_t = targets.as<float>((size_t)nitems);
+    launch_targets(d_counts, nitems, d_t, st);
+    const int rounds = std::max(1, std::min(nscenes, 16));
+    for (int r = 0; r < rounds; ++r) {
+      sizer_remember(d_feat, d_t, nitems, (int64_t)r * kReplayPerScene);
+      sizer_train(replay_feat.as<float>((size_t)kReplay * kFeat), replay_t.as<float>(kReplay),
+                  replay_n, kTrainSteps, kTrainRate);
+    }
+    double mae = 0;
+    STQ_HIP(hipMemcpyAsync(&mae, acc, sizeof(double), hipMemcpyDeviceToHost, st));
+    STQ_HIP(hipStreamSynchronize(st));
+    stats.sizing_mae = mae / (double)nitems;
+  }
+  if (total > capacity) throw StatusError(STQ_ERR_CAPACITY, "output capacity too small");
 }
 
 template <class F>

@@ -12,17 +12,6 @@ namespace {
 constexpr int kT = 256;
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
 
-// scene of a position: the last s with seg_off[s] <= pos (empty scenes skip)
-__device__ __forceinline__ int scene_of(const int64_t *__restrict__ seg_off, int nscenes,
-                                        int64_t pos) {
-  int lo = 0, hi = nscenes;  // seg_off[lo] <= pos < seg_off[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (seg_off[mid] <= pos) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
 __device__ inline void other_axes_b(int axis, int &a1, int &a2) {
   a1 = axis == 0 ? 1 : 0;
   a2 = axis == 2 ? 1 : 2;

@@ -13,13 +13,6 @@
 
 namespace stqamd {
 
-// Per-scene result of the statistics pass.
-struct SceneParams {
-  int axis;
-  int pad;
-  QParams qp;
-};
-
 // Host-built tables of one batch shape (scene offsets + window size).
 struct BatchTables {
   int nscenes = 0;

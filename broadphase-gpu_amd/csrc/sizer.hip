@@ -23,20 +23,34 @@ __device__ __forceinline__ T wave_sum(T v) {
 // One wave per item.  Rows are the item's lanes; the q range is sampled at 64
 // evenly spaced positions.  Extents enter as log(extent)/10, centres as the
 // position inside the scene's quantisation range (0..1).
+// With scene tables (batched scenes) the level range and axes are those of the
+// item's scene, and the item count is read from the device.
 template <class R>
 __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items, int64_t nitems,
+                                                 const int64_t *__restrict__ nitems_dev,
                                                  const int *__restrict__ rowlist,
                                                  const int *__restrict__ hi,
-                                                 const R *__restrict__ rec, QParams qp, int a1,
-                                                 int a2, float *__restrict__ feat,
+                                                 const R *__restrict__ rec, QParams qp0, int a10,
+                                                 int a20, const SceneParams *__restrict__ sp,
+                                                 const int64_t *__restrict__ seg_off, int nscenes,
+                                                 float *__restrict__ feat,
                                                  uint32_t *__restrict__ cand) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kT / 64) + (threadIdx.x >> 6)));
   const int nwaves = (int)gridDim.x * (kT / 64);
-  const double n1 = qp.scale[a1] / 32767.0, n2 = qp.scale[a2] / 32767.0;
-  const double o1 = qp.origin[a1], o2 = qp.origin[a2];
+  if (nitems_dev) nitems = *nitems_dev;
   for (int64_t it = wave; it < nitems; it += nwaves) {
     const Item item = items[it];
+    QParams qp = qp0;
+    int a1 = a10, a2 = a20;
+    if (sp) {
+      const SceneParams p = sp[scene_of(seg_off, nscenes, rowlist[item.rowoff])];
+      qp = p.qp;
+      a1 = p.axis == 0 ? 1 : 0;
+      a2 = p.axis == 2 ? 1 : 2;
+    }
+    const double n1 = qp.scale[a1] / 32767.0, n2 = qp.scale[a2] / 32767.0;
+    const double o1 = qp.origin[a1], o2 = qp.origin[a2];
     const bool act = lane < item.nrows;
     const int q0 = item.q0, q1 = item.q1;
     float fr = act ? 1.f : 0.f;
@@ -133,8 +147,11 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
 // ---- MLP forward -----------------------------------------------------------------
 __global__ __launch_bounds__(kT) void k_mlp_forward(const float *__restrict__ params,
                                                     const float *__restrict__ feat, int64_t n,
+                                                    const int64_t *__restrict__ n_dev,
                                                     float *__restrict__ pred) {
   __shared__ float w[kParams];
+  if (n_dev) n = *n_dev;
+  if ((int64_t)blockIdx.x * kT >= n) return;
   for (int i = threadIdx.x; i < kParams; i += kT) w[i] = params[i];
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
@@ -264,14 +281,26 @@ __global__ __launch_bounds__(kT) void k_adam(float *__restrict__ w, const float 
 
 __global__ __launch_bounds__(kT) void k_caps(const float *__restrict__ pred,
                                              const uint32_t *__restrict__ cand, int64_t n,
+                                             const int64_t *__restrict__ n_dev,
                                              float alpha, float beta,
                                              uint32_t *__restrict__ caps) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
+  if (n_dev && i >= *n_dev) {  // beyond the device-side item count
+    caps[i] = 0;
+    return;
+  }
   const float pr = fminf(fmaxf(pred[i], 0.f), 30.f);
   const float want = ceilf(alpha * expm1f(pr) + beta);
   const uint32_t c = cand[i];
   caps[i] = want >= (float)c ? c : (uint32_t)want;
+}
+
+__global__ __launch_bounds__(kT) void k_cap_budget(uint32_t *__restrict__ caps,
+                                                   const int64_t *__restrict__ cap_off, int64_t n,
+                                                   int64_t budget) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n && cap_off[i] + (int64_t)caps[i] > budget) caps[i] = 0;
 }
 
 __global__ __launch_bounds__(kT) void k_targets(const uint32_t *__restrict__ counts, int64_t n,
@@ -282,8 +311,10 @@ __global__ __launch_bounds__(kT) void k_targets(const uint32_t *__restrict__ cou
 
 __global__ __launch_bounds__(kT) void k_abs_error(const float *__restrict__ pred,
                                                   const uint32_t *__restrict__ counts, int64_t n,
+                                                  const int64_t *__restrict__ n_dev,
                                                   double *__restrict__ acc) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (n_dev) n = *n_dev;
   double e = 0.0;
   if (i < n) e = fabs((double)expm1f(fminf(fmaxf(pred[i], 0.f), 30.f)) - (double)counts[i]);
   e = wave_sum(e);
@@ -294,12 +325,16 @@ __global__ __launch_bounds__(kT) void k_compact(const uint32_t *__restrict__ cou
                                                 const uint32_t *__restrict__ caps,
                                                 const int64_t *__restrict__ cap_off,
                                                 const int64_t *__restrict__ off, int64_t n,
+                                                const int64_t *__restrict__ n_dev,
+                                                const int64_t *__restrict__ total, int64_t capacity,
                                                 const int2 *__restrict__ scratch,
                                                 int2 *__restrict__ out, int *__restrict__ ovf,
-                                                unsigned *__restrict__ novf) {
+                                                unsigned long long *__restrict__ novf) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kT / 64) + (threadIdx.x >> 6)));
   const int nwaves = (int)gridDim.x * (kT / 64);
+  if (total && *total > capacity) return;  // the output does not fit: nothing is written
+  if (n_dev) n = *n_dev;
   for (int64_t it = wave; it < n; it += nwaves) {
     const uint32_t c = counts[it];
     if (c <= caps[it]) {
@@ -307,7 +342,7 @@ __global__ __launch_bounds__(kT) void k_compact(const uint32_t *__restrict__ cou
       int2 *dst = out + off[it];
       for (uint32_t k = lane; k < c; k += 64) dst[k] = src[k];
     } else if (lane == 0) {
-      ovf[atomicAdd(novf, 1u)] = (int)it;
+      ovf[atomicAdd(novf, 1ull)] = (int)it;
     }
   }
 }
@@ -328,31 +363,42 @@ inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
 
 }  // namespace
 
-void launch_features(bool f32, const SweepArgs &sa, const void *, const void *, const QParams &qp,
-                     int axis, float *feat, uint32_t *cand, hipStream_t st) {
+void launch_features(bool f32, const SweepArgs &sa, const QParams &qp, int axis,
+                     const SceneParams *sp, const int64_t *seg_off, int nscenes, float *feat,
+                     uint32_t *cand, hipStream_t st) {
   if (sa.nitems <= 0) return;
   const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
   const unsigned g = (unsigned)std::min<int64_t>((sa.nitems + 3) / 4, 4096);
   if (f32)
     hipLaunchKernelGGL((k_features<Exact64>), dim3(g), dim3(kT), 0, st, sa.items, sa.nitems,
-                       sa.rowlist, sa.hi, (const Exact64 *)sa.rec, qp, a1, a2, feat, cand);
+                       sa.nitems_dev, sa.rowlist, sa.hi, (const Exact64 *)sa.rec, qp, a1, a2, sp,
+                       seg_off, nscenes, feat, cand);
   else
     hipLaunchKernelGGL((k_features<Rec64>), dim3(g), dim3(kT), 0, st, sa.items, sa.nitems,
-                       sa.rowlist, sa.hi, (const Rec64 *)sa.rec, qp, a1, a2, feat, cand);
+                       sa.nitems_dev, sa.rowlist, sa.hi, (const Rec64 *)sa.rec, qp, a1, a2, sp,
+                       seg_off, nscenes, feat, cand);
   STQ_HIP(hipGetLastError());
 }
 
 void launch_mlp_forward(const float *params, const float *feat, int64_t n, float *pred,
-                        hipStream_t st) {
+                        hipStream_t st, const int64_t *n_dev) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_mlp_forward, dim3(nblk(n)), dim3(kT), 0, st, params, feat, n, pred);
+  hipLaunchKernelGGL(k_mlp_forward, dim3(nblk(n)), dim3(kT), 0, st, params, feat, n, n_dev, pred);
   STQ_HIP(hipGetLastError());
 }
 
 void launch_caps(const float *pred, const uint32_t *cand, int64_t n, float alpha, float beta,
-                 uint32_t *caps, hipStream_t st) {
+                 uint32_t *caps, hipStream_t st, const int64_t *n_dev) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_caps, dim3(nblk(n)), dim3(kT), 0, st, pred, cand, n, alpha, beta, caps);
+  hipLaunchKernelGGL(k_caps, dim3(nblk(n)), dim3(kT), 0, st, pred, cand, n, n_dev, alpha, beta,
+                     caps);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_cap_budget(uint32_t *caps, const int64_t *cap_off, int64_t n, int64_t budget,
+                       hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cap_budget, dim3(nblk(n)), dim3(kT), 0, st, caps, cap_off, n, budget);
   STQ_HIP(hipGetLastError());
 }
 
@@ -381,19 +427,20 @@ void launch_adam(float *params, const float *grad, float *m, float *v, float lr,
 }
 
 void launch_abs_error(const float *pred, const uint32_t *counts, int64_t n, double *acc,
-                      hipStream_t st) {
+                      hipStream_t st, const int64_t *n_dev) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_abs_error, dim3(nblk(n)), dim3(kT), 0, st, pred, counts, n, acc);
+  hipLaunchKernelGGL(k_abs_error, dim3(nblk(n)), dim3(kT), 0, st, pred, counts, n, n_dev, acc);
   STQ_HIP(hipGetLastError());
 }
 
 void launch_compact(const uint32_t *counts, const uint32_t *caps, const int64_t *cap_off,
                     const int64_t *off, int64_t n, const int2 *scratch, int2 *out, int *ovf,
-                    unsigned *novf, hipStream_t st) {
+                    unsigned long long *novf, hipStream_t st, const int64_t *n_dev,
+                    const int64_t *total, int64_t capacity) {
   if (n <= 0) return;
   const unsigned g = (unsigned)std::min<int64_t>((n + 3) / 4, 8192);
-  hipLaunchKernelGGL(k_compact, dim3(g), dim3(kT), 0, st, counts, caps, cap_off, off, n, scratch,
-                     out, ovf, novf);
+  hipLaunchKernelGGL(k_compact, dim3(g), dim3(kT), 0, st, counts, caps, cap_off, off, n, n_dev,
+                     total, capacity, scratch, out, ovf, novf);
   STQ_HIP(hipGetLastError());
 }
 

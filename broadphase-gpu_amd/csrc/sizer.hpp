@@ -26,15 +26,23 @@ constexpr int kOffB3 = kOffW3 + kHid;
 constexpr int kParams = kOffB3 + 1;  // 6337
 
 // Features of every work item (nitems x kFeat floats) and its exact number of
-// sweep candidates (an upper bound on its pairs).
-void launch_features(bool f32, const SweepArgs &sa, const void *smin, const void *smax,
-                     const QParams &qp, int axis, float *feat, uint32_t *cand, hipStream_t st);
-// pred[i] = MLP(feat[i]) (the log1p of the predicted pair count).
+// sweep candidates (an upper bound on its pairs).  One scene: qp and axis;
+// batched scenes: sp/seg_off/nscenes give each item's scene, and the item
+// count is sa.nitems_dev when set.
+void launch_features(bool f32, const SweepArgs &sa, const QParams &qp, int axis,
+                     const SceneParams *sp, const int64_t *seg_off, int nscenes, float *feat,
+                     uint32_t *cand, hipStream_t st);
+// pred[i] = MLP(feat[i]) (the log1p of the predicted pair count).  Where an
+// n_dev pointer is given, the count is *n_dev (n is then an upper bound).
 void launch_mlp_forward(const float *params, const float *feat, int64_t n, float *pred,
-                        hipStream_t st);
-// caps[i] = min(cand[i], ceil(alpha * expm1(pred[i]) + beta)).
+                        hipStream_t st, const int64_t *n_dev = nullptr);
+// caps[i] = min(cand[i], ceil(alpha * expm1(pred[i]) + beta)); 0 for i >= *n_dev.
 void launch_caps(const float *pred, const uint32_t *cand, int64_t n, float alpha, float beta,
-                 uint32_t *caps, hipStream_t st);
+                 uint32_t *caps, hipStream_t st, const int64_t *n_dev = nullptr);
+// caps[i] = 0 where the item's slots [cap_off[i], cap_off[i] + caps[i]) end
+// beyond `budget` (the scratch size): such items are redone exactly.
+void launch_cap_budget(uint32_t *caps, const int64_t *cap_off, int64_t n, int64_t budget,
+                       hipStream_t st);
 // targets[i] = log1p(counts[i]).
 void launch_targets(const uint32_t *counts, int64_t n, float *targets, hipStream_t st);
 // Mean squared error of MLP(feat) against targets and its gradient (kParams
@@ -46,12 +54,14 @@ void launch_adam(float *params, const float *grad, float *m, float *v, float lr,
                  hipStream_t st);
 // sum |expm1(pred) - counts| over items, added to *acc.
 void launch_abs_error(const float *pred, const uint32_t *counts, int64_t n, double *acc,
-                      hipStream_t st);
+                      hipStream_t st, const int64_t *n_dev = nullptr);
 // Per item: if counts <= caps copy its pairs from scratch (at cap_off) to out
-// (at off); otherwise append the item to ovf (count in *novf).
+// (at off); otherwise append the item to ovf (count in *novf).  Nothing
+// happens when *total exceeds capacity (total may be null).
 void launch_compact(const uint32_t *counts, const uint32_t *caps, const int64_t *cap_off,
                     const int64_t *off, int64_t n, const int2 *scratch, int2 *out, int *ovf,
-                    unsigned *novf, hipStream_t st);
+                    unsigned long long *novf, hipStream_t st, const int64_t *n_dev = nullptr,
+                    const int64_t *total = nullptr, int64_t capacity = 0);
 
 // Minibatch rows k = 0..bs-1 of item (start + k) * stride mod n.
 void launch_gather_batch(const float *feat, const float *target, int64_t n, int64_t start,

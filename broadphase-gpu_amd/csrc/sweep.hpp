@@ -48,6 +48,24 @@ __device__ inline uint32_t level_up(double x, double o, double sc) {
   return (uint32_t)t;
 }
 
+// Per-scene sweep axis and level range (batched scenes).
+struct SceneParams {
+  int axis;
+  int pad;
+  QParams qp;
+};
+
+// Scene of a sorted or input position: the s with seg_off[s] <= pos <
+// seg_off[s+1] (seg_off has nscenes+1 entries, empty scenes are skipped).
+__device__ inline int scene_of(const int64_t *__restrict__ seg_off, int nscenes, int64_t pos) {
+  int lo = 0, hi = nscenes;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (seg_off[mid] <= pos) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 void launch_axis_stats(const stq_aabb_f64 *boxes, int64_t n, double *partial, int nblocks,
                        hipStream_t st);
 void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, void *keys,

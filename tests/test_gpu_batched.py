@@ -10,7 +10,7 @@ import pytest
 
 import stq_amd
 from stq_amd import _lib, scenes
-from oracle import oracle
+from oracle import mlp, oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -208,3 +208,89 @@ def test_batched_stream_512_scenes(ctx):
     finally:
         db.close()
         dout.close()
+
+
+# ---- learned sizing inside the batched pipeline ------------------------------------
+def _countfill(kw, allb, offs):
+    c = stq_amd.Context(0, graph=2, **kw)
+    try:
+        return c.find_pairs_batched(allb, offs)
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("form,kw", FORMS)
+@pytest.mark.parametrize("graph", [0, 2])
+def test_batched_learned_exact_and_same_order(form, kw, graph):
+    """The first call trains on exact counts (count-then-fill); later calls use
+    the single learned pass.  Pairs and their order equal the count-then-fill
+    pipeline's on every call."""
+    sc = mixed_scenes(120, 23)
+    allb, offs = scenes.concat_scenes(sc)
+    ref_pairs, ref_offs = _countfill(kw, allb, offs)
+    check_scenes(ref_pairs, ref_offs, sc, form)
+    c = stq_amd.Context(0, sizing=stq_amd.SIZING_LEARNED, train_scenes=10, graph=graph, **kw)
+    try:
+        for rep in range(4):
+            pairs, poffs = c.find_pairs_batched(allb, offs)
+            st = c.stats()
+            print(f"{form} graph={graph} call {rep}: items {st['n_items']} fallback {st['n_fallback_tiles']} "
+                  f"mae {st['sizing_mae']:.2f} state {st['graph']}", flush=True)
+            assert np.array_equal(poffs, ref_offs), rep
+            assert np.array_equal(pairs, ref_pairs), rep
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("form,kw", FORMS)
+def test_batched_learned_forced_fallback(form, kw):
+    """A network that predicts no pairs: every item with more than the slack
+    overflows and is redone exactly."""
+    sc = mixed_scenes(121, 12)
+    allb, offs = scenes.concat_scenes(sc)
+    ref_pairs, ref_offs = _countfill(kw, allb, offs)
+    c = stq_amd.Context(0, sizing=stq_amd.SIZING_LEARNED, train_scenes=-1, graph=0, **kw)
+    try:
+        w = np.zeros(mlp.NPARAMS, np.float32)
+        w[-1] = -5.0
+        c.sizer_set_params(w)
+        for rep in range(3):
+            pairs, poffs = c.find_pairs_batched(allb, offs)
+            st = c.stats()
+            assert st["n_fallback_tiles"] > 0
+            assert np.array_equal(pairs, ref_pairs) and np.array_equal(poffs, ref_offs), rep
+    finally:
+        c.close()
+
+
+def test_batched_learned_budget_growth():
+    """A network that predicts huge counts asks for more slots than the first
+    scratch budget holds: the items past the budget are redone exactly, and
+    the next call has room for all of them."""
+    kw = dict(precision=stq_amd.PRECISION_F32_CONFIRM)
+    items = scenes.batched_scenes(count=96, seed=8, nmin=1e3, nmax=4e4)
+    sc = [b for (_, _, b) in items]
+    allb, offs = scenes.concat_scenes(sc)
+    ref_pairs, ref_offs = _countfill(kw, allb, offs)
+    c = stq_amd.Context(0, sizing=stq_amd.SIZING_LEARNED, train_scenes=-1, graph=0, **kw)
+    try:
+        w = np.zeros(mlp.NPARAMS, np.float32)
+        w[-1] = 12.0
+        c.sizer_set_params(w)
+        need = int(ref_offs[-1])
+        db = _lib.DeviceBuffer.from_array(allb)
+        dout = _lib.DeviceBuffer(8 * need)
+        fb = []
+        for rep in range(3):
+            rc, poffs = c.find_pairs_batched_device(db.ptr, offs, dout.ptr, need)
+            assert rc == _lib.STQ_OK
+            fb.append(c.stats()["n_fallback_tiles"])
+            pairs = dout.to_array(np.int32, 2 * need).reshape(-1, 2)
+            assert np.array_equal(pairs, ref_pairs) and np.array_equal(poffs, ref_offs), rep
+        db.close()
+        dout.close()
+        print(f"{need} pairs; fallback items per call {fb}", flush=True)
+        assert fb[0] > 0 and fb[1] == 0 and fb[2] == 0
+    finally:
+        c.close()
+    check_scenes(ref_pairs, ref_offs, sc, "budget")
