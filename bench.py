@@ -47,6 +47,11 @@ def parse():
     ap.add_argument("--no-check", action="store_true", help="skip the pair-set comparison")
     ap.add_argument("--train-scenes", type=int, default=80,
                     help="scenes of the online training stream run before timing the chosen form")
+    ap.add_argument("--batched-scenes", type=int, default=512,
+                    help="scenes of the batched leg (sizes log-uniform in [1e3, 5e5]; 0 = skip)")
+    ap.add_argument("--stream-scenes", type=int, default=240,
+                    help="scenes of the learned-sizing stream leg: the first 200 train, the rest are "
+                         "held out (sizes log-uniform in [5e4, 4e6]; 0 = skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (traffic calibration)")
     return ap.parse_args()
@@ -61,6 +66,145 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor() or "unknown"
+
+
+def pair_digest(pairs):
+    """Order-independent digest of a pair list: (count, sum of a 64-bit mix of
+    each canonical (min, max) key, mod 2^64)."""
+    import numpy as np
+    p = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
+    a = np.minimum(p[:, 0], p[:, 1]).astype(np.uint64)
+    b = np.maximum(p[:, 0], p[:, 1]).astype(np.uint64)
+    z = (a << np.uint64(32)) | b
+    with np.errstate(over="ignore"):
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+        return len(p), int(z.sum(dtype=np.uint64))
+
+
+def bench_batched(args, H):
+    """512 scenes back to back (the batched stream), device-resident: the
+    chosen batched form (reduced precision, chain-length axis, learned single
+    pass, hipGraph replay) against plain FP64 count-then-fill, also batched
+    and replayed.  Per-scene pair sets are compared by digest."""
+    import numpy as np
+    import stq_amd
+    from stq_amd import _lib, scenes
+    t0 = time.perf_counter()
+    items = scenes.batched_scenes(count=args.batched_scenes, seed=3, nmin=1e3, nmax=5e5)
+    sc = [b for (_, _, b) in items]
+    allb, offs = scenes.concat_scenes(sc)
+    del items, sc
+    gen_s = time.perf_counter() - t0
+    db = _lib.DeviceBuffer.from_array(allb)
+    nbox = len(allb)
+    del allb
+    forms = {
+        "chosen": dict(precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN,
+                       sizing=stq_amd.SIZING_LEARNED, train_scenes=200),
+        "plain": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X),
+    }
+    out, digests = {}, {}
+    for name, kw in forms.items():
+        ctx = stq_amd.Context(0, **kw)
+        rc, poffs = ctx.find_pairs_batched_device(db.ptr, offs, None, 0)  # sizes (and trains)
+        train = ctx.stats()
+        need = int(poffs[-1])
+        dout = _lib.DeviceBuffer(8 * max(need, 1))
+        for _ in range(max(args.warmup, 2)):  # eager, then capture
+            ctx.find_pairs_batched_device(db.ptr, offs, dout.ptr, need)
+        ms = []
+        for _ in range(args.steps):
+            rc, p2 = ctx.find_pairs_batched_device(db.ptr, offs, dout.ptr, need)
+            assert rc == _lib.STQ_OK and np.array_equal(p2, poffs)
+            ms.append(ctx.stats()["ms_total"])
+        st = ctx.stats()
+        t = float(np.median(ms))
+        out[name] = dict(ms_per_batch=t, boxes_per_s=nbox / (t * 1e-3), pairs_per_s=need / (t * 1e-3),
+                         pairs=need, items=st["n_items"], candidates=st["n_candidates"],
+                         graph_state=st["graph"], fallback_items=st["n_fallback_tiles"],
+                         mae_pairs_per_item=st["sizing_mae"] if kw.get("sizing") else None,
+                         training_call_mae=train["sizing_mae"] if kw.get("sizing") else None)
+        if not args.no_check:
+            pairs = dout.to_array(np.int32, 2 * need).reshape(-1, 2)
+            digests[name] = [pair_digest(pairs[poffs[s]:poffs[s + 1]]) for s in range(len(poffs) - 1)]
+            del pairs
+        dout.close()
+        ctx.close()
+    db.close()
+    exact = (digests["chosen"] == digests["plain"]) if len(digests) == 2 else None
+    return {"workload": f"{args.batched_scenes} scenes, sizes log-uniform in [1e3, 5e5], uniform-like or "
+                        f"cloth-like, density factor U[0.5, 4] (seed 3): {nbox} FP64 boxes, device-resident",
+            "boxes": nbox, "generate_s": gen_s,
+            "chosen": out["chosen"], "plain_fp64_batched": out["plain"],
+            "speedup_vs_plain_fp64": out["chosen"]["boxes_per_s"] / out["plain"]["boxes_per_s"],
+            "per_scene_digests_equal": exact}
+
+
+def bench_stream(args, H):
+    """The learned-sizing stream: scenes of 5e4 to 4e6 boxes, density factor
+    log-uniform in [0.25, 8].  The first 200 run count-then-fill and train the
+    network on their exact per-item counts; the rest use the single learned
+    pass.  Reports the count error and the fallback rate on the held-out
+    scenes, and checks every 8th of them against plain FP64 by digest."""
+    import numpy as np
+    import stq_amd
+    from stq_amd import _lib, scenes
+    rng = np.random.default_rng(4)
+    ntrain = min(200, args.stream_scenes)
+    ctx = stq_amd.Context(0, precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN,
+                          sizing=stq_amd.SIZING_LEARNED, train_scenes=ntrain)
+    plain = stq_amd.Context(0, precision=stq_amd.PRECISION_F64)
+    cap = 1 << 26
+    dout = _lib.DeviceBuffer(8 * cap)
+    rows = []
+    t0 = time.perf_counter()
+    checked, mismatches = 0, 0
+    for i in range(args.stream_scenes):
+        nb = int(10 ** rng.uniform(np.log10(5e4), np.log10(4e6)))
+        rho = float(10 ** rng.uniform(np.log10(0.25), np.log10(8.0)))
+        b = scenes.density_scene(i % 2, nb, rho, seed=70_000 + i)
+        db = _lib.DeviceBuffer.from_array(b)
+        try:
+            cnt = ctx.find_pairs_device(db.ptr, len(b), dout.ptr, cap)
+        except stq_amd.StqError as e:
+            if e.status != _lib.STQ_ERR_CAPACITY:
+                raise
+            cnt = ctx.stats()["n_pairs"]
+            dout.close()
+            cap = int(cnt * 1.25)
+            dout = _lib.DeviceBuffer(8 * cap)
+            cnt = ctx.find_pairs_device(db.ptr, len(b), dout.ptr, cap)
+        st = ctx.stats()
+        rows.append((i, len(b), cnt, st["n_tiles"], st["n_fallback_tiles"], st["sizing_mae"], st["ms_total"]))
+        if i >= ntrain and (i - ntrain) % 8 == 0 and not args.no_check:
+            got = pair_digest(dout.to_array(np.int32, 2 * cnt).reshape(-1, 2))
+            c2 = plain.find_pairs_device(db.ptr, len(b), dout.ptr, cap)
+            ref = pair_digest(dout.to_array(np.int32, 2 * c2).reshape(-1, 2))
+            checked += 1
+            mismatches += int(got != ref)
+        db.close()
+    dout.close()
+    ctx.close()
+    plain.close()
+    held = [r for r in rows if r[0] >= ntrain]
+    tr = [r for r in rows if r[0] < ntrain]
+
+    def agg(rs):
+        if not rs:
+            return None
+        items = sum(r[3] for r in rs)
+        return {"scenes": len(rs), "boxes": int(sum(r[1] for r in rs)), "pairs": int(sum(r[2] for r in rs)),
+                "items": int(items), "fallback_items": int(sum(r[4] for r in rs)),
+                "fallback_rate": sum(r[4] for r in rs) / max(items, 1),
+                "mae_pairs_per_item": sum(r[5] * r[3] for r in rs) / max(items, 1),
+                "ms_total": sum(r[6] for r in rs)}
+    return {"workload": f"{args.stream_scenes} scenes (seed 4), 5e4..4e6 boxes log-uniform, density factor "
+                        f"log-uniform in [0.25, 8]; first {ntrain} train the network online",
+            "seconds": time.perf_counter() - t0, "training": agg(tr), "held_out": agg(held),
+            "held_out_checked_vs_plain_fp64": checked, "mismatches": mismatches}
 
 
 def main():
@@ -225,6 +369,10 @@ def main():
                      "kernel_ms": r["fill_sweep_ms"]},
         "cpu_baseline": cpu,
     }
+    if args.batched_scenes > 0:
+        line["batched"] = bench_batched(args, H)
+    if args.stream_scenes > 0:
+        line["learned_stream"] = bench_stream(args, H)
     print(json.dumps(line))
 
 
