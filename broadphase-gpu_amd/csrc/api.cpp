@@ -84,6 +84,14 @@ struct PinnedBuf {
   }
 };
 
+// Device counters of one call: [0] error flags, [1] candidates, [2] lane
+// steps, [3] confirmations, [4] overflowed items; [8..12) and [12..16) the
+// item-claim counters (kXcds x u32) of the first and second sweep launch.
+constexpr int kCounters = 16;
+inline unsigned *claim_counters(unsigned long long *ctr, int launch) {
+  return reinterpret_cast<unsigned *>(ctr + 8 + 4 * launch);
+}
+
 enum Ev {
   E_START, E_KEYS, E_SORT, E_GATHER, E_PLAN, E_PRED, E_COUNT, E_SCAN, E_FILL, E_COMPACT, E_END,
   E_NUM
@@ -175,7 +183,7 @@ static bool options_valid(const stq_options &o) {
          (o.window == 0 || (o.window >= 64 && o.window <= 2048 && (o.window & (o.window - 1)) == 0)) &&
          o.waves_per_cu >= 0 && o.waves_per_cu <= 64 &&
          (o.batch == 0 || o.batch == 2 || o.batch == 4 || o.batch == 8) && o.xcd_split >= 0 &&
-         o.xcd_split <= 2;
+         o.xcd_split <= 2 && o.graph >= 0 && o.graph <= 2 && o.claim >= 0 && o.claim <= 2;
 }
 
 // Axis choice and, for the reduced-precision form, the quantisation range:
@@ -258,8 +266,8 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     return;
   }
   // counters: [0] error flags, [1] candidates, [2] lane steps, [3] confirms
-  unsigned long long *ctr = counters.as<unsigned long long>(8);
-  STQ_HIP(hipMemsetAsync(ctr, 0, 8 * sizeof(unsigned long long), st));
+  unsigned long long *ctr = counters.as<unsigned long long>(kCounters);
+  STQ_HIP(hipMemsetAsync(ctr, 0, kCounters * sizeof(unsigned long long), st));
 
   QParams qp{};
   const bool learned = opt.sizing == STQ_SIZING_LEARNED;
@@ -331,6 +339,8 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   sa.counts = d_counts;
   sa.confirms = &ctr[3];
   sa.xcd_split = opt.xcd_split == 2 ? 0 : 1;
+  const bool claim = opt.claim != 2;
+  sa.work = claim ? claim_counters(ctr, 0) : nullptr;
   const int batch = opt.batch ? opt.batch : (f32 ? 8 : 2);
 
   // learned sizing: features of every item; the network predicts, or (while
@@ -401,6 +411,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     sa.confirms = nullptr;
     sa.offsets = d_off;
     sa.out = reinterpret_cast<int2 *>(dst);
+    sa.work = claim ? claim_counters(ctr, 1) : nullptr;
     launch_sweep(f32, kFill, batch, nblocks, sa, st);
   }
   STQ_HIP(hipEventRecord(ev[E_FILL], st));
@@ -468,7 +479,7 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
     throw StatusError(STQ_ERR_CAPACITY, "output capacity too small");
   }
   int *d_ovf = ovf.as<int>((size_t)nitems);
-  unsigned long long *d_novf = counters.as<unsigned long long>(8) + 4;
+  unsigned long long *d_novf = counters.as<unsigned long long>(kCounters) + 4;
   STQ_HIP(hipMemsetAsync(d_novf, 0, sizeof(unsigned long long), st));
   launch_compact(d_counts, d_caps, d_capoff, d_off, nitems, d_scratch,
                  reinterpret_cast<int2 *>(dst), d_ovf, d_novf, st);
@@ -485,6 +496,7 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
     fa.caps = nullptr;
     fa.offsets = d_off;
     fa.out = reinterpret_cast<int2 *>(dst);
+    if (fa.work) fa.work = claim_counters(counters.as<unsigned long long>(kCounters), 1);
     launch_sweep(f32, kFill, batch, nblocks, fa, st);
   }
   STQ_HIP(hipEventRecord(ev[E_COMPACT], st));
@@ -559,7 +571,7 @@ void Context::sizer_train(const float *d_feat, const float *d_target, int64_t n,
 
 void Context::finish_stats(hipStream_t st) {
   unsigned long long hc[4];
-  unsigned long long *ctr = counters.as<unsigned long long>(8);
+  unsigned long long *ctr = counters.as<unsigned long long>(kCounters);
   STQ_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
   STQ_HIP(hipStreamSynchronize(st));
   stats.n_lane_steps = (int64_t)hc[2];
@@ -727,7 +739,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   int64_t *d_off = offsets.as<int64_t>((size_t)bound + 1);
   scan_reserve_i64(std::max(ngroups, bound), scan_tmp);
   int64_t *d_pairoff = b_pairoff.as<int64_t>((size_t)nscenes + 1);
-  unsigned long long *ctr = counters.as<unsigned long long>(8);
+  unsigned long long *ctr = counters.as<unsigned long long>(kCounters);
   float *d_feat = learned ? feat.as<float>((size_t)bound * kFeat) : nullptr;
   uint32_t *d_cand = learned ? cand.as<uint32_t>((size_t)bound) : nullptr;
   float *d_pred = learned ? pred.as<float>((size_t)bound) : nullptr;
@@ -748,7 +760,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   const int batch = opt.batch ? opt.batch : (f32 ? 8 : 2);
 
   auto enqueue = [&]() {
-    launch_fill_u64(ctr, 0ull, 8, st);
+    launch_fill_u64(ctr, 0ull, kCounters, st);
     launch_seg_stats(boxes, t, bp.dev, opt.axis_policy, opt.axis_policy <= 2 ? opt.axis_policy : 0,
                      f32, st);
     launch_keys_seg(f32, boxes, t, bp.dev, d_keys, d_vals, reinterpret_cast<unsigned *>(&ctr[0]),
@@ -780,6 +792,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     sa.counts = d_counts;
     sa.confirms = &ctr[3];
     sa.xcd_split = opt.xcd_split == 2 ? 0 : 1;
+    sa.work = opt.claim != 2 ? claim_counters(ctr, 0) : nullptr;
     if (learned)
       launch_features(f32, sa, QParams{}, 0, bp.dev.params, bp.dev.seg_off, nscenes, d_feat, d_cand,
                       st);
@@ -793,6 +806,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
       sa.out = reinterpret_cast<int2 *>(out);
       sa.total_dev = total;
       sa.capacity = capacity;
+      if (sa.work) sa.work = claim_counters(ctr, 1);
       launch_sweep(f32, kFill, batch, nblocks, sa, st);
     } else {
       launch_mlp_forward(d_w, d_feat, bound, d_pred, st, sa.nitems_dev);
@@ -819,6 +833,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
       fa.out = reinterpret_cast<int2 *>(out);
       fa.total_dev = total;
       fa.capacity = capacity;
+      if (fa.work) fa.work = claim_counters(ctr, 1);
       launch_sweep(f32, kFill, batch, nblocks, fa, st);
     }
     launch_translate_dev(out, total, capacity, d_orig, st);

@@ -316,14 +316,19 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_bal
 // robin (block b on XCD b % kXcds); with xcd_split each XCD takes one
 // contiguous slice of the item list, so its L2 holds only that slice's q
 // records.
+// With work counters (one per XCD slice) the waves claim items one at a time,
+// so a wave that drew short items takes more of them; otherwise wave w takes
+// items w, w + nwaves, ...
 struct WaveSlice {
   int64_t kbeg, kend;
-  int wave, nwaves;
+  int wave, nwaves, xcd;
 };
 __device__ __forceinline__ WaveSlice wave_slice(int64_t nitems, int xcd_split) {
   WaveSlice w;
+  w.xcd = 0;
   if (xcd_split) {
     const int xcd = (int)(blockIdx.x % kXcds);
+    w.xcd = xcd;
     const int per = (int)(gridDim.x / kXcds);
     w.wave = uniform((int)((blockIdx.x / kXcds) * (kThreads / 64) + (threadIdx.x >> 6)));
     w.nwaves = per * (kThreads / 64);
@@ -336,6 +341,18 @@ __device__ __forceinline__ WaveSlice wave_slice(int64_t nitems, int xcd_split) {
     w.kend = nitems;
   }
   return w;
+}
+
+// Claims the next item of the slice: lane 0's atomic result, read out only
+// when the item is needed (claim_issue at the start of the previous item,
+// claim_take at its end), so the atomic's latency overlaps the sweep.
+__device__ __forceinline__ unsigned claim_issue(unsigned *ctr) {
+  unsigned v = 0;
+  if ((threadIdx.x & 63) == 0) v = atomicAdd(ctr, 1u);
+  return v;
+}
+__device__ __forceinline__ int64_t claim_take(const WaveSlice &ws, unsigned v) {
+  return ws.kbeg + (int64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)v);
 }
 
 // Bounds of the unchecked middle part: every active lane has p < q <= hi_p
@@ -394,7 +411,10 @@ __global__ __launch_bounds__(kThreads) void k_sweep64(SweepArgs a, const Rec64 *
   const int lane = threadIdx.x & 63;
   if (MODE == kFill && a.total_dev && *a.total_dev > a.capacity) return;
   const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split);
-  for (int64_t k = ws.kbeg + ws.wave; k < ws.kend; k += ws.nwaves) {
+  unsigned *wc = a.work ? a.work + ws.xcd : nullptr;
+  int64_t k = wc ? claim_take(ws, claim_issue(wc)) : ws.kbeg + ws.wave;
+  while (k < ws.kend) {
+    const unsigned pend = wc ? claim_issue(wc) : 0u;
     const int64_t it = a.sel ? (int64_t)a.sel[k] : k;
     const Item item = a.items[it];
     const bool act = lane < item.nrows;
@@ -484,6 +504,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep64(SweepArgs a, const Rec64 *
       walk(q0, q1, true);
     }
     if (MODE != kFill && lane == 0) a.counts[it] = em.cnt;
+    k = wc ? claim_take(ws, pend) : k + ws.nwaves;
   }
 }
 
@@ -530,7 +551,10 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
   unsigned long long nconf = 0;
   if (MODE == kFill && a.total_dev && *a.total_dev > a.capacity) return;
   const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split);
-  for (int64_t k = ws.kbeg + ws.wave; k < ws.kend; k += ws.nwaves) {
+  unsigned *wc = a.work ? a.work + ws.xcd : nullptr;
+  int64_t k = wc ? claim_take(ws, claim_issue(wc)) : ws.kbeg + ws.wave;
+  while (k < ws.kend) {
+    const unsigned pend = wc ? claim_issue(wc) : 0u;
     const int64_t it = a.sel ? (int64_t)a.sel[k] : k;
     const Item item = a.items[it];
     const bool act = lane < item.nrows;
@@ -668,6 +692,7 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
     }
     if (nq) drain();
     if (MODE != kFill && lane == 0) a.counts[it] = em.cnt;
+    k = wc ? claim_take(ws, pend) : k + ws.nwaves;
   }
   if (MODE != kFill && lane == 0 && nconf) atomicAdd(a.confirms, nconf);
 }

@@ -103,6 +103,9 @@ struct SweepArgs {
   const int64_t *nitems_dev = nullptr;
   const int64_t *total_dev = nullptr;
   int64_t capacity = 0;
+  // kXcds zeroed claim counters (one per XCD slice): waves claim items
+  // dynamically; null: a fixed stride
+  unsigned *work = nullptr;
 };
 
 // Pairs come out as (input id of the row, sorted q); launch_translate turns

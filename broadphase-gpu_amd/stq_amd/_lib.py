@@ -54,7 +54,8 @@ class Options(ctypes.Structure):
         ("xcd_split", ctypes.c_int32),
         ("train_scenes", ctypes.c_int32),
         ("graph", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("claim", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 

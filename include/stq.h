@@ -133,7 +133,10 @@ typedef struct stq_options {
                             of an unchanged batch is captured in a
                             hipGraph, later ones replay it), 1 = capture
                             from the first call, 2 = never          */
-  int32_t reserved[5];
+  int32_t claim;         /* sweep work split: 0 = auto (waves claim
+                            items from per-XCD counters), 1 = claim,
+                            2 = fixed stride                        */
+  int32_t reserved[4];
 } stq_options;
 
 typedef struct stq_stats {
