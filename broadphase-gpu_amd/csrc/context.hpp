@@ -1,0 +1,202 @@
+// context.hpp -- the library context (one per stq_create): device, stream,
+// grow-only scratch, learned-sizing state, batched-pipeline plans and graphs,
+// and the autotuning table.  Calls on one context are serialised by the
+// caller.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "batch.hpp"
+#include "sizer.hpp"
+#include "stq_internal.hpp"
+#include "sweep.hpp"
+
+namespace stqamd {
+
+// Learned sizing: Adam steps (minibatches of up to 8192 items) and learning
+// rate per training scene; slots per item = kCapScale * prediction + kCapSlack.
+constexpr int kTrainSteps = 64;
+constexpr float kTrainRate = 2e-3f;
+constexpr float kCapScale = 2.0f;
+constexpr float kCapSlack = 32.f;
+// Replay ring of training rows: each training scene adds a strided sample of
+// up to kReplayPerScene of its items.
+constexpr int64_t kReplay = 262144;
+constexpr int64_t kReplayPerScene = 8192;
+
+// Batched scenes: planning window when none is set.
+constexpr int kBatchWindow = 256;
+
+// One batch shape (scene offsets, window, chunk): its host and device tables
+// and the graph captured for it.
+struct BatchPlan {
+  std::vector<int64_t> offsets;
+  int W = 0, chunk = 0;
+  BatchTables t;
+  BatchDev dev;
+  int64_t item_bound = 0;  // no more items than this, whatever the boxes
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t graph_sig = 0;  // launch signature the graph was captured with
+  uint64_t eager_sig = 0;  // signature of the last eager run
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    graph_sig = 0;
+  }
+};
+
+// Grow-only pinned host buffer (results read back after a batched call).
+struct PinnedBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void *get(size_t need) {
+    if (need > bytes) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      bytes = 0;
+      if (hipHostMalloc(&p, need, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        throw StatusError(STQ_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
+      }
+      bytes = need;
+    }
+    return p;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// Device counters of one call: [0] error flags, [1] candidates, [2] lane
+// steps, [3] confirmations, [4] overflowed items; [8..12) and [12..16) the
+// item-claim counters (kXcds x u32) of the first and second sweep launch.
+constexpr int kCounters = 16;
+inline unsigned *claim_counters(unsigned long long *ctr, int launch) {
+  return reinterpret_cast<unsigned *>(ctr + 8 + 4 * launch);
+}
+
+enum Ev {
+  E_START, E_KEYS, E_SORT, E_GATHER, E_PLAN, E_PRED, E_COUNT, E_SCAN, E_FILL, E_COMPACT, E_END,
+  E_NUM
+};
+
+// Launch parameters the autotuner picks for one call shape class.
+struct TuneEntry {
+  int axis_policy, window, chunk, batch, waves_per_cu;
+  double ms;  // the winning trial's count-only time
+};
+// Shape class of a call: precision, batched or not, floor(log2 boxes),
+// floor(log2 scenes).
+uint64_t tune_key(int precision, bool batched, int64_t n, int nscenes);
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  stq_options opt{};
+  stq_stats stats{};
+  std::string last_error;
+  int cus = 256;
+  hipEvent_t ev[E_NUM] = {};
+
+  DevBuf keys, vals, smin, smax, geo, meta, full, hi, rowlist, groups, nchunks, ioff, items,
+      counts, offsets, scan_tmp, counters, axis_partial, h2d_boxes, d2h_pairs;
+  SortScratch sort;
+  // learned sizing: parameters, Adam state, per-item buffers
+  DevBuf sz_params, sz_m, sz_v, sz_grad, sz_partial, sz_loss, feat, cand, pred, caps, cap_off,
+      targets, bfeat, btargets, scratch_pairs, ovf, mae_acc, replay_feat, replay_t;
+  bool sz_ready = false;
+  int adam_t = 0;
+  int64_t replay_n = 0, replay_pos = 0, train_calls = 0;
+  int64_t scenes_seen = 0;
+  bool learned_pass = false;  // the last call used the single learned pass
+  // batched pipeline
+  DevBuf b_tables, b_skey, b_orig, b_gchunk, b_pairoff;
+  PinnedBuf b_host;
+  BatchPlan bplan;
+  int64_t b_budget = 0;      // scratch pairs of the learned batched pass
+  int64_t b_last_pairs = 0;  // pairs and items of the last batched call
+  int64_t b_last_items = 0;
+  // autotuning: launch parameters per call shape class (tune.cpp)
+  std::map<uint64_t, TuneEntry> tune_table;
+  double last_tune_ms = 0;
+
+  ~Context() {
+    bplan.drop_graph();
+    b_host.release();
+    DevBuf *bb[] = {&b_tables, &b_skey, &b_orig, &b_gchunk, &b_pairoff};
+    for (DevBuf *b : bb) b->release();
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    DevBuf *bufs[] = {&keys,  &vals,    &smin,    &smax,     &geo,          &meta,
+                      &full,  &hi,      &rowlist, &groups,   &nchunks,      &ioff,
+                      &items, &counts,  &offsets, &scan_tmp, &counters,     &axis_partial,
+                      &h2d_boxes, &d2h_pairs};
+    for (DevBuf *b : bufs) b->release();
+    sort.alt_keys.release();
+    sort.alt_vals.release();
+    sort.counts.release();
+    sort.hist.release();
+    sort.misc.release();
+    DevBuf *szb[] = {&sz_params, &sz_m, &sz_v, &sz_grad, &sz_partial, &sz_loss, &feat, &cand,
+                     &pred, &caps, &cap_off, &targets, &bfeat, &btargets, &scratch_pairs, &ovf,
+                     &mae_acc, &replay_feat, &replay_t};
+    for (DevBuf *b : szb) b->release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int pick_axis(const stq_aabb_f64 *boxes, int64_t n, bool need_q, QParams *qp);
+  void finish_stats(hipStream_t st);
+  void sizer_ensure();
+  void run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const float *d_feat,
+                   const uint32_t *d_cand, stq_pair *out, int64_t capacity, stq_pair **alloc_out,
+                   int64_t *count);
+  void sizer_train(const float *d_feat, const float *d_target, int64_t n, int steps, float lr);
+  void sizer_remember(const float *d_feat, const float *d_target, int64_t n, int64_t start = 0);
+  float sizer_gradient(const float *d_feat, const float *d_target, int64_t n);
+  void run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t capacity,
+           stq_pair **alloc_out, int64_t *count);
+  void batch_plan(const int64_t *off, int nscenes, int W, int chunk);
+  void run_batched(const stq_aabb_f64 *boxes, const int64_t *off, int nscenes, stq_pair *out,
+                   int64_t capacity, int64_t *pair_off);
+  // With opt.autotune: tunes the call's shape class on this input if the
+  // table has no entry yet (trial runs, results discarded), then copies the
+  // entry into opt.  The caller restores opt after the call.
+  void autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int64_t *off, int nscenes,
+                          bool batched);
+  int tune_save(const char *path) const;
+  int tune_load(const char *path);
+};
+
+// Restores a context's options when a call ends (the autotuner overrides
+// them for the duration of one call).
+struct OptionsScope {
+  Context *c;
+  stq_options saved;
+  explicit OptionsScope(Context *ctx) : c(ctx), saved(ctx->opt) {}
+  ~OptionsScope() { c->opt = saved; }
+};
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int d) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != d) STQ_HIP(hipSetDevice(d));
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+}  // namespace stqamd

@@ -1,0 +1,168 @@
+// tune.cpp -- the autotuner.  For each call shape class (precision, batched
+// or not, floor(log2 boxes), floor(log2 scenes)) it picks the sweep-axis
+// policy, the planning window, the item chunk, the scalar batch of the sweep
+// loop (how many q records are in flight per wave) and the resident waves per
+// CU, by timing count-only trial runs on the first input of that class:
+// coordinate descent, each candidate timed twice with HIP events, a change
+// kept only when it is at least 2% faster.  The table can be saved to and
+// loaded from a text file.
+//
+// Replaces the reference's setup() (src/util.cu:9-55), which derives one
+// block size and shared-memory size from device attributes.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "context.hpp"
+
+namespace stqamd {
+
+namespace {
+constexpr int64_t kTuneMinBoxes = 1 << 14;  // smaller calls are not worth tuning
+constexpr double kKeep = 0.98;              // a candidate must win by 2%
+
+int floor_log2(int64_t v) { return v > 0 ? 63 - __builtin_clzll((unsigned long long)v) : 0; }
+}  // namespace
+
+uint64_t tune_key(int precision, bool batched, int64_t n, int nscenes) {
+  return (uint64_t)(precision & 15) | ((uint64_t)(batched ? 1 : 0) << 4) |
+         ((uint64_t)floor_log2(n) << 8) | ((uint64_t)floor_log2(nscenes) << 16);
+}
+
+void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int64_t *off,
+                                 int nscenes, bool batched) {
+  last_tune_ms = 0;
+  if (!opt.autotune) return;
+  const uint64_t key = tune_key(opt.precision, batched, n, nscenes);
+  auto it = tune_table.find(key);
+  if (it == tune_table.end()) {
+    if (n < kTuneMinBoxes) return;
+    const auto t0 = std::chrono::steady_clock::now();
+    const stq_options user = opt;
+    const bool f32 = user.precision == STQ_PRECISION_F32_CONFIRM;
+    std::vector<int64_t> pair_off(batched ? (size_t)nscenes + 1 : 0);
+    auto trial = [&](const TuneEntry &e) {
+      opt = user;
+      opt.sizing = STQ_SIZING_COUNT_FILL;
+      opt.graph = 2;
+      opt.axis_policy = e.axis_policy;
+      opt.window = e.window;
+      opt.chunk = e.chunk;
+      opt.batch = e.batch;
+      opt.waves_per_cu = e.waves_per_cu;
+      double best = INFINITY;
+      for (int rep = 0; rep < 2; ++rep) {
+        try {  // capacity 0: the count pass runs, no pair is written
+          int64_t cnt = 0;
+          if (batched)
+            run_batched(boxes, off, nscenes, nullptr, 0, pair_off.data());
+          else
+            run(boxes, n, nullptr, 0, nullptr, &cnt);
+        } catch (const StatusError &err) {
+          if (err.status != STQ_ERR_CAPACITY) {
+            opt = user;
+            throw;
+          }
+        }
+        best = std::min(best, stats.ms_total);
+      }
+      return best;
+    };
+    TuneEntry best{STQ_AXIS_CHAIN, batched ? kBatchWindow : 0, 2048, f32 ? 8 : 2, 32, 0};
+    best.ms = trial(best);
+    auto consider = [&](TuneEntry e) {
+      if (std::memcmp(&e, &best, offsetof(TuneEntry, ms)) == 0) return;
+      e.ms = trial(e);
+      if (e.ms < kKeep * best.ms) best = e;
+    };
+    for (int a : {STQ_AXIS_VARIANCE, STQ_AXIS_X}) {
+      TuneEntry e = best;
+      e.axis_policy = a;
+      consider(e);
+    }
+    for (int w : {0, 256, 512, 1024, 2048}) {
+      if (batched && w == 0) continue;  // no automatic window in the batched form
+      TuneEntry e = best;
+      e.window = w;
+      consider(e);
+    }
+    for (int c : {1024, 4096}) {
+      TuneEntry e = best;
+      e.chunk = c;
+      consider(e);
+    }
+    for (int b : f32 ? std::initializer_list<int>{4, 8} : std::initializer_list<int>{2, 4}) {
+      TuneEntry e = best;
+      e.batch = b;
+      consider(e);
+    }
+    for (int w : {16, 64}) {
+      TuneEntry e = best;
+      e.waves_per_cu = w;
+      consider(e);
+    }
+    opt = user;
+    it = tune_table.emplace(key, best).first;
+    last_tune_ms = std::chrono::duration<double, std::milli>(
+                       std::chrono::steady_clock::now() - t0).count();
+  }
+  const TuneEntry &e = it->second;
+  opt.axis_policy = e.axis_policy;
+  opt.window = e.window;
+  opt.chunk = e.chunk;
+  opt.batch = e.batch;
+  opt.waves_per_cu = e.waves_per_cu;
+}
+
+// File format: a header line, then one line per entry:
+//   <key hex> <axis_policy> <window> <chunk> <batch> <waves_per_cu> <ms>
+int Context::tune_save(const char *path) const {
+  FILE *f = std::fopen(path, "w");
+  if (!f) return STQ_ERR_INVALID_ARGUMENT;
+  std::fprintf(f, "stq-autotune 1\n");
+  for (const auto &kv : tune_table) {
+    const TuneEntry &e = kv.second;
+    std::fprintf(f, "%llx %d %d %d %d %d %.6f\n", (unsigned long long)kv.first, e.axis_policy,
+                 e.window, e.chunk, e.batch, e.waves_per_cu, e.ms);
+  }
+  return std::fclose(f) == 0 ? STQ_OK : STQ_ERR_INVALID_ARGUMENT;
+}
+
+int Context::tune_load(const char *path) {
+  FILE *f = std::fopen(path, "r");
+  if (!f) return STQ_ERR_INVALID_ARGUMENT;
+  char head[64] = {0};
+  int version = 0;
+  if (std::fscanf(f, "%63s %d", head, &version) != 2 || std::strcmp(head, "stq-autotune") != 0 ||
+      version != 1) {
+    std::fclose(f);
+    return STQ_ERR_INVALID_ARGUMENT;
+  }
+  std::map<uint64_t, TuneEntry> table;
+  unsigned long long key;
+  TuneEntry e;
+  int r;
+  while ((r = std::fscanf(f, "%llx %d %d %d %d %d %lf", &key, &e.axis_policy, &e.window, &e.chunk,
+                          &e.batch, &e.waves_per_cu, &e.ms)) == 7) {
+    const bool ok = e.axis_policy >= 0 && e.axis_policy <= 4 && e.chunk > 0 &&
+                    (e.window == 0 || (e.window >= 64 && e.window <= 2048 &&
+                                       (e.window & (e.window - 1)) == 0)) &&
+                    (e.batch == 2 || e.batch == 4 || e.batch == 8) && e.waves_per_cu > 0 &&
+                    e.waves_per_cu <= 64;
+    if (!ok) {
+      std::fclose(f);
+      return STQ_ERR_INVALID_ARGUMENT;
+    }
+    table[key] = e;
+  }
+  const bool clean_end = r == EOF;
+  std::fclose(f);
+  if (!clean_end) return STQ_ERR_INVALID_ARGUMENT;
+  for (const auto &kv : table) tune_table[kv.first] = kv.second;
+  return STQ_OK;
+}
+
+}  // namespace stqamd

@@ -8,6 +8,7 @@ this module only moves numpy arrays across the C-ABI.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -75,6 +76,21 @@ class Context:
     def set_options(self, opts: Options | None = None, **kw):
         o = opts if opts is not None else options(**kw)
         self._check(self._L.stq_set_options(self.ptr, ctypes.byref(o)), "stq_set_options")
+
+    # -- autotuning table ------------------------------------------------------
+    def autotune_save(self, path):
+        self._check(self._L.stq_autotune_save(self.ptr, os.fsencode(path)), "stq_autotune_save")
+
+    def autotune_load(self, path):
+        self._check(self._L.stq_autotune_load(self.ptr, os.fsencode(path)), "stq_autotune_load")
+
+    def autotune_clear(self):
+        self._check(self._L.stq_autotune_clear(self.ptr), "stq_autotune_clear")
+
+    def autotune_entries(self) -> int:
+        n = ctypes.c_int32(0)
+        self._check(self._L.stq_autotune_entries(self.ptr, ctypes.byref(n)), "stq_autotune_entries")
+        return n.value
 
     def stats(self) -> dict:
         s = Stats()

@@ -31,7 +31,8 @@ SIZING_LEARNED = 1
 EXPORTS = [
     "stq_options_default", "stq_create", "stq_destroy", "stq_set_options",
     "stq_find_pairs_device", "stq_find_pairs_device_alloc", "stq_find_pairs_host",
-    "stq_find_pairs_batched_device", "stq_device_alloc", "stq_copy_to_device",
+    "stq_find_pairs_batched_device", "stq_autotune_save", "stq_autotune_load",
+    "stq_autotune_clear", "stq_autotune_entries", "stq_device_alloc", "stq_copy_to_device",
     "stq_copy_to_host", "stq_free_device", "stq_free_host",
     "stq_sizer_num_params", "stq_sizer_num_features", "stq_sizer_get_params",
     "stq_sizer_set_params", "stq_sizer_reset", "stq_sizer_forward", "stq_sizer_gradient",
@@ -85,7 +86,8 @@ class Stats(ctypes.Structure):
         ("ms_compact", ctypes.c_double),
         ("ms_predict", ctypes.c_double),
         ("sizing_mae", ctypes.c_double),
-        ("reserved", ctypes.c_double * 6),
+        ("ms_tune", ctypes.c_double),
+        ("reserved", ctypes.c_double * 5),
     ]
 
     def as_dict(self):
@@ -113,6 +115,10 @@ def lib():
         "stq_find_pairs_device_alloc": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
         "stq_find_pairs_host": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
         "stq_find_pairs_batched_device": (ctypes.c_int, [P, P, P, I32, P, I64, P]),
+        "stq_autotune_save": (ctypes.c_int, [P, ctypes.c_char_p]),
+        "stq_autotune_load": (ctypes.c_int, [P, ctypes.c_char_p]),
+        "stq_autotune_clear": (ctypes.c_int, [P]),
+        "stq_autotune_entries": (ctypes.c_int, [P, ctypes.POINTER(I32)]),
         "stq_device_alloc": (ctypes.c_int, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
         "stq_copy_to_device": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
         "stq_copy_to_host": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),

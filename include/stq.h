@@ -166,7 +166,9 @@ typedef struct stq_stats {
   double ms_compact;        /* output compaction                            */
   double ms_predict;        /* sizing network inference                     */
   double sizing_mae;        /* mean |predicted - exact| pairs per tile      */
-  double reserved[6];
+  double ms_tune;           /* host time the autotuner spent in this call
+                               (0 unless the call's shape class was new)   */
+  double reserved[5];
 } stq_stats;
 
 /* Fills *opt with the defaults: plain FP64, axis x, count-then-fill. */
@@ -213,6 +215,19 @@ int stq_find_pairs_batched_device(void *ctx, const stq_aabb_f64 *d_boxes,
                                   const int64_t *h_scene_offsets,
                                   int32_t nscenes, stq_pair *d_out,
                                   int64_t capacity, int64_t *h_pair_offsets);
+
+/* ---- autotuning (no counterpart in the reference, whose setup(),
+   src/util.cu:9-55, derives one block size from device attributes) ----
+   With options.autotune = 1, the first call of each shape class (precision,
+   batched or not, floor(log2 boxes), floor(log2 scenes)) with at least 16384
+   boxes runs count-only trials on its own input and picks the sweep-axis
+   policy, window, chunk, batch and waves_per_cu; those values then replace
+   the options' own for every call of that class.  The table belongs to the
+   context and can be saved to and loaded from a text file. */
+int stq_autotune_save(void *ctx, const char *path);
+int stq_autotune_load(void *ctx, const char *path);
+int stq_autotune_clear(void *ctx);
+int stq_autotune_entries(void *ctx, int32_t *count);
 
 /* Device memory plumbing for callers that do not link the HIP runtime
    themselves (foreign-function callers, the C++ drop-in of
