@@ -84,6 +84,24 @@ def pair_digest(pairs):
         return len(p), int(z.sum(dtype=np.uint64))
 
 
+def tuned_table(ctx):
+    """The context's autotuning table as a list of dicts (read back through
+    the library's own save format)."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "tune.txt")
+        ctx.autotune_save(path)
+        with open(path) as fh:
+            lines = fh.read().split("\n")[1:]
+    keys = ("key", "axis_policy", "window", "chunk", "batch", "waves_per_cu", "trial_ms")
+    out = []
+    for ln in lines:
+        if ln.strip():
+            v = ln.split()
+            out.append(dict(zip(keys, [v[0]] + [int(x) for x in v[1:6]] + [float(v[6])])))
+    return out
+
+
 def bench_batched(args, H):
     """512 scenes back to back (the batched stream), device-resident: the
     chosen batched form (reduced precision, chain-length axis, learned single
@@ -103,14 +121,15 @@ def bench_batched(args, H):
     del allb
     forms = {
         "chosen": dict(precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN,
-                       sizing=stq_amd.SIZING_LEARNED, train_scenes=200),
+                       sizing=stq_amd.SIZING_LEARNED, train_scenes=200, autotune=1),
         "plain": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X),
     }
     out, digests = {}, {}
     for name, kw in forms.items():
         ctx = stq_amd.Context(0, **kw)
-        rc, poffs = ctx.find_pairs_batched_device(db.ptr, offs, None, 0)  # sizes (and trains)
+        rc, poffs = ctx.find_pairs_batched_device(db.ptr, offs, None, 0)  # sizes, tunes, trains
         train = ctx.stats()
+        tuned = tuned_table(ctx) if kw.get("autotune") else None
         need = int(poffs[-1])
         dout = _lib.DeviceBuffer(8 * max(need, 1))
         for _ in range(max(args.warmup, 2)):  # eager, then capture
@@ -126,7 +145,8 @@ def bench_batched(args, H):
                          pairs=need, items=st["n_items"], candidates=st["n_candidates"],
                          graph_state=st["graph"], fallback_items=st["n_fallback_tiles"],
                          mae_pairs_per_item=st["sizing_mae"] if kw.get("sizing") else None,
-                         training_call_mae=train["sizing_mae"] if kw.get("sizing") else None)
+                         training_call_mae=train["sizing_mae"] if kw.get("sizing") else None,
+                         tune_ms=train["ms_tune"], autotuned=tuned)
         if not args.no_check:
             pairs = dout.to_array(np.int32, 2 * need).reshape(-1, 2)
             digests[name] = [pair_digest(pairs[poffs[s]:poffs[s + 1]]) for s in range(len(poffs) - 1)]
@@ -233,7 +253,8 @@ def main():
 
     forms = {
         "chosen": dict(precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN,
-                       sizing=stq_amd.SIZING_LEARNED, train_scenes=max(args.train_scenes, 1)),
+                       sizing=stq_amd.SIZING_LEARNED, train_scenes=max(args.train_scenes, 1),
+                       autotune=1),
         "plain": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X,
                       sizing=stq_amd.SIZING_COUNT_FILL),
     }
@@ -269,6 +290,7 @@ def main():
             if e.status != _lib.STQ_ERR_CAPACITY:
                 raise
         cap = ctx.stats()["n_pairs"]
+        tuned = tuned_table(ctx) if forms[name].get("autotune") else None
         d_out = stq_amd.DeviceBuffer(8 * max(cap, 1))
         for _ in range(args.warmup):
             ctx.find_pairs_device(d_boxes.ptr, n, d_out.ptr, cap)
@@ -287,7 +309,7 @@ def main():
         res[name] = dict(ms_per_step=ms, boxes_per_s=n / (ms * 1e-3), pairs_per_s=cnt / (ms * 1e-3),
                          pairs=cnt, stream_ms=float(np.median(totals)),
                          fill_sweep_ms=float(np.mean(sweeps)), stats=st, form=forms[name],
-                         fallback_items=int(np.max(fallback)) if fallback else 0)
+                         fallback_items=int(np.max(fallback)) if fallback else 0, autotuned=tuned)
         if not args.no_check:
             outputs[name] = d_out.to_array(np.int32, 2 * cnt).reshape(-1, 2)
         d_out.close()
