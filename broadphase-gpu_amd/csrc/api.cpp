@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "context.hpp"
+#include "plan.hpp"
 
 namespace stqamd {
 
@@ -27,7 +28,8 @@ static bool options_valid(const stq_options &o) {
          (o.window == 0 || (o.window >= 64 && o.window <= 2048 && (o.window & (o.window - 1)) == 0)) &&
          o.waves_per_cu >= 0 && o.waves_per_cu <= 64 &&
          (o.batch == 0 || o.batch == 2 || o.batch == 4 || o.batch == 8) && o.xcd_split >= 0 &&
-         o.xcd_split <= 2 && o.graph >= 0 && o.graph <= 2 && o.claim >= 0 && o.claim <= 2;
+         o.xcd_split <= 2 && o.graph >= 0 && o.graph <= 2 && o.claim >= 0 && o.claim <= 2 &&
+         o.tails >= 0 && o.tails <= 2;
 }
 
 // Axis choice and, for the reduced-precision form, the quantisation range:
@@ -154,16 +156,19 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   const int64_t ngroups = (n + 63) / 64;
   int *d_rowlist = rowlist.as<int>((size_t)n);
   int4 *d_groups = groups.as<int4>((size_t)ngroups);
+  int4 *d_tg = tgroups.as<int4>((size_t)ngroups);
+  int2 *d_gch = gchunks.as<int2>((size_t)ngroups);
   uint32_t *d_nch = nchunks.as<uint32_t>((size_t)ngroups);
   int64_t *d_ioff = ioff.as<int64_t>((size_t)ngroups + 1);
-  launch_plan(d_hi, n, W, chunk, d_rowlist, d_groups, d_nch, &ctr[2], st);
+  launch_plan(d_hi, n, W, chunk, opt.tails != 2, d_rowlist, d_groups, d_tg, d_gch, d_nch, &ctr[2],
+              st);
   scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
   int64_t nitems = 0;
   STQ_HIP(hipMemcpyAsync(&nitems, d_ioff + ngroups, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   STQ_HIP(hipStreamSynchronize(st));
   stats.n_items = nitems;
   Item *d_items = items.as<Item>((size_t)std::max<int64_t>(nitems, 1));
-  launch_items(d_groups, d_ioff, ngroups, chunk, d_items, st);
+  launch_items(d_groups, d_tg, d_gch, d_ioff, ngroups, d_items, st);
   STQ_HIP(hipEventRecord(ev[E_PLAN], st));
 
   const int wpc = opt.waves_per_cu > 0 ? opt.waves_per_cu : 32;
@@ -458,7 +463,8 @@ void Context::batch_plan(const int64_t *off, int nscenes, int W, int chunk) {
     const int64_t m = off[s + 1] - off[s];
     if (m < 2) continue;
     const int64_t g = t.scene_first_group[s + 1] - t.scene_first_group[s];
-    bound += g * std::min<int64_t>(kMaxChunks, (m - 1 + chunk - 1) / chunk);
+    // body and tail: each at most kCapChunks items of at least `chunk` positions
+    bound += 2 * g * std::min<int64_t>(kCapChunks, (m - 1 + chunk - 1) / chunk);
   }
   bp.item_bound = bound;
 
@@ -575,7 +581,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   int *d_hi = hi.as<int>((size_t)n);
   int *d_rowlist = rowlist.as<int>((size_t)n);
   int4 *d_groups = groups.as<int4>((size_t)ngroups);
-  int *d_gchunk = b_gchunk.as<int>((size_t)ngroups);
+  int2 *d_gchunk = gchunks.as<int2>((size_t)ngroups);
+  int4 *d_tg = tgroups.as<int4>((size_t)ngroups);
   uint32_t *d_nch = nchunks.as<uint32_t>((size_t)ngroups);
   int64_t *d_ioff = ioff.as<int64_t>((size_t)ngroups + 1);
   Item *d_items = items.as<Item>((size_t)bound);
@@ -620,9 +627,10 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     }
     launch_gather_seg(f32, boxes, d_vals, t, bp.dev, d_smin, d_smax, d_rec, d_qrec, d_orig, st);
     launch_chain_end_seg(f32, d_smin, d_smax, t, bp.dev, d_hi, &ctr[1], st);
-    launch_plan_seg(d_hi, t, bp.dev, W, chunk, d_rowlist, d_groups, d_gchunk, d_nch, &ctr[2], st);
+    launch_plan_seg(d_hi, t, bp.dev, W, chunk, opt.tails != 2, d_rowlist, d_groups, d_tg, d_gchunk,
+                    d_nch, &ctr[2], st);
     scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
-    launch_items_seg(d_groups, d_gchunk, d_ioff, ngroups, d_items, st);
+    launch_items(d_groups, d_tg, d_gchunk, d_ioff, ngroups, d_items, st);
     launch_fill_u32(d_counts, 0u, bound, st);
     SweepArgs sa;
     sa.items = d_items;

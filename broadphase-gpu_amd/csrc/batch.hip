@@ -4,6 +4,7 @@
 #include <cmath>
 
 #include "batch.hpp"
+#include "plan.hpp"
 
 namespace stqamd {
 
@@ -228,9 +229,10 @@ __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
                                                  const int64_t *__restrict__ win_start,
                                                  const int *__restrict__ win_cnt,
                                                  const int64_t *__restrict__ win_gbase, int W,
-                                                 int chunk, int *__restrict__ rowlist,
+                                                 int chunk, int tails, int *__restrict__ rowlist,
                                                  int4 *__restrict__ groups,
-                                                 int *__restrict__ gchunk,
+                                                 int4 *__restrict__ tgroups,
+                                                 int2 *__restrict__ gchunk,
                                                  uint32_t *__restrict__ nchunks,
                                                  unsigned long long *__restrict__ lane_steps) {
   __shared__ uint64_t key[kMaxWin];
@@ -267,46 +269,22 @@ __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
   for (int k = w; k < ng; k += kT / 64) {
     const int j = 64 * k + lane;
     const bool act = j < cnt;
-    int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
-    int h = act ? (int)(key[j] >> 32) : INT_MIN;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      p = min(p, __shfl_xor(p, o, kWave));
-      h = max(h, __shfl_xor(h, o, kWave));
-    }
+    const int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
+    const int h = act ? (int)(key[j] >> 32) : INT_MIN;
     const int nrows = min(64, cnt - 64 * k);
-    const int q0 = p + 1, q1 = h;
+    // capped: the item buffer is sized on the host from the scene sizes
+    const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, true,
+                                    tails != 0);
     const int64_t g = gbase + k;
     if (lane == 0) {
-      groups[g] = make_int4((int)(wbase + 64 * k), nrows, q0, q1);
-      const int64_t span = (int64_t)q1 - q0 + 1;
-      int ch = chunk;
-      if (span > 0 && (span + ch - 1) / ch > kMaxChunks) ch = (int)((span + kMaxChunks - 1) / kMaxChunks);
-      gchunk[g] = ch;
-      nchunks[g] = span > 0 ? (uint32_t)((span + ch - 1) / ch) : 0u;
-      if (span > 0) steps += (unsigned long long)span * 64ull;
+      groups[g] = gp.body;
+      tgroups[g] = gp.tail;
+      gchunk[g] = gp.chunk;
+      nchunks[g] = gp.nitems;
+      steps += gp.steps;
     }
   }
   if (lane == 0 && steps) atomicAdd(lane_steps, steps);
-}
-
-__global__ __launch_bounds__(kT) void k_items_seg(const int4 *__restrict__ groups,
-                                                  const int *__restrict__ gchunk,
-                                                  const int64_t *__restrict__ ioff,
-                                                  int64_t ngroups, Item *__restrict__ items) {
-  const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (g >= ngroups) return;
-  const int4 gr = groups[g];
-  const int ch = gchunk[g];
-  const int64_t o = ioff[g], e = ioff[g + 1];
-  for (int64_t c = 0; c < e - o; ++c) {
-    Item it;
-    it.rowoff = gr.x;
-    it.nrows = gr.y;
-    it.q0 = gr.z + (int)(c * ch);
-    it.q1 = min(it.q0 + ch - 1, gr.w);
-    items[o + c] = it;
-  }
 }
 
 __global__ __launch_bounds__(kT) void k_scene_pairs(const int64_t *__restrict__ off,
@@ -433,20 +411,12 @@ void launch_chain_end_seg(bool f32, const void *smin, const void *smax, const Ba
 }
 
 void launch_plan_seg(const int *hi, const BatchTables &t, const BatchDev &d, int W, int chunk,
-                     int *rowlist, int4 *groups, int *gchunk, uint32_t *nchunks,
-                     unsigned long long *lane_steps, hipStream_t st) {
+                     bool tails, int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk,
+                     uint32_t *nchunks, unsigned long long *lane_steps, hipStream_t st) {
   const int nw = (int)t.win_start.size();
   if (nw <= 0) return;
   hipLaunchKernelGGL(k_plan_seg, dim3(nw), dim3(kT), 0, st, hi, d.win_start, d.win_cnt, d.win_gbase,
-                     W, chunk, rowlist, groups, gchunk, nchunks, lane_steps);
-  STQ_HIP(hipGetLastError());
-}
-
-void launch_items_seg(const int4 *groups, const int *gchunk, const int64_t *ioff, int64_t ngroups,
-                      Item *items, hipStream_t st) {
-  if (ngroups <= 0) return;
-  hipLaunchKernelGGL(k_items_seg, dim3(nblk(ngroups)), dim3(kT), 0, st, groups, gchunk, ioff,
-                     ngroups, items);
+                     W, chunk, tails ? 1 : 0, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
   STQ_HIP(hipGetLastError());
 }
 

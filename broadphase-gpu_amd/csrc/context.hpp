@@ -110,7 +110,7 @@ struct Context {
   hipEvent_t ev[E_NUM] = {};
 
   DevBuf keys, vals, smin, smax, geo, meta, full, hi, rowlist, groups, nchunks, ioff, items,
-      counts, offsets, scan_tmp, counters, axis_partial, h2d_boxes, d2h_pairs;
+      counts, offsets, scan_tmp, counters, axis_partial, h2d_boxes, d2h_pairs, tgroups, gchunks;
   SortScratch sort;
   // learned sizing: parameters, Adam state, per-item buffers
   DevBuf sz_params, sz_m, sz_v, sz_grad, sz_partial, sz_loss, feat, cand, pred, caps, cap_off,
@@ -121,7 +121,7 @@ struct Context {
   int64_t scenes_seen = 0;
   bool learned_pass = false;  // the last call used the single learned pass
   // batched pipeline
-  DevBuf b_tables, b_skey, b_orig, b_gchunk, b_pairoff;
+  DevBuf b_tables, b_skey, b_orig, b_pairoff;
   PinnedBuf b_host;
   BatchPlan bplan;
   int64_t b_budget = 0;      // scratch pairs of the learned batched pass
@@ -134,7 +134,7 @@ struct Context {
   ~Context() {
     bplan.drop_graph();
     b_host.release();
-    DevBuf *bb[] = {&b_tables, &b_skey, &b_orig, &b_gchunk, &b_pairoff};
+    DevBuf *bb[] = {&b_tables, &b_skey, &b_orig, &b_pairoff, &tgroups, &gchunks};
     for (DevBuf *b : bb) b->release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);

@@ -1,0 +1,124 @@
+// plan.hpp -- planning of one 64-row group (device code, shared by the
+// one-scene and the batched planners).
+//
+// The rows of a group are sorted by chain end.  Swept as one body, the group
+// costs 64 lane steps per position of [min p + 1, max hi]; when a few rows
+// have much longer chains than the rest (heavy-tailed extents), most lanes
+// idle through their tails.  So the group is cut after the row k that
+// minimises
+//     (hi_k - q0 + 1)                                  body: 64 lanes per step
+//   + kTailCost * sum_{j > k} (hi_j - hi_k) / 64       tails: one row per lane-step
+// The body is swept as before over [q0, hi_k]; the rows above k become a tail
+// item over (hi_k, max hi], swept with positions across the lanes and the
+// rows taken in turn.  Without long rows the cut is the last row (no tail).
+#pragma once
+
+#include <climits>
+
+#include "stq_internal.hpp"
+
+namespace stqamd {
+
+constexpr int kTailFlag = 1 << 30;  // in Item::nrows: a tail item
+constexpr int kRowsMask = 127;      // Item::nrows & kRowsMask = rows of the item
+constexpr double kTailCost = 2.0;   // a tail (row, q) test against a body lane step
+constexpr int kCapChunks = 64;      // capped planning: at most this many items per body or tail
+
+// Group record: body {rowoff, nrows, q0, q1}, tail {rowoff, rows, q0, q1}
+// (rows = 0: none), chunk sizes {body, tail}.
+struct GroupPlan {
+  int4 body, tail;
+  int2 chunk;
+  uint32_t nitems;
+  unsigned long long steps;  // lane steps of the body + (row, q) tests of the tail
+};
+
+__device__ inline int64_t chunk_count(int64_t span, int ch) {
+  return span > 0 ? (span + ch - 1) / ch : 0;
+}
+
+// All 64 lanes call this; lane j < nrows holds row j of the group (position p,
+// chain end h, non-decreasing in j).  The result is the same in every lane.
+__device__ inline GroupPlan plan_group(int lane, int nrows, int p, int h, int rowoff, int chunk,
+                                       bool cap, bool tails) {
+  const bool act = lane < nrows;
+  int pm = act ? p : INT_MAX, hm = act ? h : INT_MIN;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    pm = min(pm, __shfl_xor(pm, o, kWave));
+    hm = max(hm, __shfl_xor(hm, o, kWave));
+  }
+  const int q0 = pm + 1, hmax = hm;
+  int kc = nrows - 1;  // last row of the body
+  if (tails) {
+    // inclusive prefix sum of the chain ends
+    long long pre = act ? (long long)h : 0;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const long long y = __shfl_up(pre, o, kWave);
+      if (lane >= o) pre += y;
+    }
+    const long long tot = __shfl(pre, kWave - 1, kWave);
+    double cost = INFINITY;
+    if (act) {
+      const double body = (double)max((long long)h - q0 + 1, 0ll);
+      const double tail = (double)(tot - pre) - (double)(nrows - 1 - lane) * (double)h;
+      cost = body + kTailCost * tail * (1.0 / 64.0);
+    }
+    double cmin = cost;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cmin = fmin(cmin, __shfl_xor(cmin, o, kWave));
+    const uint64_t at = __builtin_amdgcn_ballot_w64(act && cost == cmin);
+    kc = 63 - __builtin_clzll(at);  // the largest such row: fewest tail rows
+  }
+  const int hk = __shfl(h, kc, kWave);
+  const int R = nrows - 1 - kc;
+  GroupPlan g;
+  const int q1 = R > 0 ? hk : hmax;
+  g.body = make_int4(rowoff, nrows, q0, q1);
+  const int64_t sb = (int64_t)q1 - q0 + 1;
+  int cb = chunk;
+  if (cap && chunk_count(sb, cb) > kCapChunks) cb = (int)((sb + kCapChunks - 1) / kCapChunks);
+  int64_t nb = chunk_count(sb, cb);
+  unsigned long long steps = sb > 0 ? (unsigned long long)sb * 64ull : 0ull;
+  int ct = 64;
+  int64_t nt = 0;
+  if (R > 0) {
+    g.tail = make_int4(rowoff + kc + 1, R, hk + 1, hmax);
+    const int64_t st = (int64_t)hmax - hk;
+    // a tail item tests about as many (row, q) pairs as a body item of `chunk`
+    ct = (int)min<int64_t>(((int64_t)chunk * 64 / R + 63) / 64 * 64, INT_MAX / 2);
+    if (cap && chunk_count(st, ct) > kCapChunks) ct = (int)((st + kCapChunks - 1) / kCapChunks);
+    nt = chunk_count(st, ct);
+    steps += (unsigned long long)R * (unsigned long long)st;
+  } else {
+    g.tail = make_int4(0, 0, 0, -1);
+  }
+  g.chunk = make_int2(cb, ct);
+  g.nitems = (uint32_t)(nb + nt);
+  g.steps = steps;
+  return g;
+}
+
+// Items of one group at items[o .. o + n): the body's chunks, then the tail's.
+__device__ inline void write_group_items(const int4 body, const int4 tail, const int2 ch,
+                                         int64_t o, int64_t n, Item *__restrict__ items) {
+  const int64_t nb = chunk_count((int64_t)body.w - body.z + 1, ch.x);
+  for (int64_t c = 0; c < n; ++c) {
+    Item it;
+    if (c < nb) {
+      it.rowoff = body.x;
+      it.nrows = body.y;
+      it.q0 = body.z + (int)(c * ch.x);
+      it.q1 = (int)min<int64_t>((int64_t)it.q0 + ch.x - 1, body.w);
+    } else {
+      it.rowoff = tail.x;
+      it.nrows = tail.y | kTailFlag;
+      it.q0 = tail.z + (int)((c - nb) * ch.y);
+      it.q1 = (int)min<int64_t>((int64_t)it.q0 + ch.y - 1, tail.w);
+    }
+    items[o + c] = it;
+  }
+}
+
+}  // namespace stqamd

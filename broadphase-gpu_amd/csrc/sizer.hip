@@ -4,6 +4,7 @@
 #include <climits>
 #include <cmath>
 
+#include "plan.hpp"
 #include "sizer.hpp"
 
 namespace stqamd {
@@ -51,7 +52,8 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
     }
     const double n1 = qp.scale[a1] / 32767.0, n2 = qp.scale[a2] / 32767.0;
     const double o1 = qp.origin[a1], o2 = qp.origin[a2];
-    const bool act = lane < item.nrows;
+    const int nrows = item.nrows & kRowsMask;  // tail items: their rows
+    const bool act = lane < nrows;
     const int q0 = item.q0, q1 = item.q1;
     float fr = act ? 1.f : 0.f;
     float le1 = 0, le2 = 0, c1 = 0, c2 = 0, tv = 0, te = 0, tf = 0, chain = 0;

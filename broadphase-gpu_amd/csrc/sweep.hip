@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <climits>
 
+#include "plan.hpp"
 #include "stq_internal.hpp"
 #include "sweep.hpp"
 
@@ -226,13 +227,17 @@ __global__ __launch_bounds__(kThreads) void k_chain_end(const T *__restrict__ sm
 // ---- work planning ----------------------------------------------------------------
 // One block per window of W sorted rows (W a power of two, 64..2048): rows are
 // ordered by (hi, position) with a bitonic sort in LDS and cut into groups of
-// 64.  Group g covers rowlist[64g .. 64g+nrows) and the q range
-// [min p + 1, max hi].  nchunks[g] = number of chunk-sized items.
+// 64.  Group g covers rowlist[64g .. 64g+nrows); plan_group splits it into a
+// body over [min p + 1, cut] and, for rows with much longer chains, a tail.
+// nchunks[g] = number of items of the group.
 constexpr int kMaxWindow = 2048;
 
 __global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi, int64_t n, int W,
-                                                   int chunk, int *__restrict__ rowlist,
+                                                   int chunk, int tails,
+                                                   int *__restrict__ rowlist,
                                                    int4 *__restrict__ groups,
+                                                   int4 *__restrict__ tgroups,
+                                                   int2 *__restrict__ gchunk,
                                                    uint32_t *__restrict__ nchunks,
                                                    unsigned long long *__restrict__ lane_steps) {
   __shared__ uint64_t key[kMaxWindow];
@@ -270,41 +275,32 @@ __global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi, i
   for (int k = w; k < ng; k += kThreads / 64) {
     const int j = 64 * k + lane;
     const bool act = j < cnt;
-    int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
-    int h = act ? (int)(key[j] >> 32) : INT_MIN;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      p = min(p, __shfl_xor(p, o, kWave));
-      h = max(h, __shfl_xor(h, o, kWave));
-    }
+    const int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
+    const int h = act ? (int)(key[j] >> 32) : INT_MIN;
     const int nrows = min(64, cnt - 64 * k);
-    const int q0 = p + 1, q1 = h;
+    const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, false,
+                                    tails != 0);
     const int64_t g = (wbase >> 6) + k;
     if (lane == 0) {
-      groups[g] = make_int4((int)(wbase + 64 * k), nrows, q0, q1);
-      const int64_t span = (int64_t)q1 - q0 + 1;
-      nchunks[g] = span > 0 ? (uint32_t)((span + chunk - 1) / chunk) : 0u;
-      if (span > 0) steps += (unsigned long long)span * 64ull;
+      groups[g] = gp.body;
+      tgroups[g] = gp.tail;
+      gchunk[g] = gp.chunk;
+      nchunks[g] = gp.nitems;
+      steps += gp.steps;
     }
   }
   if (lane == 0 && steps) atomicAdd(lane_steps, steps);
 }
 
+// Items of every group: the body's chunks, then the tail's (items[ioff[g]..]).
 __global__ __launch_bounds__(kThreads) void k_items(const int4 *__restrict__ groups,
+                                                    const int4 *__restrict__ tgroups,
+                                                    const int2 *__restrict__ gchunk,
                                                     const int64_t *__restrict__ ioff, int64_t ngroups,
-                                                    int chunk, Item *__restrict__ items) {
+                                                    Item *__restrict__ items) {
   const int64_t g = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (g >= ngroups) return;
-  const int4 gr = groups[g];
-  const int64_t o = ioff[g], e = ioff[g + 1];
-  for (int64_t c = 0; c < e - o; ++c) {
-    Item it;
-    it.rowoff = gr.x;
-    it.nrows = gr.y;
-    it.q0 = gr.z + (int)(c * chunk);
-    it.q1 = min(it.q0 + chunk - 1, gr.w);
-    items[o + c] = it;
-  }
+  write_group_items(groups[g], tgroups[g], gchunk[g], ioff[g], ioff[g + 1] - ioff[g], items);
 }
 
 // ---- the sweep ----------------------------------------------------------------------
@@ -401,6 +397,65 @@ __device__ __forceinline__ uint64_t geo_mask(const R &m, const R &q) {
          ballot(m.lo2 <= q.hi2);
 }
 
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// ---- tail items ----
+// A tail item holds up to 64 rows (the longest chains of a group, in order of
+// chain end) and a position range [q0, q1].  Positions go across the lanes,
+// 64 at a time; each row in turn is tested against them, its record broadcast
+// from the lane that holds it.  A row whose chain ends before the current 64
+// positions is skipped from then on.  Emission order: positions block, row,
+// lane.
+template <int MODE>
+__device__ __forceinline__ void tail64(const SweepArgs &a, const Rec64 *__restrict__ rec,
+                                       int2 *__restrict__ out, const Item &item, int lane,
+                                       Emit<MODE> &em) {
+  const int R = item.nrows & kRowsMask;
+  int rp = INT_MAX, rh = INT_MIN;
+  Rec64 rr;
+  rr.lo1 = rr.lo2 = INFINITY;
+  rr.hi1 = rr.hi2 = -INFINITY;
+  rr.v0 = rr.v1 = rr.v2 = rr.type = 0;
+  uint32_t rop = 0;
+  if (lane < R) {
+    rp = a.rowlist[item.rowoff + lane];
+    rh = a.hi[rp];
+    rr = rec[rp];
+    if (MODE != kCount) rop = a.orig[rp];
+  }
+  int r0 = 0;
+  for (int qb = item.q0; qb <= item.q1; qb += kWave) {
+    while (r0 < R && readlane_i(rh, r0) < qb) ++r0;
+    const int q = qb + lane;
+    const bool inq = q <= item.q1;
+    Rec64 mq;
+    mq.lo1 = mq.lo2 = INFINITY;
+    mq.hi1 = mq.hi2 = -INFINITY;
+    mq.v0 = mq.v1 = mq.v2 = mq.type = 0;
+    if (inq) mq = rec[q];
+    for (int r = r0; r < R; ++r) {
+      const int p = readlane_i(rp, r), hp = readlane_i(rh, r);
+      const double lo1 = readlane_f64(rr.lo1, r), hi1 = readlane_f64(rr.hi1, r);
+      const double lo2 = readlane_f64(rr.lo2, r), hi2 = readlane_f64(rr.hi2, r);
+      const int v0 = readlane_i(rr.v0, r), v1 = readlane_i(rr.v1, r), v2 = readlane_i(rr.v2, r);
+      const int acc = accept_bits(readlane_i(rr.type, r));
+      const bool h = inq & (q > p) & (q <= hp) & (mq.lo1 <= hi1) & (mq.hi1 >= lo1) &
+                     (mq.lo2 <= hi2) & (mq.hi2 >= lo2) & ((acc & mq.type) != 0) &
+                     (v0 != mq.v0) & (v0 != mq.v1) & (v0 != mq.v2) & (v1 != mq.v0) &
+                     (v1 != mq.v1) & (v1 != mq.v2) & (v2 != mq.v0) & (v2 != mq.v1) &
+                     (v2 != mq.v2);
+      const uint64_t hm = ballot(h);
+      if (hm) em.put(h, hm, (uint32_t)readlane_i((int)rop, r), q, out);
+    }
+  }
+}
+
 // The record and output pointers are separate restrict-qualified arguments:
 // that is what lets the compiler keep the uniform record loads on the scalar
 // path next to the pair stores.
@@ -417,91 +472,95 @@ __global__ __launch_bounds__(kThreads) void k_sweep64(SweepArgs a, const Rec64 *
     const unsigned pend = wc ? claim_issue(wc) : 0u;
     const int64_t it = a.sel ? (int64_t)a.sel[k] : k;
     const Item item = a.items[it];
-    const bool act = lane < item.nrows;
-    const int p = act ? a.rowlist[item.rowoff + lane] : INT_MAX;
-    const int myhi = act ? a.hi[p] : INT_MIN;
-    Rec64 mine;
-    mine.lo1 = mine.lo2 = INFINITY;  // empty box for idle lanes
-    mine.hi1 = mine.hi2 = -INFINITY;
-    mine.v0 = mine.v1 = mine.v2 = mine.type = 0;
-    uint32_t op = 0;
-    if (act) {
-      mine = rec[p];
-      if (MODE != kCount) op = a.orig[p];
-    }
-    const int acc = act ? accept_bits(mine.type) : 0;
     Emit<MODE> em;
     if (MODE != kCount) em.base0 = em.base = a.offsets[it];
     if (MODE == kLearned) em.cap = a.caps[it];
-
-    // rows in m passed the geometric test against sorted q (record rq); idle
-    // lanes can pass it only against unbounded boxes and never the type test
-    auto hit = [&](int q, const Rec64 &rq, uint64_t m) __attribute__((always_inline)) {
-      const bool h = (((m >> lane) & 1ull) != 0) & ((acc & rq.type) != 0) &
-                     (mine.v0 != rq.v0) & (mine.v0 != rq.v1) & (mine.v0 != rq.v2) &
-                     (mine.v1 != rq.v0) & (mine.v1 != rq.v1) & (mine.v1 != rq.v2) &
-                     (mine.v2 != rq.v0) & (mine.v2 != rq.v1) & (mine.v2 != rq.v2);
-      const uint64_t hm = ballot(h);
-      if (hm) em.put(h, hm, op, q, out);
-    };
-    auto batch = [&](const Rec64 *b, int q, bool check) __attribute__((always_inline)) {
-      uint64_t m[B];
-      uint64_t any = 0;
-#pragma unroll
-      for (int j = 0; j < B; ++j) {
-        m[j] = geo_mask(mine, b[j]);
-        if (check) m[j] &= ballot(q + j > p) & ballot(q + j <= myhi);
-        any |= m[j];
-      }
-      if (any) {
-#pragma unroll
-        for (int j = 0; j < B; ++j)
-          if (m[j]) hit(q + j, b[j], m[j]);
-      }
-    };
-    auto walk = [&](int qa, int qb, bool check) __attribute__((always_inline)) {
-      int q = qa;
-      int nb = (qb - qa + 1) / B;
-      if (nb > 0) {
-        Rec64 b0[B], b1[B];
-#pragma unroll
-        for (int j = 0; j < B; ++j) b0[j] = rec[q + j];
-        while (true) {
-          q += zero * b0[0].v0;  // waits for b0 on every path
-          if (nb > 1) {
-#pragma unroll
-            for (int j = 0; j < B; ++j) b1[j] = rec[q + B + j];
-          }
-          batch(b0, q, check);
-          q += B;
-          if (--nb == 0) break;
-          q += zero * b1[0].v0;
-          if (nb > 1) {
-#pragma unroll
-            for (int j = 0; j < B; ++j) b0[j] = rec[q + B + j];
-          }
-          batch(b1, q, check);
-          q += B;
-          if (--nb == 0) break;
-        }
-      }
-      for (; q <= qb; ++q) {
-        const Rec64 r = rec[q];
-        uint64_t m = geo_mask(mine, r);
-        if (check) m &= ballot(q > p) & ballot(q <= myhi);
-        if (m) hit(q, r, m);
-      }
-    };
-    int pmax, hmin;
-    row_span(act, p, myhi, pmax, hmin);
-    const int q0 = item.q0, q1 = item.q1;
-    const int b0 = max(q0, pmax + 1), b1 = min(q1, hmin);
-    if (b0 <= b1) {
-      if (q0 < b0) walk(q0, b0 - 1, true);
-      walk(b0, b1, false);
-      if (b1 < q1) walk(b1 + 1, q1, true);
+    if (item.nrows & kTailFlag) {
+      tail64<MODE>(a, rec, out, item, lane, em);
     } else {
-      walk(q0, q1, true);
+      const bool act = lane < item.nrows;
+      const int p = act ? a.rowlist[item.rowoff + lane] : INT_MAX;
+      const int myhi = act ? a.hi[p] : INT_MIN;
+      Rec64 mine;
+      mine.lo1 = mine.lo2 = INFINITY;  // empty box for idle lanes
+      mine.hi1 = mine.hi2 = -INFINITY;
+      mine.v0 = mine.v1 = mine.v2 = mine.type = 0;
+      uint32_t op = 0;
+      if (act) {
+        mine = rec[p];
+        if (MODE != kCount) op = a.orig[p];
+      }
+      const int acc = act ? accept_bits(mine.type) : 0;
+
+      // rows in m passed the geometric test against sorted q (record rq); idle
+      // lanes can pass it only against unbounded boxes and never the type test
+      auto hit = [&](int q, const Rec64 &rq, uint64_t m) __attribute__((always_inline)) {
+        const bool h = (((m >> lane) & 1ull) != 0) & ((acc & rq.type) != 0) &
+                       (mine.v0 != rq.v0) & (mine.v0 != rq.v1) & (mine.v0 != rq.v2) &
+                       (mine.v1 != rq.v0) & (mine.v1 != rq.v1) & (mine.v1 != rq.v2) &
+                       (mine.v2 != rq.v0) & (mine.v2 != rq.v1) & (mine.v2 != rq.v2);
+        const uint64_t hm = ballot(h);
+        if (hm) em.put(h, hm, op, q, out);
+      };
+      auto batch = [&](const Rec64 *b, int q, bool check) __attribute__((always_inline)) {
+        uint64_t m[B];
+        uint64_t any = 0;
+  #pragma unroll
+        for (int j = 0; j < B; ++j) {
+          m[j] = geo_mask(mine, b[j]);
+          if (check) m[j] &= ballot(q + j > p) & ballot(q + j <= myhi);
+          any |= m[j];
+        }
+        if (any) {
+  #pragma unroll
+          for (int j = 0; j < B; ++j)
+            if (m[j]) hit(q + j, b[j], m[j]);
+        }
+      };
+      auto walk = [&](int qa, int qb, bool check) __attribute__((always_inline)) {
+        int q = qa;
+        int nb = (qb - qa + 1) / B;
+        if (nb > 0) {
+          Rec64 b0[B], b1[B];
+  #pragma unroll
+          for (int j = 0; j < B; ++j) b0[j] = rec[q + j];
+          while (true) {
+            q += zero * b0[0].v0;  // waits for b0 on every path
+            if (nb > 1) {
+  #pragma unroll
+              for (int j = 0; j < B; ++j) b1[j] = rec[q + B + j];
+            }
+            batch(b0, q, check);
+            q += B;
+            if (--nb == 0) break;
+            q += zero * b1[0].v0;
+            if (nb > 1) {
+  #pragma unroll
+              for (int j = 0; j < B; ++j) b0[j] = rec[q + B + j];
+            }
+            batch(b1, q, check);
+            q += B;
+            if (--nb == 0) break;
+          }
+        }
+        for (; q <= qb; ++q) {
+          const Rec64 r = rec[q];
+          uint64_t m = geo_mask(mine, r);
+          if (check) m &= ballot(q > p) & ballot(q <= myhi);
+          if (m) hit(q, r, m);
+        }
+      };
+      int pmax, hmin;
+      row_span(act, p, myhi, pmax, hmin);
+      const int q0 = item.q0, q1 = item.q1;
+      const int b0 = max(q0, pmax + 1), b1 = min(q1, hmin);
+      if (b0 <= b1) {
+        if (q0 < b0) walk(q0, b0 - 1, true);
+        walk(b0, b1, false);
+        if (b1 < q1) walk(b1 + 1, q1, true);
+      } else {
+        walk(q0, q1, true);
+      }
     }
     if (MODE != kFill && lane == 0) a.counts[it] = em.cnt;
     k = wc ? claim_take(ws, pend) : k + ws.nwaves;
@@ -532,11 +591,56 @@ __device__ __forceinline__ bool level_overlap(uint32_t phi, uint32_t plo, uint32
   return ((__builtin_bit_cast(uint32_t, d1) | __builtin_bit_cast(uint32_t, d2)) & 0x80008000u) == 0;
 }
 
-__device__ __forceinline__ double readlane_f64(double v, int l) {
-  const uint64_t u = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+// Tail items of the reduced-precision sweep: the level test per (row, q), then
+// every candidate confirmed at once against both exact records.
+template <int MODE>
+__device__ __forceinline__ void tailq(const SweepArgs &a, const Exact64 *__restrict__ rec,
+                                      const QRec *__restrict__ qrec, int2 *__restrict__ out,
+                                      const Item &item, int lane, Emit<MODE> &em,
+                                      unsigned long long &nconf) {
+  const int R = item.nrows & kRowsMask;
+  int rp = INT_MAX, rh = INT_MIN;
+  uint32_t rlo = 0x7fff7fffu, rhi = 0u, rop = 0;
+  if (lane < R) {
+    rp = a.rowlist[item.rowoff + lane];
+    rh = a.hi[rp];
+    const QRec x = qrec[rp];
+    rlo = x.lo;
+    rhi = x.hi;
+    if (MODE != kCount) rop = a.orig[rp];
+  }
+  int r0 = 0;
+  for (int qb = item.q0; qb <= item.q1; qb += kWave) {
+    while (r0 < R && readlane_i(rh, r0) < qb) ++r0;
+    const int q = qb + lane;
+    const bool inq = q <= item.q1;
+    uint32_t qlo = 0x7fff7fffu, qhi = 0u;
+    if (inq) {
+      const QRec x = qrec[q];
+      qlo = x.lo;
+      qhi = x.hi;
+    }
+    for (int r = r0; r < R; ++r) {
+      const int p = readlane_i(rp, r), hp = readlane_i(rh, r);
+      const uint32_t plo = (uint32_t)readlane_i((int)rlo, r), phi = (uint32_t)readlane_i((int)rhi, r);
+      const bool c = inq & (q > p) & (q <= hp) & level_overlap(phi, plo, qlo, qhi);
+      if (ballot(c)) {
+        const Exact64 xp = rec[p];
+        bool hg = false, h = false;
+        if (c) {
+          const Exact64 xq = rec[q];
+          hg = ((accept_bits(xp.type) & xq.type) != 0) & (xp.hi0 >= xq.lo0) & (xp.lo0 <= xq.hi0) &
+               (xp.hi1 >= xq.lo1) & (xp.lo1 <= xq.hi1) & (xp.hi2 >= xq.lo2) & (xp.lo2 <= xq.hi2);
+          h = hg & (xp.v0 != xq.v0) & (xp.v0 != xq.v1) & (xp.v0 != xq.v2) & (xp.v1 != xq.v0) &
+              (xp.v1 != xq.v1) & (xp.v1 != xq.v2) & (xp.v2 != xq.v0) & (xp.v2 != xq.v1) &
+              (xp.v2 != xq.v2);
+        }
+        nconf += (unsigned long long)__popcll(ballot(hg));
+        const uint64_t hm = ballot(h);
+        if (hm) em.put(h, hm, (uint32_t)readlane_i((int)rop, r), q, out);
+      }
+    }
+  }
 }
 
 template <int MODE, int B>
@@ -557,140 +661,144 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
     const unsigned pend = wc ? claim_issue(wc) : 0u;
     const int64_t it = a.sel ? (int64_t)a.sel[k] : k;
     const Item item = a.items[it];
-    const bool act = lane < item.nrows;
-    const int p = act ? a.rowlist[item.rowoff + lane] : INT_MAX;
-    const int myhi = act ? a.hi[p] : INT_MIN;
-    QRec mq;
-    mq.lo = 0x7fff7fffu;  // idle lanes: levels above every upper bound
-    mq.hi = 0u;
-    if (act) mq = qrec[p];
     Emit<MODE> em;
     if (MODE != kCount) em.base0 = em.base = a.offsets[it];
     if (MODE == kLearned) em.cap = a.caps[it];
+    if (item.nrows & kTailFlag) {
+      tailq<MODE>(a, rec, qrec, out, item, lane, em, nconf);
+    } else {
+      const bool act = lane < item.nrows;
+      const int p = act ? a.rowlist[item.rowoff + lane] : INT_MAX;
+      const int myhi = act ? a.hi[p] : INT_MIN;
+      QRec mq;
+      mq.lo = 0x7fff7fffu;  // idle lanes: levels above every upper bound
+      mq.hi = 0u;
+      if (act) mq = qrec[p];
 
-    // the candidate queue: entry e = (position eq, row mask emlo | emhi << 32)
-    int eq = 0;
-    uint32_t emlo = 0, emhi = 0;
-    int nq = 0;
-    // Draining expands the entries into (row, q) candidates, one per lane, in
-    // (entry, row lane) order through this wave's LDS slots, then tests all of
-    // them at once: both exact records are fetched per lane.
-    auto test_slots = [&](int T) __attribute__((always_inline)) {
-      bool h = false, hg = false;
-      int cp = 0, cq = 0;
-      if (lane < T) {
-        const int2 c = slots[lane];
-        cp = c.x;
-        cq = c.y;
-        const Exact64 xp = rec[cp], xq = rec[cq];
-        hg = ((accept_bits(xp.type) & xq.type) != 0) & (xp.hi0 >= xq.lo0) & (xp.lo0 <= xq.hi0) &
-             (xp.hi1 >= xq.lo1) & (xp.lo1 <= xq.hi1) & (xp.hi2 >= xq.lo2) & (xp.lo2 <= xq.hi2);
-        h = hg;
-        h = h & (xp.v0 != xq.v0) & (xp.v0 != xq.v1) & (xp.v0 != xq.v2) & (xp.v1 != xq.v0) &
-            (xp.v1 != xq.v1) & (xp.v1 != xq.v2) & (xp.v2 != xq.v0) & (xp.v2 != xq.v1) &
-            (xp.v2 != xq.v2);
-      }
-      nconf += (unsigned long long)__popcll(ballot(hg));
-      const uint64_t hm = ballot(h);
-      if (hm) {
-        uint32_t o = 0;
-        if (MODE != kCount && h) o = a.orig[cp];
-        em.put(h, hm, o, cq, out);
-      }
-    };
-    auto drain = [&]() __attribute__((always_inline)) {
-      int T = 0;
-      for (int e = 0; e < nq; ++e) {
-        const int q = __builtin_amdgcn_readlane(eq, e);
-        const uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)emhi, e) << 32) |
-                           (uint32_t)__builtin_amdgcn_readlane((int)emlo, e);
-        const int c = __popcll(m);
-        if (T + c > kWave) {
-          __builtin_amdgcn_wave_barrier();
-          test_slots(T);
-          __builtin_amdgcn_wave_barrier();
-          T = 0;
+      // the candidate queue: entry e = (position eq, row mask emlo | emhi << 32)
+      int eq = 0;
+      uint32_t emlo = 0, emhi = 0;
+      int nq = 0;
+      // Draining expands the entries into (row, q) candidates, one per lane, in
+      // (entry, row lane) order through this wave's LDS slots, then tests all of
+      // them at once: both exact records are fetched per lane.
+      auto test_slots = [&](int T) __attribute__((always_inline)) {
+        bool h = false, hg = false;
+        int cp = 0, cq = 0;
+        if (lane < T) {
+          const int2 c = slots[lane];
+          cp = c.x;
+          cq = c.y;
+          const Exact64 xp = rec[cp], xq = rec[cq];
+          hg = ((accept_bits(xp.type) & xq.type) != 0) & (xp.hi0 >= xq.lo0) & (xp.lo0 <= xq.hi0) &
+               (xp.hi1 >= xq.lo1) & (xp.lo1 <= xq.hi1) & (xp.hi2 >= xq.lo2) & (xp.lo2 <= xq.hi2);
+          h = hg;
+          h = h & (xp.v0 != xq.v0) & (xp.v0 != xq.v1) & (xp.v0 != xq.v2) & (xp.v1 != xq.v0) &
+              (xp.v1 != xq.v1) & (xp.v1 != xq.v2) & (xp.v2 != xq.v0) & (xp.v2 != xq.v1) &
+              (xp.v2 != xq.v2);
         }
-        if ((m >> lane) & 1ull) slots[T + mbcnt64(m)] = make_int2(p, q);
-        T += c;
-      }
-      __builtin_amdgcn_wave_barrier();
-      test_slots(T);
-      __builtin_amdgcn_wave_barrier();
-      nq = 0;
-    };
-    auto push = [&](int q, uint64_t m) __attribute__((always_inline)) {
-      if (lane == nq) {
-        eq = q;
-        emlo = (uint32_t)m;
-        emhi = (uint32_t)(m >> 32);
-      }
-      ++nq;
-    };
-    auto batch = [&](const QRec *b, int q, bool check) __attribute__((always_inline)) {
-      uint64_t m[B];
-      uint64_t any = 0;
-#pragma unroll
-      for (int j = 0; j < B; ++j) {
-        m[j] = ballot(level_overlap(mq.hi, mq.lo, b[j].lo, b[j].hi));
-        if (check) m[j] &= ballot(q + j > p) & ballot(q + j <= myhi);
-        any |= m[j];
-      }
-      if (any) {
-#pragma unroll
-        for (int j = 0; j < B; ++j)
-          if (m[j]) push(q + j, m[j]);
-        if (nq >= kDrainAt) drain();
-      }
-    };
-    auto walk = [&](int qa, int qb, bool check) __attribute__((always_inline)) {
-      int q = qa;
-      int nb = (qb - qa + 1) / B;
-      if (nb > 0) {
-        QRec b0[B], b1[B];
-#pragma unroll
-        for (int j = 0; j < B; ++j) b0[j] = qrec[q + j];
-        while (true) {
-          q += zero * (int)b0[0].lo;  // waits for b0 on every path
-          if (nb > 1) {
-#pragma unroll
-            for (int j = 0; j < B; ++j) b1[j] = qrec[q + B + j];
-          }
-          batch(b0, q, check);
-          q += B;
-          if (--nb == 0) break;
-          q += zero * (int)b1[0].lo;
-          if (nb > 1) {
-#pragma unroll
-            for (int j = 0; j < B; ++j) b0[j] = qrec[q + B + j];
-          }
-          batch(b1, q, check);
-          q += B;
-          if (--nb == 0) break;
+        nconf += (unsigned long long)__popcll(ballot(hg));
+        const uint64_t hm = ballot(h);
+        if (hm) {
+          uint32_t o = 0;
+          if (MODE != kCount && h) o = a.orig[cp];
+          em.put(h, hm, o, cq, out);
         }
-      }
-      for (; q <= qb; ++q) {
-        const QRec r = qrec[q];
-        uint64_t m = ballot(level_overlap(mq.hi, mq.lo, r.lo, r.hi));
-        if (check) m &= ballot(q > p) & ballot(q <= myhi);
-        if (m) {
-          push(q, m);
+      };
+      auto drain = [&]() __attribute__((always_inline)) {
+        int T = 0;
+        for (int e = 0; e < nq; ++e) {
+          const int q = __builtin_amdgcn_readlane(eq, e);
+          const uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)emhi, e) << 32) |
+                             (uint32_t)__builtin_amdgcn_readlane((int)emlo, e);
+          const int c = __popcll(m);
+          if (T + c > kWave) {
+            __builtin_amdgcn_wave_barrier();
+            test_slots(T);
+            __builtin_amdgcn_wave_barrier();
+            T = 0;
+          }
+          if ((m >> lane) & 1ull) slots[T + mbcnt64(m)] = make_int2(p, q);
+          T += c;
+        }
+        __builtin_amdgcn_wave_barrier();
+        test_slots(T);
+        __builtin_amdgcn_wave_barrier();
+        nq = 0;
+      };
+      auto push = [&](int q, uint64_t m) __attribute__((always_inline)) {
+        if (lane == nq) {
+          eq = q;
+          emlo = (uint32_t)m;
+          emhi = (uint32_t)(m >> 32);
+        }
+        ++nq;
+      };
+      auto batch = [&](const QRec *b, int q, bool check) __attribute__((always_inline)) {
+        uint64_t m[B];
+        uint64_t any = 0;
+  #pragma unroll
+        for (int j = 0; j < B; ++j) {
+          m[j] = ballot(level_overlap(mq.hi, mq.lo, b[j].lo, b[j].hi));
+          if (check) m[j] &= ballot(q + j > p) & ballot(q + j <= myhi);
+          any |= m[j];
+        }
+        if (any) {
+  #pragma unroll
+          for (int j = 0; j < B; ++j)
+            if (m[j]) push(q + j, m[j]);
           if (nq >= kDrainAt) drain();
         }
+      };
+      auto walk = [&](int qa, int qb, bool check) __attribute__((always_inline)) {
+        int q = qa;
+        int nb = (qb - qa + 1) / B;
+        if (nb > 0) {
+          QRec b0[B], b1[B];
+  #pragma unroll
+          for (int j = 0; j < B; ++j) b0[j] = qrec[q + j];
+          while (true) {
+            q += zero * (int)b0[0].lo;  // waits for b0 on every path
+            if (nb > 1) {
+  #pragma unroll
+              for (int j = 0; j < B; ++j) b1[j] = qrec[q + B + j];
+            }
+            batch(b0, q, check);
+            q += B;
+            if (--nb == 0) break;
+            q += zero * (int)b1[0].lo;
+            if (nb > 1) {
+  #pragma unroll
+              for (int j = 0; j < B; ++j) b0[j] = qrec[q + B + j];
+            }
+            batch(b1, q, check);
+            q += B;
+            if (--nb == 0) break;
+          }
+        }
+        for (; q <= qb; ++q) {
+          const QRec r = qrec[q];
+          uint64_t m = ballot(level_overlap(mq.hi, mq.lo, r.lo, r.hi));
+          if (check) m &= ballot(q > p) & ballot(q <= myhi);
+          if (m) {
+            push(q, m);
+            if (nq >= kDrainAt) drain();
+          }
+        }
+      };
+      int pmax, hmin;
+      row_span(act, p, myhi, pmax, hmin);
+      const int q0 = item.q0, q1 = item.q1;
+      const int b0 = max(q0, pmax + 1), b1 = min(q1, hmin);
+      if (b0 <= b1) {
+        if (q0 < b0) walk(q0, b0 - 1, true);
+        walk(b0, b1, false);
+        if (b1 < q1) walk(b1 + 1, q1, true);
+      } else {
+        walk(q0, q1, true);
       }
-    };
-    int pmax, hmin;
-    row_span(act, p, myhi, pmax, hmin);
-    const int q0 = item.q0, q1 = item.q1;
-    const int b0 = max(q0, pmax + 1), b1 = min(q1, hmin);
-    if (b0 <= b1) {
-      if (q0 < b0) walk(q0, b0 - 1, true);
-      walk(b0, b1, false);
-      if (b1 < q1) walk(b1 + 1, q1, true);
-    } else {
-      walk(q0, q1, true);
+      if (nq) drain();
     }
-    if (nq) drain();
     if (MODE != kFill && lane == 0) a.counts[it] = em.cnt;
     k = wc ? claim_take(ws, pend) : k + ws.nwaves;
   }
@@ -755,20 +863,21 @@ void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, i
   STQ_HIP(hipGetLastError());
 }
 
-void launch_plan(const int *hi, int64_t n, int W, int chunk, int *rowlist, int4 *groups,
-                 uint32_t *nchunks, unsigned long long *lane_steps, hipStream_t st) {
+void launch_plan(const int *hi, int64_t n, int W, int chunk, bool tails, int *rowlist,
+                 int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
+                 unsigned long long *lane_steps, hipStream_t st) {
   if (n <= 0) return;
   const int64_t nwin = (n + W - 1) / W;
   hipLaunchKernelGGL(k_plan, dim3((unsigned)nwin), dim3(kThreads), 0, st, hi, n, W, chunk,
-                     rowlist, groups, nchunks, lane_steps);
+                     tails ? 1 : 0, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
   STQ_HIP(hipGetLastError());
 }
 
-void launch_items(const int4 *groups, const int64_t *ioff, int64_t ngroups, int chunk,
-                  Item *items, hipStream_t st) {
+void launch_items(const int4 *groups, const int4 *tgroups, const int2 *gchunk, const int64_t *ioff,
+                  int64_t ngroups, Item *items, hipStream_t st) {
   if (ngroups <= 0) return;
-  hipLaunchKernelGGL(k_items, dim3(blocks_for(ngroups)), dim3(kThreads), 0, st, groups, ioff,
-                     ngroups, chunk, items);
+  hipLaunchKernelGGL(k_items, dim3(blocks_for(ngroups)), dim3(kThreads), 0, st, groups, tgroups,
+                     gchunk, ioff, ngroups, items);
   STQ_HIP(hipGetLastError());
 }
 

@@ -75,10 +75,12 @@ void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, i
                    hipStream_t st);
 void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, int *hi,
                       unsigned long long *cand, hipStream_t st);
-void launch_plan(const int *hi, int64_t n, int W, int chunk, int *rowlist, int4 *groups,
-                 uint32_t *nchunks, unsigned long long *lane_steps, hipStream_t st);
-void launch_items(const int4 *groups, const int64_t *ioff, int64_t ngroups, int chunk,
-                  Item *items, hipStream_t st);
+void launch_plan(const int *hi, int64_t n, int W, int chunk, bool tails, int *rowlist,
+                 int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
+                 unsigned long long *lane_steps, hipStream_t st);
+// Items of every group from the group records (body chunks, then tail chunks).
+void launch_items(const int4 *groups, const int4 *tgroups, const int2 *gchunk, const int64_t *ioff,
+                  int64_t ngroups, Item *items, hipStream_t st);
 enum SweepMode { kCount = 0, kFill = 1, kLearned = 2 };
 
 struct SweepArgs {

@@ -56,7 +56,8 @@ class Options(ctypes.Structure):
         ("train_scenes", ctypes.c_int32),
         ("graph", ctypes.c_int32),
         ("claim", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("tails", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 

@@ -136,7 +136,10 @@ typedef struct stq_options {
   int32_t claim;         /* sweep work split: 0 = auto (waves claim
                             items from per-XCD counters), 1 = claim,
                             2 = fixed stride                        */
-  int32_t reserved[4];
+  int32_t tails;         /* long-chain rows swept as tail items with
+                            positions across lanes: 0 = auto (on),
+                            1 = on, 2 = off                         */
+  int32_t reserved[3];
 } stq_options;
 
 typedef struct stq_stats {

@@ -210,3 +210,51 @@ def test_cloth_full_size_properties(ctx):
     mine = np.stack([pos[i[keep]], pos[j[keep]]], 1)
     assert len(ref) > 100000
     assert np.array_equal(oracle.canonical(mine), ref)
+
+
+def _long_row_scene(n=40_000, seed=21, frac=0.004, scale=300.0, extent=2.0):
+    """Small uniform boxes plus a few whose sweep-axis extent is `scale` times
+    longer: their rows have chains far longer than their group's other rows."""
+    b = scenes.uniform_scene(n, seed=seed, extent_scale=extent)
+    rng = np.random.default_rng(seed)
+    k = rng.choice(n, size=max(1, int(frac * n)), replace=False)
+    c = 0.5 * (b["min"][k] + b["max"][k])
+    h = 0.5 * (b["max"][k] - b["min"][k])
+    h[:, 0] *= scale
+    b["min"][k] = c - h
+    b["max"][k] = c + h
+    return b
+
+
+@pytest.mark.parametrize("form,kw", FORMS)
+@pytest.mark.parametrize("sizing", [stq_amd.SIZING_COUNT_FILL, stq_amd.SIZING_LEARNED])
+def test_long_rows_tail_items(ctx, form, kw, sizing):
+    b = _long_row_scene()
+    ref = oracle.reference_set(b)
+    steps = {}
+    outs = {}
+    for tails in (2, 1):
+        ctx.sizer_reset()
+        for rep in range(3):  # learned: one training call, then the single pass
+            got = run(ctx, b, axis_policy=0, tails=tails, sizing=sizing, train_scenes=1, **kw)
+            assert same_set(got, ref), (form, sizing, tails, rep)
+        steps[tails] = ctx.stats()["n_lane_steps"]
+        outs[tails] = got
+        assert np.array_equal(run(ctx, b, axis_policy=0, tails=tails, sizing=sizing, train_scenes=1,
+                                  **kw), got)
+    print(f"{form} sizing={sizing}: lane steps without tails {steps[2]}, with {steps[1]}", flush=True)
+    assert steps[1] < steps[2]
+
+
+@pytest.mark.parametrize("form,kw", FORMS)
+def test_long_rows_tail_items_forced_fallback(ctx, form, kw):
+    """Every item (tail items too) overflows its predicted slots and is redone."""
+    b = _long_row_scene(seed=22, extent=12.0, scale=40.0)
+    ref = oracle.reference_set(b)
+    w = np.zeros(6337, np.float32)
+    w[-1] = -5.0
+    ctx.sizer_set_params(w)
+    got = run(ctx, b, axis_policy=0, sizing=stq_amd.SIZING_LEARNED, train_scenes=-1, **kw)
+    assert ctx.stats()["n_fallback_tiles"] > 0
+    assert same_set(got, ref)
+    assert np.array_equal(got, run(ctx, b, axis_policy=0, **kw))
