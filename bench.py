@@ -93,12 +93,12 @@ def tuned_table(ctx):
         ctx.autotune_save(path)
         with open(path) as fh:
             lines = fh.read().split("\n")[1:]
-    keys = ("key", "axis_policy", "window", "chunk", "batch", "waves_per_cu", "trial_ms")
+    keys = ("key", "axis_policy", "window", "chunk", "batch", "waves_per_cu", "tail_cost", "trial_ms")
     out = []
     for ln in lines:
         if ln.strip():
             v = ln.split()
-            out.append(dict(zip(keys, [v[0]] + [int(x) for x in v[1:6]] + [float(v[6])])))
+            out.append(dict(zip(keys, [v[0]] + [int(x) for x in v[1:7]] + [float(v[7])])))
     return out
 
 

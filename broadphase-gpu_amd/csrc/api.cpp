@@ -29,7 +29,12 @@ static bool options_valid(const stq_options &o) {
          o.waves_per_cu >= 0 && o.waves_per_cu <= 64 &&
          (o.batch == 0 || o.batch == 2 || o.batch == 4 || o.batch == 8) && o.xcd_split >= 0 &&
          o.xcd_split <= 2 && o.graph >= 0 && o.graph <= 2 && o.claim >= 0 && o.claim <= 2 &&
-         o.tails >= 0 && o.tails <= 2;
+         o.tails >= 0 && o.tails <= 2 && o.tail_cost >= 0 && o.tail_cost <= 100000;
+}
+
+// Weight of a tail test in the group cut; 0 turns tails off.
+static float tail_cost(const stq_options &o) {
+  return o.tails == 2 ? 0.f : (o.tail_cost > 0 ? (float)o.tail_cost * 0.01f : 4.f);
 }
 
 // Axis choice and, for the reduced-precision form, the quantisation range:
@@ -160,7 +165,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   int2 *d_gch = gchunks.as<int2>((size_t)ngroups);
   uint32_t *d_nch = nchunks.as<uint32_t>((size_t)ngroups);
   int64_t *d_ioff = ioff.as<int64_t>((size_t)ngroups + 1);
-  launch_plan(d_hi, n, W, chunk, opt.tails != 2, d_rowlist, d_groups, d_tg, d_gch, d_nch, &ctr[2],
+  launch_plan(d_hi, n, W, chunk, tail_cost(opt), d_rowlist, d_groups, d_tg, d_gch, d_nch, &ctr[2],
               st);
   scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
   int64_t nitems = 0;
@@ -627,7 +632,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     }
     launch_gather_seg(f32, boxes, d_vals, t, bp.dev, d_smin, d_smax, d_rec, d_qrec, d_orig, st);
     launch_chain_end_seg(f32, d_smin, d_smax, t, bp.dev, d_hi, &ctr[1], st);
-    launch_plan_seg(d_hi, t, bp.dev, W, chunk, opt.tails != 2, d_rowlist, d_groups, d_tg, d_gchunk,
+    launch_plan_seg(d_hi, t, bp.dev, W, chunk, tail_cost(opt), d_rowlist, d_groups, d_tg, d_gchunk,
                     d_nch, &ctr[2], st);
     scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
     launch_items(d_groups, d_tg, d_gchunk, d_ioff, ngroups, d_items, st);

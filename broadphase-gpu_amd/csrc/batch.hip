@@ -229,7 +229,8 @@ __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
                                                  const int64_t *__restrict__ win_start,
                                                  const int *__restrict__ win_cnt,
                                                  const int64_t *__restrict__ win_gbase, int W,
-                                                 int chunk, int tails, int *__restrict__ rowlist,
+                                                 int chunk, float tail_cost,
+                                                 int *__restrict__ rowlist,
                                                  int4 *__restrict__ groups,
                                                  int4 *__restrict__ tgroups,
                                                  int2 *__restrict__ gchunk,
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
     const int nrows = min(64, cnt - 64 * k);
     // capped: the item buffer is sized on the host from the scene sizes
     const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, true,
-                                    tails != 0);
+                                    tail_cost);
     const int64_t g = gbase + k;
     if (lane == 0) {
       groups[g] = gp.body;
@@ -411,12 +412,12 @@ void launch_chain_end_seg(bool f32, const void *smin, const void *smax, const Ba
 }
 
 void launch_plan_seg(const int *hi, const BatchTables &t, const BatchDev &d, int W, int chunk,
-                     bool tails, int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk,
+                     float tail_cost, int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk,
                      uint32_t *nchunks, unsigned long long *lane_steps, hipStream_t st) {
   const int nw = (int)t.win_start.size();
   if (nw <= 0) return;
   hipLaunchKernelGGL(k_plan_seg, dim3(nw), dim3(kT), 0, st, hi, d.win_start, d.win_cnt, d.win_gbase,
-                     W, chunk, tails ? 1 : 0, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
+                     W, chunk, tail_cost, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
   STQ_HIP(hipGetLastError());
 }
 

@@ -62,7 +62,7 @@ void launch_chain_end_seg(bool f32, const void *smin, const void *smax, const Ba
 // has a bound the host computes from the scene sizes.  Items are then written
 // by launch_items (sweep.hpp).
 void launch_plan_seg(const int *hi, const BatchTables &t, const BatchDev &d, int W, int chunk,
-                     bool tails, int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk,
+                     float tail_cost, int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk,
                      uint32_t *nchunks, unsigned long long *lane_steps, hipStream_t st);
 // pair_off[s] = off[ioff[scene_first_group[s]]] for s = 0..nscenes.
 void launch_scene_pairs(const int64_t *off, const int64_t *ioff, const BatchTables &t,

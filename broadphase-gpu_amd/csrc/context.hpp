@@ -93,7 +93,7 @@ enum Ev {
 
 // Launch parameters the autotuner picks for one call shape class.
 struct TuneEntry {
-  int axis_policy, window, chunk, batch, waves_per_cu;
+  int axis_policy, window, chunk, batch, waves_per_cu, tail_cost;
   double ms;  // the winning trial's count-only time
 };
 // Shape class of a call: precision, batched or not, floor(log2 boxes),

@@ -7,7 +7,7 @@
 // idle through their tails.  So the group is cut after the row k that
 // minimises
 //     (hi_k - q0 + 1)                                  body: 64 lanes per step
-//   + kTailCost * sum_{j > k} (hi_j - hi_k) / 64       tails: one row per lane-step
+//   + tail_cost * sum_{j > k} (hi_j - hi_k) / 64      tails: one row per lane-step
 // The body is swept as before over [q0, hi_k]; the rows above k become a tail
 // item over (hi_k, max hi], swept with positions across the lanes and the
 // rows taken in turn.  Without long rows the cut is the last row (no tail).
@@ -21,7 +21,6 @@ namespace stqamd {
 
 constexpr int kTailFlag = 1 << 30;  // in Item::nrows: a tail item
 constexpr int kRowsMask = 127;      // Item::nrows & kRowsMask = rows of the item
-constexpr double kTailCost = 2.0;   // a tail (row, q) test against a body lane step
 constexpr int kCapChunks = 64;      // capped planning: at most this many items per body or tail
 
 // Group record: body {rowoff, nrows, q0, q1}, tail {rowoff, rows, q0, q1}
@@ -39,8 +38,9 @@ __device__ inline int64_t chunk_count(int64_t span, int ch) {
 
 // All 64 lanes call this; lane j < nrows holds row j of the group (position p,
 // chain end h, non-decreasing in j).  The result is the same in every lane.
+// tail_cost <= 0: no tails.
 __device__ inline GroupPlan plan_group(int lane, int nrows, int p, int h, int rowoff, int chunk,
-                                       bool cap, bool tails) {
+                                       bool cap, float tail_cost) {
   const bool act = lane < nrows;
   int pm = act ? p : INT_MAX, hm = act ? h : INT_MIN;
 #pragma unroll
@@ -50,7 +50,7 @@ __device__ inline GroupPlan plan_group(int lane, int nrows, int p, int h, int ro
   }
   const int q0 = pm + 1, hmax = hm;
   int kc = nrows - 1;  // last row of the body
-  if (tails) {
+  if (tail_cost > 0.f) {
     // inclusive prefix sum of the chain ends
     long long pre = act ? (long long)h : 0;
 #pragma unroll
@@ -63,7 +63,7 @@ __device__ inline GroupPlan plan_group(int lane, int nrows, int p, int h, int ro
     if (act) {
       const double body = (double)max((long long)h - q0 + 1, 0ll);
       const double tail = (double)(tot - pre) - (double)(nrows - 1 - lane) * (double)h;
-      cost = body + kTailCost * tail * (1.0 / 64.0);
+      cost = body + (double)tail_cost * tail * (1.0 / 64.0);
     }
     double cmin = cost;
 #pragma unroll

@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kThreads) void k_chain_end(const T *__restrict__ sm
 constexpr int kMaxWindow = 2048;
 
 __global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi, int64_t n, int W,
-                                                   int chunk, int tails,
+                                                   int chunk, float tail_cost,
                                                    int *__restrict__ rowlist,
                                                    int4 *__restrict__ groups,
                                                    int4 *__restrict__ tgroups,
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi, i
     const int h = act ? (int)(key[j] >> 32) : INT_MIN;
     const int nrows = min(64, cnt - 64 * k);
     const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, false,
-                                    tails != 0);
+                                    tail_cost);
     const int64_t g = (wbase >> 6) + k;
     if (lane == 0) {
       groups[g] = gp.body;
@@ -863,13 +863,13 @@ void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, i
   STQ_HIP(hipGetLastError());
 }
 
-void launch_plan(const int *hi, int64_t n, int W, int chunk, bool tails, int *rowlist,
+void launch_plan(const int *hi, int64_t n, int W, int chunk, float tail_cost, int *rowlist,
                  int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
                  unsigned long long *lane_steps, hipStream_t st) {
   if (n <= 0) return;
   const int64_t nwin = (n + W - 1) / W;
   hipLaunchKernelGGL(k_plan, dim3((unsigned)nwin), dim3(kThreads), 0, st, hi, n, W, chunk,
-                     tails ? 1 : 0, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
+                     tail_cost, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
   STQ_HIP(hipGetLastError());
 }
 

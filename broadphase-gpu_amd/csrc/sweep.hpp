@@ -75,7 +75,8 @@ void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, i
                    hipStream_t st);
 void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, int *hi,
                       unsigned long long *cand, hipStream_t st);
-void launch_plan(const int *hi, int64_t n, int W, int chunk, bool tails, int *rowlist,
+// tail_cost: weight of tail tests in the group cut (plan.hpp); 0 = no tails.
+void launch_plan(const int *hi, int64_t n, int W, int chunk, float tail_cost, int *rowlist,
                  int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
                  unsigned long long *lane_steps, hipStream_t st);
 // Items of every group from the group records (body chunks, then tail chunks).

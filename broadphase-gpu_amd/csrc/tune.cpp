@@ -1,8 +1,8 @@
 // tune.cpp -- the autotuner.  For each call shape class (precision, batched
 // or not, floor(log2 boxes), floor(log2 scenes)) it picks the sweep-axis
 // policy, the planning window, the item chunk, the scalar batch of the sweep
-// loop (how many q records are in flight per wave) and the resident waves per
-// CU, by timing count-only trial runs on the first input of that class:
+// loop (how many q records are in flight per wave), the resident waves per CU
+// and the weight of tail tests in the group cut, by timing count-only trial runs on the first input of that class:
 // coordinate descent, each candidate timed twice with HIP events, a change
 // kept only when it is at least 2% faster.  The table can be saved to and
 // loaded from a text file.
@@ -53,6 +53,7 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
       opt.chunk = e.chunk;
       opt.batch = e.batch;
       opt.waves_per_cu = e.waves_per_cu;
+      opt.tail_cost = e.tail_cost;
       double best = INFINITY;
       for (int rep = 0; rep < 2; ++rep) {
         try {  // capacity 0: the count pass runs, no pair is written
@@ -71,7 +72,7 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
       }
       return best;
     };
-    TuneEntry best{STQ_AXIS_CHAIN, batched ? kBatchWindow : 0, 2048, f32 ? 8 : 2, 32, 0};
+    TuneEntry best{STQ_AXIS_CHAIN, batched ? kBatchWindow : 0, 2048, f32 ? 8 : 2, 32, 400, 0};
     best.ms = trial(best);
     auto consider = [&](TuneEntry e) {
       if (std::memcmp(&e, &best, offsetof(TuneEntry, ms)) == 0) return;
@@ -104,6 +105,11 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
       e.waves_per_cu = w;
       consider(e);
     }
+    for (int tc : {200, 800}) {
+      TuneEntry e = best;
+      e.tail_cost = tc;
+      consider(e);
+    }
     opt = user;
     it = tune_table.emplace(key, best).first;
     last_tune_ms = std::chrono::duration<double, std::milli>(
@@ -115,18 +121,19 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
   opt.chunk = e.chunk;
   opt.batch = e.batch;
   opt.waves_per_cu = e.waves_per_cu;
+  opt.tail_cost = e.tail_cost;
 }
 
 // File format: a header line, then one line per entry:
-//   <key hex> <axis_policy> <window> <chunk> <batch> <waves_per_cu> <ms>
+//   <key hex> <axis_policy> <window> <chunk> <batch> <waves_per_cu> <tail_cost> <ms>
 int Context::tune_save(const char *path) const {
   FILE *f = std::fopen(path, "w");
   if (!f) return STQ_ERR_INVALID_ARGUMENT;
-  std::fprintf(f, "stq-autotune 1\n");
+  std::fprintf(f, "stq-autotune 2\n");
   for (const auto &kv : tune_table) {
     const TuneEntry &e = kv.second;
-    std::fprintf(f, "%llx %d %d %d %d %d %.6f\n", (unsigned long long)kv.first, e.axis_policy,
-                 e.window, e.chunk, e.batch, e.waves_per_cu, e.ms);
+    std::fprintf(f, "%llx %d %d %d %d %d %d %.6f\n", (unsigned long long)kv.first, e.axis_policy,
+                 e.window, e.chunk, e.batch, e.waves_per_cu, e.tail_cost, e.ms);
   }
   return std::fclose(f) == 0 ? STQ_OK : STQ_ERR_INVALID_ARGUMENT;
 }
@@ -137,7 +144,7 @@ int Context::tune_load(const char *path) {
   char head[64] = {0};
   int version = 0;
   if (std::fscanf(f, "%63s %d", head, &version) != 2 || std::strcmp(head, "stq-autotune") != 0 ||
-      version != 1) {
+      version != 2) {
     std::fclose(f);
     return STQ_ERR_INVALID_ARGUMENT;
   }
@@ -145,13 +152,13 @@ int Context::tune_load(const char *path) {
   unsigned long long key;
   TuneEntry e;
   int r;
-  while ((r = std::fscanf(f, "%llx %d %d %d %d %d %lf", &key, &e.axis_policy, &e.window, &e.chunk,
-                          &e.batch, &e.waves_per_cu, &e.ms)) == 7) {
+  while ((r = std::fscanf(f, "%llx %d %d %d %d %d %d %lf", &key, &e.axis_policy, &e.window,
+                          &e.chunk, &e.batch, &e.waves_per_cu, &e.tail_cost, &e.ms)) == 8) {
     const bool ok = e.axis_policy >= 0 && e.axis_policy <= 4 && e.chunk > 0 &&
                     (e.window == 0 || (e.window >= 64 && e.window <= 2048 &&
                                        (e.window & (e.window - 1)) == 0)) &&
                     (e.batch == 2 || e.batch == 4 || e.batch == 8) && e.waves_per_cu > 0 &&
-                    e.waves_per_cu <= 64;
+                    e.waves_per_cu <= 64 && e.tail_cost >= 0 && e.tail_cost <= 100000;
     if (!ok) {
       std::fclose(f);
       return STQ_ERR_INVALID_ARGUMENT;

@@ -57,7 +57,8 @@ class Options(ctypes.Structure):
         ("graph", ctypes.c_int32),
         ("claim", ctypes.c_int32),
         ("tails", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("tail_cost", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

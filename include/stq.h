@@ -139,7 +139,10 @@ typedef struct stq_options {
   int32_t tails;         /* long-chain rows swept as tail items with
                             positions across lanes: 0 = auto (on),
                             1 = on, 2 = off                         */
-  int32_t reserved[3];
+  int32_t tail_cost;     /* weight of a tail (row, position) test
+                            against a body lane step, in percent;
+                            0 = 400                                 */
+  int32_t reserved[2];
 } stq_options;
 
 typedef struct stq_stats {
@@ -224,7 +227,7 @@ int stq_find_pairs_batched_device(void *ctx, const stq_aabb_f64 *d_boxes,
    With options.autotune = 1, the first call of each shape class (precision,
    batched or not, floor(log2 boxes), floor(log2 scenes)) with at least 16384
    boxes runs count-only trials on its own input and picks the sweep-axis
-   policy, window, chunk, batch and waves_per_cu; those values then replace
+   policy, window, chunk, batch, waves_per_cu and tail_cost; those values then replace
    the options' own for every call of that class.  The table belongs to the
    context and can be saved to and loaded from a text file. */
 int stq_autotune_save(void *ctx, const char *path);

@@ -34,7 +34,7 @@ def test_tune_reuse_save_load(tmp_path, prec):
         path = tmp_path / "tune.txt"
         c.autotune_save(path)
         text = path.read_text()
-        assert text.startswith("stq-autotune 1\n") and len(text.splitlines()) == 2
+        assert text.startswith("stq-autotune 2\n") and len(text.splitlines()) == 2
         c.autotune_clear()
         assert c.autotune_entries() == 0
         c.autotune_load(path)
@@ -50,9 +50,9 @@ def test_load_rejects_malformed(tmp_path):
     c = stq_amd.Context(0, autotune=1)
     try:
         bad = tmp_path / "bad.txt"
-        for text in ["nonsense\n", "stq-autotune 1\n1 9 0 2048 8 32 1.0\n",
-                     "stq-autotune 1\n1 4 300 2048 8 32 1.0\n", "stq-autotune 1\n1 4 0 2048 8\n",
-                     "stq-autotune 2\n"]:
+        for text in ["nonsense\n", "stq-autotune 2\n1 9 0 2048 8 32 200 1.0\n",
+                     "stq-autotune 2\n1 4 300 2048 8 32 200 1.0\n", "stq-autotune 2\n1 4 0 2048 8\n",
+                     "stq-autotune 2\n1 4 0 2048 8 32 -5 1.0\n", "stq-autotune 1\n"]:
             bad.write_text(text)
             with pytest.raises(stq_amd.StqError):
                 c.autotune_load(bad)
