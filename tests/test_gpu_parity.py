@@ -210,6 +210,27 @@ def test_cloth_full_size_properties(ctx):
     mine = np.stack([pos[i[keep]], pos[j[keep]]], 1)
     assert len(ref) > 100000
     assert np.array_equal(oracle.canonical(mine), ref)
+    # and the whole set against the oracle's full sweep
+    assert np.array_equal(c64, oracle.reference_set(b))
+
+
+@pytest.mark.parametrize("form,kw", FORMS)
+def test_clustered_8m_full_size(ctx, form, kw):
+    """The 8M-box clustered scene with Pareto extents (64 clusters, sigma
+    0.02): the whole pair set against the oracle, with tail items on (the
+    default) and with the learned single pass."""
+    b = scenes.clustered_scene()
+    assert len(b) == 8_000_000
+    ref = oracle.reference_set(b)
+    got = run(ctx, b, axis_policy=0, **kw)
+    st = ctx.stats()
+    print(f"clustered 8M {form}: {len(got)} pairs, {st['ms_total']:.1f} ms, items {st['n_items']}, "
+          f"lane steps {st['n_lane_steps']:.3e}, candidates {st['n_candidates']:.3e}", flush=True)
+    assert same_set(got, ref)
+    ctx.sizer_reset()
+    for rep in range(2):
+        got = run(ctx, b, axis_policy=0, sizing=stq_amd.SIZING_LEARNED, train_scenes=1, **kw)
+        assert same_set(got, ref), rep
 
 
 def _long_row_scene(n=40_000, seed=21, frac=0.004, scale=300.0, extent=2.0):
