@@ -5,17 +5,25 @@ FP64 boxes, ~48.4M pairs), boxes already resident in HBM, pairs written to a
 device buffer.  Two forms run on the same scene:
 
   plain  : FP64 sort keys and comparisons, sweep axis x, exact count pass +
-           fill pass (the reference method, src/simulation.cu:555-782)
+           fill pass, no tail items, default launch parameters (the
+           reference method, src/simulation.cu:555-782, on this library's
+           kernels)
   chosen : FP32 outward-rounded keys and chain ends, 15-bit outward levels in
            the sweep's hot loop, FP64 confirmation of every candidate,
-           chain-length axis policy, and learned sizing: one emitting pass
-           into per-item slots predicted by the network (trained online on
-           the exact counts of a stream of other scenes before timing)
+           autotuned launch parameters, tail items for long chains, and
+           learned sizing: one emitting pass into per-item slots predicted by
+           the network (trained online on the exact counts of a stream of
+           other scenes before timing)
 
 The JSON line reports the chosen form's boxes/s as `value`, its pairs/s, the
 plain form's numbers and the speedup, whether both pair sets are identical,
 the roofline of the dominant kernel (the emitting sweep) and a CPU baseline
-(the oracle's restatement of the reference CPU sweep on this host).
+(the oracle's restatement of the reference CPU sweep on this host, with a
+1-thread figure).  Further legs, reported in the same line: the 8M-box
+clustered heavy-tailed scene (`skewed`), 512 scenes in one batched call
+(`batched`), a bounded sample of the learned-sizing stream
+(`learned_stream`) and the PCIe-inclusive host-to-host time
+(`end_to_end_host`).
 """
 from __future__ import annotations
 
@@ -52,6 +60,10 @@ def parse():
     ap.add_argument("--stream-scenes", type=int, default=240,
                     help="scenes of the learned-sizing stream leg: the first 200 train, the rest are "
                          "held out (sizes log-uniform in [5e4, 4e6]; 0 = skip)")
+    ap.add_argument("--skewed", type=int, default=1,
+                    help="run the clustered heavy-tailed 8M-box leg (0 = skip)")
+    ap.add_argument("--cpu-sample-1t", type=float, default=0.02,
+                    help="fraction of sorted rows the 1-thread CPU timing sweeps (0 = skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (traffic calibration)")
     return ap.parse_args()
@@ -122,7 +134,7 @@ def bench_batched(args, H):
     forms = {
         "chosen": dict(precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN,
                        sizing=stq_amd.SIZING_LEARNED, train_scenes=200, autotune=1),
-        "plain": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X),
+        "plain": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X, tails=2),
     }
     out, digests = {}, {}
     for name, kw in forms.items():
@@ -163,6 +175,54 @@ def bench_batched(args, H):
             "per_scene_digests_equal": exact}
 
 
+def bench_skewed(args, H):
+    """8M boxes around 64 Gaussian clusters (sigma 0.02) with Pareto(1.5)
+    half-extents (min 5e-5, cap 0.05): a few very long chains.  The chosen
+    form (reduced precision, autotuned, tail items) against plain FP64 with
+    default parameters, both count-then-fill; the pair sets are compared."""
+    import numpy as np
+    import stq_amd
+    from stq_amd import _lib, scenes
+    b = scenes.clustered_scene()
+    n = len(b)
+    db = _lib.DeviceBuffer.from_array(b)
+    del b
+    forms = {"chosen": dict(precision=stq_amd.PRECISION_F32_CONFIRM, autotune=1),
+             "plain": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X, tails=2)}
+    out, dig = {}, {}
+    for name, kw in forms.items():
+        ctx = stq_amd.Context(0, **kw)
+        try:
+            ctx.find_pairs_device(db.ptr, n, None, 0)
+        except stq_amd.StqError as e:
+            if e.status != _lib.STQ_ERR_CAPACITY:
+                raise
+        tune = ctx.stats()["ms_tune"]
+        cap = ctx.stats()["n_pairs"]
+        dout = _lib.DeviceBuffer(8 * max(cap, 1))
+        for _ in range(max(args.warmup, 1)):
+            ctx.find_pairs_device(db.ptr, n, dout.ptr, cap)
+        ms = []
+        for _ in range(args.steps):
+            ctx.find_pairs_device(db.ptr, n, dout.ptr, cap)
+            ms.append(ctx.stats()["ms_total"])
+        st = ctx.stats()
+        t = float(np.median(ms))
+        out[name] = dict(ms=t, boxes_per_s=n / (t * 1e-3), pairs=cap, candidates=st["n_candidates"],
+                         lane_steps=st["n_lane_steps"], items=st["n_items"], tune_ms=tune,
+                         autotuned=tuned_table(ctx) if kw.get("autotune") else None)
+        if not args.no_check:
+            dig[name] = pair_digest(dout.to_array(np.int32, 2 * cap).reshape(-1, 2))
+        dout.close()
+        ctx.close()
+    db.close()
+    return {"workload": f"{n} boxes, 64 Gaussian clusters (sigma 0.02), Pareto(1.5) half-extents in "
+                        f"[5e-5, 0.05] (seed 2), device-resident",
+            "chosen": out["chosen"], "plain_fp64": out["plain"],
+            "speedup_vs_plain_fp64": out["chosen"]["boxes_per_s"] / out["plain"]["boxes_per_s"],
+            "pair_sets_equal": (dig["chosen"] == dig["plain"]) if len(dig) == 2 else None}
+
+
 def bench_stream(args, H):
     """The learned-sizing stream: scenes of 5e4 to 4e6 boxes, density factor
     log-uniform in [0.25, 8].  The first 200 run count-then-fill and train the
@@ -176,7 +236,7 @@ def bench_stream(args, H):
     ntrain = min(200, args.stream_scenes)
     ctx = stq_amd.Context(0, precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN,
                           sizing=stq_amd.SIZING_LEARNED, train_scenes=ntrain)
-    plain = stq_amd.Context(0, precision=stq_amd.PRECISION_F64)
+    plain = stq_amd.Context(0, precision=stq_amd.PRECISION_F64, tails=2)
     cap = 1 << 26
     dout = _lib.DeviceBuffer(8 * cap)
     rows = []
@@ -256,7 +316,7 @@ def main():
                        sizing=stq_amd.SIZING_LEARNED, train_scenes=max(args.train_scenes, 1),
                        autotune=1),
         "plain": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X,
-                      sizing=stq_amd.SIZING_COUNT_FILL),
+                      sizing=stq_amd.SIZING_COUNT_FILL, tails=2),
     }
     want = [f for f in args.forms.split(",") if f]
     res = {}
@@ -348,7 +408,16 @@ def main():
         t1 = time.perf_counter()
         sweep_s = (t1 - t0) - sort_s
         est = sort_s + sweep_s * (n / nrows)
-        cpu = {"value": n / est, "unit": "boxes/s", "cores": threads, "kind": "port",
+        one = None
+        if args.cpu_sample_1t > 0:
+            nr1 = max(1, int(n * args.cpu_sample_1t))
+            t0 = time.perf_counter()
+            _, sort1 = oracle.sweep(boxes, axis=0, nthreads=1, nrows=nr1, return_sort_time=True)
+            sw1 = time.perf_counter() - t0 - sort1
+            one = {"value": n / (sort1 + sw1 * (n / nr1)), "unit": "boxes/s",
+                   "sample": f"sort of all boxes ({sort1:.2f} s) + sweep of the first {nr1} sorted rows "
+                             f"({sw1:.2f} s), scaled to all rows"}
+        cpu = {"value": n / est, "unit": "boxes/s", "cores": threads, "kind": "port", "one_thread": one,
                "sample": (f"oracle restatement of src/cpu/sweep.cpp (FP64, axis x): parallel sort of all "
                           f"{n} boxes ({sort_s:.2f} s) + chain sweep of the first {nrows} sorted rows "
                           f"({sweep_s:.2f} s, {len(pairs)} pairs), scaled to all rows; "
@@ -391,6 +460,22 @@ def main():
                      "kernel_ms": r["fill_sweep_ms"]},
         "cpu_baseline": cpu,
     }
+    if "chosen" in forms and "chosen" in want:
+        # PCIe-inclusive: host boxes in, host pairs out (not the benchmark value)
+        c2 = stq_amd.Context(0, **forms["chosen"])
+        c2.set_options(**{**forms["chosen"], "sizing": stq_amd.SIZING_COUNT_FILL})
+        c2.find_pairs_host(boxes)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            c2.find_pairs_host(boxes)
+            ts.append(time.perf_counter() - t0)
+        c2.close()
+        line["end_to_end_host"] = {"ms": 1e3 * float(np.median(ts)),
+                                   "what": "stq_find_pairs_host, reduced precision, count then fill: "
+                                           "H2D of the boxes, the call, D2H of the pairs"}
+    if args.skewed:
+        line["skewed"] = bench_skewed(args, H)
     if args.batched_scenes > 0:
         line["batched"] = bench_batched(args, H)
     if args.stream_scenes > 0:

@@ -8,7 +8,8 @@
 // minimises
 //     (hi_k - q0 + 1)                                  body: 64 lanes per step
 //   + tail_cost * sum_{j > k} (hi_j - hi_k) / 64      tails: one row per lane-step
-// The body is swept as before over [q0, hi_k]; the rows above k become a tail
+//   + kTailFixed if k is not the last row              the tail item's own set-up
+// (all in wave steps).  The body is swept as before over [q0, hi_k]; the rows above k become a tail
 // item over (hi_k, max hi], swept with positions across the lanes and the
 // rows taken in turn.  Without long rows the cut is the last row (no tail).
 #pragma once
@@ -22,6 +23,7 @@ namespace stqamd {
 constexpr int kTailFlag = 1 << 30;  // in Item::nrows: a tail item
 constexpr int kRowsMask = 127;      // Item::nrows & kRowsMask = rows of the item
 constexpr int kCapChunks = 64;      // capped planning: at most this many items per body or tail
+constexpr double kTailFixed = 256.0;  // wave steps charged for making a tail item at all
 
 // Group record: body {rowoff, nrows, q0, q1}, tail {rowoff, rows, q0, q1}
 // (rows = 0: none), chunk sizes {body, tail}.
@@ -63,7 +65,7 @@ __device__ inline GroupPlan plan_group(int lane, int nrows, int p, int h, int ro
     if (act) {
       const double body = (double)max((long long)h - q0 + 1, 0ll);
       const double tail = (double)(tot - pre) - (double)(nrows - 1 - lane) * (double)h;
-      cost = body + (double)tail_cost * tail * (1.0 / 64.0);
+      cost = body + (double)tail_cost * tail * (1.0 / 64.0) + (lane < nrows - 1 ? kTailFixed : 0.0);
     }
     double cmin = cost;
 #pragma unroll
