@@ -64,6 +64,8 @@ def parse():
                     help="run the clustered heavy-tailed 8M-box leg (0 = skip)")
     ap.add_argument("--cpu-sample-1t", type=float, default=0.02,
                     help="fraction of sorted rows the 1-thread CPU timing sweeps (0 = skip)")
+    ap.add_argument("--host-e2e", type=int, default=1,
+                    help="time the PCIe-inclusive host-to-host call (0 = skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (traffic calibration)")
     return ap.parse_args()
@@ -460,7 +462,7 @@ def main():
                      "kernel_ms": r["fill_sweep_ms"]},
         "cpu_baseline": cpu,
     }
-    if "chosen" in forms and "chosen" in want:
+    if args.host_e2e and "chosen" in want:
         # PCIe-inclusive: host boxes in, host pairs out (not the benchmark value)
         c2 = stq_amd.Context(0, **forms["chosen"])
         c2.set_options(**{**forms["chosen"], "sizing": stq_amd.SIZING_COUNT_FILL})
