@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "build.hpp"
 #include "context.hpp"
 #include "plan.hpp"
 
@@ -1006,6 +1007,61 @@ int stq_find_pairs_batched_device(void *ctx, const stq_aabb_f64 *d_boxes,
     stqamd::tuned_call(c, d_boxes, nb, h_scene_offsets, nscenes, true, [&] {
       c->run_batched(d_boxes, h_scene_offsets, nscenes, d_out, capacity, h_pair_offsets);
     });
+  });
+}
+
+static bool mesh_args_valid(const double *v0, const double *v1, int64_t nv, const int32_t *e,
+                            int64_t ne, const int32_t *f, int64_t nf) {
+  if (nv < 0 || ne < 0 || nf < 0) return false;
+  if (nv > 0 && (!v0 || !v1)) return false;
+  if ((ne > 0 && !e) || (nf > 0 && !f)) return false;
+  return true;
+}
+
+// Builds the boxes on the context's stream and checks the index flag.
+static void build_boxes(Context *c, const double *v0, const double *v1, int64_t nv,
+                        const int32_t *e, int64_t ne, const int32_t *f, int64_t nf, double r,
+                        bool float_build, stq_aabb_f64 *out) {
+  unsigned *err = reinterpret_cast<unsigned *>(
+      c->counters.as<unsigned long long>(stqamd::kCounters) + stqamd::kCounters - 1);
+  STQ_HIP(hipMemsetAsync(err, 0, sizeof(unsigned), c->stream));
+  stqamd::launch_build_boxes(v0, v1, nv, e, ne, f, nf, r, float_build, out, err, c->stream);
+  unsigned h = 0;
+  STQ_HIP(hipMemcpyAsync(&h, err, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  STQ_HIP(hipStreamSynchronize(c->stream));
+  if (h) throw stqamd::StatusError(STQ_ERR_INVALID_ARGUMENT, "edge or face index outside [0, nv)");
+}
+
+int stq_build_swept_boxes_device(void *ctx, const double *d_v0, const double *d_v1, int64_t nv,
+                                 const int32_t *d_edges, int64_t ne, const int32_t *d_faces,
+                                 int64_t nf, double inflation_radius, int32_t float_build,
+                                 stq_aabb_f64 *d_boxes) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !mesh_args_valid(d_v0, d_v1, nv, d_edges, ne, d_faces, nf) ||
+      (nv + ne + nf > 0 && !d_boxes) || float_build < 0 || float_build > 1)
+    return STQ_ERR_INVALID_ARGUMENT;
+  if (nv + ne + nf > INT32_MAX || nv > INT32_MAX) return STQ_ERR_TOO_LARGE;
+  return stqamd::guarded(c, [&] {
+    build_boxes(c, d_v0, d_v1, nv, d_edges, ne, d_faces, nf, inflation_radius, float_build != 0,
+                d_boxes);
+  });
+}
+
+int stq_find_pairs_mesh_device(void *ctx, const double *d_v0, const double *d_v1, int64_t nv,
+                               const int32_t *d_edges, int64_t ne, const int32_t *d_faces,
+                               int64_t nf, double inflation_radius, stq_pair *d_out,
+                               int64_t capacity, int64_t *count) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !count || !mesh_args_valid(d_v0, d_v1, nv, d_edges, ne, d_faces, nf) || capacity < 0 ||
+      (capacity > 0 && !d_out))
+    return STQ_ERR_INVALID_ARGUMENT;
+  const int64_t n = nv + ne + nf;
+  if (n > INT32_MAX) return STQ_ERR_TOO_LARGE;
+  return stqamd::guarded(c, [&] {
+    stq_aabb_f64 *boxes = c->mesh_boxes.as<stq_aabb_f64>((size_t)std::max<int64_t>(n, 1));
+    build_boxes(c, d_v0, d_v1, nv, d_edges, ne, d_faces, nf, inflation_radius, false, boxes);
+    stqamd::tuned_call(c, boxes, n, nullptr, 1, false,
+                       [&] { c->run(boxes, n, d_out, capacity, nullptr, count); });
   });
 }
 

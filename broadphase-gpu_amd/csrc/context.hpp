@@ -121,7 +121,7 @@ struct Context {
   int64_t scenes_seen = 0;
   bool learned_pass = false;  // the last call used the single learned pass
   // batched pipeline
-  DevBuf b_tables, b_skey, b_orig, b_pairoff;
+  DevBuf b_tables, b_skey, b_orig, b_pairoff, mesh_boxes;
   PinnedBuf b_host;
   BatchPlan bplan;
   int64_t b_budget = 0;      // scratch pairs of the learned batched pass
@@ -134,7 +134,7 @@ struct Context {
   ~Context() {
     bplan.drop_graph();
     b_host.release();
-    DevBuf *bb[] = {&b_tables, &b_skey, &b_orig, &b_pairoff, &tgroups, &gchunks};
+    DevBuf *bb[] = {&b_tables, &b_skey, &b_orig, &b_pairoff, &tgroups, &gchunks, &mesh_boxes};
     for (DevBuf *b : bb) b->release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);

@@ -77,6 +77,56 @@ class Context:
         o = opts if opts is not None else options(**kw)
         self._check(self._L.stq_set_options(self.ptr, ctypes.byref(o)), "stq_set_options")
 
+    # -- meshes ----------------------------------------------------------------
+    @staticmethod
+    def _mesh_buffers(V0, V1, E, F):
+        v0 = np.ascontiguousarray(V0, dtype=np.float64).reshape(-1, 3)
+        v1 = np.ascontiguousarray(V1, dtype=np.float64).reshape(-1, 3)
+        e = np.ascontiguousarray(E, dtype=np.int32).reshape(-1, 2)
+        f = np.ascontiguousarray(F, dtype=np.int32).reshape(-1, 3)
+        assert v0.shape == v1.shape
+        bufs = [DeviceBuffer.from_array(a) for a in (v0, v1, e, f)]
+        return bufs, (len(v0), len(e), len(f))
+
+    def build_swept_boxes(self, V0, V1, E, F, radius=0.0, float_build=False) -> np.ndarray:
+        """Swept boxes of a moving mesh built on the GPU (vertices, edges, faces)."""
+        bufs, (nv, ne, nf) = self._mesh_buffers(V0, V1, E, F)
+        n = nv + ne + nf
+        out = DeviceBuffer(72 * max(n, 1))
+        try:
+            self._check(self._L.stq_build_swept_boxes_device(
+                self.ptr, bufs[0].ptr, bufs[1].ptr, nv, bufs[2].ptr, ne, bufs[3].ptr, nf, float(radius),
+                int(bool(float_build)), out.ptr), "stq_build_swept_boxes_device")
+            return out.to_array(AABB_DTYPE, n)
+        finally:
+            out.close()
+            for b in bufs:
+                b.close()
+
+    def find_pairs_mesh(self, V0, V1, E, F, radius=0.0) -> np.ndarray:
+        """Pairs of the mesh's swept boxes (ids = box positions: vertices,
+        then edges, then faces)."""
+        bufs, (nv, ne, nf) = self._mesh_buffers(V0, V1, E, F)
+        cap = 1 << 20
+        try:
+            while True:
+                dout = DeviceBuffer(8 * cap)
+                cnt = ctypes.c_int64(0)
+                rc = self._L.stq_find_pairs_mesh_device(self.ptr, bufs[0].ptr, bufs[1].ptr, nv, bufs[2].ptr,
+                                                        ne, bufs[3].ptr, nf, float(radius), dout.ptr, cap,
+                                                        ctypes.byref(cnt))
+                if rc == _lib.STQ_ERR_CAPACITY:
+                    cap = int(cnt.value)
+                    dout.close()
+                    continue
+                self._check(rc, "stq_find_pairs_mesh_device")
+                res = dout.to_array(np.int32, 2 * cnt.value).reshape(-1, 2)
+                dout.close()
+                return res
+        finally:
+            for b in bufs:
+                b.close()
+
     # -- autotuning table ------------------------------------------------------
     def autotune_save(self, path):
         self._check(self._L.stq_autotune_save(self.ptr, os.fsencode(path)), "stq_autotune_save")

@@ -31,7 +31,8 @@ SIZING_LEARNED = 1
 EXPORTS = [
     "stq_options_default", "stq_create", "stq_destroy", "stq_set_options",
     "stq_find_pairs_device", "stq_find_pairs_device_alloc", "stq_find_pairs_host",
-    "stq_find_pairs_batched_device", "stq_autotune_save", "stq_autotune_load",
+    "stq_find_pairs_batched_device", "stq_build_swept_boxes_device", "stq_find_pairs_mesh_device",
+    "stq_autotune_save", "stq_autotune_load",
     "stq_autotune_clear", "stq_autotune_entries", "stq_device_alloc", "stq_copy_to_device",
     "stq_copy_to_host", "stq_free_device", "stq_free_host",
     "stq_sizer_num_params", "stq_sizer_num_features", "stq_sizer_get_params",
@@ -117,6 +118,10 @@ def lib():
         "stq_find_pairs_device_alloc": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
         "stq_find_pairs_host": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
         "stq_find_pairs_batched_device": (ctypes.c_int, [P, P, P, I32, P, I64, P]),
+        "stq_build_swept_boxes_device": (ctypes.c_int, [P, P, P, I64, P, I64, P, I64, ctypes.c_double,
+                                                         I32, P]),
+        "stq_find_pairs_mesh_device": (ctypes.c_int, [P, P, P, I64, P, I64, P, I64, ctypes.c_double, P,
+                                                       I64, ctypes.POINTER(I64)]),
         "stq_autotune_save": (ctypes.c_int, [P, ctypes.c_char_p]),
         "stq_autotune_load": (ctypes.c_int, [P, ctypes.c_char_p]),
         "stq_autotune_clear": (ctypes.c_int, [P]),

@@ -222,6 +222,32 @@ int stq_find_pairs_batched_device(void *ctx, const stq_aabb_f64 *d_boxes,
                                   int32_t nscenes, stq_pair *d_out,
                                   int64_t capacity, int64_t *h_pair_offsets);
 
+/* ---- swept boxes of a moving triangle mesh ----
+   Replaces the reference's constructBoxes (src/io.cu:15-28) with addVertices,
+   addEdges and addFaces (src/aabb.cu:63-174), which run on the host there.
+   d_v0/d_v1: nv vertices at t0 and t1, row-major (x, y, z) doubles;
+   d_edges: ne pairs of vertex indices; d_faces: nf triples.  Writes
+   nv + ne + nf boxes to d_boxes: vertices, then edges, then faces; box s has
+   id s and ref_id = its index within its kind; vertex ids follow the
+   reference convention.  Bounds cover both positions of every vertex of the
+   simplex, inflated by `inflation_radius`:
+     float_build = 0: FP64 (the reference built with CCD_USE_DOUBLE);
+     float_build = 1: the reference's FP32 build, bit for bit (rounded to
+                      float, one ulp outward as nextafterf, radius rounded
+                      up, float arithmetic), stored as doubles.
+   STQ_ERR_INVALID_ARGUMENT if an edge or face index is outside [0, nv). */
+int stq_build_swept_boxes_device(void *ctx, const double *d_v0, const double *d_v1, int64_t nv,
+                                 const int32_t *d_edges, int64_t ne, const int32_t *d_faces,
+                                 int64_t nf, double inflation_radius, int32_t float_build,
+                                 stq_aabb_f64 *d_boxes);
+/* The boxes above (FP64 build) straight into stq_find_pairs_device: pair ids
+   are box positions (vertices, then edges, then faces).  The boxes live in
+   the context's scratch. */
+int stq_find_pairs_mesh_device(void *ctx, const double *d_v0, const double *d_v1, int64_t nv,
+                               const int32_t *d_edges, int64_t ne, const int32_t *d_faces,
+                               int64_t nf, double inflation_radius, stq_pair *d_out,
+                               int64_t capacity, int64_t *count);
+
 /* ---- autotuning (no counterpart in the reference, whose setup(),
    src/util.cu:9-55, derives one block size from device attributes) ----
    With options.autotune = 1, the first call of each shape class (precision,

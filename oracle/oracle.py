@@ -53,6 +53,9 @@ def lib():
         L.oracle_pair_predicate.argtypes = [P, P]
         L.oracle_free.restype = None
         L.oracle_free.argtypes = [P]
+        L.oracle_build_swept_boxes.restype = ctypes.c_int
+        L.oracle_build_swept_boxes.argtypes = [P, P, I64, P, I64, P, I64, ctypes.c_double,
+                                               ctypes.c_int, P]
         L.oracle_sizeof_aabb.restype = ctypes.c_int
         L.oracle_max_threads.restype = ctypes.c_int
         assert L.oracle_sizeof_aabb() == 72
@@ -137,3 +140,21 @@ def pair_predicate(boxes, i, j):
 def reference_set(boxes, axis=0, nthreads=None):
     """The canonical FP64 pair set of the reference sweep."""
     return canonical(sweep(boxes, axis=axis, nthreads=nthreads))
+
+
+def build_swept_boxes(V0, V1, E, F, radius=0.0, float_build=False):
+    """Swept boxes of a mesh (vertices, then edges, then faces) as the
+    reference's constructBoxes builds them (src/io.cu:15-28,
+    src/aabb.cu:56-174); float_build reproduces its float build."""
+    from stq_amd.scenes import AABB_DTYPE
+    v0 = np.ascontiguousarray(V0, dtype=np.float64).reshape(-1, 3)
+    v1 = np.ascontiguousarray(V1, dtype=np.float64).reshape(-1, 3)
+    e = np.ascontiguousarray(E, dtype=np.int32).reshape(-1, 2)
+    f = np.ascontiguousarray(F, dtype=np.int32).reshape(-1, 3)
+    out = np.zeros(len(v0) + len(e) + len(f), dtype=AABB_DTYPE)
+    rc = lib().oracle_build_swept_boxes(v0.ctypes.data, v1.ctypes.data, len(v0), e.ctypes.data, len(e),
+                                        f.ctypes.data, len(f), float(radius), int(bool(float_build)),
+                                        out.ctypes.data)
+    if rc != 0:
+        raise ValueError("edge or face index outside the vertex range")
+    return out

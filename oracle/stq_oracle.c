@@ -316,3 +316,87 @@ int64_t oracle_canonicalize(int32_t *pairs, int64_t n) {
 void oracle_free(void *p) { free(p); }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+/* ---- swept-box construction: src/aabb.cu:56-174, src/io.cu:15-28 ------------
+ * constructBoxes appends vertex boxes, then edge boxes, then face boxes
+ * (io.cu:25-27).  Box i of a kind covers the t0 and t1 positions of its
+ * simplex's vertices (2, 4 or 6 points), inflated by r:
+ *   FP64 build (CCD_USE_DOUBLE):  min = min(points) - r, max = max(points) + r
+ *   float build:  min = nextafter_down((float)min(points)) - nextafter_up((float)r)
+ *                 max = nextafter_up((float)max(points))  + nextafter_up((float)r)
+ *                 in float arithmetic, nextafter_up/down as in src/aabb.cu:56-61
+ * vertexIds: vertex {i, -i-1, -i-1} (aabb.cu:113), edge {e0, e1, -e0-1}
+ * (aabb.cu:80), face {f0, f1, f2} (aabb.cu:157); id = position of the box in
+ * the output, ref_id = i (the index within its kind; aabb.cu:93,127,169).
+ * Vertices are row-major (x, y, z) per vertex.  Returns 0, or -1 when an edge
+ * or face index is outside [0, nv). */
+static float ref_next_up(float x) { return nextafterf(x, x + 3.40282347e+38f); }
+static float ref_next_down(float x) { return nextafterf(x, x - 3.40282347e+38f); }
+
+static void hull_box(const double *v0, const double *v1, const int32_t *idx, int k, double r,
+                     int float_build, double *mn, double *mx) {
+  for (int a = 0; a < 3; ++a) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int j = 0; j < k; ++j) {
+      const double p = v0[3 * (int64_t)idx[j] + a], q = v1[3 * (int64_t)idx[j] + a];
+      lo = fmin(lo, fmin(p, q));
+      hi = fmax(hi, fmax(p, q));
+    }
+    if (float_build) {
+      const float rf = ref_next_up((float)r);
+      volatile float l = ref_next_down((float)lo) - rf; /* float arithmetic */
+      volatile float h = ref_next_up((float)hi) + rf;
+      mn[a] = (double)l;
+      mx[a] = (double)h;
+    } else {
+      mn[a] = lo - r;
+      mx[a] = hi + r;
+    }
+  }
+}
+
+int oracle_build_swept_boxes(const double *v0, const double *v1, int64_t nv, const int32_t *edges,
+                             int64_t ne, const int32_t *faces, int64_t nf, double r,
+                             int float_build, oracle_aabb *out) {
+  for (int64_t i = 0; i < ne; ++i)
+    for (int j = 0; j < 2; ++j)
+      if (edges[2 * i + j] < 0 || edges[2 * i + j] >= nv) return -1;
+  for (int64_t i = 0; i < nf; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (faces[3 * i + j] < 0 || faces[3 * i + j] >= nv) return -1;
+  const int64_t n = nv + ne + nf;
+#pragma omp parallel for schedule(static)
+  for (int64_t s = 0; s < n; ++s) {
+    oracle_aabb *b = &out[s];
+    memset(b, 0, sizeof(*b));
+    b->id = (int32_t)s;
+    int32_t idx[3];
+    int k;
+    if (s < nv) {
+      const int32_t i = (int32_t)s;
+      idx[0] = i;
+      k = 1;
+      b->vids[0] = i;
+      b->vids[1] = -i - 1;
+      b->vids[2] = -i - 1;
+      b->ref_id = i;
+    } else if (s < nv + ne) {
+      const int64_t i = s - nv;
+      idx[0] = edges[2 * i];
+      idx[1] = edges[2 * i + 1];
+      k = 2;
+      b->vids[0] = idx[0];
+      b->vids[1] = idx[1];
+      b->vids[2] = -idx[0] - 1;
+      b->ref_id = (int32_t)i;
+    } else {
+      const int64_t i = s - nv - ne;
+      for (int j = 0; j < 3; ++j) idx[j] = faces[3 * i + j];
+      k = 3;
+      for (int j = 0; j < 3; ++j) b->vids[j] = idx[j];
+      b->ref_id = (int32_t)i;
+    }
+    hull_box(v0, v1, idx, k, r, float_build, b->min, b->max);
+  }
+  return 0;
+}

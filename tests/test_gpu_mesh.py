@@ -1,0 +1,89 @@
+"""Swept boxes of a moving mesh built on the GPU, against the oracle's
+restatement of the reference construction (bit for bit, both builds), and
+the fused mesh-to-pairs call against the pair-set oracle."""
+import numpy as np
+import pytest
+
+import stq_amd
+from stq_amd import _lib, scenes
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+SPECIAL = np.array([0.0, -0.0, 1e-45, -1e-45, 1e-40, 0.1, -0.1, 1.0, 3.0e38, -3.0e38, 3.5e38, -3.5e38,
+                    1e300, -1e300, 16777217.0, 0.333333333333], dtype=np.float64)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = stq_amd.Context(0)
+    yield c
+    c.close()
+
+
+def _mesh(seed, nv=3000):
+    rng = np.random.default_rng(seed)
+    V0 = rng.uniform(-2, 2, (nv, 3))
+    V0.ravel()[:len(SPECIAL)] = SPECIAL
+    V1 = V0 + rng.normal(0, 1e-2, V0.shape)
+    V1.ravel()[len(SPECIAL):2 * len(SPECIAL)] = SPECIAL[::-1]
+    E = rng.integers(0, nv, (5000, 2))
+    F = rng.integers(0, nv, (4000, 3))
+    return V0, V1, E, F
+
+
+def _same_boxes(a, b):
+    for k in ("id", "vids", "ref_id"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("min", "max"):  # bit patterns (NaN-free here; -0.0 vs +0.0 distinguished)
+        assert np.array_equal(a[k].view(np.uint64), b[k].view(np.uint64)), k
+
+
+@pytest.mark.parametrize("float_build", [False, True])
+@pytest.mark.parametrize("radius", [0.0, 1e-3, 1e-45])
+def test_build_matches_oracle(ctx, float_build, radius):
+    V0, V1, E, F = _mesh(4)
+    got = ctx.build_swept_boxes(V0, V1, E, F, radius=radius, float_build=float_build)
+    ref = oracle.build_swept_boxes(V0, V1, E, F, radius=radius, float_build=float_build)
+    _same_boxes(got, ref)
+
+
+def test_build_rejects_bad_indices(ctx):
+    V0, V1, E, F = _mesh(5, nv=100)
+    E, F = E % 100, F % 100
+    E[7, 0] = 100
+    with pytest.raises(stq_amd.StqError) as e:
+        ctx.build_swept_boxes(V0, V1, E, F)
+    assert e.value.status == _lib.STQ_ERR_INVALID_ARGUMENT
+    E[7, 0] = 0
+    F[9, 2] = -3
+    with pytest.raises(stq_amd.StqError):
+        ctx.find_pairs_mesh(V0, V1, E, F)
+    # the context stays usable
+    F[9, 2] = 0
+    assert len(ctx.find_pairs_mesh(V0, V1, E, F)) >= 0
+
+
+@pytest.mark.parametrize("prec", [stq_amd.PRECISION_F64, stq_amd.PRECISION_F32_CONFIRM])
+def test_mesh_pairs(ctx, prec):
+    """Cloth mesh 150 x 150 with an inflation radius: the fused call equals the
+    pair-set oracle on the oracle-built boxes."""
+    V0, E, F = scenes.cloth_mesh(150)
+    V1 = V0 + np.random.default_rng(8).normal(0, 4e-3, V0.shape)
+    ctx.set_options(precision=prec)
+    got = ctx.find_pairs_mesh(V0, V1, E, F, radius=1e-3)
+    ref = oracle.reference_set(oracle.build_swept_boxes(V0, V1, E, F, radius=1e-3))
+    assert len(ref) > 10000
+    c = oracle.canonical(got)
+    assert len(c) == len(got) and np.array_equal(c, ref)
+
+
+def test_mesh_pairs_full_cloth(ctx):
+    """The 708 x 708 cloth from its mesh equals the box path on the same boxes,
+    pair for pair and in order."""
+    V0, E, F = scenes.cloth_mesh(708)
+    V1 = V0 + np.random.default_rng(1).normal(0.0, 2e-3, size=V0.shape)
+    ctx.set_options(precision=stq_amd.PRECISION_F32_CONFIRM)
+    got = ctx.find_pairs_mesh(V0, V1, E, F)
+    boxes = scenes.cloth_scene()
+    assert np.array_equal(got, ctx.find_pairs(boxes))

@@ -156,3 +156,94 @@ def test_candidates_count_matches_definition():
     hi = np.searchsorted(smin, smax, side="right") - 1
     lens = hi - np.arange(len(b))
     assert tot == int(lens.sum()) and mc == int(lens.max())
+
+
+# ---- swept-box construction (src/io.cu:15-28, src/aabb.cu:56-174) -----------------
+SPECIAL = np.array([0.0, -0.0, 1e-45, -1e-45, 1e-40, 0.1, -0.1, 1.0, 3.0e38, -3.0e38, 3.5e38, -3.5e38,
+                    1e300, -1e300, 16777217.0, 0.333333333333], dtype=np.float64)
+
+
+def _mesh(seed, nv=300):
+    rng = np.random.default_rng(seed)
+    V0 = rng.uniform(-2, 2, (nv, 3))
+    V0.ravel()[:len(SPECIAL)] = SPECIAL
+    V1 = V0 + rng.normal(0, 1e-2, V0.shape)
+    V1.ravel()[len(SPECIAL):2 * len(SPECIAL)] = SPECIAL[::-1]
+    E = rng.integers(0, nv, (500, 2))
+    F = rng.integers(0, nv, (400, 3))
+    return V0, V1, E, F
+
+
+def test_build_fp64_matches_hull():
+    """FP64 build: exactly min/max over the simplex's t0/t1 points -/+ r, the
+    id/vertex-id/ref_id conventions, vertices then edges then faces."""
+    V0, V1, E, F = _mesh(1)
+    for r in (0.0, 1e-3):
+        b = oracle.build_swept_boxes(V0, V1, E, F, radius=r)
+        nv, ne = len(V0), len(E)
+        assert len(b) == nv + ne + len(F)
+        assert np.array_equal(b["id"], np.arange(len(b)))
+        lo = np.minimum(V0, V1)
+        hi = np.maximum(V0, V1)
+        for s0, idx in ((0, np.arange(nv)[:, None]), (nv, E), (nv + ne, F)):
+            m = len(idx)
+            assert np.array_equal(b["min"][s0:s0 + m], lo[idx].min(axis=1) - r)
+            assert np.array_equal(b["max"][s0:s0 + m], hi[idx].max(axis=1) + r)
+            assert np.array_equal(b["ref_id"][s0:s0 + m], np.arange(m))
+        i = np.arange(nv)
+        assert np.array_equal(b["vids"][:nv], np.stack([i, -i - 1, -i - 1], 1))
+        assert np.array_equal(b["vids"][nv:nv + ne], np.stack([E[:, 0], E[:, 1], -E[:, 0] - 1], 1))
+        assert np.array_equal(b["vids"][nv + ne:], F)
+    # r = 0 equals the scene generator's own construction
+    Vc, Ec, Fc = scenes.cloth_mesh(30)
+    Vc1 = Vc + np.random.default_rng(3).normal(0, 2e-3, Vc.shape)
+    a = oracle.build_swept_boxes(Vc, Vc1, Ec, Fc)
+    g = scenes.swept_boxes(Vc, Vc1, Ec, Fc)
+    for k in ("id", "min", "max", "vids"):
+        assert np.array_equal(a[k], g[k])
+
+
+def _np_next(x, up):
+    big = np.float32(3.40282347e38)
+    with np.errstate(over="ignore", invalid="ignore"):
+        return np.nextafter(x, (x + big) if up else (x - big)).astype(np.float32)
+
+
+def test_build_float_matches_numpy_nextafter():
+    """Float build against numpy's float32 nextafter and float32 arithmetic
+    (src/aabb.cu:56-61,86-91,120-125,162-167), special values included."""
+    V0, V1, E, F = _mesh(2)
+    nv, ne = len(V0), len(E)
+    lo = np.minimum(V0, V1)
+    hi = np.maximum(V0, V1)
+    for r in (0.0, 1e-3, 1e-45):
+        b = oracle.build_swept_boxes(V0, V1, E, F, radius=r, float_build=True)
+        with np.errstate(over="ignore", invalid="ignore"):
+            rf = _np_next(np.float32(r), True)
+            for s0, idx in ((0, np.arange(nv)[:, None]), (nv, E), (nv + ne, F)):
+                m = len(idx)
+                wl = _np_next(lo[idx].min(axis=1).astype(np.float32), False) - rf
+                wh = _np_next(hi[idx].max(axis=1).astype(np.float32), True) + rf
+                got_l = b["min"][s0:s0 + m]
+                got_h = b["max"][s0:s0 + m]
+                assert np.array_equal(got_l, wl.astype(np.float64)), r
+                assert np.array_equal(got_h, wh.astype(np.float64)), r
+    # outward (for finite float images and r = 0): the float box contains the FP64 box
+    b64 = oracle.build_swept_boxes(V0, V1, E, F)
+    b32 = oracle.build_swept_boxes(V0, V1, E, F, float_build=True)
+    fin = np.isfinite(b32["min"]) & np.isfinite(b32["max"]) & (np.abs(b64["min"]) < 3.4e38) & \
+        (np.abs(b64["max"]) < 3.4e38)
+    assert np.all(b32["min"][fin] <= b64["min"][fin]) and np.all(b32["max"][fin] >= b64["max"][fin])
+
+
+def test_build_rejects_bad_indices():
+    V0, V1, E, F = _mesh(3, nv=20)
+    E = E % 20
+    F = F % 20
+    E[3, 1] = 20
+    with pytest.raises(ValueError):
+        oracle.build_swept_boxes(V0, V1, E, F)
+    E[3, 1] = 0
+    F[2, 0] = -1
+    with pytest.raises(ValueError):
+        oracle.build_swept_boxes(V0, V1, E, F)
