@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "build.hpp"
+#include "narrow.hpp"
 #include "context.hpp"
 #include "plan.hpp"
 
@@ -1010,6 +1011,18 @@ int stq_find_pairs_batched_device(void *ctx, const stq_aabb_f64 *d_boxes,
   });
 }
 
+static unsigned *flag_word(Context *c) {
+  return reinterpret_cast<unsigned *>(
+      c->counters.as<unsigned long long>(stqamd::kCounters) + stqamd::kCounters - 1);
+}
+
+static void check_flag(Context *c, unsigned *err, const char *what) {
+  unsigned h = 0;
+  STQ_HIP(hipMemcpyAsync(&h, err, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  STQ_HIP(hipStreamSynchronize(c->stream));
+  if (h) throw stqamd::StatusError(STQ_ERR_INVALID_ARGUMENT, what);
+}
+
 static bool mesh_args_valid(const double *v0, const double *v1, int64_t nv, const int32_t *e,
                             int64_t ne, const int32_t *f, int64_t nf) {
   if (nv < 0 || ne < 0 || nf < 0) return false;
@@ -1022,14 +1035,10 @@ static bool mesh_args_valid(const double *v0, const double *v1, int64_t nv, cons
 static void build_boxes(Context *c, const double *v0, const double *v1, int64_t nv,
                         const int32_t *e, int64_t ne, const int32_t *f, int64_t nf, double r,
                         bool float_build, stq_aabb_f64 *out) {
-  unsigned *err = reinterpret_cast<unsigned *>(
-      c->counters.as<unsigned long long>(stqamd::kCounters) + stqamd::kCounters - 1);
+  unsigned *err = flag_word(c);
   STQ_HIP(hipMemsetAsync(err, 0, sizeof(unsigned), c->stream));
   stqamd::launch_build_boxes(v0, v1, nv, e, ne, f, nf, r, float_build, out, err, c->stream);
-  unsigned h = 0;
-  STQ_HIP(hipMemcpyAsync(&h, err, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  STQ_HIP(hipStreamSynchronize(c->stream));
-  if (h) throw stqamd::StatusError(STQ_ERR_INVALID_ARGUMENT, "edge or face index outside [0, nv)");
+  check_flag(c, err, "edge or face index outside [0, nv)");
 }
 
 int stq_build_swept_boxes_device(void *ctx, const double *d_v0, const double *d_v1, int64_t nv,
@@ -1064,6 +1073,37 @@ int stq_find_pairs_mesh_device(void *ctx, const double *d_v0, const double *d_v1
                        [&] { c->run(boxes, n, d_out, capacity, nullptr, count); });
   });
 }
+
+int stq_orient_pairs_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n, stq_pair *d_pairs,
+                            int64_t count) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || n < 0 || count < 0 || (n > 0 && !d_boxes) || (count > 0 && !d_pairs))
+    return STQ_ERR_INVALID_ARGUMENT;
+  return stqamd::guarded(c, [&] {
+    unsigned *err = flag_word(c);
+    STQ_HIP(hipMemsetAsync(err, 0, sizeof(unsigned), c->stream));
+    stqamd::launch_orient_pairs(d_boxes, n, d_pairs, count, err, c->stream);
+    check_flag(c, err, "pair of another kind than vertex-face or edge-edge, or id out of range");
+  });
+}
+
+int stq_ccd_queries_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n, const double *d_v0,
+                           const double *d_v1, int64_t nv, const stq_pair *d_pairs, int64_t count,
+                           double min_separation, stq_ccd_query *d_queries) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || n < 0 || nv < 0 || count < 0 || (n > 0 && !d_boxes) || (nv > 0 && (!d_v0 || !d_v1)) ||
+      (count > 0 && (!d_pairs || !d_queries)))
+    return STQ_ERR_INVALID_ARGUMENT;
+  return stqamd::guarded(c, [&] {
+    unsigned *err = flag_word(c);
+    STQ_HIP(hipMemsetAsync(err, 0, sizeof(unsigned), c->stream));
+    stqamd::launch_ccd_queries(d_boxes, n, d_v0, d_v1, nv, d_pairs, count, min_separation,
+                               d_queries, err, c->stream);
+    check_flag(c, err, "pair of another kind, id out of range, or vertex id outside [0, nv)");
+  });
+}
+
+int stq_sizeof_ccd_query(void) { return (int)sizeof(stq_ccd_query); }
 
 int stq_autotune_save(void *ctx, const char *path) {
   Context *c = static_cast<Context *>(ctx);

@@ -1,0 +1,18 @@
+// narrow.hpp -- orientation of pairs and narrow-phase query records (narrow.hip).
+#pragma once
+
+#include "stq_internal.hpp"
+
+namespace stqamd {
+
+// In place: vertex-face pairs become (vertex, face), edge-edge pairs
+// (min, max).  *err |= 1 for a pair of another kind or an id outside [0, n).
+void launch_orient_pairs(const stq_aabb_f64 *boxes, int64_t n, stq_pair *pairs, int64_t count,
+                         unsigned *err, hipStream_t st);
+// One stq_ccd_query per pair (oriented as above); *err |= 1 as above or when
+// a vertex id is outside [0, nv) (that record is zero).
+void launch_ccd_queries(const stq_aabb_f64 *boxes, int64_t n, const double *v0, const double *v1,
+                        int64_t nv, const stq_pair *pairs, int64_t count, double ms,
+                        stq_ccd_query *out, unsigned *err, hipStream_t st);
+
+}  // namespace stqamd

@@ -127,6 +127,37 @@ class Context:
             for b in bufs:
                 b.close()
 
+    def orient_pairs(self, boxes, pairs) -> np.ndarray:
+        """Pairs oriented for the narrow phase: (vertex, face), edge-edge (min, max)."""
+        b = np.ascontiguousarray(boxes)
+        p = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        db, dp = DeviceBuffer.from_array(b), DeviceBuffer.from_array(p)
+        try:
+            self._check(self._L.stq_orient_pairs_device(self.ptr, db.ptr, len(b), dp.ptr, len(p)),
+                        "stq_orient_pairs_device")
+            return dp.to_array(np.int32, 2 * len(p)).reshape(-1, 2)
+        finally:
+            db.close()
+            dp.close()
+
+    def ccd_queries(self, boxes, V0, V1, pairs, min_separation=0.0) -> np.ndarray:
+        """Narrow-phase query records (CCD_QUERY_DTYPE) of the pairs."""
+        b = np.ascontiguousarray(boxes)
+        v0 = np.ascontiguousarray(V0, dtype=np.float64).reshape(-1, 3)
+        v1 = np.ascontiguousarray(V1, dtype=np.float64).reshape(-1, 3)
+        p = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        bufs = [DeviceBuffer.from_array(a) for a in (b, v0, v1, p)]
+        out = DeviceBuffer(_lib.CCD_QUERY_DTYPE.itemsize * max(len(p), 1))
+        try:
+            self._check(self._L.stq_ccd_queries_device(self.ptr, bufs[0].ptr, len(b), bufs[1].ptr, bufs[2].ptr,
+                                                       len(v0), bufs[3].ptr, len(p), float(min_separation),
+                                                       out.ptr), "stq_ccd_queries_device")
+            return out.to_array(_lib.CCD_QUERY_DTYPE, len(p))
+        finally:
+            out.close()
+            for x in bufs:
+                x.close()
+
     # -- autotuning table ------------------------------------------------------
     def autotune_save(self, path):
         self._check(self._L.stq_autotune_save(self.ptr, os.fsencode(path)), "stq_autotune_save")

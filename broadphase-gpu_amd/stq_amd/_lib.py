@@ -32,6 +32,7 @@ EXPORTS = [
     "stq_options_default", "stq_create", "stq_destroy", "stq_set_options",
     "stq_find_pairs_device", "stq_find_pairs_device_alloc", "stq_find_pairs_host",
     "stq_find_pairs_batched_device", "stq_build_swept_boxes_device", "stq_find_pairs_mesh_device",
+    "stq_orient_pairs_device", "stq_ccd_queries_device", "stq_sizeof_ccd_query",
     "stq_autotune_save", "stq_autotune_load",
     "stq_autotune_clear", "stq_autotune_entries", "stq_device_alloc", "stq_copy_to_device",
     "stq_copy_to_host", "stq_free_device", "stq_free_host",
@@ -41,6 +42,16 @@ EXPORTS = [
     "stq_get_stats", "stq_last_error", "stq_status_string",
     "stq_sizeof_aabb", "stq_sizeof_options", "stq_sizeof_stats",
 ]
+
+
+# stq_ccd_query (include/stq.h): the reference's CCDData layout, 256 bytes.
+def _ccd_dtype():
+    import numpy as np
+    return np.dtype([("vs", "<f8", (4, 3)), ("ve", "<f8", (4, 3)), ("ms", "<f8"), ("err", "<f8", (3,)),
+                     ("tol", "<f8", (3,)), ("nbr_checks", "<i4"), ("pad", "<i4")])
+
+
+CCD_QUERY_DTYPE = _ccd_dtype()
 
 
 class Options(ctypes.Structure):
@@ -122,6 +133,9 @@ def lib():
                                                          I32, P]),
         "stq_find_pairs_mesh_device": (ctypes.c_int, [P, P, P, I64, P, I64, P, I64, ctypes.c_double, P,
                                                        I64, ctypes.POINTER(I64)]),
+        "stq_orient_pairs_device": (ctypes.c_int, [P, P, I64, P, I64]),
+        "stq_ccd_queries_device": (ctypes.c_int, [P, P, I64, P, P, I64, P, I64, ctypes.c_double, P]),
+        "stq_sizeof_ccd_query": (ctypes.c_int, []),
         "stq_autotune_save": (ctypes.c_int, [P, ctypes.c_char_p]),
         "stq_autotune_load": (ctypes.c_int, [P, ctypes.c_char_p]),
         "stq_autotune_clear": (ctypes.c_int, [P]),
@@ -154,6 +168,7 @@ def lib():
     assert L.stq_sizeof_aabb() == 72
     assert L.stq_sizeof_options() == ctypes.sizeof(Options)
     assert L.stq_sizeof_stats() == ctypes.sizeof(Stats)
+    assert L.stq_sizeof_ccd_query() == CCD_QUERY_DTYPE.itemsize
     _LIB = L
     return L
 

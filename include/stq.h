@@ -248,6 +248,35 @@ int stq_find_pairs_mesh_device(void *ctx, const double *d_v0, const double *d_v1
                                int64_t nf, double inflation_radius, stq_pair *d_out,
                                int64_t capacity, int64_t *count);
 
+/* ---- the narrow-phase hand-off ----
+   The reference hands its pairs to a continuous-collision narrow phase
+   oriented as (vertex box, face box) for vertex-face pairs and (smaller id,
+   larger id) for edge-edge pairs (src/simulation.cu:510-524), and sizes that
+   narrow phase's CCDData queries (src/memory.cuh:48-67,96-161). */
+typedef struct stq_ccd_query {
+  double v0s[3], v1s[3], v2s[3], v3s[3]; /* the pair's four vertices at t0 */
+  double v0e[3], v1e[3], v2e[3], v3e[3]; /* ... and at t1                   */
+  double ms;                             /* minimum separation              */
+  double err[3];                         /* zero: set by the narrow phase   */
+  double tol[3];                         /* zero: set by the narrow phase   */
+  int32_t nbr_checks;                    /* zero                            */
+  int32_t pad;
+} stq_ccd_query; /* 256 bytes: CCDData built without CCD_TOI_PER_QUERY */
+
+/* In place, on the device: vertex-face pairs become (vertex, face), edge-edge
+   pairs (min, max).  STQ_ERR_INVALID_ARGUMENT if a pair is of another kind
+   or an id is outside [0, n). */
+int stq_orient_pairs_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n, stq_pair *d_pairs,
+                            int64_t count);
+/* One query per pair (oriented as above): vertex-face: the vertex, then the
+   face's three vertices; edge-edge: the first edge's two vertices, then the
+   second's; positions from d_v0 (t0) and d_v1 (t1), row-major (x, y, z).
+   The vertices are the boxes' vertex ids. */
+int stq_ccd_queries_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n, const double *d_v0,
+                           const double *d_v1, int64_t nv, const stq_pair *d_pairs, int64_t count,
+                           double min_separation, stq_ccd_query *d_queries);
+int stq_sizeof_ccd_query(void);
+
 /* ---- autotuning (no counterpart in the reference, whose setup(),
    src/util.cu:9-55, derives one block size from device attributes) ----
    With options.autotune = 1, the first call of each shape class (precision,

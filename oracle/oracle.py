@@ -56,6 +56,8 @@ def lib():
         L.oracle_build_swept_boxes.restype = ctypes.c_int
         L.oracle_build_swept_boxes.argtypes = [P, P, I64, P, I64, P, I64, ctypes.c_double,
                                                ctypes.c_int, P]
+        L.oracle_orient_pairs.restype = ctypes.c_int
+        L.oracle_orient_pairs.argtypes = [P, I64, P, I64]
         L.oracle_sizeof_aabb.restype = ctypes.c_int
         L.oracle_max_threads.restype = ctypes.c_int
         assert L.oracle_sizeof_aabb() == 72
@@ -158,3 +160,13 @@ def build_swept_boxes(V0, V1, E, F, radius=0.0, float_build=False):
     if rc != 0:
         raise ValueError("edge or face index outside the vertex range")
     return out
+
+
+def orient_pairs(boxes, pairs):
+    """Pairs oriented as the reference hands them to the narrow phase
+    (src/simulation.cu:510-524): (vertex, face); edge-edge (min, max)."""
+    b = _as_boxes(boxes)
+    p = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2)).copy()
+    if lib().oracle_orient_pairs(b.ctypes.data, len(b), p.ctypes.data, len(p)) != 0:
+        raise ValueError("pair of another kind than vertex-face or edge-edge, or id out of range")
+    return p

@@ -400,3 +400,26 @@ int oracle_build_swept_boxes(const double *v0, const double *v1, int64_t nv, con
   }
   return 0;
 }
+
+/* ---- narrow-phase orientation: src/simulation.cu:510-524 --------------------
+ * vertex a, face b -> (a, b); face a, vertex b -> (b, a); edge, edge ->
+ * (min, max); any other pair is dropped by the reference (here: -1). */
+int oracle_orient_pairs(const oracle_aabb *boxes, int64_t n, int32_t *pairs, int64_t count) {
+  for (int64_t k = 0; k < count; ++k) {
+    const int32_t a = pairs[2 * k], b = pairs[2 * k + 1];
+    if (a < 0 || b < 0 || a >= n || b >= n) return -1;
+    const int32_t *va = boxes[a].vids, *vb = boxes[b].vids;
+    if (is_vertex(va) && is_face(vb)) {
+      /* unchanged */
+    } else if (is_edge(va) && is_edge(vb)) {
+      pairs[2 * k] = a < b ? a : b;
+      pairs[2 * k + 1] = a < b ? b : a;
+    } else if (is_face(va) && is_vertex(vb)) {
+      pairs[2 * k] = b;
+      pairs[2 * k + 1] = a;
+    } else {
+      return -1;
+    }
+  }
+  return 0;
+}

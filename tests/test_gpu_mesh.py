@@ -87,3 +87,42 @@ def test_mesh_pairs_full_cloth(ctx):
     got = ctx.find_pairs_mesh(V0, V1, E, F)
     boxes = scenes.cloth_scene()
     assert np.array_equal(got, ctx.find_pairs(boxes))
+
+
+def _mesh_pairs(ctx, g=80, seed=11, sigma=5e-3):
+    V0, E, F = scenes.cloth_mesh(g)
+    V1 = V0 + np.random.default_rng(seed).normal(0, sigma, V0.shape)
+    boxes = oracle.build_swept_boxes(V0, V1, E, F)
+    ctx.set_options(precision=stq_amd.PRECISION_F32_CONFIRM)
+    pairs = ctx.find_pairs_mesh(V0, V1, E, F)
+    return V0, V1, boxes, pairs
+
+
+def test_orient_pairs_matches_oracle(ctx):
+    V0, V1, boxes, pairs = _mesh_pairs(ctx)
+    assert len(pairs) > 1000
+    got = ctx.orient_pairs(boxes, pairs)
+    assert np.array_equal(got, oracle.orient_pairs(boxes, pairs))
+    # orienting twice changes nothing; a vertex-vertex pair is rejected
+    assert np.array_equal(ctx.orient_pairs(boxes, got), got)
+    with pytest.raises(stq_amd.StqError):
+        ctx.orient_pairs(boxes, np.array([[0, 1]], np.int32))
+
+
+def test_ccd_queries(ctx):
+    """Every record holds the pair's four vertices (vertex-face: the vertex,
+    then the face's; edge-edge: first edge, then second) at t0 and t1."""
+    V0, V1, boxes, pairs = _mesh_pairs(ctx, seed=12)
+    q = ctx.ccd_queries(boxes, V0, V1, pairs, min_separation=1e-4)
+    o = oracle.orient_pairs(boxes, pairs)
+    va, vb = boxes["vids"][o[:, 0]], boxes["vids"][o[:, 1]]
+    vf = vb[:, 2] >= 0  # second box is a face: vertex-face
+    verts = np.where(vf[:, None], np.stack([va[:, 0], vb[:, 0], vb[:, 1], vb[:, 2]], 1),
+                     np.stack([va[:, 0], va[:, 1], vb[:, 0], vb[:, 1]], 1))
+    assert np.array_equal(q["vs"], V0[verts]) and np.array_equal(q["ve"], V1[verts])
+    assert np.all(q["ms"] == 1e-4)
+    assert not q["err"].any() and not q["tol"].any() and not q["nbr_checks"].any()
+    assert vf.any() and (~vf).any()
+    # a vertex id outside the vertex arrays is reported
+    with pytest.raises(stq_amd.StqError):
+        ctx.ccd_queries(boxes, V0[:10], V1[:10], pairs)

@@ -247,3 +247,19 @@ def test_build_rejects_bad_indices():
     F[2, 0] = -1
     with pytest.raises(ValueError):
         oracle.build_swept_boxes(V0, V1, E, F)
+
+
+def test_orient_pairs_known_answers():
+    """(vertex, face) whatever the input order; edge-edge (min, max); other kinds rejected."""
+    b = scenes.empty_boxes(6)
+    b["vids"][0] = (0, -1, -1)   # vertex
+    b["vids"][1] = (1, 2, 3)     # face
+    b["vids"][2] = (4, 5, -5)    # edge
+    b["vids"][3] = (6, 7, -7)    # edge
+    b["vids"][4] = (8, -9, -9)   # vertex
+    b["vids"][5] = (9, 10, 11)   # face
+    got = oracle.orient_pairs(b, [[0, 1], [1, 0], [3, 2], [2, 3], [5, 4], [4, 5]])
+    assert got.tolist() == [[0, 1], [0, 1], [2, 3], [2, 3], [4, 5], [4, 5]]
+    for bad in ([[0, 4]], [[1, 5]], [[0, 2]], [[2, 5]], [[0, 6]], [[-1, 0]]):
+        with pytest.raises(ValueError):
+            oracle.orient_pairs(b, bad)
