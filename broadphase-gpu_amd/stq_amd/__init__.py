@@ -311,6 +311,55 @@ class Context:
             db.close()
 
 
+def _take_host(ptr, dtype, count):
+    """Copies `count` items from a library-malloc'd block and frees it."""
+    L = _lib.lib()
+    try:
+        if count and ptr.value:
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(np.ctypeslib.as_ctypes_type(dtype))),
+                                         shape=(count,)).copy()
+        return np.zeros(0, dtype=dtype)
+    finally:
+        L.stq_free_host(ptr)
+
+
+def read_mesh(path):
+    """(V, F) of an OBJ or PLY triangle mesh (the reference's parseMesh input)."""
+    L = _lib.lib()
+    v, f = ctypes.c_void_p(), ctypes.c_void_p()
+    nv, nf = ctypes.c_int64(0), ctypes.c_int64(0)
+    rc = L.stq_mesh_read(os.fsencode(path), ctypes.byref(v), ctypes.byref(nv), ctypes.byref(f), ctypes.byref(nf))
+    if rc != 0:
+        raise StqError(rc, f"stq_mesh_read: {L.stq_status_string(rc).decode()} ({path})")
+    V = _take_host(v, np.float64, 3 * nv.value).reshape(-1, 3)
+    F = _take_host(f, np.int32, 3 * nf.value).reshape(-1, 3)
+    return V, F
+
+
+def mesh_edges(F):
+    """Unique undirected edges of a triangle list, in igl::edges order."""
+    L = _lib.lib()
+    f = np.ascontiguousarray(F, dtype=np.int32).reshape(-1, 3)
+    e = ctypes.c_void_p()
+    ne = ctypes.c_int64(0)
+    rc = L.stq_mesh_edges(f.ctypes.data, len(f), ctypes.byref(e), ctypes.byref(ne))
+    if rc != 0:
+        raise StqError(rc, "stq_mesh_edges: " + L.stq_status_string(rc).decode())
+    return _take_host(e, np.int32, 2 * ne.value).reshape(-1, 2)
+
+
+def recall(pairs, json_path, canonical=False):
+    """(true positives, total) of a JSON ground-truth pair list."""
+    L = _lib.lib()
+    p = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+    tp, tot = ctypes.c_int64(0), ctypes.c_int64(0)
+    rc = L.stq_recall(p.ctypes.data, len(p), os.fsencode(json_path), int(bool(canonical)), ctypes.byref(tp),
+                      ctypes.byref(tot))
+    if rc != 0:
+        raise StqError(rc, "stq_recall: " + L.stq_status_string(rc).decode())
+    return tp.value, tot.value
+
+
 def canonical_sort(pairs: np.ndarray) -> np.ndarray:
     """Lexicographic order of (first, second) pairs (numpy, for comparisons)."""
     p = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)

@@ -33,6 +33,7 @@ EXPORTS = [
     "stq_find_pairs_device", "stq_find_pairs_device_alloc", "stq_find_pairs_host",
     "stq_find_pairs_batched_device", "stq_build_swept_boxes_device", "stq_find_pairs_mesh_device",
     "stq_orient_pairs_device", "stq_ccd_queries_device", "stq_sizeof_ccd_query",
+    "stq_mesh_read", "stq_mesh_edges", "stq_recall",
     "stq_autotune_save", "stq_autotune_load",
     "stq_autotune_clear", "stq_autotune_entries", "stq_device_alloc", "stq_copy_to_device",
     "stq_copy_to_host", "stq_free_device", "stq_free_host",
@@ -136,6 +137,10 @@ def lib():
         "stq_orient_pairs_device": (ctypes.c_int, [P, P, I64, P, I64]),
         "stq_ccd_queries_device": (ctypes.c_int, [P, P, I64, P, P, I64, P, I64, ctypes.c_double, P]),
         "stq_sizeof_ccd_query": (ctypes.c_int, []),
+        "stq_mesh_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(P), ctypes.POINTER(I64),
+                                         ctypes.POINTER(P), ctypes.POINTER(I64)]),
+        "stq_mesh_edges": (ctypes.c_int, [P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
+        "stq_recall": (ctypes.c_int, [P, I64, ctypes.c_char_p, I32, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
         "stq_autotune_save": (ctypes.c_int, [P, ctypes.c_char_p]),
         "stq_autotune_load": (ctypes.c_int, [P, ctypes.c_char_p]),
         "stq_autotune_clear": (ctypes.c_int, [P]),

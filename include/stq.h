@@ -277,6 +277,23 @@ int stq_ccd_queries_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n, co
                            double min_separation, stq_ccd_query *d_queries);
 int stq_sizeof_ccd_query(void);
 
+/* ---- mesh input and ground truth (host only, no context) ----
+   stq_mesh_read replaces parseMesh's igl::readOBJ / igl::readPLY
+   (src/io.cu:30-51): OBJ ("v" and triangular "f" lines) or PLY (ascii or
+   binary little endian, vertex x/y/z, triangular faces).  *h_vertices gets
+   nv rows of (x, y, z) doubles, *h_faces nf triples of 0-based indices.
+   stq_mesh_edges replaces igl::edges: every undirected edge of the faces once
+   as (smaller, larger), ordered by the larger index, then the smaller.
+   stq_recall replaces compare_mathematica (src/groundtruth.cu:32-67): the
+   number of pairs of the JSON list [[i, j], ...] found among h_pairs, and
+   the list's size (canonical != 0: both sides as (min, max)).
+   Outputs are malloc'd; release them with stq_free_host. */
+int stq_mesh_read(const char *path, double **h_vertices, int64_t *nv, int32_t **h_faces,
+                  int64_t *nf);
+int stq_mesh_edges(const int32_t *h_faces, int64_t nf, int32_t **h_edges, int64_t *ne);
+int stq_recall(const stq_pair *h_pairs, int64_t count, const char *json_path, int32_t canonical,
+               int64_t *true_positives, int64_t *total);
+
 /* ---- autotuning (no counterpart in the reference, whose setup(),
    src/util.cu:9-55, derives one block size from device attributes) ----
    With options.autotune = 1, the first call of each shape class (precision,

@@ -107,7 +107,57 @@ ROUNDING = [
 ]
 
 
+# A tetrahedron, written in the three mesh formats the reference reads
+# (src/io.cu:36-45).  Vertex coordinates are exact in binary32 so that the
+# float-typed PLY holds them unchanged.  Its edges, hand-derived from the four
+# faces: {0,1} {0,2} {0,3} {1,2} {1,3} {2,3}, listed as igl::edges reads them
+# off the column-major adjacency matrix (by larger index, then smaller).
+TETRA_V = [[0.0, 0.0, 0.0], [1.5, 0.0, 0.25], [0.0, 2.0, -0.5], [0.75, 0.5, 3.0]]
+TETRA_F = [[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]]
+TETRA_E = [[0, 1], [0, 2], [1, 2], [0, 3], [1, 3], [2, 3]]
+# A ground-truth list in the reference's JSON form (src/groundtruth.cu:43-47)
+# and pairs to check it against: (1, 3) and (2, 7) are found as given, (5, 0)
+# only when both sides are taken as (min, max); (4, 4) is not in the list.
+TRUTH = [[0, 5], [1, 3], [2, 7]]
+FOUND = [[1, 3], [2, 7], [5, 0], [4, 4]]
+
+
+def write_meshes():
+    with open(os.path.join(HERE, "tetra.obj"), "w") as fh:
+        fh.write("# tetrahedron\n")
+        for v in TETRA_V:
+            fh.write("v %r %r %r\n" % tuple(v))
+        fh.write("vn 0 0 1\n")
+        fh.write("f 1 2 3\n")
+        fh.write("f 1/1/1 2/2/2 4/4/4\n")   # v/vt/vn form
+        fh.write("f 1//1 3//1 4//1\n")      # v//vn form
+        fh.write("f -3 -2 -1\n")            # relative indices: 2 3 4
+    with open(os.path.join(HERE, "tetra_ascii.ply"), "w") as fh:
+        fh.write("ply\nformat ascii 1.0\ncomment tetrahedron\nelement vertex 4\n"
+                 "property float x\nproperty float y\nproperty float z\n"
+                 "element face 4\nproperty list uchar int vertex_indices\nend_header\n")
+        for v in TETRA_V:
+            fh.write("%r %r %r\n" % tuple(v))
+        for f in TETRA_F:
+            fh.write("3 %d %d %d\n" % tuple(f))
+    with open(os.path.join(HERE, "tetra_binary.ply"), "wb") as fh:
+        fh.write(b"ply\nformat binary_little_endian 1.0\nelement vertex 4\n"
+                 b"property double x\nproperty double y\nproperty double z\nproperty uchar red\n"
+                 b"element face 4\nproperty list uchar uint vertex_indices\nend_header\n")
+        for k, v in enumerate(TETRA_V):
+            fh.write(struct.pack("<dddB", v[0], v[1], v[2], 10 * k))
+        for f in TETRA_F:
+            fh.write(struct.pack("<BIII", 3, *f))
+    with open(os.path.join(HERE, "mesh_expected.json"), "w") as fh:
+        json.dump({"V": TETRA_V, "F": TETRA_F, "E": TETRA_E}, fh, indent=1)
+    with open(os.path.join(HERE, "truth_tiny.json"), "w") as fh:
+        json.dump(TRUTH, fh)
+    with open(os.path.join(HERE, "recall_expected.json"), "w") as fh:
+        json.dump({"pairs": FOUND, "as_given": [2, 3], "canonical": [3, 3]}, fh)
+
+
 def main():
+    write_meshes()
     out = {name: {"boxes": b, "pairs": p} for name, (b, p) in CASES.items()}
     with open(os.path.join(HERE, "known_answers.json"), "w") as fh:
         json.dump(out, fh, indent=1)

@@ -1,6 +1,8 @@
 """Swept boxes of a moving mesh built on the GPU, against the oracle's
 restatement of the reference construction (bit for bit, both builds), and
 the fused mesh-to-pairs call against the pair-set oracle."""
+import json
+
 import numpy as np
 import pytest
 
@@ -126,3 +128,33 @@ def test_ccd_queries(ctx):
     # a vertex id outside the vertex arrays is reported
     with pytest.raises(stq_amd.StqError):
         ctx.ccd_queries(boxes, V0[:10], V1[:10], pairs)
+
+
+def test_mesh_files_end_to_end(ctx, tmp_path):
+    """t0/t1 meshes written as OBJ and PLY, read back, edges extracted, pairs
+    found on the GPU, recall against the oracle's pair list as JSON: all found."""
+    V0, _, F = scenes.cloth_mesh(60)
+    V1 = V0 + np.random.default_rng(13).normal(0, 4e-3, V0.shape)
+    with open(tmp_path / "t0.obj", "w") as fh:
+        for v in V0:
+            fh.write("v %r %r %r\n" % tuple(float(x) for x in v))
+        for f in F:
+            fh.write("f %d %d %d\n" % tuple(f + 1))
+    with open(tmp_path / "t1.ply", "wb") as fh:
+        fh.write(b"ply\nformat binary_little_endian 1.0\nelement vertex %d\n" % len(V1) +
+                 b"property double x\nproperty double y\nproperty double z\n" +
+                 b"element face %d\nproperty list uchar int vertex_indices\nend_header\n" % len(F))
+        fh.write(np.ascontiguousarray(V1, dtype="<f8").tobytes())
+        rec = np.zeros(len(F), dtype=[("n", "u1"), ("i", "<i4", (3,))])
+        rec["n"], rec["i"] = 3, F
+        fh.write(rec.tobytes())
+    A0, G0 = stq_amd.read_mesh(tmp_path / "t0.obj")
+    A1, G1 = stq_amd.read_mesh(tmp_path / "t1.ply")
+    assert np.array_equal(A0, V0) and np.array_equal(A1, V1) and np.array_equal(G0, G1)
+    E = stq_amd.mesh_edges(G0)
+    ctx.set_options(precision=stq_amd.PRECISION_F32_CONFIRM)
+    pairs = ctx.find_pairs_mesh(A0, A1, E, G0)
+    ref = oracle.reference_set(oracle.build_swept_boxes(A0, A1, E, G0))
+    (tmp_path / "truth.json").write_text(json.dumps(ref.tolist()))
+    assert stq_amd.recall(pairs, tmp_path / "truth.json") == (len(ref), len(ref))
+    assert len(pairs) == len(ref)
