@@ -22,8 +22,10 @@ the roofline of the dominant kernel (the emitting sweep) and a CPU baseline
 1-thread figure).  Further legs, reported in the same line: the 8M-box
 clustered heavy-tailed scene (`skewed`), 512 scenes in one batched call
 (`batched`), a bounded sample of the learned-sizing stream
-(`learned_stream`) and the PCIe-inclusive host-to-host time
-(`end_to_end_host`).
+(`learned_stream`), the PCIe-inclusive host-to-host time
+(`end_to_end_host`), and the kernels either side of the broad phase on the
+cloth mesh: swept-box build, pair orientation and narrow-phase query records
+(`handoff`), each against the HBM roofline.
 """
 from __future__ import annotations
 
@@ -40,6 +42,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "boxes/s & pairs/s at exact pair set: chosen form vs plain FP64 STQ, 1 MI355X"
 HBM_PEAK = 8.0e12
+
+
+def progress(msg):
+    """One line on stderr when a leg ends (the JSON line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -66,6 +73,8 @@ def parse():
                     help="fraction of sorted rows the 1-thread CPU timing sweeps (0 = skip)")
     ap.add_argument("--host-e2e", type=int, default=1,
                     help="time the PCIe-inclusive host-to-host call (0 = skip)")
+    ap.add_argument("--handoff", type=int, default=1,
+                    help="time the swept-box build and the narrow-phase hand-off kernels (1 = on)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (traffic calibration)")
     return ap.parse_args()
@@ -289,6 +298,77 @@ def bench_stream(args, H):
             "held_out_checked_vs_plain_fp64": checked, "mismatches": mismatches}
 
 
+def bench_handoff(args, H):
+    """The kernels either side of the broad phase on the cloth mesh: swept-box
+    build (FP64) from the vertices at t0/t1 and the edge and face lists, then,
+    on the broad phase's pairs, their orientation for the narrow phase and the
+    256-byte query records.  Each kernel's time comes from HIP events on the
+    library's stream (stats ms_total), median of `steps` calls; roofline
+    against HBM with these algorithmic bytes per launch:
+      build:   48 nv (V0, V1) + 8 ne + 12 nf (indices) + 72 n (boxes written)
+      orient:  16 P (pairs read and written) + 12 n (vertex ids of the boxes)
+      queries: 8 P (pairs) + 256 P (records written) + 12 n + 48 nv
+    """
+    import numpy as np
+    import stq_amd
+    from stq_amd import _lib, scenes
+
+    rng = np.random.default_rng(args.seed)
+    V0, E, F = scenes.cloth_mesh(args.grid)
+    V1 = V0 + rng.normal(0.0, 2e-3, size=V0.shape)
+    nv, ne, nf = len(V0), len(E), len(F)
+    n = nv + ne + nf
+    bufs = [stq_amd.DeviceBuffer.from_array(np.ascontiguousarray(a)) for a in
+            (V0, V1, E.astype(np.int32), F.astype(np.int32))]
+    d_boxes = stq_amd.DeviceBuffer(72 * n)
+    ctx = stq_amd.Context(0, precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN)
+
+    def timed(fn):
+        for _ in range(max(args.warmup, 1)):
+            fn()
+        ms = []
+        for _ in range(max(args.steps, 1)):
+            fn()
+            ms.append(ctx.stats()["ms_total"])
+        return float(np.median(ms))
+
+    def roof(alg, ms):
+        gbs = alg / (ms * 1e-3) / 1e9
+        return {"ms": ms, "algorithmic_bytes": alg, "achieved_GBs": gbs, "frac_hbm": gbs * 1e9 / HBM_PEAK}
+
+    out = {"workload": f"cloth {args.grid}x{args.grid}: nv={nv}, ne={ne}, nf={nf}"}
+    ms = timed(lambda: ctx.build_swept_boxes_device(bufs[0].ptr, bufs[1].ptr, nv, bufs[2].ptr, ne,
+                                                    bufs[3].ptr, nf, d_boxes.ptr))
+    out["build"] = roof(48.0 * nv + 8.0 * ne + 12.0 * nf + 72.0 * n, ms)
+    if not args.no_check:
+        # ids exactly, bounds by value: numpy's maximum/minimum may give the
+        # other zero where -0.0 meets +0.0, which no overlap test can see
+        ref = scenes.swept_boxes(V0, V1, E, F)
+        got = d_boxes.to_array(scenes.AABB_DTYPE, n)
+        out["build"]["equal_to_host_build"] = bool(all(np.array_equal(got[k], ref[k]) for k in
+                                                       ("id", "min", "max", "vids", "ref_id")))
+        del ref, got
+    try:
+        ctx.find_pairs_device(d_boxes.ptr, n, None, 0)
+    except stq_amd.StqError as e:
+        if e.status != _lib.STQ_ERR_CAPACITY:
+            raise
+    P = ctx.stats()["n_pairs"]
+    d_pairs = stq_amd.DeviceBuffer(8 * max(P, 1))
+    ctx.find_pairs_device(d_boxes.ptr, n, d_pairs.ptr, P)
+    ms = timed(lambda: ctx.orient_pairs_device(d_boxes.ptr, n, d_pairs.ptr, P))
+    out["orient"] = roof(16.0 * P + 12.0 * n, ms) | {"pairs": P}
+    d_q = stq_amd.DeviceBuffer(_lib.CCD_QUERY_DTYPE.itemsize * max(P, 1))
+    ms = timed(lambda: ctx.ccd_queries_device(d_boxes.ptr, n, bufs[0].ptr, bufs[1].ptr, nv, d_pairs.ptr, P,
+                                              d_q.ptr))
+    out["ccd_queries"] = roof(264.0 * P + 12.0 * n + 48.0 * nv, ms) | {"pairs": P}
+    for b in bufs + [d_boxes, d_pairs, d_q]:
+        b.close()
+    ctx.close()
+    H.hipDeviceSynchronize()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -376,6 +456,7 @@ def main():
             outputs[name] = d_out.to_array(np.int32, 2 * cnt).reshape(-1, 2)
         d_out.close()
         ctx.close()
+        progress(f"form {name}: {ms:.2f} ms per call, {cnt} pairs")
 
     exact = None
     if not args.no_check and len(outputs) == 2:
@@ -419,6 +500,7 @@ def main():
             one = {"value": n / (sort1 + sw1 * (n / nr1)), "unit": "boxes/s",
                    "sample": f"sort of all boxes ({sort1:.2f} s) + sweep of the first {nr1} sorted rows "
                              f"({sw1:.2f} s), scaled to all rows"}
+        progress(f"cpu baseline: {n / est:.3g} boxes/s on {threads} threads")
         cpu = {"value": n / est, "unit": "boxes/s", "cores": threads, "kind": "port", "one_thread": one,
                "sample": (f"oracle restatement of src/cpu/sweep.cpp (FP64, axis x): parallel sort of all "
                           f"{n} boxes ({sort_s:.2f} s) + chain sweep of the first {nrows} sorted rows "
@@ -478,10 +560,16 @@ def main():
                                            "H2D of the boxes, the call, D2H of the pairs"}
     if args.skewed:
         line["skewed"] = bench_skewed(args, H)
+        progress("skewed leg done")
     if args.batched_scenes > 0:
         line["batched"] = bench_batched(args, H)
+        progress("batched leg done")
     if args.stream_scenes > 0:
         line["learned_stream"] = bench_stream(args, H)
+        progress("learned stream leg done")
+    if args.handoff:
+        line["handoff"] = bench_handoff(args, H)
+        progress("hand-off leg done")
     print(json.dumps(line))
 
 

@@ -1032,13 +1032,28 @@ static bool mesh_args_valid(const double *v0, const double *v1, int64_t nv, cons
 }
 
 // Builds the boxes on the context's stream and checks the index flag.
+// A call around the path (box build, orientation, query records): its stats are
+// the record count and the kernel's time on the stream (ms_total), bracketed by
+// HIP events; `launch` enqueues the kernel, then the error flag is read back.
+extern "C++" template <class F>
+static void timed_kernel(Context *c, int64_t n, unsigned *err, const char *what, F &&launch) {
+  c->stats = stq_stats{};
+  c->stats.n_boxes = n;
+  STQ_HIP(hipMemsetAsync(err, 0, sizeof(unsigned), c->stream));
+  STQ_HIP(hipEventRecord(c->ev[stqamd::E_START], c->stream));
+  launch();
+  STQ_HIP(hipEventRecord(c->ev[stqamd::E_END], c->stream));
+  check_flag(c, err, what);
+  c->stats.ms_total = stqamd::ev_ms(c->ev[stqamd::E_START], c->ev[stqamd::E_END]);
+}
+
 static void build_boxes(Context *c, const double *v0, const double *v1, int64_t nv,
                         const int32_t *e, int64_t ne, const int32_t *f, int64_t nf, double r,
                         bool float_build, stq_aabb_f64 *out) {
-  unsigned *err = flag_word(c);
-  STQ_HIP(hipMemsetAsync(err, 0, sizeof(unsigned), c->stream));
-  stqamd::launch_build_boxes(v0, v1, nv, e, ne, f, nf, r, float_build, out, err, c->stream);
-  check_flag(c, err, "edge or face index outside [0, nv)");
+  timed_kernel(c, nv + ne + nf, flag_word(c), "edge or face index outside [0, nv)", [&] {
+    stqamd::launch_build_boxes(v0, v1, nv, e, ne, f, nf, r, float_build, out, flag_word(c),
+                               c->stream);
+  });
 }
 
 int stq_build_swept_boxes_device(void *ctx, const double *d_v0, const double *d_v1, int64_t nv,
@@ -1081,9 +1096,9 @@ int stq_orient_pairs_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n, s
     return STQ_ERR_INVALID_ARGUMENT;
   return stqamd::guarded(c, [&] {
     unsigned *err = flag_word(c);
-    STQ_HIP(hipMemsetAsync(err, 0, sizeof(unsigned), c->stream));
-    stqamd::launch_orient_pairs(d_boxes, n, d_pairs, count, err, c->stream);
-    check_flag(c, err, "pair of another kind than vertex-face or edge-edge, or id out of range");
+    timed_kernel(c, count, err,
+                 "pair of another kind than vertex-face or edge-edge, or id out of range",
+                 [&] { stqamd::launch_orient_pairs(d_boxes, n, d_pairs, count, err, c->stream); });
   });
 }
 
@@ -1096,10 +1111,11 @@ int stq_ccd_queries_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n, co
     return STQ_ERR_INVALID_ARGUMENT;
   return stqamd::guarded(c, [&] {
     unsigned *err = flag_word(c);
-    STQ_HIP(hipMemsetAsync(err, 0, sizeof(unsigned), c->stream));
-    stqamd::launch_ccd_queries(d_boxes, n, d_v0, d_v1, nv, d_pairs, count, min_separation,
-                               d_queries, err, c->stream);
-    check_flag(c, err, "pair of another kind, id out of range, or vertex id outside [0, nv)");
+    timed_kernel(c, count, err,
+                 "pair of another kind, id out of range, or vertex id outside [0, nv)", [&] {
+                   stqamd::launch_ccd_queries(d_boxes, n, d_v0, d_v1, nv, d_pairs, count,
+                                              min_separation, d_queries, err, c->stream);
+                 });
   });
 }
 

@@ -103,6 +103,24 @@ class Context:
             for b in bufs:
                 b.close()
 
+    # device-resident forms (device pointers as ints); stats() then holds the
+    # kernel time on the stream in ms_total
+    def build_swept_boxes_device(self, d_v0, d_v1, nv, d_e, ne, d_f, nf, d_boxes, radius=0.0,
+                                 float_build=False):
+        self._check(self._L.stq_build_swept_boxes_device(
+            self.ptr, d_v0, d_v1, nv, d_e, ne, d_f, nf, float(radius), int(bool(float_build)), d_boxes),
+            "stq_build_swept_boxes_device")
+
+    def orient_pairs_device(self, d_boxes, n, d_pairs, count):
+        self._check(self._L.stq_orient_pairs_device(self.ptr, d_boxes, n, d_pairs, count),
+                    "stq_orient_pairs_device")
+
+    def ccd_queries_device(self, d_boxes, n, d_v0, d_v1, nv, d_pairs, count, d_queries,
+                           min_separation=0.0):
+        self._check(self._L.stq_ccd_queries_device(self.ptr, d_boxes, n, d_v0, d_v1, nv, d_pairs, count,
+                                                   float(min_separation), d_queries),
+                    "stq_ccd_queries_device")
+
     def find_pairs_mesh(self, V0, V1, E, F, radius=0.0) -> np.ndarray:
         """Pairs of the mesh's swept boxes (ids = box positions: vertices,
         then edges, then faces)."""

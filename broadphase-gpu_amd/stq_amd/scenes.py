@@ -104,9 +104,12 @@ def cloth_mesh(g: int):
 
 def swept_boxes(V0: np.ndarray, V1: np.ndarray, E: np.ndarray, F: np.ndarray) -> np.ndarray:
     """FP64 swept boxes: hull of the t0/t1 positions of each simplex's vertices
-    (the CCD_USE_DOUBLE branch of src/aabb.cu:63-174, without inflation)."""
+    (the CCD_USE_DOUBLE branch of src/aabb.cu:63-174, without inflation).
+    ref_id is the index within the box's kind, as the reference records it
+    (src/aabb.cu:94,128,170)."""
     nv, ne, nf = len(V0), len(E), len(F)
     b = empty_boxes(nv + ne + nf)
+    b["ref_id"] = np.concatenate([np.arange(nv), np.arange(ne), np.arange(nf)]).astype(np.int32)
     vmin = np.minimum(V0, V1)
     vmax = np.maximum(V0, V1)
     b["min"][:nv] = vmin

@@ -235,7 +235,10 @@ int stq_find_pairs_batched_device(void *ctx, const stq_aabb_f64 *d_boxes,
      float_build = 1: the reference's FP32 build, bit for bit (rounded to
                       float, one ulp outward as nextafterf, radius rounded
                       up, float arithmetic), stored as doubles.
-   STQ_ERR_INVALID_ARGUMENT if an edge or face index is outside [0, nv). */
+   STQ_ERR_INVALID_ARGUMENT if an edge or face index is outside [0, nv).
+   After this call and the two hand-off calls below, stq_get_stats gives the
+   records written in n_boxes (boxes here, pairs there) and the kernel's time
+   on the context's stream in ms_total; the other fields are zero. */
 int stq_build_swept_boxes_device(void *ctx, const double *d_v0, const double *d_v1, int64_t nv,
                                  const int32_t *d_edges, int64_t ne, const int32_t *d_faces,
                                  int64_t nf, double inflation_radius, int32_t float_build,

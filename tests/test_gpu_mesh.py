@@ -46,6 +46,8 @@ def _same_boxes(a, b):
 def test_build_matches_oracle(ctx, float_build, radius):
     V0, V1, E, F = _mesh(4)
     got = ctx.build_swept_boxes(V0, V1, E, F, radius=radius, float_build=float_build)
+    st = ctx.stats()  # the call's record count and its kernel time on the stream
+    assert st["n_boxes"] == len(V0) + len(E) + len(F) and st["ms_total"] > 0
     ref = oracle.build_swept_boxes(V0, V1, E, F, radius=radius, float_build=float_build)
     _same_boxes(got, ref)
 
@@ -116,6 +118,8 @@ def test_ccd_queries(ctx):
     then the face's; edge-edge: first edge, then second) at t0 and t1."""
     V0, V1, boxes, pairs = _mesh_pairs(ctx, seed=12)
     q = ctx.ccd_queries(boxes, V0, V1, pairs, min_separation=1e-4)
+    st = ctx.stats()
+    assert st["n_boxes"] == len(pairs) and st["ms_total"] > 0
     o = oracle.orient_pairs(boxes, pairs)
     va, vb = boxes["vids"][o[:, 0]], boxes["vids"][o[:, 1]]
     vf = vb[:, 2] >= 0  # second box is a face: vertex-face
