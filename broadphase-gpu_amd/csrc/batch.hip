@@ -198,28 +198,14 @@ __global__ __launch_bounds__(kT) void k_chain_end_seg(const T *__restrict__ smin
                                                       const int64_t *__restrict__ seg_off,
                                                       int nscenes, int *__restrict__ hi,
                                                       unsigned long long *__restrict__ cand) {
-  const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
   unsigned long long len = 0;
-  if (s < n) {
+  for (int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x; s < n; s += (int64_t)gridDim.x * kT) {
     const int64_t end = seg_off[scene_of(seg_off, nscenes, s) + 1];
-    const T x = smax[s];
-    int64_t lo = s, step = 1, up;
-    while (true) {
-      const int64_t probe = lo + step;
-      if (probe >= end) { up = end; break; }
-      if (smin[probe] <= x) { lo = probe; step <<= 1; }
-      else { up = probe; break; }
-    }
-    while (up - lo > 1) {
-      const int64_t mid = lo + ((up - lo) >> 1);
-      if (smin[mid] <= x) lo = mid; else up = mid;
-    }
+    const int64_t lo = last_le(smin, s, end, smax[s]);
     hi[s] = (int)lo;
-    len = (unsigned long long)(lo - s);
+    len += (unsigned long long)(lo - s);
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) len += __shfl_xor(len, o, kWave);
-  if ((threadIdx.x & 63) == 0 && len) atomicAdd(cand, len);
+  block_atomic_add<kT>(len, cand);
 }
 
 // ---- planning -------------------------------------------------------------------
@@ -228,8 +214,8 @@ constexpr int kMaxWin = 2048;
 __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
                                                  const int64_t *__restrict__ win_start,
                                                  const int *__restrict__ win_cnt,
-                                                 const int64_t *__restrict__ win_gbase, int W,
-                                                 int chunk, float tail_cost,
+                                                 const int64_t *__restrict__ win_gbase, int nwin,
+                                                 int W, int chunk, float tail_cost,
                                                  int *__restrict__ rowlist,
                                                  int4 *__restrict__ groups,
                                                  int4 *__restrict__ tgroups,
@@ -237,55 +223,58 @@ __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
                                                  uint32_t *__restrict__ nchunks,
                                                  unsigned long long *__restrict__ lane_steps) {
   __shared__ uint64_t key[kMaxWin];
-  const int64_t wbase = win_start[blockIdx.x];
-  const int cnt = win_cnt[blockIdx.x];
-  const int64_t gbase = win_gbase[blockIdx.x];
-  for (int j = threadIdx.x; j < W; j += kT) {
-    uint64_t k = ~0ull;
-    if (j < cnt) k = ((uint64_t)(uint32_t)hi[wbase + j] << 32) | (uint32_t)j;
-    key[j] = k;
-  }
-  __syncthreads();
-  for (int size = 2; size <= W; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int j = threadIdx.x; j < W; j += kT) {
-        const int pj = j ^ stride;
-        if (pj > j) {
-          const bool upd = (j & size) == 0;
-          const uint64_t a = key[j], b = key[pj];
-          if ((a > b) == upd) {
-            key[j] = b;
-            key[pj] = a;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long steps = 0;
+  for (int b = blockIdx.x; b < nwin; b += gridDim.x) {  // grid-stride over windows
+    const int64_t wbase = win_start[b];
+    const int cnt = win_cnt[b];
+    const int64_t gbase = win_gbase[b];
+    for (int j = threadIdx.x; j < W; j += kT) {
+      uint64_t k = ~0ull;
+      if (j < cnt) k = ((uint64_t)(uint32_t)hi[wbase + j] << 32) | (uint32_t)j;
+      key[j] = k;
+    }
+    __syncthreads();
+    for (int size = 2; size <= W; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int j = threadIdx.x; j < W; j += kT) {
+          const int pj = j ^ stride;
+          if (pj > j) {
+            const bool upd = (j & size) == 0;
+            const uint64_t a = key[j], c = key[pj];
+            if ((a > c) == upd) {
+              key[j] = c;
+              key[pj] = a;
+            }
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
-  }
-  for (int j = threadIdx.x; j < cnt; j += kT)
-    rowlist[wbase + j] = (int)(wbase + (int64_t)(key[j] & 0xffffffffu));
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int ng = (cnt + 63) / 64;
-  unsigned long long steps = 0;
-  for (int k = w; k < ng; k += kT / 64) {
-    const int j = 64 * k + lane;
-    const bool act = j < cnt;
-    const int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
-    const int h = act ? (int)(key[j] >> 32) : INT_MIN;
-    const int nrows = min(64, cnt - 64 * k);
-    // capped: the item buffer is sized on the host from the scene sizes
-    const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, true,
-                                    tail_cost);
-    const int64_t g = gbase + k;
-    if (lane == 0) {
-      groups[g] = gp.body;
-      tgroups[g] = gp.tail;
-      gchunk[g] = gp.chunk;
-      nchunks[g] = gp.nitems;
-      steps += gp.steps;
+    for (int j = threadIdx.x; j < cnt; j += kT)
+      rowlist[wbase + j] = (int)(wbase + (int64_t)(key[j] & 0xffffffffu));
+    const int ng = (cnt + 63) / 64;
+    for (int k = w; k < ng; k += kT / 64) {
+      const int j = 64 * k + lane;
+      const bool act = j < cnt;
+      const int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
+      const int h = act ? (int)(key[j] >> 32) : INT_MIN;
+      const int nrows = min(64, cnt - 64 * k);
+      // capped: the item buffer is sized on the host from the scene sizes
+      const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, true,
+                                      tail_cost);
+      const int64_t g = gbase + k;
+      if (lane == 0) {
+        groups[g] = gp.body;
+        tgroups[g] = gp.tail;
+        gchunk[g] = gp.chunk;
+        nchunks[g] = gp.nitems;
+        steps += gp.steps;
+      }
     }
+    __syncthreads();  // the next window reuses key[]
   }
-  if (lane == 0 && steps) atomicAdd(lane_steps, steps);
+  block_atomic_add<kT>(steps, lane_steps);
 }
 
 __global__ __launch_bounds__(kT) void k_scene_pairs(const int64_t *__restrict__ off,
@@ -401,13 +390,13 @@ void launch_gather_seg(bool f32, const stq_aabb_f64 *boxes, const uint32_t *orde
 void launch_chain_end_seg(bool f32, const void *smin, const void *smax, const BatchTables &t,
                           const BatchDev &d, int *hi, unsigned long long *cand, hipStream_t st) {
   if (t.n <= 0) return;
+  const unsigned g = std::min(nblk(t.n), kReduceGrid);
   if (f32)
-    hipLaunchKernelGGL(k_chain_end_seg<float>, dim3(nblk(t.n)), dim3(kT), 0, st,
-                       (const float *)smin, (const float *)smax, t.n, d.seg_off, t.nscenes, hi, cand);
+    hipLaunchKernelGGL(k_chain_end_seg<float>, dim3(g), dim3(kT), 0, st, (const float *)smin,
+                       (const float *)smax, t.n, d.seg_off, t.nscenes, hi, cand);
   else
-    hipLaunchKernelGGL(k_chain_end_seg<double>, dim3(nblk(t.n)), dim3(kT), 0, st,
-                       (const double *)smin, (const double *)smax, t.n, d.seg_off, t.nscenes, hi,
-                       cand);
+    hipLaunchKernelGGL(k_chain_end_seg<double>, dim3(g), dim3(kT), 0, st, (const double *)smin,
+                       (const double *)smax, t.n, d.seg_off, t.nscenes, hi, cand);
   STQ_HIP(hipGetLastError());
 }
 
@@ -416,8 +405,9 @@ void launch_plan_seg(const int *hi, const BatchTables &t, const BatchDev &d, int
                      uint32_t *nchunks, unsigned long long *lane_steps, hipStream_t st) {
   const int nw = (int)t.win_start.size();
   if (nw <= 0) return;
-  hipLaunchKernelGGL(k_plan_seg, dim3(nw), dim3(kT), 0, st, hi, d.win_start, d.win_cnt, d.win_gbase,
-                     W, chunk, tail_cost, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
+  hipLaunchKernelGGL(k_plan_seg, dim3(std::min((unsigned)nw, kReduceGrid)), dim3(kT), 0, st, hi,
+                     d.win_start, d.win_cnt, d.win_gbase, nw, W, chunk, tail_cost, rowlist, groups,
+                     tgroups, gchunk, nchunks, lane_steps);
   STQ_HIP(hipGetLastError());
 }
 

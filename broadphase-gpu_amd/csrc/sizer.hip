@@ -20,6 +20,25 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// Sum over the 64 lanes of a fully active wave with DPP row moves instead of
+// LDS permutes: pairs, quads (quad_perm), 8 and 16 lanes (row half-mirror,
+// row mirror), then rows 0 -> 1 and 2 -> 3 (row_bcast:15 into rows 1 and 3)
+// and rows 0..1 -> 2..3 (row_bcast:31 into rows 2 and 3); lane 63 then holds
+// the total, which is broadcast.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_move(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWS, 0xf, false));
+}
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+  v += dpp_move<0xB1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_move<0x4E, 0xf>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_move<0x141, 0xf>(v);  // row_half_mirror
+  v += dpp_move<0x140, 0xf>(v);  // row_mirror
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // ---- features ----------------------------------------------------------------
 // One wave per item.  Rows are the item's lanes; the q range is sampled at 64
 // evenly spaced positions.  Extents enter as log(extent)/10, centres as the
@@ -62,8 +81,8 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
       const int p = rowlist[item.rowoff + lane];
       const int h = hi[p];
       const R r = rec[p];
-      le1 = (float)(log(fmax(r.hi1 - r.lo1, 1e-30)) * 0.1);
-      le2 = (float)(log(fmax(r.hi2 - r.lo2, 1e-30)) * 0.1);
+      le1 = logf(fmaxf((float)(r.hi1 - r.lo1), 1e-30f)) * 0.1f;
+      le2 = logf(fmaxf((float)(r.hi2 - r.lo2), 1e-30f)) * 0.1f;
       c1 = (float)((0.5 * (r.lo1 + r.hi1) - o1) * n1);
       c2 = (float)((0.5 * (r.lo2 + r.hi2) - o2) * n2);
       tv = r.type == 1 ? 1.f : 0.f;
@@ -76,29 +95,29 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
     const int span = q1 - q0;
     const int qs = q0 + (span > 0 ? (int)(((long long)lane * span) / 63) : 0);
     const R s = rec[qs];
-    const float se1 = (float)(log(fmax(s.hi1 - s.lo1, 1e-30)) * 0.1);
-    const float se2 = (float)(log(fmax(s.hi2 - s.lo2, 1e-30)) * 0.1);
+    const float se1 = logf(fmaxf((float)(s.hi1 - s.lo1), 1e-30f)) * 0.1f;
+    const float se2 = logf(fmaxf((float)(s.hi2 - s.lo2), 1e-30f)) * 0.1f;
     const float sc1 = (float)((0.5 * (s.lo1 + s.hi1) - o1) * n1);
     const float sc2 = (float)((0.5 * (s.lo2 + s.hi2) - o2) * n2);
     const float sv = s.type == 1 ? 1.f : 0.f, se = s.type == 2 ? 1.f : 0.f,
                 sf = s.type == 4 ? 1.f : 0.f;
-    const float nr = fmaxf(wave_sum(fr), 1.f);
-    const float m_le1 = wave_sum(le1) / nr, m_le2 = wave_sum(le2) / nr;
-    const float v_le1 = wave_sum(le1 * le1) / nr - m_le1 * m_le1;
-    const float v_le2 = wave_sum(le2 * le2) / nr - m_le2 * m_le2;
-    const float m_c1 = wave_sum(c1) / nr, m_c2 = wave_sum(c2) / nr;
-    const float v_c1 = wave_sum(c1 * c1) / nr - m_c1 * m_c1;
-    const float v_c2 = wave_sum(c2 * c2) / nr - m_c2 * m_c2;
-    const float f_v = wave_sum(tv) / nr, f_e = wave_sum(te) / nr, f_f = wave_sum(tf) / nr;
-    const float m_chain = wave_sum(chain) / nr;
+    const float nr = fmaxf(wave_sum_dpp(fr), 1.f);
+    const float m_le1 = wave_sum_dpp(le1) / nr, m_le2 = wave_sum_dpp(le2) / nr;
+    const float v_le1 = wave_sum_dpp(le1 * le1) / nr - m_le1 * m_le1;
+    const float v_le2 = wave_sum_dpp(le2 * le2) / nr - m_le2 * m_le2;
+    const float m_c1 = wave_sum_dpp(c1) / nr, m_c2 = wave_sum_dpp(c2) / nr;
+    const float v_c1 = wave_sum_dpp(c1 * c1) / nr - m_c1 * m_c1;
+    const float v_c2 = wave_sum_dpp(c2 * c2) / nr - m_c2 * m_c2;
+    const float f_v = wave_sum_dpp(tv) / nr, f_e = wave_sum_dpp(te) / nr, f_f = wave_sum_dpp(tf) / nr;
+    const float m_chain = wave_sum_dpp(chain) / nr;
     const long long cand_it = wave_sum(cnd);
-    const float m_se1 = wave_sum(se1) / 64.f, m_se2 = wave_sum(se2) / 64.f;
-    const float v_se1 = wave_sum(se1 * se1) / 64.f - m_se1 * m_se1;
-    const float v_se2 = wave_sum(se2 * se2) / 64.f - m_se2 * m_se2;
-    const float m_sc1 = wave_sum(sc1) / 64.f, m_sc2 = wave_sum(sc2) / 64.f;
-    const float v_sc1 = wave_sum(sc1 * sc1) / 64.f - m_sc1 * m_sc1;
-    const float v_sc2 = wave_sum(sc2 * sc2) / 64.f - m_sc2 * m_sc2;
-    const float g_v = wave_sum(sv) / 64.f, g_e = wave_sum(se) / 64.f, g_f = wave_sum(sf) / 64.f;
+    const float m_se1 = wave_sum_dpp(se1) / 64.f, m_se2 = wave_sum_dpp(se2) / 64.f;
+    const float v_se1 = wave_sum_dpp(se1 * se1) / 64.f - m_se1 * m_se1;
+    const float v_se2 = wave_sum_dpp(se2 * se2) / 64.f - m_se2 * m_se2;
+    const float m_sc1 = wave_sum_dpp(sc1) / 64.f, m_sc2 = wave_sum_dpp(sc2) / 64.f;
+    const float v_sc1 = wave_sum_dpp(sc1 * sc1) / 64.f - m_sc1 * m_sc1;
+    const float v_sc2 = wave_sum_dpp(sc2 * sc2) / 64.f - m_sc2 * m_sc2;
+    const float g_v = wave_sum_dpp(sv) / 64.f, g_e = wave_sum_dpp(se) / 64.f, g_f = wave_sum_dpp(sf) / 64.f;
     if (lane == 0) {
       const float steps = 64.f * (float)(span + 1);
       const float e1r = expf(10.f * m_le1) * n1, e1q = expf(10.f * m_se1) * n1;
@@ -315,12 +334,11 @@ __global__ __launch_bounds__(kT) void k_abs_error(const float *__restrict__ pred
                                                   const uint32_t *__restrict__ counts, int64_t n,
                                                   const int64_t *__restrict__ n_dev,
                                                   double *__restrict__ acc) {
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (n_dev) n = *n_dev;
   double e = 0.0;
-  if (i < n) e = fabs((double)expm1f(fminf(fmaxf(pred[i], 0.f), 30.f)) - (double)counts[i]);
-  e = wave_sum(e);
-  if ((threadIdx.x & 63) == 0 && e != 0.0) atomicAdd(acc, e);
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    e += fabs((double)expm1f(fminf(fmaxf(pred[i], 0.f), 30.f)) - (double)counts[i]);
+  block_atomic_add<kT>(e, acc);
 }
 
 __global__ __launch_bounds__(kT) void k_compact(const uint32_t *__restrict__ counts,
@@ -431,7 +449,8 @@ void launch_adam(float *params, const float *grad, float *m, float *v, float lr,
 void launch_abs_error(const float *pred, const uint32_t *counts, int64_t n, double *acc,
                       hipStream_t st, const int64_t *n_dev) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_abs_error, dim3(nblk(n)), dim3(kT), 0, st, pred, counts, n, n_dev, acc);
+  hipLaunchKernelGGL(k_abs_error, dim3(std::min(nblk(n), kReduceGrid)), dim3(kT), 0, st, pred,
+                     counts, n, n_dev, acc);
   STQ_HIP(hipGetLastError());
 }
 

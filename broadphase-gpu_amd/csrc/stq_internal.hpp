@@ -26,6 +26,27 @@ namespace stqamd {
 
 constexpr int kWave = 64;
 
+// Most blocks a grid-stride kernel that ends in one atomic per block launches
+// (8 blocks of 256 threads per CU on 256 CUs): atomics to one address
+// serialise in one L2 channel, so a counter takes at most this many.
+constexpr unsigned kReduceGrid = 2048;
+
+// Adds the sum of v over the block (NT threads, all of them call this) to
+// *dst with one atomic: wave sums by shuffles, then the block's waves in LDS.
+template <int NT, class T>
+__device__ __forceinline__ void block_atomic_add(T v, T *dst) {
+  __shared__ T part[NT / kWave];
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T s = part[0];
+    for (int w = 1; w < NT / kWave; ++w) s += part[w];
+    if (s != T(0)) atomicAdd(dst, s);
+  }
+}
+
 // Type bits of a vertex-id triple (src/aabb.cu:12-30).
 __host__ __device__ inline int type_bit(int v1, int v2) {
   return v2 >= 0 ? 4 : (v1 >= 0 ? 2 : 1);

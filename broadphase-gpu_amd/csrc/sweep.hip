@@ -192,42 +192,28 @@ __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__r
 }
 
 // ---- chain ends -----------------------------------------------------------------
-// hi[s] = last q with smin[q] <= smax[s] (q >= s since smin[s] <= smax[s]);
-// galloping search forward from s, then bisection.  Adds sum(hi - s) to *cand.
+// hi[s] = last q with smin[q] <= smax[s] (q >= s since smin[s] <= smax[s]),
+// by last_le; grid-stride over a capped grid, adding sum(hi - s) to *cand
+// with one atomic per block.
 template <class T>
 __global__ __launch_bounds__(kThreads) void k_chain_end(const T *__restrict__ smin,
                                                         const T *__restrict__ smax, int64_t n,
                                                         int *__restrict__ hi,
                                                         unsigned long long *__restrict__ cand) {
-  const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   unsigned long long len = 0;
-  if (s < n) {
-    const T x = smax[s];
-    int64_t lo = s, step = 1, up;
-    while (true) {
-      const int64_t probe = lo + step;
-      if (probe >= n) { up = n; break; }
-      if (smin[probe] <= x) { lo = probe; step <<= 1; }
-      else { up = probe; break; }
-    }
-    // smin[lo] <= x, and up == n or smin[up] > x
-    while (up - lo > 1) {
-      const int64_t mid = lo + ((up - lo) >> 1);
-      if (smin[mid] <= x) lo = mid; else up = mid;
-    }
+  for (int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * kThreads) {
+    const int64_t lo = last_le(smin, s, n, smax[s]);
     hi[s] = (int)lo;
-    len = (unsigned long long)(lo - s);
+    len += (unsigned long long)(lo - s);
   }
-  // wave reduction then one atomic per wave
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) len += __shfl_xor(len, o, kWave);
-  if ((threadIdx.x & 63) == 0 && len) atomicAdd(cand, len);
+  block_atomic_add<kThreads>(len, cand);
 }
 
 // ---- work planning ----------------------------------------------------------------
-// One block per window of W sorted rows (W a power of two, 64..2048): rows are
-// ordered by (hi, position) with a bitonic sort in LDS and cut into groups of
-// 64.  Group g covers rowlist[64g .. 64g+nrows); plan_group splits it into a
+// Blocks take windows of W sorted rows (W a power of two, 64..2048) in
+// grid-stride order: rows are ordered by (hi, position) with a bitonic sort in
+// LDS and cut into groups of 64.  Group g covers rowlist[64g .. 64g+nrows); plan_group splits it into a
 // body over [min p + 1, cut] and, for rows with much longer chains, a tail.
 // nchunks[g] = number of items of the group.
 constexpr int kMaxWindow = 2048;
@@ -241,55 +227,59 @@ __global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi, i
                                                    uint32_t *__restrict__ nchunks,
                                                    unsigned long long *__restrict__ lane_steps) {
   __shared__ uint64_t key[kMaxWindow];
-  const int64_t wbase = (int64_t)blockIdx.x * W;
-  const int cnt = (int)std::min<int64_t>(W, n - wbase);
-  for (int j = threadIdx.x; j < W; j += kThreads) {
-    uint64_t k = ~0ull;
-    if (j < cnt) k = ((uint64_t)(uint32_t)hi[wbase + j] << 32) | (uint32_t)j;
-    key[j] = k;
-  }
-  __syncthreads();
-  // bitonic sort ascending of W keys
-  for (int size = 2; size <= W; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int j = threadIdx.x; j < W; j += kThreads) {
-        const int pj = j ^ stride;
-        if (pj > j) {
-          const bool up = (j & size) == 0;
-          const uint64_t a = key[j], b = key[pj];
-          if ((a > b) == up) {
-            key[j] = b;
-            key[pj] = a;
+  const int64_t nwin = (n + W - 1) / W;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long steps = 0;
+  for (int64_t b = blockIdx.x; b < nwin; b += gridDim.x) {  // grid-stride over windows
+    const int64_t wbase = b * W;
+    const int cnt = (int)std::min<int64_t>(W, n - wbase);
+    for (int j = threadIdx.x; j < W; j += kThreads) {
+      uint64_t k = ~0ull;
+      if (j < cnt) k = ((uint64_t)(uint32_t)hi[wbase + j] << 32) | (uint32_t)j;
+      key[j] = k;
+    }
+    __syncthreads();
+    // bitonic sort ascending of W keys
+    for (int size = 2; size <= W; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int j = threadIdx.x; j < W; j += kThreads) {
+          const int pj = j ^ stride;
+          if (pj > j) {
+            const bool up = (j & size) == 0;
+            const uint64_t a = key[j], c = key[pj];
+            if ((a > c) == up) {
+              key[j] = c;
+              key[pj] = a;
+            }
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
-  }
-  for (int j = threadIdx.x; j < cnt; j += kThreads)
-    rowlist[wbase + j] = (int)(wbase + (int64_t)(key[j] & 0xffffffffu));
-  // one wave per group
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int ng = (cnt + 63) / 64;
-  unsigned long long steps = 0;
-  for (int k = w; k < ng; k += kThreads / 64) {
-    const int j = 64 * k + lane;
-    const bool act = j < cnt;
-    const int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
-    const int h = act ? (int)(key[j] >> 32) : INT_MIN;
-    const int nrows = min(64, cnt - 64 * k);
-    const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, false,
-                                    tail_cost);
-    const int64_t g = (wbase >> 6) + k;
-    if (lane == 0) {
-      groups[g] = gp.body;
-      tgroups[g] = gp.tail;
-      gchunk[g] = gp.chunk;
-      nchunks[g] = gp.nitems;
-      steps += gp.steps;
+    for (int j = threadIdx.x; j < cnt; j += kThreads)
+      rowlist[wbase + j] = (int)(wbase + (int64_t)(key[j] & 0xffffffffu));
+    // one wave per group
+    const int ng = (cnt + 63) / 64;
+    for (int k = w; k < ng; k += kThreads / 64) {
+      const int j = 64 * k + lane;
+      const bool act = j < cnt;
+      const int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
+      const int h = act ? (int)(key[j] >> 32) : INT_MIN;
+      const int nrows = min(64, cnt - 64 * k);
+      const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, false,
+                                      tail_cost);
+      const int64_t g = (wbase >> 6) + k;
+      if (lane == 0) {
+        groups[g] = gp.body;
+        tgroups[g] = gp.tail;
+        gchunk[g] = gp.chunk;
+        nchunks[g] = gp.nitems;
+        steps += gp.steps;
+      }
     }
+    __syncthreads();  // the next window reuses key[]
   }
-  if (lane == 0 && steps) atomicAdd(lane_steps, steps);
+  block_atomic_add<kThreads>(steps, lane_steps);
 }
 
 // Items of every group: the body's chunks, then the tail's (items[ioff[g]..]).
@@ -854,12 +844,13 @@ void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, i
 void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, int *hi,
                       unsigned long long *cand, hipStream_t st) {
   if (n <= 0) return;
+  const unsigned g = std::min(blocks_for(n), kReduceGrid);
   if (f32)
-    hipLaunchKernelGGL(k_chain_end<float>, dim3(blocks_for(n)), dim3(kThreads), 0, st,
-                       (const float *)smin, (const float *)smax, n, hi, cand);
+    hipLaunchKernelGGL(k_chain_end<float>, dim3(g), dim3(kThreads), 0, st, (const float *)smin,
+                       (const float *)smax, n, hi, cand);
   else
-    hipLaunchKernelGGL(k_chain_end<double>, dim3(blocks_for(n)), dim3(kThreads), 0, st,
-                       (const double *)smin, (const double *)smax, n, hi, cand);
+    hipLaunchKernelGGL(k_chain_end<double>, dim3(g), dim3(kThreads), 0, st, (const double *)smin,
+                       (const double *)smax, n, hi, cand);
   STQ_HIP(hipGetLastError());
 }
 
@@ -868,8 +859,9 @@ void launch_plan(const int *hi, int64_t n, int W, int chunk, float tail_cost, in
                  unsigned long long *lane_steps, hipStream_t st) {
   if (n <= 0) return;
   const int64_t nwin = (n + W - 1) / W;
-  hipLaunchKernelGGL(k_plan, dim3((unsigned)nwin), dim3(kThreads), 0, st, hi, n, W, chunk,
-                     tail_cost, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
+  hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(nwin, kReduceGrid)), dim3(kThreads),
+                     0, st, hi, n, W, chunk, tail_cost, rowlist, groups, tgroups, gchunk, nchunks,
+                     lane_steps);
   STQ_HIP(hipGetLastError());
 }
 

@@ -66,6 +66,48 @@ __device__ inline int scene_of(const int64_t *__restrict__ seg_off, int nscenes,
   return lo;
 }
 
+// Chain end: the last q in [s, end) with smin[q] <= x, given smin[s] <= x and
+// smin non-decreasing.  Each round issues K independent probes (indices
+// clamped into [lo, up), so every load is in bounds) and keeps the bracket
+// smin[lo] <= x, up == end or smin[up] > x: galloping with the step growing
+// (K+1)-fold, then (K+1)-ary cuts.  A chain of L takes about 2 log_{K+1} L
+// dependent load rounds instead of the 2 log_2 L of a binary search.
+template <class T>
+__device__ __forceinline__ int64_t last_le(const T *__restrict__ smin, int64_t s, int64_t end,
+                                           T x) {
+  constexpr int K = 4;
+  int64_t lo = s, up = end, t = 1;
+  while (true) {
+    int c = 0;
+#pragma unroll
+    for (int i = 1; i <= K; ++i) {
+      const int64_t p = lo + t * i;
+      c += (p < up) & (smin[p < up ? p : up - 1] <= x);
+    }
+    if (c < K) {
+      const int64_t nlo = lo + t * c, nup = lo + t * (c + 1);
+      lo = nlo;
+      up = nup < up ? nup : up;
+      break;
+    }
+    lo += t * K;
+    t *= K + 1;
+  }
+  while (up - lo > 1) {
+    const int64_t st = (up - lo + K) / (K + 1);
+    int c = 0;
+#pragma unroll
+    for (int i = 1; i <= K; ++i) {
+      const int64_t p = lo + st * i;
+      c += (p < up) & (smin[p < up ? p : up - 1] <= x);
+    }
+    const int64_t nlo = lo + st * c, nup = lo + st * (c + 1);
+    lo = nlo;
+    up = nup < up ? nup : up;
+  }
+  return lo;
+}
+
 void launch_axis_stats(const stq_aabb_f64 *boxes, int64_t n, double *partial, int nblocks,
                        hipStream_t st);
 void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, void *keys,
