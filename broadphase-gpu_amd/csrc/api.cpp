@@ -272,7 +272,6 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   }
   STQ_HIP(hipEventRecord(ev[E_FILL], st));
   STQ_HIP(hipEventRecord(ev[E_COMPACT], st));
-  launch_translate(dst, total, order, st);
   STQ_HIP(hipEventRecord(ev[E_END], st));
   finish_stats(st);
 }
@@ -356,7 +355,6 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
     launch_sweep(f32, kFill, batch, nblocks, fa, st);
   }
   STQ_HIP(hipEventRecord(ev[E_COMPACT], st));
-  launch_translate(dst, total, sa.orig, st);
   STQ_HIP(hipEventRecord(ev[E_END], st));
   double mae = 0;
   STQ_HIP(hipMemcpyAsync(&mae, acc, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -439,7 +437,7 @@ void Context::finish_stats(hipStream_t st) {
   stats.ms_plan = ev_ms(ev[E_GATHER], ev[E_PLAN]);
   stats.ms_predict = ev_ms(ev[E_PLAN], ev[E_PRED]);
   if (learned_pass) {
-    // predict | (scratch) | learned pass | scan | compact + refill | translate
+    // predict | (scratch) | learned pass | scan | compact + refill
     stats.ms_sweep_count = 0;
     stats.ms_sweep_fill = ev_ms(ev[E_COUNT], ev[E_FILL]);
     stats.ms_scan = ev_ms(ev[E_FILL], ev[E_SCAN]);
@@ -695,7 +693,6 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
       if (fa.work) fa.work = claim_counters(ctr, 1);
       launch_sweep(f32, kFill, batch, nblocks, fa, st);
     }
-    launch_translate_dev(out, total, capacity, d_orig, st);
     launch_scene_pairs(d_off, d_ioff, t, bp.dev, d_pairoff, st);
     STQ_HIP(hipMemcpyAsync(hbuf, d_pairoff, sizeof(int64_t) * ((size_t)nscenes + 1),
                            hipMemcpyDeviceToHost, st));

@@ -285,19 +285,6 @@ __global__ __launch_bounds__(kT) void k_scene_pairs(const int64_t *__restrict__ 
   if (s <= nscenes) pair_off[s] = off[ioff[sfg[s]]];
 }
 
-__global__ __launch_bounds__(kT) void k_translate_dev(int2 *__restrict__ pairs,
-                                                      const int64_t *__restrict__ total,
-                                                      int64_t capacity,
-                                                      const uint32_t *__restrict__ orig) {
-  const int64_t cnt = *total;
-  if (cnt > capacity) return;  // nothing was written
-  const int64_t stride = (int64_t)gridDim.x * kT;
-  for (int64_t k = (int64_t)blockIdx.x * kT + threadIdx.x; k < cnt; k += stride) {
-    const int2 e = pairs[k];
-    const int oq = (int)orig[e.y];
-    pairs[k] = make_int2(min(e.x, oq), max(e.x, oq));
-  }
-}
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
@@ -418,14 +405,6 @@ void launch_scene_pairs(const int64_t *off, const int64_t *ioff, const BatchTabl
   STQ_HIP(hipGetLastError());
 }
 
-void launch_translate_dev(stq_pair *pairs, const int64_t *total, int64_t bound,
-                          const uint32_t *orig, hipStream_t st) {
-  if (bound <= 0) return;
-  const unsigned g = (unsigned)std::min<int64_t>(nblk(bound), 16384);
-  hipLaunchKernelGGL(k_translate_dev, dim3(g), dim3(kT), 0, st, reinterpret_cast<int2 *>(pairs),
-                     total, bound, orig);
-  STQ_HIP(hipGetLastError());
-}
 
 void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t st) {
   if (n <= 0) return;

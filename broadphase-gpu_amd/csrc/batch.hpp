@@ -67,10 +67,6 @@ void launch_plan_seg(const int *hi, const BatchTables &t, const BatchDev &d, int
 // pair_off[s] = off[ioff[scene_first_group[s]]] for s = 0..nscenes.
 void launch_scene_pairs(const int64_t *off, const int64_t *ioff, const BatchTables &t,
                         const BatchDev &d, int64_t *pair_off, hipStream_t st);
-// pairs[k] for k < *total (device count): (row id, sorted q) -> canonical ids;
-// nothing happens when *total > bound (the output capacity).
-void launch_translate_dev(stq_pair *pairs, const int64_t *total, int64_t bound,
-                          const uint32_t *orig, hipStream_t st);
 void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t st);
 void launch_fill_u64(unsigned long long *p, unsigned long long v, int64_t n, hipStream_t st);
 

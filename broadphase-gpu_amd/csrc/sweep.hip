@@ -355,19 +355,22 @@ __device__ __forceinline__ void row_span(bool act, int p, int myhi, int &pmax, i
   hmin = uniform(hmin);
 }
 
-// Emission of the pairs of the rows in hm with sorted q (wave aggregated).
+// Emission of the pairs of the rows in hm (wave aggregated), as canonical
+// (min, max) pairs of input ids: op is the row's input id, oq that of the
+// partner (both unused when counting).
 template <int MODE>
 struct Emit {
   unsigned cnt = 0;
   int64_t base0 = 0, base = 0;
   unsigned cap = 0;
-  __device__ __forceinline__ void put(bool h, uint64_t hm, uint32_t op, int q, int2 *out) {
+  __device__ __forceinline__ void put(bool h, uint64_t hm, uint32_t op, uint32_t oq, int2 *out) {
+    const int2 pr = make_int2((int)min(op, oq), (int)max(op, oq));
     if (MODE == kFill) {
-      if (h) out[base + mbcnt64(hm)] = make_int2((int)op, q);
+      if (h) out[base + mbcnt64(hm)] = pr;
       base += __popcll(hm);
     } else if (MODE == kLearned) {
       const unsigned slot = cnt + mbcnt64(hm);
-      if (h && slot < cap) out[base0 + slot] = make_int2((int)op, q);
+      if (h && slot < cap) out[base0 + slot] = pr;
       cnt += (unsigned)__popcll(hm);
     } else {
       cnt += (unsigned)__popcll(hm);
@@ -428,7 +431,11 @@ __device__ __forceinline__ void tail64(const SweepArgs &a, const Rec64 *__restri
     mq.lo1 = mq.lo2 = INFINITY;
     mq.hi1 = mq.hi2 = -INFINITY;
     mq.v0 = mq.v1 = mq.v2 = mq.type = 0;
-    if (inq) mq = rec[q];
+    uint32_t oq = 0;
+    if (inq) {
+      mq = rec[q];
+      if (MODE != kCount) oq = a.orig[q];
+    }
     for (int r = r0; r < R; ++r) {
       const int p = readlane_i(rp, r), hp = readlane_i(rh, r);
       const double lo1 = readlane_f64(rr.lo1, r), hi1 = readlane_f64(rr.hi1, r);
@@ -441,7 +448,7 @@ __device__ __forceinline__ void tail64(const SweepArgs &a, const Rec64 *__restri
                      (v1 != mq.v1) & (v1 != mq.v2) & (v2 != mq.v0) & (v2 != mq.v1) &
                      (v2 != mq.v2);
       const uint64_t hm = ballot(h);
-      if (hm) em.put(h, hm, (uint32_t)readlane_i((int)rop, r), q, out);
+      if (hm) em.put(h, hm, (uint32_t)readlane_i((int)rop, r), oq, out);
     }
   }
 }
@@ -490,7 +497,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep64(SweepArgs a, const Rec64 *
                        (mine.v1 != rq.v0) & (mine.v1 != rq.v1) & (mine.v1 != rq.v2) &
                        (mine.v2 != rq.v0) & (mine.v2 != rq.v1) & (mine.v2 != rq.v2);
         const uint64_t hm = ballot(h);
-        if (hm) em.put(h, hm, op, q, out);
+        if (hm) em.put(h, hm, op, MODE != kCount ? a.orig[q] : 0u, out);
       };
       auto batch = [&](const Rec64 *b, int q, bool check) __attribute__((always_inline)) {
         uint64_t m[B];
@@ -627,7 +634,9 @@ __device__ __forceinline__ void tailq(const SweepArgs &a, const Exact64 *__restr
         }
         nconf += (unsigned long long)__popcll(ballot(hg));
         const uint64_t hm = ballot(h);
-        if (hm) em.put(h, hm, (uint32_t)readlane_i((int)rop, r), q, out);
+        if (hm)
+          em.put(h, hm, (uint32_t)readlane_i((int)rop, r), MODE != kCount && h ? a.orig[q] : 0u,
+                 out);
       }
     }
   }
@@ -691,8 +700,12 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
         const uint64_t hm = ballot(h);
         if (hm) {
           uint32_t o = 0;
-          if (MODE != kCount && h) o = a.orig[cp];
-          em.put(h, hm, o, cq, out);
+          uint32_t oq = 0;
+          if (MODE != kCount && h) {
+            o = a.orig[cp];
+            oq = a.orig[cq];
+          }
+          em.put(h, hm, o, oq, out);
         }
       };
       auto drain = [&]() __attribute__((always_inline)) {
@@ -793,16 +806,6 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
     k = wc ? claim_take(ws, pend) : k + ws.nwaves;
   }
   if (MODE != kFill && lane == 0 && nconf) atomicAdd(a.confirms, nconf);
-}
-
-// (input id of the row, sorted q) -> canonical (min id, max id).
-__global__ __launch_bounds__(kThreads) void k_translate(int2 *__restrict__ pairs, int64_t count,
-                                                        const uint32_t *__restrict__ orig) {
-  const int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (k >= count) return;
-  const int2 e = pairs[k];
-  const int oq = (int)orig[e.y];
-  pairs[k] = make_int2(min(e.x, oq), max(e.x, oq));
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
@@ -907,11 +910,5 @@ void launch_sweep(bool f32, int mode, int batch, int nblocks, const SweepArgs &a
   STQ_HIP(hipGetLastError());
 }
 
-void launch_translate(stq_pair *pairs, int64_t count, const uint32_t *orig, hipStream_t st) {
-  if (count <= 0) return;
-  hipLaunchKernelGGL(k_translate, dim3(blocks_for(count)), dim3(kThreads), 0, st,
-                     reinterpret_cast<int2 *>(pairs), count, orig);
-  STQ_HIP(hipGetLastError());
-}
 
 }  // namespace stqamd

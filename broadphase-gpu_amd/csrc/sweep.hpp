@@ -153,10 +153,9 @@ struct SweepArgs {
   unsigned *work = nullptr;
 };
 
-// Pairs come out as (input id of the row, sorted q); launch_translate turns
-// them into canonical input-id pairs.
+// Pairs come out as canonical (min, max) pairs of input ids (orig[] of both
+// sorted positions), in the emission order of the sweep.
 void launch_sweep(bool f32, int mode, int batch, int nblocks, const SweepArgs &args,
                   hipStream_t st);
-void launch_translate(stq_pair *pairs, int64_t count, const uint32_t *orig, hipStream_t st);
 
 }  // namespace stqamd
