@@ -36,6 +36,9 @@
 
 namespace stqamd {
 
+// v_writelane_b32: lane l of the result is `src` (uniform), the others keep `old`.
+__device__ int writelane_i32(int src, int l, int old) __asm("llvm.amdgcn.writelane.i32");
+
 namespace {
 
 constexpr int kThreads = 256;
@@ -339,6 +342,19 @@ __device__ __forceinline__ unsigned claim_issue(unsigned *ctr) {
 }
 __device__ __forceinline__ int64_t claim_take(const WaveSlice &ws, unsigned v) {
   return ws.kbeg + (int64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// Rows of a batch of positions q .. q+nb-1 near the ends of an item's range
+// (reduced-precision sweep; its batches of 8 gain from one mask per batch):
+// some = rows whose chain (p, hi] meets the batch, full = rows whose chain
+// holds all of it.  False when no row meets the batch (it is skipped); when
+// some == full the batch needs no per-position range test.
+__device__ __forceinline__ bool batch_rows(int p, int myhi, int q, int nb, uint64_t &some,
+                                           uint64_t &full) {
+  some = ballot(p < q + nb - 1) & ballot(myhi >= q);
+  if (some == 0) return false;
+  full = ballot(p < q) & ballot(myhi >= q + nb - 1);
+  return true;
 }
 
 // Bounds of the unchecked middle part: every active lane has p < q <= hi_p
@@ -730,20 +746,25 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
         nq = 0;
       };
       auto push = [&](int q, uint64_t m) __attribute__((always_inline)) {
-        if (lane == nq) {
-          eq = q;
-          emlo = (uint32_t)m;
-          emhi = (uint32_t)(m >> 32);
-        }
+        eq = writelane_i32(q, nq, eq);
+        emlo = (uint32_t)writelane_i32((int)(uint32_t)m, nq, (int)emlo);
+        emhi = (uint32_t)writelane_i32((int)(uint32_t)(m >> 32), nq, (int)emhi);
         ++nq;
       };
       auto batch = [&](const QRec *b, int q, bool check) __attribute__((always_inline)) {
+        uint64_t keep = ~0ull;
+        bool per_q = false;
+        if (check) {
+          uint64_t full;
+          if (!batch_rows(p, myhi, q, B, keep, full)) return;
+          per_q = keep != full;
+        }
         uint64_t m[B];
         uint64_t any = 0;
   #pragma unroll
         for (int j = 0; j < B; ++j) {
-          m[j] = ballot(level_overlap(mq.hi, mq.lo, b[j].lo, b[j].hi));
-          if (check) m[j] &= ballot(q + j > p) & ballot(q + j <= myhi);
+          m[j] = ballot(level_overlap(mq.hi, mq.lo, b[j].lo, b[j].hi)) & keep;
+          if (per_q) m[j] &= ballot(q + j > p) & ballot(q + j <= myhi);
           any |= m[j];
         }
         if (any) {
