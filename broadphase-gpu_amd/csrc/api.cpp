@@ -1095,7 +1095,11 @@ int stq_orient_pairs_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n, s
     unsigned *err = flag_word(c);
     timed_kernel(c, count, err,
                  "pair of another kind than vertex-face or edge-edge, or id out of range",
-                 [&] { stqamd::launch_orient_pairs(d_boxes, n, d_pairs, count, err, c->stream); });
+                 [&] {
+                   uint8_t *kinds = c->box_kinds.as<uint8_t>((size_t)std::max<int64_t>(n, 1));
+                   stqamd::launch_box_kinds(d_boxes, n, kinds, c->stream);
+                   stqamd::launch_orient_pairs(kinds, n, d_pairs, count, err, c->stream);
+                 });
   });
 }
 
@@ -1110,7 +1114,9 @@ int stq_ccd_queries_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n, co
     unsigned *err = flag_word(c);
     timed_kernel(c, count, err,
                  "pair of another kind, id out of range, or vertex id outside [0, nv)", [&] {
-                   stqamd::launch_ccd_queries(d_boxes, n, d_v0, d_v1, nv, d_pairs, count,
+                   uint8_t *kinds = c->box_kinds.as<uint8_t>((size_t)std::max<int64_t>(n, 1));
+                   stqamd::launch_box_kinds(d_boxes, n, kinds, c->stream);
+                   stqamd::launch_ccd_queries(d_boxes, kinds, n, d_v0, d_v1, nv, d_pairs, count,
                                               min_separation, d_queries, err, c->stream);
                  });
   });

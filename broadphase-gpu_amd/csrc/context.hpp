@@ -122,6 +122,7 @@ struct Context {
   bool learned_pass = false;  // the last call used the single learned pass
   // batched pipeline
   DevBuf b_tables, b_skey, b_orig, b_pairoff, mesh_boxes;
+  DevBuf box_kinds;  // narrow-phase hand-off: one kind byte per box
   PinnedBuf b_host;
   BatchPlan bplan;
   int64_t b_budget = 0;      // scratch pairs of the learned batched pass
@@ -134,7 +135,8 @@ struct Context {
   ~Context() {
     bplan.drop_graph();
     b_host.release();
-    DevBuf *bb[] = {&b_tables, &b_skey, &b_orig, &b_pairoff, &tgroups, &gchunks, &mesh_boxes};
+    DevBuf *bb[] = {&b_tables, &b_skey,  &b_orig,     &b_pairoff,
+                    &tgroups,  &gchunks, &mesh_boxes, &box_kinds};
     for (DevBuf *b : bb) b->release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
