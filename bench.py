@@ -42,6 +42,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "boxes/s & pairs/s at exact pair set: chosen form vs plain FP64 STQ, 1 MI355X"
 HBM_PEAK = 8.0e12
+# one wave64 VALU instruction per cycle per CU (four 16-lane SIMDs, four cycles
+# each), 256 CUs at 2.4 GHz: the issue ceiling of the comparison-bound sweep
+VALU_ISSUE_PEAK = 256 * 2.4e9
 
 
 def progress(msg):
@@ -472,14 +475,24 @@ def main():
     P = r["pairs"]
     alg_bytes = 60.0 * n + 8.0 * P
     achieved = alg_bytes / (r["fill_sweep_ms"] * 1e-3)
-    traffic = None
+    # HBM bytes and VALU wave instructions per launch of this kernel, from the
+    # committed rocprof counter runs of the same workload
+    traffic, valu = None, None
     tpath = os.path.join(ROOT, "profiles", "sweep_traffic_cloth708.json")
     if os.path.exists(tpath):
         try:
             with open(tpath) as fh:
-                traffic = json.load(fh).get(main_form, {}).get("hbm_bytes_per_launch")
+                entry = json.load(fh).get(main_form, {})
+            traffic = entry.get("hbm_bytes_per_launch")
+            valu = entry.get("valu_wave_instructions_per_launch")
         except (OSError, ValueError):
-            traffic = None
+            traffic, valu = None, None
+    issue = None
+    if valu:
+        rate = valu / (r["fill_sweep_ms"] * 1e-3)
+        issue = {"bound": "valu issue", "achieved": rate, "peak": VALU_ISSUE_PEAK,
+                 "unit": "wave instructions/s", "frac": rate / VALU_ISSUE_PEAK,
+                 "valu_per_launch": valu}
 
     cpu = None
     if args.cpu_sample > 0:
@@ -541,7 +554,8 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": "emitting sweep (k_qsweep learned pass for the chosen form)",
                      "algorithmic_bytes_per_launch": alg_bytes,
-                     "kernel_ms": r["fill_sweep_ms"]},
+                     "kernel_ms": r["fill_sweep_ms"],
+                     "issue": issue},
         "cpu_baseline": cpu,
     }
     if args.host_e2e and "chosen" in want:
