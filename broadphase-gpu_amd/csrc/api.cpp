@@ -310,17 +310,22 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
   launch_sweep(f32, kLearned, batch, nblocks, sa, st);
   STQ_HIP(hipEventRecord(ev[E_FILL], st));
   scan_u32_to_i64(d_counts, d_off, nitems, scan_tmp, st);
-  int64_t total = 0;
-  STQ_HIP(hipMemcpyAsync(&total, d_off + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  const int64_t *d_total = d_off + nitems;
   double *acc = mae_acc.as<double>(1);
   STQ_HIP(hipMemsetAsync(acc, 0, sizeof(double), st));
   launch_abs_error(d_pred, d_counts, nitems, acc, st);
   STQ_HIP(hipEventRecord(ev[E_SCAN], st));
-  STQ_HIP(hipStreamSynchronize(st));
-  *count = total;
-  stats.n_pairs = total;
+  // Compaction and the exact refill check the device total against the
+  // capacity themselves (nothing is written when it does not fit), and the
+  // refill reads the number of overflowed items from the device: the only
+  // host synchronisation left is the final one.  A library-allocated output
+  // needs the total first.
   stq_pair *dst = out;
+  int64_t cap_dev = capacity;
   if (alloc_out) {
+    int64_t total = 0;
+    STQ_HIP(hipMemcpyAsync(&total, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    STQ_HIP(hipStreamSynchronize(st));
     void *p = nullptr;
     if (hipMalloc(&p, sizeof(stq_pair) * (size_t)std::max<int64_t>(total, 1)) != hipSuccess) {
       (void)hipGetLastError();
@@ -328,38 +333,42 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
     }
     dst = static_cast<stq_pair *>(p);
     *alloc_out = dst;
-  } else if (total > capacity) {
-    for (int e = E_COMPACT; e < E_NUM; ++e) STQ_HIP(hipEventRecord(ev[e], st));
-    finish_stats(st);
-    throw StatusError(STQ_ERR_CAPACITY, "output capacity too small");
+    cap_dev = total;
   }
   int *d_ovf = ovf.as<int>((size_t)nitems);
   unsigned long long *d_novf = counters.as<unsigned long long>(kCounters) + 4;
   STQ_HIP(hipMemsetAsync(d_novf, 0, sizeof(unsigned long long), st));
   launch_compact(d_counts, d_caps, d_capoff, d_off, nitems, d_scratch,
-                 reinterpret_cast<int2 *>(dst), d_ovf, d_novf, st);
-  unsigned long long novf = 0;
-  STQ_HIP(hipMemcpyAsync(&novf, d_novf, sizeof(novf), hipMemcpyDeviceToHost, st));
-  STQ_HIP(hipStreamSynchronize(st));
-  stats.n_fallback_tiles = novf;
-  if (novf > 0) {
+                 reinterpret_cast<int2 *>(dst), d_ovf, d_novf, st, nullptr, d_total, cap_dev);
+  {
     SweepArgs fa = sa;
     fa.sel = d_ovf;
-    fa.nitems = novf;
+    fa.nitems = nitems;
+    fa.nitems_dev = reinterpret_cast<const int64_t *>(d_novf);
     fa.counts = nullptr;
     fa.confirms = nullptr;
     fa.caps = nullptr;
     fa.offsets = d_off;
     fa.out = reinterpret_cast<int2 *>(dst);
+    fa.total_dev = d_total;
+    fa.capacity = cap_dev;
     if (fa.work) fa.work = claim_counters(counters.as<unsigned long long>(kCounters), 1);
     launch_sweep(f32, kFill, batch, nblocks, fa, st);
   }
+  int64_t total = 0;
+  unsigned long long novf = 0;
+  STQ_HIP(hipMemcpyAsync(&total, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipMemcpyAsync(&novf, d_novf, sizeof(novf), hipMemcpyDeviceToHost, st));
   STQ_HIP(hipEventRecord(ev[E_COMPACT], st));
   STQ_HIP(hipEventRecord(ev[E_END], st));
   double mae = 0;
   STQ_HIP(hipMemcpyAsync(&mae, acc, sizeof(double), hipMemcpyDeviceToHost, st));
   finish_stats(st);
+  *count = total;
+  stats.n_pairs = total;
+  stats.n_fallback_tiles = (int64_t)novf;
   stats.sizing_mae = mae / (double)std::max<int64_t>(nitems, 1);
+  if (!alloc_out && total > capacity) throw StatusError(STQ_ERR_CAPACITY, "output capacity too small");
 }
 
 void Context::sizer_ensure() {
@@ -570,7 +579,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   if (learned) sizer_ensure();
   if (single_pass && b_budget == 0)
     b_budget = std::max<int64_t>(int64_t(1) << 22,
-                                 (int64_t)kCapScale * b_last_pairs + (int64_t)kCapSlack * b_last_items);
+                                 (int64_t)kBatchCapScale * b_last_pairs +
+                                     (int64_t)kBatchCapSlack * b_last_items);
 
   // every buffer is reserved here: the launches below allocate nothing
   const size_t ts = f32 ? 4 : 8;
@@ -667,7 +677,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
       launch_sweep(f32, kFill, batch, nblocks, sa, st);
     } else {
       launch_mlp_forward(d_w, d_feat, bound, d_pred, st, sa.nitems_dev);
-      launch_caps(d_pred, d_cand, bound, kCapScale, kCapSlack, d_caps, st, sa.nitems_dev);
+      launch_caps(d_pred, d_cand, bound, kBatchCapScale, kBatchCapSlack, d_caps, st, sa.nitems_dev);
       scan_u32_to_i64(d_caps, d_capoff, bound, scan_tmp, st);
       launch_cap_budget(d_caps, d_capoff, bound, b_budget, st);
       sa.offsets = d_capoff;

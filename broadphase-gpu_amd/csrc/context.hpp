@@ -19,11 +19,17 @@
 namespace stqamd {
 
 // Learned sizing: Adam steps (minibatches of up to 8192 items) and learning
-// rate per training scene; slots per item = kCapScale * prediction + kCapSlack.
+// rate per training scene; slots per item = min(candidates, scale * prediction
+// + slack).  One scene: 3 * p + 64, which leaves few items to the exact refill
+// (a short, low-occupancy launch of the densest items) for a scratch of about
+// three times the pairs.  Batched scenes: 2 * p + 32; there overflow is rare
+// and the scratch budget of the whole batch stays at twice its pairs.
 constexpr int kTrainSteps = 64;
 constexpr float kTrainRate = 2e-3f;
-constexpr float kCapScale = 2.0f;
-constexpr float kCapSlack = 32.f;
+constexpr float kCapScale = 3.0f;
+constexpr float kCapSlack = 64.f;
+constexpr float kBatchCapScale = 2.0f;
+constexpr float kBatchCapSlack = 32.f;
 // Replay ring of training rows: each training scene adds a strided sample of
 // up to kReplayPerScene of its items.
 constexpr int64_t kReplay = 262144;
