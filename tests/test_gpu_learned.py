@@ -86,7 +86,7 @@ def test_learned_pipeline_is_exact(ctx, prec):
 
 
 def test_forced_fallback_is_exact(ctx):
-    """Predictions of zero pairs give every item 16 slots: items with more pairs
+    """Predictions of zero pairs give every item 64 slots: items with more pairs
     overflow and are redone exactly."""
     ctx.sizer_reset()
     w = np.zeros(mlp.NPARAMS, np.float32)
@@ -101,6 +101,38 @@ def test_forced_fallback_is_exact(ctx):
     # same order as the count-then-fill path
     ctx.set_options(precision=stq_amd.PRECISION_F32_CONFIRM)
     assert np.array_equal(ctx.find_pairs(b), got)
+
+
+def test_learned_capacity_reported(ctx):
+    """The learned single pass into a caller's device buffer that is too small:
+    the call reports the count needed and writes nothing (compaction and the
+    exact refill of overflowed items check the total on the device); with that
+    capacity the set is exact and in the count-then-fill order."""
+    ctx.sizer_reset()
+    w = np.zeros(mlp.NPARAMS, np.float32)
+    w[-1] = -5.0  # predictions of about zero pairs: many items overflow too
+    ctx.sizer_set_params(w)
+    ctx.set_options(precision=stq_amd.PRECISION_F32_CONFIRM, sizing=stq_amd.SIZING_LEARNED, train_scenes=-1)
+    b = scenes.cloth_scene(g=150, seed=6, sigma=1e-2)
+    ref = oracle.reference_set(b)
+    db = stq_amd.DeviceBuffer.from_array(b)
+    sentinel = np.full(2 * 1000, -7, np.int32)
+    small = stq_amd.DeviceBuffer.from_array(sentinel)
+    with pytest.raises(stq_amd.StqError) as e:
+        ctx.find_pairs_device(db.ptr, len(b), small.ptr, 1000)
+    assert e.value.status == 3
+    st = ctx.stats()
+    assert st["n_pairs"] == len(ref) > 1000  # nothing fits, so no item is compacted or refilled
+    assert np.array_equal(small.to_array(np.int32, 2 * 1000), sentinel)
+    out = stq_amd.DeviceBuffer(8 * len(ref))
+    cnt = ctx.find_pairs_device(db.ptr, len(b), out.ptr, len(ref))
+    got = out.to_array(np.int32, 2 * cnt).reshape(-1, 2)
+    assert cnt == len(ref) and ctx.stats()["n_fallback_tiles"] > 0
+    assert np.array_equal(oracle.canonical(got), ref)
+    ctx.set_options(precision=stq_amd.PRECISION_F32_CONFIRM)
+    assert np.array_equal(ctx.find_pairs(b), got)
+    for x in (db, small, out):
+        x.close()
 
 
 def test_trained_network_beats_untrained(ctx):
