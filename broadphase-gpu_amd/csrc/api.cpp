@@ -646,7 +646,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
                     d_nch, &ctr[2], st);
     scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
     launch_items(d_groups, d_tg, d_gchunk, d_ioff, ngroups, d_items, st);
-    launch_fill_u32(d_counts, 0u, bound, st);
+    // item-level arrays are used up to the device item count only: every item
+    // below it gets its count from the sweep, and the scans stop there
     SweepArgs sa;
     sa.items = d_items;
     sa.nitems = bound;
@@ -666,7 +667,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     const int64_t *total = d_off + bound;
     if (!single_pass) {
       launch_sweep(f32, kCount, batch, nblocks, sa, st);
-      scan_u32_to_i64(d_counts, d_off, bound, scan_tmp, st);
+      scan_u32_to_i64(d_counts, d_off, bound, scan_tmp, st, sa.nitems_dev);
       sa.counts = nullptr;
       sa.confirms = nullptr;
       sa.offsets = d_off;
@@ -678,13 +679,13 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     } else {
       launch_mlp_forward(d_w, d_feat, bound, d_pred, st, sa.nitems_dev);
       launch_caps(d_pred, d_cand, bound, kBatchCapScale, kBatchCapSlack, d_caps, st, sa.nitems_dev);
-      scan_u32_to_i64(d_caps, d_capoff, bound, scan_tmp, st);
-      launch_cap_budget(d_caps, d_capoff, bound, b_budget, st);
+      scan_u32_to_i64(d_caps, d_capoff, bound, scan_tmp, st, sa.nitems_dev);
+      launch_cap_budget(d_caps, d_capoff, bound, b_budget, st, sa.nitems_dev);
       sa.offsets = d_capoff;
       sa.caps = d_caps;
       sa.out = d_scratch;
       launch_sweep(f32, kLearned, batch, nblocks, sa, st);
-      scan_u32_to_i64(d_counts, d_off, bound, scan_tmp, st);
+      scan_u32_to_i64(d_counts, d_off, bound, scan_tmp, st, sa.nitems_dev);
       launch_fill_u64(reinterpret_cast<unsigned long long *>(d_acc), 0ull, 1, st);
       launch_abs_error(d_pred, d_counts, bound, d_acc, st, sa.nitems_dev);
       launch_compact(d_counts, d_caps, d_capoff, d_off, bound, d_scratch,

@@ -205,9 +205,23 @@ __device__ inline T block_exclusive_scan(T v, T *total) {
   return x - v + add;
 }
 
+// n_dev (optional): the device-side count; n is then the host's upper bound.
+__device__ inline int64_t scan_len(int64_t n, const int64_t *n_dev) {
+  if (!n_dev) return n;
+  const int64_t m = *n_dev;
+  return m < n ? (m > 0 ? m : 0) : n;
+}
+
 template <class T>
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ in,
-                                                               int64_t n, T *__restrict__ partial) {
+                                                               int64_t n,
+                                                               const int64_t *__restrict__ n_dev,
+                                                               T *__restrict__ partial) {
+  n = scan_len(n, n_dev);
+  if ((int64_t)blockIdx.x * kScanTile >= n) {  // beyond the count: never read
+    if (threadIdx.x == 0) partial[blockIdx.x] = 0;
+    return;
+  }
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
   T s = 0;
 #pragma unroll
@@ -219,14 +233,20 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__
 }
 
 template <class T>
-__global__ __launch_bounds__(kScanThreads) void k_scan_partials(T *__restrict__ partial, int64_t m) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(T *__restrict__ partial, int64_t m,
+                                                                 int64_t n,
+                                                                 const int64_t *__restrict__ n_dev) {
+  // m block partials; only those of blocks below the count are scanned, and
+  // the total goes to partial[m]
+  const int64_t live = (scan_len(n, n_dev) + kScanTile - 1) / kScanTile;
+  const int64_t ml = live < m ? live : m;
   T carry = 0;
-  for (int64_t base = 0; base < m; base += kScanThreads) {
+  for (int64_t base = 0; base < ml; base += kScanThreads) {
     int64_t i = base + threadIdx.x;
-    T v = i < m ? partial[i] : (T)0;
+    T v = i < ml ? partial[i] : (T)0;
     T tot;
     T ex = block_exclusive_scan<T>(v, &tot);
-    if (i < m) partial[i] = ex + carry;
+    if (i < ml) partial[i] = ex + carry;
     carry += tot;
   }
   if (threadIdx.x == 0) partial[m] = carry;
@@ -234,8 +254,17 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partials(T *__restrict__ 
 
 template <class T>
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *__restrict__ in, int64_t n,
+                                                             const int64_t *__restrict__ n_dev,
                                                              const T *__restrict__ partial,
                                                              T *__restrict__ out) {
+  const int64_t nb = gridDim.x;
+  const int64_t cap = n;
+  n = scan_len(n, n_dev);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {  // the total, at the count and at the bound
+    out[n] = partial[nb];
+    if (n != cap) out[cap] = partial[nb];
+  }
+  if ((int64_t)blockIdx.x * kScanTile >= n) return;
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
   T v[kScanItems];
   T s = 0;
@@ -251,10 +280,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *__re
     if (base + k < n) out[base + k] = ex;
     ex += v[k];
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
-    const int64_t nb = gridDim.x;
-    out[n] = partial[nb];
-  }
 }
 
 template <class T>
@@ -264,12 +289,14 @@ T *scan_scratch_reserve(int64_t n, DevBuf &tmp) {
 }
 
 template <class T>
-void scan_impl(const uint32_t *in, T *out, int64_t n, DevBuf &tmp, hipStream_t st) {
+void scan_impl(const uint32_t *in, T *out, int64_t n, DevBuf &tmp, hipStream_t st,
+               const int64_t *n_dev = nullptr) {
   const int64_t nb = n > 0 ? (n + kScanTile - 1) / kScanTile : 1;
   T *partial = scan_scratch_reserve<T>(n, tmp);
-  hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n, partial);
-  hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(kScanThreads), 0, st, partial, nb);
-  hipLaunchKernelGGL(k_scan_down<T>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n,
+  hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n, n_dev,
+                     partial);
+  hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(kScanThreads), 0, st, partial, nb, n, n_dev);
+  hipLaunchKernelGGL(k_scan_down<T>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n, n_dev,
                      (const T *)partial, out);
   STQ_HIP(hipGetLastError());
 }
@@ -378,8 +405,9 @@ void radix_sort_reserve(int64_t n, int key_bytes, SortScratch &s) {
   scan_scratch_reserve<unsigned>(256 * ntiles, s.misc);
 }
 void scan_reserve_i64(int64_t n, DevBuf &tmp) { scan_scratch_reserve<int64_t>(n, tmp); }
-void scan_u32_to_i64(const uint32_t *in, int64_t *out, int64_t n, DevBuf &tmp, hipStream_t st) {
-  scan_impl<int64_t>(in, out, n, tmp, st);
+void scan_u32_to_i64(const uint32_t *in, int64_t *out, int64_t n, DevBuf &tmp, hipStream_t st,
+                     const int64_t *n_dev) {
+  scan_impl<int64_t>(in, out, n, tmp, st, n_dev);
 }
 void scan_u32(const uint32_t *in, uint32_t *out, int64_t n, DevBuf &tmp, hipStream_t st) {
   scan_impl<unsigned>(in, out, n, tmp, st);

@@ -59,38 +59,71 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
   const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kT / 64) + (threadIdx.x >> 6)));
   const int nwaves = (int)gridDim.x * (kT / 64);
   if (nitems_dev) nitems = *nitems_dev;
-  for (int64_t it = wave; it < nitems; it += nwaves) {
+  // Each wave takes a contiguous run of items.  A group's items (its body
+  // chunks, then its tail chunks) are consecutive and share their rows, so
+  // the row statistics are computed once per run of items with equal rows.
+  const int64_t per = (nitems + nwaves - 1) / nwaves;
+  const int64_t ib = (int64_t)wave * per;
+  const int64_t ie = ib + per < nitems ? ib + per : nitems;
+  int prow = -1, pnrows = -1;  // rows of the cached statistics
+  bool act = false;
+  int p = 0, h = 0;            // this lane's row and its chain end
+  double n1 = 0, n2 = 0, o1 = 0, o2 = 0;
+  float nr = 1.f, m_le1 = 0, m_le2 = 0, v_le1 = 0, v_le2 = 0, m_c1 = 0, m_c2 = 0, v_c1 = 0,
+        v_c2 = 0, f_v = 0, f_e = 0, f_f = 0, m_chain = 0;
+  for (int64_t it = ib; it < ie; ++it) {
     const Item item = items[it];
-    QParams qp = qp0;
-    int a1 = a10, a2 = a20;
-    if (sp) {
-      const SceneParams p = sp[scene_of(seg_off, nscenes, rowlist[item.rowoff])];
-      qp = p.qp;
-      a1 = p.axis == 0 ? 1 : 0;
-      a2 = p.axis == 2 ? 1 : 2;
-    }
-    const double n1 = qp.scale[a1] / 32767.0, n2 = qp.scale[a2] / 32767.0;
-    const double o1 = qp.origin[a1], o2 = qp.origin[a2];
     const int nrows = item.nrows & kRowsMask;  // tail items: their rows
-    const bool act = lane < nrows;
     const int q0 = item.q0, q1 = item.q1;
-    float fr = act ? 1.f : 0.f;
-    float le1 = 0, le2 = 0, c1 = 0, c2 = 0, tv = 0, te = 0, tf = 0, chain = 0;
+    if (item.rowoff != prow || nrows != pnrows) {
+      prow = item.rowoff;
+      pnrows = nrows;
+      QParams qp = qp0;
+      int a1 = a10, a2 = a20;
+      if (sp) {
+        const SceneParams sc = sp[scene_of(seg_off, nscenes, rowlist[item.rowoff])];
+        qp = sc.qp;
+        a1 = sc.axis == 0 ? 1 : 0;
+        a2 = sc.axis == 2 ? 1 : 2;
+      }
+      n1 = qp.scale[a1] / 32767.0;
+      n2 = qp.scale[a2] / 32767.0;
+      o1 = qp.origin[a1];
+      o2 = qp.origin[a2];
+      act = lane < nrows;
+      float fr = act ? 1.f : 0.f;
+      float le1 = 0, le2 = 0, c1 = 0, c2 = 0, tv = 0, te = 0, tf = 0, chain = 0;
+      if (act) {
+        p = rowlist[item.rowoff + lane];
+        h = hi[p];
+        const R r = rec[p];
+        le1 = logf(fmaxf((float)(r.hi1 - r.lo1), 1e-30f)) * 0.1f;
+        le2 = logf(fmaxf((float)(r.hi2 - r.lo2), 1e-30f)) * 0.1f;
+        c1 = (float)((0.5 * (r.lo1 + r.hi1) - o1) * n1);
+        c2 = (float)((0.5 * (r.lo2 + r.hi2) - o2) * n2);
+        tv = r.type == 1 ? 1.f : 0.f;
+        te = r.type == 2 ? 1.f : 0.f;
+        tf = r.type == 4 ? 1.f : 0.f;
+        chain = (float)(h - p);
+      }
+      nr = fmaxf(wave_sum_dpp(fr), 1.f);
+      m_le1 = wave_sum_dpp(le1) / nr;
+      m_le2 = wave_sum_dpp(le2) / nr;
+      v_le1 = wave_sum_dpp(le1 * le1) / nr - m_le1 * m_le1;
+      v_le2 = wave_sum_dpp(le2 * le2) / nr - m_le2 * m_le2;
+      m_c1 = wave_sum_dpp(c1) / nr;
+      m_c2 = wave_sum_dpp(c2) / nr;
+      v_c1 = wave_sum_dpp(c1 * c1) / nr - m_c1 * m_c1;
+      v_c2 = wave_sum_dpp(c2 * c2) / nr - m_c2 * m_c2;
+      f_v = wave_sum_dpp(tv) / nr;
+      f_e = wave_sum_dpp(te) / nr;
+      f_f = wave_sum_dpp(tf) / nr;
+      m_chain = wave_sum_dpp(chain) / nr;
+    }
     long long cnd = 0;
     if (act) {
-      const int p = rowlist[item.rowoff + lane];
-      const int h = hi[p];
-      const R r = rec[p];
-      le1 = logf(fmaxf((float)(r.hi1 - r.lo1), 1e-30f)) * 0.1f;
-      le2 = logf(fmaxf((float)(r.hi2 - r.lo2), 1e-30f)) * 0.1f;
-      c1 = (float)((0.5 * (r.lo1 + r.hi1) - o1) * n1);
-      c2 = (float)((0.5 * (r.lo2 + r.hi2) - o2) * n2);
-      tv = r.type == 1 ? 1.f : 0.f;
-      te = r.type == 2 ? 1.f : 0.f;
-      tf = r.type == 4 ? 1.f : 0.f;
       const int lo = max(q0, p + 1), up = min(q1, h);
       cnd = up >= lo ? (long long)(up - lo + 1) : 0;
-      chain = (float)(h - p);
     }
     const int span = q1 - q0;
     const int qs = q0 + (span > 0 ? (int)(((long long)lane * span) / 63) : 0);
@@ -101,15 +134,6 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
     const float sc2 = (float)((0.5 * (s.lo2 + s.hi2) - o2) * n2);
     const float sv = s.type == 1 ? 1.f : 0.f, se = s.type == 2 ? 1.f : 0.f,
                 sf = s.type == 4 ? 1.f : 0.f;
-    const float nr = fmaxf(wave_sum_dpp(fr), 1.f);
-    const float m_le1 = wave_sum_dpp(le1) / nr, m_le2 = wave_sum_dpp(le2) / nr;
-    const float v_le1 = wave_sum_dpp(le1 * le1) / nr - m_le1 * m_le1;
-    const float v_le2 = wave_sum_dpp(le2 * le2) / nr - m_le2 * m_le2;
-    const float m_c1 = wave_sum_dpp(c1) / nr, m_c2 = wave_sum_dpp(c2) / nr;
-    const float v_c1 = wave_sum_dpp(c1 * c1) / nr - m_c1 * m_c1;
-    const float v_c2 = wave_sum_dpp(c2 * c2) / nr - m_c2 * m_c2;
-    const float f_v = wave_sum_dpp(tv) / nr, f_e = wave_sum_dpp(te) / nr, f_f = wave_sum_dpp(tf) / nr;
-    const float m_chain = wave_sum_dpp(chain) / nr;
     const long long cand_it = wave_sum(cnd);
     const float m_se1 = wave_sum_dpp(se1) / 64.f, m_se2 = wave_sum_dpp(se2) / 64.f;
     const float v_se1 = wave_sum_dpp(se1 * se1) / 64.f - m_se1 * m_se1;
@@ -166,20 +190,11 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
 }
 
 // ---- MLP forward -----------------------------------------------------------------
-__global__ __launch_bounds__(kT) void k_mlp_forward(const float *__restrict__ params,
-                                                    const float *__restrict__ feat, int64_t n,
-                                                    const int64_t *__restrict__ n_dev,
-                                                    float *__restrict__ pred) {
-  __shared__ float w[kParams];
-  if (n_dev) n = *n_dev;
-  if ((int64_t)blockIdx.x * kT >= n) return;
-  for (int i = threadIdx.x; i < kParams; i += kT) w[i] = params[i];
-  __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
+// One item: 32 features -> 64 -> 64 -> 1 (ReLU), weights w in LDS.
+__device__ __forceinline__ float mlp_forward_one(const float *w, const float *__restrict__ fi) {
   float x[kFeat];
 #pragma unroll
-  for (int k = 0; k < kFeat; ++k) x[k] = feat[i * kFeat + k];
+  for (int k = 0; k < kFeat; ++k) x[k] = fi[k];
   float h1[kHid];
   for (int j = 0; j < kHid; ++j) {
     float s = w[kOffB1 + j];
@@ -194,8 +209,24 @@ __global__ __launch_bounds__(kT) void k_mlp_forward(const float *__restrict__ pa
     for (int k = 0; k < kHid; ++k) s += w[kOffW2 + j * kHid + k] * h1[k];
     out += w[kOffW3 + j] * fmaxf(s, 0.f);
   }
-  pred[i] = out;
+  return out;
 }
+
+__global__ __launch_bounds__(kT) void k_mlp_forward(const float *__restrict__ params,
+                                                    const float *__restrict__ feat, int64_t n,
+                                                    const int64_t *__restrict__ n_dev,
+                                                    float *__restrict__ pred) {
+  __shared__ float w[kParams];
+  if (n_dev) n = *n_dev;
+  if ((int64_t)blockIdx.x * kT >= n) return;
+  for (int i = threadIdx.x; i < kParams; i += kT) w[i] = params[i];
+  __syncthreads();
+  // grid-stride over the items: the weights are staged once per block
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    pred[i] = mlp_forward_one(w, feat + i * kFeat);
+}
+
+
 
 // ---- MLP gradient --------------------------------------------------------------------
 // One wave per block, 32 items per block.  Phase 1: lanes 0..31 run forward
@@ -305,23 +336,22 @@ __global__ __launch_bounds__(kT) void k_caps(const float *__restrict__ pred,
                                              const int64_t *__restrict__ n_dev,
                                              float alpha, float beta,
                                              uint32_t *__restrict__ caps) {
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
-  if (n_dev && i >= *n_dev) {  // beyond the device-side item count
-    caps[i] = 0;
-    return;
+  if (n_dev) n = *n_dev;  // items beyond the device-side count are never read
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    const float pr = fminf(fmaxf(pred[i], 0.f), 30.f);
+    const float want = ceilf(alpha * expm1f(pr) + beta);
+    const uint32_t c = cand[i];
+    caps[i] = want >= (float)c ? c : (uint32_t)want;
   }
-  const float pr = fminf(fmaxf(pred[i], 0.f), 30.f);
-  const float want = ceilf(alpha * expm1f(pr) + beta);
-  const uint32_t c = cand[i];
-  caps[i] = want >= (float)c ? c : (uint32_t)want;
 }
 
 __global__ __launch_bounds__(kT) void k_cap_budget(uint32_t *__restrict__ caps,
                                                    const int64_t *__restrict__ cap_off, int64_t n,
+                                                   const int64_t *__restrict__ n_dev,
                                                    int64_t budget) {
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i < n && cap_off[i] + (int64_t)caps[i] > budget) caps[i] = 0;
+  if (n_dev) n = *n_dev;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    if (cap_off[i] + (int64_t)caps[i] > budget) caps[i] = 0;
 }
 
 __global__ __launch_bounds__(kT) void k_targets(const uint32_t *__restrict__ counts, int64_t n,
@@ -403,22 +433,24 @@ void launch_features(bool f32, const SweepArgs &sa, const QParams &qp, int axis,
 void launch_mlp_forward(const float *params, const float *feat, int64_t n, float *pred,
                         hipStream_t st, const int64_t *n_dev) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_mlp_forward, dim3(nblk(n)), dim3(kT), 0, st, params, feat, n, n_dev, pred);
+  hipLaunchKernelGGL(k_mlp_forward, dim3(std::min(nblk(n), 4096u)), dim3(kT), 0, st, params, feat, n,
+                     n_dev, pred);
   STQ_HIP(hipGetLastError());
 }
 
 void launch_caps(const float *pred, const uint32_t *cand, int64_t n, float alpha, float beta,
                  uint32_t *caps, hipStream_t st, const int64_t *n_dev) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_caps, dim3(nblk(n)), dim3(kT), 0, st, pred, cand, n, n_dev, alpha, beta,
-                     caps);
+  hipLaunchKernelGGL(k_caps, dim3(std::min(nblk(n), 8192u)), dim3(kT), 0, st, pred, cand, n, n_dev,
+                     alpha, beta, caps);
   STQ_HIP(hipGetLastError());
 }
 
 void launch_cap_budget(uint32_t *caps, const int64_t *cap_off, int64_t n, int64_t budget,
-                       hipStream_t st) {
+                       hipStream_t st, const int64_t *n_dev) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_cap_budget, dim3(nblk(n)), dim3(kT), 0, st, caps, cap_off, n, budget);
+  hipLaunchKernelGGL(k_cap_budget, dim3(std::min(nblk(n), 8192u)), dim3(kT), 0, st, caps, cap_off, n,
+                     n_dev, budget);
   STQ_HIP(hipGetLastError());
 }
 

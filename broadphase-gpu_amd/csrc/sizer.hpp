@@ -36,13 +36,14 @@ void launch_features(bool f32, const SweepArgs &sa, const QParams &qp, int axis,
 // n_dev pointer is given, the count is *n_dev (n is then an upper bound).
 void launch_mlp_forward(const float *params, const float *feat, int64_t n, float *pred,
                         hipStream_t st, const int64_t *n_dev = nullptr);
-// caps[i] = min(cand[i], ceil(alpha * expm1(pred[i]) + beta)); 0 for i >= *n_dev.
+// caps[i] = min(cand[i], ceil(alpha * expm1(pred[i]) + beta)) for i < n
+// (n = *n_dev when given; nothing is written beyond).
 void launch_caps(const float *pred, const uint32_t *cand, int64_t n, float alpha, float beta,
                  uint32_t *caps, hipStream_t st, const int64_t *n_dev = nullptr);
 // caps[i] = 0 where the item's slots [cap_off[i], cap_off[i] + caps[i]) end
 // beyond `budget` (the scratch size): such items are redone exactly.
 void launch_cap_budget(uint32_t *caps, const int64_t *cap_off, int64_t n, int64_t budget,
-                       hipStream_t st);
+                       hipStream_t st, const int64_t *n_dev = nullptr);
 // targets[i] = log1p(counts[i]).
 void launch_targets(const uint32_t *counts, int64_t n, float *targets, hipStream_t st);
 // Mean squared error of MLP(feat) against targets and its gradient (kParams

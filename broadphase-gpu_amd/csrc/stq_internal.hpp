@@ -192,9 +192,12 @@ void radix_sort_u32_sync_free(uint32_t *keys, uint32_t *vals, int64_t n, int bit
 void radix_sort_reserve(int64_t n, int key_bytes, SortScratch &s);
 
 // Exclusive prefix sum of n uint32 counts into n+1 int64 offsets (out[n] is
-// the total).  Scratch is taken from `tmp`.
+// the total).  Scratch is taken from `tmp`.  With n_dev, only the first
+// m = min(*n_dev, n) counts are summed (n is then the host's upper bound):
+// out[0..m] are written, the total goes to both out[m] and out[n], and the
+// blocks beyond m return at once.
 void scan_u32_to_i64(const uint32_t *in, int64_t *out, int64_t n, DevBuf &tmp,
-                     hipStream_t st);
+                     hipStream_t st, const int64_t *n_dev = nullptr);
 // Allocates the scratch of scan_u32_to_i64 over n counts.
 void scan_reserve_i64(int64_t n, DevBuf &tmp);
 // Exclusive prefix sum of uint32 into uint32 (n+1 outputs), for sort counts.
