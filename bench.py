@@ -23,9 +23,10 @@ the roofline of the dominant kernel (the emitting sweep) and a CPU baseline
 clustered heavy-tailed scene (`skewed`), 512 scenes in one batched call
 (`batched`), a bounded sample of the learned-sizing stream
 (`learned_stream`), the PCIe-inclusive host-to-host time
-(`end_to_end_host`), and the kernels either side of the broad phase on the
+(`end_to_end_host`), the kernels either side of the broad phase on the
 cloth mesh: swept-box build, pair orientation and narrow-phase query records
-(`handoff`), each against the HBM roofline.
+(`handoff`), each against the HBM roofline, and the 100,000-box uniform scene
+with its whole CPU sweep (`small`).
 """
 from __future__ import annotations
 
@@ -78,6 +79,8 @@ def parse():
                     help="time the PCIe-inclusive host-to-host call (0 = skip)")
     ap.add_argument("--handoff", type=int, default=1,
                     help="time the swept-box build and the narrow-phase hand-off kernels (1 = on)")
+    ap.add_argument("--small", type=int, default=1,
+                    help="time the 100,000-box uniform scene, GPU forms and CPU oracle (1 = on)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (traffic calibration)")
     return ap.parse_args()
@@ -299,6 +302,57 @@ def bench_stream(args, H):
                         f"log-uniform in [0.25, 8]; first {ntrain} train the network online",
             "seconds": time.perf_counter() - t0, "training": agg(tr), "held_out": agg(held),
             "held_out_checked_vs_plain_fp64": checked, "mismatches": mismatches}
+
+
+def bench_small(args, H):
+    """The 100,000-box uniform scene (centres in the unit cube, half-extents
+    10^U(-4,-2), 1-3 vertex ids from 60,000): the reduced-precision form
+    (autotuned, count then fill; learned sizing has nothing to size at about
+    1,000 pairs) and plain FP64 on the GPU, median of `steps` calls, and the
+    oracle's CPU sweep of the whole scene on the host threads."""
+    import numpy as np
+    import stq_amd
+    from stq_amd import _lib, scenes
+    b = scenes.uniform_scene(100_000, seed=0)
+    n = len(b)
+    db = stq_amd.DeviceBuffer.from_array(b)
+    forms = {"chosen": dict(precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN, autotune=1),
+             "plain": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X, tails=2)}
+    out, sets = {"workload": f"uniform {n} boxes (seed 0)"}, {}
+    for name, kw in forms.items():
+        ctx = stq_amd.Context(0, **kw)
+        try:
+            ctx.find_pairs_device(db.ptr, n, None, 0)
+        except stq_amd.StqError as e:
+            if e.status != _lib.STQ_ERR_CAPACITY:
+                raise
+        cap = max(ctx.stats()["n_pairs"], 1)
+        d_out = stq_amd.DeviceBuffer(8 * cap)
+        for _ in range(max(args.warmup, 2)):
+            ctx.find_pairs_device(db.ptr, n, d_out.ptr, cap)
+        ms = []
+        for _ in range(max(args.steps, 1)):
+            cnt = ctx.find_pairs_device(db.ptr, n, d_out.ptr, cap)
+            ms.append(ctx.stats()["ms_total"])
+        t = float(np.median(ms))
+        out[name] = {"ms": t, "boxes_per_s": n / (t * 1e-3), "pairs": cnt}
+        sets[name] = pair_digest(d_out.to_array(np.int32, 2 * cnt))
+        d_out.close()
+        ctx.close()
+    out["speedup_vs_plain_fp64"] = out["plain"]["ms"] / out["chosen"]["ms"]
+    out["pair_sets_equal"] = sets["chosen"] == sets["plain"]
+    if args.cpu_sample > 0:
+        from oracle import oracle
+        threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 64))
+        oracle.sweep(b, axis=0, nthreads=threads)  # warm
+        t0 = time.perf_counter()
+        pairs = oracle.sweep(b, axis=0, nthreads=threads)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"ms": cpu_s * 1e3, "boxes_per_s": n / cpu_s, "cores": threads, "kind": "port",
+                               "sample": "oracle restatement of src/cpu/sweep.cpp over the whole scene",
+                               "pairs_equal_gpu": pair_digest(pairs) == sets["plain"]}
+    db.close()
+    return out
 
 
 def bench_handoff(args, H):
@@ -584,6 +638,9 @@ def main():
     if args.handoff:
         line["handoff"] = bench_handoff(args, H)
         progress("hand-off leg done")
+    if args.small:
+        line["small"] = bench_small(args, H)
+        progress("small scene leg done")
     print(json.dumps(line))
 
 

@@ -190,43 +190,103 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
 }
 
 // ---- MLP forward -----------------------------------------------------------------
-// One item: 32 features -> 64 -> 64 -> 1 (ReLU), weights w in LDS.
-__device__ __forceinline__ float mlp_forward_one(const float *w, const float *__restrict__ fi) {
-  float x[kFeat];
-#pragma unroll
-  for (int k = 0; k < kFeat; ++k) x[k] = fi[k];
-  float h1[kHid];
-  for (int j = 0; j < kHid; ++j) {
-    float s = w[kOffB1 + j];
-#pragma unroll
-    for (int k = 0; k < kFeat; ++k) s += w[kOffW1 + j * kFeat + k] * x[k];
-    h1[j] = fmaxf(s, 0.f);
-  }
-  float out = w[kOffB3];
-  for (int j = 0; j < kHid; ++j) {
-    float s = w[kOffB2 + j];
-#pragma unroll
-    for (int k = 0; k < kHid; ++k) s += w[kOffW2 + j * kHid + k] * h1[k];
-    out += w[kOffW3 + j] * fmaxf(s, 0.f);
-  }
-  return out;
-}
+
+// Inference on the matrix cores: each wave takes 16 items at a time and runs
+// both hidden layers as FP32 GEMMs with v_mfma_f32_16x16x4f32
+// (D[16x16] += A[16x4] B[4x16]; lane l holds A[l%16][l/16], B[l/16][l%16]
+// and D[4(l/16)+r][l%16], r = 0..3):
+//   H1[16 x 64] = relu(X[16 x 32] W1^T + b1): 4 column tiles x 8 k-steps,
+//   H2[16 x 64] = relu(H1 W2^T + b2):        4 column tiles x 16 k-steps,
+// with H1 passed from the D layout to the A layout through the wave's LDS
+// tile, then out = H2 . w3 + b3 as a sum over the 16 lanes of each quarter.
+// A lane's B operands (its 96 weights), biases and w3 entries are the same for
+// every group, so they are loaded into registers once per wave.
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+constexpr int kMfmaItems = 16;
+constexpr int kH1Row = kHid + 2;  // LDS row stride of the H1 tile: a half wave reads 32 banks
 
 __global__ __launch_bounds__(kT) void k_mlp_forward(const float *__restrict__ params,
                                                     const float *__restrict__ feat, int64_t n,
                                                     const int64_t *__restrict__ n_dev,
                                                     float *__restrict__ pred) {
-  __shared__ float w[kParams];
+  __shared__ float h1s[kT / 64][kMfmaItems][kH1Row];
   if (n_dev) n = *n_dev;
-  if ((int64_t)blockIdx.x * kT >= n) return;
-  for (int i = threadIdx.x; i < kParams; i += kT) w[i] = params[i];
-  __syncthreads();
-  // grid-stride over the items: the weights are staged once per block
-  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
-    pred[i] = mlp_forward_one(w, feat + i * kFeat);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 15, quad = lane >> 4;
+  const int64_t ngroups = (n + kMfmaItems - 1) / kMfmaItems;
+  const int64_t gstride = (int64_t)gridDim.x * (kT / 64);
+  int64_t g = (int64_t)blockIdx.x * (kT / 64) + wv;
+  if (g >= ngroups) return;
+  float w1r[kHid / 16][kFeat / 4], w2r[kHid / 16][kHid / 4], b1r[kHid / 16], b2r[kHid / 16],
+      w3r[kHid / 16];
+#pragma unroll
+  for (int nt = 0; nt < kHid / 16; ++nt) {
+    const int j = 16 * nt + col;
+#pragma unroll
+    for (int ks = 0; ks < kFeat / 4; ++ks) w1r[nt][ks] = params[kOffW1 + j * kFeat + 4 * ks + quad];
+#pragma unroll
+    for (int ks = 0; ks < kHid / 4; ++ks) w2r[nt][ks] = params[kOffW2 + j * kHid + 4 * ks + quad];
+    b1r[nt] = params[kOffB1 + j];
+    b2r[nt] = params[kOffB2 + j];
+    w3r[nt] = params[kOffW3 + j];
+  }
+  const float b3 = params[kOffB3];
+  float(*tile)[kH1Row] = h1s[wv];
+  // A operand of layer 1: item i0 + col, features 4 ks + quad; the next
+  // group's features are fetched while this group runs
+  float xn[kFeat / 4];
+  auto fetch = [&](int64_t gg) __attribute__((always_inline)) {
+    const int64_t ia = gg * kMfmaItems + col;
+#pragma unroll
+    for (int ks = 0; ks < kFeat / 4; ++ks) xn[ks] = ia < n ? feat[ia * kFeat + 4 * ks + quad] : 0.f;
+  };
+  fetch(g);
+  for (; g < ngroups; g += gstride) {
+    const int64_t i0 = g * kMfmaItems;
+    float xa[kFeat / 4];
+#pragma unroll
+    for (int ks = 0; ks < kFeat / 4; ++ks) xa[ks] = xn[ks];
+    if (g + gstride < ngroups) fetch(g + gstride);
+#pragma unroll
+    for (int nt = 0; nt < kHid / 16; ++nt) {
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < kFeat / 4; ++ks)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[ks], w1r[nt][ks], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) tile[4 * quad + r][16 * nt + col] = fmaxf(acc[r] + b1r[nt], 0.f);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // A operand of layer 2: H1[col][4 ks + quad]
+    float ha[kHid / 4];
+#pragma unroll
+    for (int ks = 0; ks < kHid / 4; ++ks) ha[ks] = tile[col][4 * ks + quad];
+    // the next group writes the tile only after every lane has read it
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int nt = 0; nt < kHid / 16; ++nt) {
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < kHid / 4; ++ks)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ha[ks], w2r[nt][ks], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) part[r] += w3r[nt] * fmaxf(acc[r] + b2r[nt], 0.f);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float v = part[r];
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+      const int64_t item = i0 + 4 * quad + r;
+      if (col == 0 && item < n) pred[item] = v + b3;
+    }
+  }
 }
-
-
 
 // ---- MLP gradient --------------------------------------------------------------------
 // One wave per block, 32 items per block.  Phase 1: lanes 0..31 run forward
@@ -433,8 +493,11 @@ void launch_features(bool f32, const SweepArgs &sa, const QParams &qp, int axis,
 void launch_mlp_forward(const float *params, const float *feat, int64_t n, float *pred,
                         hipStream_t st, const int64_t *n_dev) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_mlp_forward, dim3(std::min(nblk(n), 4096u)), dim3(kT), 0, st, params, feat, n,
-                     n_dev, pred);
+  // 16 items per wave step; about as many blocks as are resident at once
+  // (3 waves per SIMD at this register count), each wave looping over groups
+  const int64_t per_block = (kT / 64) * kMfmaItems;
+  const unsigned g = (unsigned)std::min<int64_t>((n + per_block - 1) / per_block, 768);
+  hipLaunchKernelGGL(k_mlp_forward, dim3(g), dim3(kT), 0, st, params, feat, n, n_dev, pred);
   STQ_HIP(hipGetLastError());
 }
 
