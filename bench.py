@@ -81,6 +81,10 @@ def parse():
                     help="time the swept-box build and the narrow-phase hand-off kernels (1 = on)")
     ap.add_argument("--small", type=int, default=1,
                     help="time the 100,000-box uniform scene, GPU forms and CPU oracle (1 = on)")
+    ap.add_argument("--tune-table", default="",
+                    help="autotune table of the cloth forms: loaded when the file exists (no tuning), "
+                         "else written after tuning; counter runs of one workload use it to share "
+                         "launch parameters")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (traffic calibration)")
     return ap.parse_args()
@@ -463,6 +467,10 @@ def main():
     training = None
     for name in want:
         ctx = stq_amd.Context(0, **forms[name])
+        table = args.tune_table if forms[name].get("autotune") else None
+        loaded = bool(table) and os.path.exists(table)
+        if loaded:  # same launch parameters as the run that wrote the table
+            ctx.autotune_load(table)
         if forms[name]["sizing"] == stq_amd.SIZING_LEARNED:
             # the online training phase: exact counts of a stream of other
             # scenes (cloth-like and uniform-like, mixed sizes and densities)
@@ -489,6 +497,8 @@ def main():
             if e.status != _lib.STQ_ERR_CAPACITY:
                 raise
         cap = ctx.stats()["n_pairs"]
+        if table and not loaded:
+            ctx.autotune_save(table)
         tuned = tuned_table(ctx) if forms[name].get("autotune") else None
         d_out = stq_amd.DeviceBuffer(8 * max(cap, 1))
         for _ in range(args.warmup):
