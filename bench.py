@@ -249,7 +249,9 @@ def bench_stream(args, H):
     log-uniform in [0.25, 8].  The first 200 run count-then-fill and train the
     network on their exact per-item counts; the rest use the single learned
     pass.  Reports the count error and the fallback rate on the held-out
-    scenes, and checks every 8th of them against plain FP64 by digest."""
+    scenes, and checks every 8th of them against plain FP64 by digest; on
+    those it also times the same precision with count-then-fill sizing, the
+    form the learned single pass replaces."""
     import numpy as np
     import stq_amd
     from stq_amd import _lib, scenes
@@ -258,6 +260,9 @@ def bench_stream(args, H):
     ctx = stq_amd.Context(0, precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN,
                           sizing=stq_amd.SIZING_LEARNED, train_scenes=ntrain)
     plain = stq_amd.Context(0, precision=stq_amd.PRECISION_F64, tails=2)
+    ctf = stq_amd.Context(0, precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN,
+                          sizing=stq_amd.SIZING_COUNT_FILL)
+    ctf_ms, learned_ms = 0.0, 0.0
     cap = 1 << 26
     dout = _lib.DeviceBuffer(8 * cap)
     rows = []
@@ -284,12 +289,18 @@ def bench_stream(args, H):
             got = pair_digest(dout.to_array(np.int32, 2 * cnt).reshape(-1, 2))
             c2 = plain.find_pairs_device(db.ptr, len(b), dout.ptr, cap)
             ref = pair_digest(dout.to_array(np.int32, 2 * c2).reshape(-1, 2))
+            c3 = ctf.find_pairs_device(db.ptr, len(b), dout.ptr, cap)
+            ctf_ms += ctf.stats()["ms_total"]
+            learned_ms += st["ms_total"]
             checked += 1
-            mismatches += int(got != ref)
+            mismatches += int(got != ref) + int(c3 != cnt)
         db.close()
+        if (i + 1) % 100 == 0:
+            progress(f"stream: {i + 1} of {args.stream_scenes} scenes")
     dout.close()
     ctx.close()
     plain.close()
+    ctf.close()
     held = [r for r in rows if r[0] >= ntrain]
     tr = [r for r in rows if r[0] < ntrain]
 
@@ -305,7 +316,9 @@ def bench_stream(args, H):
     return {"workload": f"{args.stream_scenes} scenes (seed 4), 5e4..4e6 boxes log-uniform, density factor "
                         f"log-uniform in [0.25, 8]; first {ntrain} train the network online",
             "seconds": time.perf_counter() - t0, "training": agg(tr), "held_out": agg(held),
-            "held_out_checked_vs_plain_fp64": checked, "mismatches": mismatches}
+            "held_out_checked_vs_plain_fp64": checked, "mismatches": mismatches,
+            "checked_ms": {"learned_single_pass": learned_ms, "count_then_fill": ctf_ms,
+                           "speedup": ctf_ms / learned_ms if learned_ms > 0 else None}}
 
 
 def bench_small(args, H):

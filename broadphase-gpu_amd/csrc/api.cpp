@@ -62,9 +62,9 @@ int Context::pick_axis(const stq_aabb_f64 *boxes, int64_t n, bool need_q, QParam
       const double lo = s[12 + a], up = s[15 + a];
       const double range = up - lo;
       qp->origin[a] = std::isfinite(lo) ? lo : 0.0;
-      // 32767 levels across the scene; a degenerate or unbounded range puts
-      // every box on the full level range (scale 0)
-      qp->scale[a] = (range > 0 && std::isfinite(range)) ? 32767.0 / range : 0.0;
+      // kLevelMax levels across the scene; a degenerate or unbounded range
+      // puts every box on the full level range (scale 0)
+      qp->scale[a] = (range > 0 && std::isfinite(range)) ? kLevelMax / range : 0.0;
       if (!std::isfinite(qp->scale[a])) qp->scale[a] = 0.0;
     }
   }

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kT) void k_seg_finalize(const double *__restrict__ 
   for (int a = 0; a < 3; ++a) {
     const double lo = v[12 + a], up = v[15 + a], range = up - lo;
     p.qp.origin[a] = isfinite(lo) ? lo : 0.0;
-    double sc = (range > 0 && isfinite(range)) ? 32767.0 / range : 0.0;
+    double sc = (range > 0 && isfinite(range)) ? kLevelMax / range : 0.0;
     p.qp.scale[a] = isfinite(sc) ? sc : 0.0;
   }
   const double dn = (double)(seg_off[s + 1] - seg_off[s]);
@@ -169,12 +169,8 @@ __global__ __launch_bounds__(kT) void k_gather_seg(const stq_aabb_f64 *__restric
     r.v2 = b.vertex_ids[2];
     r.type = type_bit(r.v1, r.v2);
     static_cast<Exact64 *>(rec)[s] = r;
-    QRec q;
-    q.lo = level_down(r.lo1, p.qp.origin[a1], p.qp.scale[a1]) |
-           (level_down(r.lo2, p.qp.origin[a2], p.qp.scale[a2]) << 16);
-    q.hi = level_up(r.hi1, p.qp.origin[a1], p.qp.scale[a1]) |
-           (level_up(r.hi2, p.qp.origin[a2], p.qp.scale[a2]) << 16);
-    qrec[s] = q;
+    qrec[s] = make_qrec(r.lo1, r.hi1, r.lo2, r.hi2, r.type, p.qp.origin[a1], p.qp.scale[a1],
+                        p.qp.origin[a2], p.qp.scale[a2]);
   } else {
     static_cast<double *>(smin)[s] = b.min[axis];
     static_cast<double *>(smax)[s] = b.max[axis];

@@ -86,8 +86,8 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
         a1 = sc.axis == 0 ? 1 : 0;
         a2 = sc.axis == 2 ? 1 : 2;
       }
-      n1 = qp.scale[a1] / 32767.0;
-      n2 = qp.scale[a2] / 32767.0;
+      n1 = qp.scale[a1] / kLevelMax;
+      n2 = qp.scale[a2] / kLevelMax;
       o1 = qp.origin[a1];
       o2 = qp.origin[a2];
       act = lane < nrows;

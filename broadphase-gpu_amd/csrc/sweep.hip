@@ -36,9 +36,6 @@
 
 namespace stqamd {
 
-// v_writelane_b32: lane l of the result is `src` (uniform), the others keep `old`.
-__device__ int writelane_i32(int src, int l, int old) __asm("llvm.amdgcn.writelane.i32");
-
 namespace {
 
 constexpr int kThreads = 256;
@@ -186,12 +183,8 @@ __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__r
   r.v2 = b.vertex_ids[2];
   r.type = type_bit(r.v1, r.v2);
   rec[s] = r;
-  QRec q;
-  q.lo = level_down(r.lo1, qp.origin[a1], qp.scale[a1]) |
-         (level_down(r.lo2, qp.origin[a2], qp.scale[a2]) << 16);
-  q.hi = level_up(r.hi1, qp.origin[a1], qp.scale[a1]) |
-         (level_up(r.hi2, qp.origin[a2], qp.scale[a2]) << 16);
-  qrec[s] = q;
+  qrec[s] = make_qrec(r.lo1, r.hi1, r.lo2, r.hi2, r.type, qp.origin[a1], qp.scale[a1],
+                      qp.origin[a2], qp.scale[a2]);
 }
 
 // ---- chain ends -----------------------------------------------------------------
@@ -584,20 +577,19 @@ __global__ __launch_bounds__(kThreads) void k_sweep64(SweepArgs a, const Rec64 *
 // Hot loop: the 8-byte level records of q (QRec) arrive by scalar loads, B per
 // batch, one batch ahead as above; the overlap test of both non-sweep axes is
 // two packed 16-bit subtractions and one sign-bit test:
-//   (hi_p - lo_q) | (hi_q - lo_p) has no negative half  <=>  levels overlap.
-// Positions whose level test passes for some row are appended, with their row
-// mask, to a per-wave queue held in VGPR lanes (entry e in lane e, written by
-// v_writelane).  When, after a batch, the queue holds kDrainAt entries or
-// more (or the range ends) it is drained: the entries are expanded, in queue
-// order, into one (row, q) candidate per LDS slot, and up to 64 candidates
-// are confirmed at once, one per lane, each lane fetching both exact records:
-// exact FP64 overlap on all three axes, type filter, shared-vertex filter.
-// One load latency is paid per round of up to 64 candidates, and the emission
-// order is the order of q, as in the plain sweep.  A deep queue (48 entries)
-// fills the confirmation rounds; it measured 1-2% faster than 16 on the cloth
-// and the batched scenes.
+//   (hi_p - lo_q) | (hi_q - lo_p) has no negative half  <=>  levels overlap,
+// with the kind codes of row p and column q folded in (make_qrec), so kind
+// pairs that the type filter drops fail here already.
+// The (row, q) candidates of the positions that pass are written straight
+// into the wave's LDS slots (the row lanes of the mask, each at its rank in
+// it, under that mask as the execution mask).  After a batch, while 64 or more
+// wait, 64 of them are confirmed at once, one per lane, each lane fetching
+// both exact records: exact FP64 overlap on all three axes, type filter,
+// shared-vertex filter.  The slots hold fewer than 64 waiting candidates plus
+// a whole batch (B * 64), so no mask outlives its batch.  One load latency is
+// paid per round of 64 candidates, and the emission order is the order of q,
+// then of row lanes, as in the plain sweep.
 typedef short short2v __attribute__((ext_vector_type(2)));
-constexpr int kDrainAt = 48;  // a batch adds at most B more: kDrainAt + B <= 64 lanes
 
 __device__ __forceinline__ bool level_overlap(uint32_t phi, uint32_t plo, uint32_t qlo,
                                               uint32_t qhi) {
@@ -619,7 +611,7 @@ __device__ __forceinline__ void tailq(const SweepArgs &a, const Exact64 *__restr
   if (lane < R) {
     rp = a.rowlist[item.rowoff + lane];
     rh = a.hi[rp];
-    const QRec x = qrec[rp];
+    const QRec x = row_form(qrec[rp]);
     rlo = x.lo;
     rhi = x.hi;
     if (MODE != kCount) rop = a.orig[rp];
@@ -664,8 +656,9 @@ template <int MODE, int B>
 __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 *__restrict__ rec,
                                                      const QRec *__restrict__ qrec,
                                                      int2 *__restrict__ out) {
-  static_assert(kDrainAt + B <= kWave, "queue entries live one per lane");
-  __shared__ int2 slot_mem[kThreads / kWave][kWave];
+  // LDS slots per wave: fewer than 64 waiting candidates plus one batch's
+  // B * 64 (4.5 KB at B = 8)
+  __shared__ int2 slot_mem[kThreads / kWave][kWave * (B + 1)];
   const int zero = a.zero;
   const int lane = threadIdx.x & 63;
   int2 *slots = slot_mem[threadIdx.x >> 6];
@@ -688,22 +681,20 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
       const int p = act ? a.rowlist[item.rowoff + lane] : INT_MAX;
       const int myhi = act ? a.hi[p] : INT_MIN;
       QRec mq;
-      mq.lo = 0x7fff7fffu;  // idle lanes: levels above every upper bound
+      mq.lo = 0x7fff7fffu;  // idle lanes: an empty level box that meets no column (make_qrec)
       mq.hi = 0u;
-      if (act) mq = qrec[p];
+      if (act) mq = row_form(qrec[p]);
 
-      // the candidate queue: entry e = (position eq, row mask emlo | emhi << 32)
-      int eq = 0;
-      uint32_t emlo = 0, emhi = 0;
-      int nq = 0;
-      // Draining expands the entries into (row, q) candidates, one per lane, in
-      // (entry, row lane) order through this wave's LDS slots, then tests all of
-      // them at once: both exact records are fetched per lane.
-      auto test_slots = [&](int T) __attribute__((always_inline)) {
+      // Candidates wait in this wave's LDS slots as (row, q), in emission
+      // order (position, then row lane).  Whenever 64 or more wait, they are
+      // confirmed 64 at a time, one per lane, each lane fetching both exact
+      // records; the rest (fewer than 64) move to the front.
+      int T = 0;
+      auto test_slots = [&](int base, int n) __attribute__((always_inline)) {
         bool h = false, hg = false;
         int cp = 0, cq = 0;
-        if (lane < T) {
-          const int2 c = slots[lane];
+        if (lane < n) {
+          const int2 c = slots[base + lane];
           cp = c.x;
           cq = c.y;
           const Exact64 xp = rec[cp], xq = rec[cq];
@@ -726,32 +717,30 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
           em.put(h, hm, o, oq, out);
         }
       };
-      auto drain = [&]() __attribute__((always_inline)) {
-        int T = 0;
-        for (int e = 0; e < nq; ++e) {
-          const int q = __builtin_amdgcn_readlane(eq, e);
-          const uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)emhi, e) << 32) |
-                             (uint32_t)__builtin_amdgcn_readlane((int)emlo, e);
-          const int c = __popcll(m);
-          if (T + c > kWave) {
-            __builtin_amdgcn_wave_barrier();
-            test_slots(T);
-            __builtin_amdgcn_wave_barrier();
-            T = 0;
-          }
-          if ((m >> lane) & 1ull) slots[T + mbcnt64(m)] = make_int2(p, q);
-          T += c;
-        }
+      auto drain_full = [&]() __attribute__((always_inline)) {
         __builtin_amdgcn_wave_barrier();
-        test_slots(T);
+        int base = 0;
+        do {
+          test_slots(base, kWave);
+          base += kWave;
+        } while (T - base >= kWave);
         __builtin_amdgcn_wave_barrier();
-        nq = 0;
+        const bool mv = lane < T - base;
+        int2 v = make_int2(0, 0);
+        if (mv) v = slots[base + lane];
+        __builtin_amdgcn_wave_barrier();
+        if (mv) slots[lane] = v;
+        __builtin_amdgcn_wave_barrier();
+        T -= base;
       };
-      auto push = [&](int q, uint64_t m) __attribute__((always_inline)) {
-        eq = writelane_i32(q, nq, eq);
-        emlo = (uint32_t)writelane_i32((int)(uint32_t)m, nq, (int)emlo);
-        emhi = (uint32_t)writelane_i32((int)(uint32_t)(m >> 32), nq, (int)emhi);
-        ++nq;
+      // the rows of m at position q into the next slots: each row lane of m
+      // writes its own slot, at its rank in m
+      auto append = [&](int q, uint64_t m) __attribute__((always_inline)) {
+        if (__builtin_amdgcn_inverse_ballot_w64(m))
+          slots[__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)T))] =
+              make_int2(p, q);
+        T += __popcll(m);
       };
       auto batch = [&](const QRec *b, int q, bool check) __attribute__((always_inline)) {
         uint64_t keep = ~0ull;
@@ -772,8 +761,8 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
         if (any) {
   #pragma unroll
           for (int j = 0; j < B; ++j)
-            if (m[j]) push(q + j, m[j]);
-          if (nq >= kDrainAt) drain();
+            if (m[j]) append(q + j, m[j]);
+          if (T >= kWave) drain_full();
         }
       };
       auto walk = [&](int qa, int qb, bool check) __attribute__((always_inline)) {
@@ -807,8 +796,8 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
           uint64_t m = ballot(level_overlap(mq.hi, mq.lo, r.lo, r.hi));
           if (check) m &= ballot(q > p) & ballot(q <= myhi);
           if (m) {
-            push(q, m);
-            if (nq >= kDrainAt) drain();
+            append(q, m);
+            if (T >= kWave) drain_full();
           }
         }
       };
@@ -823,7 +812,11 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
       } else {
         walk(q0, q1, true);
       }
-      if (nq) drain();
+      if (T) {
+        __builtin_amdgcn_wave_barrier();
+        test_slots(0, T);
+        __builtin_amdgcn_wave_barrier();
+      }
     }
     if (MODE != kFill && lane == 0) a.counts[it] = em.cnt;
     k = wc ? claim_take(ws, pend) : k + ws.nwaves;

@@ -13,10 +13,11 @@ struct alignas(16) Rec64 {
   double lo1, hi1, lo2, hi2;
   int v0, v1, v2, type;
 };
-// Reduced-precision form: the two non-sweep axes quantised outward to 15-bit
+// Reduced-precision form: the two non-sweep axes quantised outward to 14-bit
 // levels, packed (axis 1 in the low half, axis 2 in the high half) for the
-// lower and the upper bounds; and the exact FP64 box (sweep axis first) with
-// its ids for the confirmation of every candidate the levels let through.
+// lower and the upper bounds, with a kind code in bit 14 of each half (below);
+// and the exact FP64 box (sweep axis first) with its ids for the confirmation
+// of every candidate the levels let through.
 struct alignas(8) QRec {
   uint32_t lo, hi;
 };
@@ -34,18 +35,57 @@ struct QParams {
 // Per block: 12 sums (per axis: sum(min+max), sum(min^2+max^2), sum(max-min),
 // sum(centre^2)) then per axis min(min) and max(max): kAxisStats doubles.
 constexpr int kAxisStats = 18;
-// Outward 15-bit levels: t = (x - origin) * scale is monotone in x under IEEE
+// Outward 14-bit levels: t = (x - origin) * scale is monotone in x under IEEE
 // rounding, so floor(t(lo)) <= ceil(t(hi')) whenever lo <= hi': a pair that
 // overlaps in FP64 always overlaps in levels (a superset test).
+constexpr double kLevelMax = 16383.0;
 __device__ inline uint32_t level_down(double x, double o, double sc) {
   if (sc == 0.0) return 0u;
-  const double t = fmin(fmax(floor((x - o) * sc), 0.0), 32767.0);
+  const double t = fmin(fmax(floor((x - o) * sc), 0.0), kLevelMax);
   return (uint32_t)t;
 }
 __device__ inline uint32_t level_up(double x, double o, double sc) {
-  if (sc == 0.0) return 32767u;
-  const double t = fmin(fmax(ceil((x - o) * sc), 0.0), 32767.0);
+  if (sc == 0.0) return (uint32_t)kLevelMax;
+  const double t = fmin(fmax(ceil((x - o) * sc), 0.0), kLevelMax);
   return (uint32_t)t;
+}
+
+// Kind codes.  The sweep tests a row p against a column q (p sorted before q)
+// by packed 16-bit differences of (level + 16384 * code) per half: equal codes
+// leave the level test as it is, and unequal codes make one of the two
+// differences negative whatever the levels (levels are in [0, 16383]).  A box
+// is stored in its column form; a row takes its row form.  Codes per half
+// (axis 1, axis 2):
+//   column: face (0,0), vertex (1,0), edge (0,1)
+//   row:    vertex (0,0), face (1,0), edge (0,1)
+// so row and column codes are equal exactly for vertex-face, face-vertex and
+// edge-edge, the pairs the type filter keeps (src/aabb.cu:12-40): the level
+// test drops the other kind pairs at no cost.  No column code is (1,1), so in
+// every half a column's upper level is below 0x7fff or its lower level above
+// 0: the empty row {lo 0x7fff7fff, hi 0} meets no column.
+constexpr uint32_t kCodeBits = 0x40004000u;
+__host__ __device__ inline uint32_t column_code(int typebit) {
+  return typebit == 1 ? 0x00004000u : (typebit == 2 ? 0x40000000u : 0u);
+}
+__host__ __device__ inline uint32_t row_code(int typebit) {
+  return typebit == 4 ? 0x00004000u : (typebit == 2 ? 0x40000000u : 0u);
+}
+// Row form of a stored (column form) record.
+__device__ __forceinline__ QRec row_form(QRec c) {
+  const uint32_t k = c.lo & kCodeBits;
+  const int t = k == 0x00004000u ? 1 : (k == 0x40000000u ? 2 : 4);
+  c.lo = (c.lo & ~kCodeBits) | row_code(t);
+  c.hi = (c.hi & ~kCodeBits) | row_code(t);
+  return c;
+}
+// Column form of a box: levels of the two non-sweep axes and the kind code.
+__device__ inline QRec make_qrec(double lo1, double hi1, double lo2, double hi2, int typebit,
+                                 double o1, double s1, double o2, double s2) {
+  const uint32_t code = column_code(typebit);
+  QRec q;
+  q.lo = (level_down(lo1, o1, s1) | (level_down(lo2, o2, s2) << 16)) | code;
+  q.hi = (level_up(hi1, o1, s1) | (level_up(hi2, o2, s2) << 16)) | code;
+  return q;
 }
 
 // Per-scene sweep axis and level range (batched scenes).
