@@ -338,16 +338,11 @@ __device__ __forceinline__ int64_t claim_take(const WaveSlice &ws, unsigned v) {
 }
 
 // Rows of a batch of positions q .. q+nb-1 near the ends of an item's range
-// (reduced-precision sweep; its batches of 8 gain from one mask per batch):
-// some = rows whose chain (p, hi] meets the batch, full = rows whose chain
-// holds all of it.  False when no row meets the batch (it is skipped); when
-// some == full the batch needs no per-position range test.
-__device__ __forceinline__ bool batch_rows(int p, int myhi, int q, int nb, uint64_t &some,
-                                           uint64_t &full) {
-  some = ballot(p < q + nb - 1) & ballot(myhi >= q);
-  if (some == 0) return false;
-  full = ballot(p < q) & ballot(myhi >= q + nb - 1);
-  return true;
+// (reduced-precision sweep): the rows whose chain (p, hi] meets the batch.
+// Which of them hold a given position is tested only for positions whose
+// level test passed for some row.
+__device__ __forceinline__ uint64_t batch_rows(int p, int myhi, int q, int nb) {
+  return ballot(p < q + nb - 1) & ballot(myhi >= q);
 }
 
 // Bounds of the unchecked middle part: every active lane has p < q <= hi_p
@@ -744,24 +739,23 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
       };
       auto batch = [&](const QRec *b, int q, bool check) __attribute__((always_inline)) {
         uint64_t keep = ~0ull;
-        bool per_q = false;
         if (check) {
-          uint64_t full;
-          if (!batch_rows(p, myhi, q, B, keep, full)) return;
-          per_q = keep != full;
+          keep = batch_rows(p, myhi, q, B);
+          if (keep == 0) return;
         }
         uint64_t m[B];
         uint64_t any = 0;
   #pragma unroll
         for (int j = 0; j < B; ++j) {
           m[j] = ballot(level_overlap(mq.hi, mq.lo, b[j].lo, b[j].hi)) & keep;
-          if (per_q) m[j] &= ballot(q + j > p) & ballot(q + j <= myhi);
           any |= m[j];
         }
         if (any) {
   #pragma unroll
-          for (int j = 0; j < B; ++j)
+          for (int j = 0; j < B; ++j) {
+            if (check && m[j]) m[j] &= ballot(q + j > p) & ballot(q + j <= myhi);
             if (m[j]) append(q + j, m[j]);
+          }
           if (T >= kWave) drain_full();
         }
       };
@@ -794,7 +788,7 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
         for (; q <= qb; ++q) {
           const QRec r = qrec[q];
           uint64_t m = ballot(level_overlap(mq.hi, mq.lo, r.lo, r.hi));
-          if (check) m &= ballot(q > p) & ballot(q <= myhi);
+          if (check && m) m &= ballot(q > p) & ballot(q <= myhi);
           if (m) {
             append(q, m);
             if (T >= kWave) drain_full();
