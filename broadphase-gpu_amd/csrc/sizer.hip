@@ -45,6 +45,62 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
 // position inside the scene's quantisation range (0..1).
 // With scene tables (batched scenes) the level range and axes are those of the
 // item's scene, and the item count is read from the device.
+//
+// The wave reductions of an item leave kStat uniform statistics; they are
+// parked in the wave's LDS and, once 64 items are parked (or the wave's run
+// ends), lane j turns item j's statistics into its 32 features: the
+// transcendental part (logs, exps, roots, divisions: several hundred
+// instructions) runs once per 64 items instead of once per item.
+enum StatField {
+  kSNr, kSCandLo, kSCandHi, kSSpan, kSMle1, kSVle1, kSMle2, kSVle2, kSMc1, kSVc1, kSMc2, kSVc2,
+  kSFv, kSFe, kSFf, kSMse1, kSVse1, kSMse2, kSVse2, kSMsc1, kSVsc1, kSMsc2, kSVsc2, kSGv, kSGe,
+  kSGf, kSMchain, kSN1Lo, kSN1Hi, kSN2Lo, kSN2Hi, kStat
+};
+// LDS words per parked item: an odd pitch, so the lanes reading one field of
+// their items hit distinct banks
+constexpr int kStatPitch = kStat | 1;
+
+__device__ __forceinline__ double stat_f64(const float *s, int lo) {
+  const uint64_t u = (uint64_t)(uint32_t)__float_as_int(s[lo]) |
+                     ((uint64_t)(uint32_t)__float_as_int(s[lo + 1]) << 32);
+  return __builtin_bit_cast(double, u);
+}
+
+// Features of one item from its parked statistics.
+__device__ __forceinline__ void item_features(const float *__restrict__ st, float *__restrict__ f,
+                                              uint32_t *__restrict__ cand_out) {
+  const float nr = st[kSNr];
+  const long long cand_it = (long long)((uint64_t)(uint32_t)__float_as_int(st[kSCandLo]) |
+                                        ((uint64_t)(uint32_t)__float_as_int(st[kSCandHi]) << 32));
+  const int span = __float_as_int(st[kSSpan]);
+  const float m_le1 = st[kSMle1], v_le1 = st[kSVle1], m_le2 = st[kSMle2], v_le2 = st[kSVle2];
+  const float m_c1 = st[kSMc1], v_c1 = st[kSVc1], m_c2 = st[kSMc2], v_c2 = st[kSVc2];
+  const float f_v = st[kSFv], f_e = st[kSFe], f_f = st[kSFf];
+  const float m_se1 = st[kSMse1], v_se1 = st[kSVse1], m_se2 = st[kSMse2], v_se2 = st[kSVse2];
+  const float m_sc1 = st[kSMsc1], v_sc1 = st[kSVsc1], m_sc2 = st[kSMsc2], v_sc2 = st[kSVsc2];
+  const float g_v = st[kSGv], g_e = st[kSGe], g_f = st[kSGf], m_chain = st[kSMchain];
+  const double n1 = stat_f64(st, kSN1Lo), n2 = stat_f64(st, kSN2Lo);
+  const float steps = 64.f * (float)(span + 1);
+  const float e1r = expf(10.f * m_le1) * n1, e1q = expf(10.f * m_se1) * n1;
+  const float e2r = expf(10.f * m_le2) * n2, e2q = expf(10.f * m_se2) * n2;
+  const float sd_c1 = sqrtf(fmaxf(v_c1, 0.f)) + sqrtf(fmaxf(v_sc1, 0.f));
+  const float sd_c2 = sqrtf(fmaxf(v_c2, 0.f)) + sqrtf(fmaxf(v_sc2, 0.f));
+  const float py = fminf(1.f, (e1r + e1q) / (sd_c1 + fabsf(m_c1 - m_sc1) + 1e-12f));
+  const float pz = fminf(1.f, (e2r + e2q) / (sd_c2 + fabsf(m_c2 - m_sc2) + 1e-12f));
+  const float tc = f_v * g_f + f_f * g_v + f_e * g_e;
+  float4 *f4 = reinterpret_cast<float4 *>(f);
+  f4[0] = make_float4(nr / 64.f, log1pf((float)cand_it) * 0.1f, log1pf(steps) * 0.1f,
+                      (float)cand_it / steps);
+  f4[1] = make_float4(m_le1, sqrtf(fmaxf(v_le1, 0.f)), m_le2, sqrtf(fmaxf(v_le2, 0.f)));
+  f4[2] = make_float4(m_c1, sqrtf(fmaxf(v_c1, 0.f)), m_c2, sqrtf(fmaxf(v_c2, 0.f)));
+  f4[3] = make_float4(f_v, f_e, f_f, m_se1);
+  f4[4] = make_float4(sqrtf(fmaxf(v_se1, 0.f)), m_se2, sqrtf(fmaxf(v_se2, 0.f)), m_sc1);
+  f4[5] = make_float4(sqrtf(fmaxf(v_sc1, 0.f)), m_sc2, sqrtf(fmaxf(v_sc2, 0.f)), g_v);
+  f4[6] = make_float4(g_e, g_f, logf(py + 1e-9f) * 0.1f, logf(pz + 1e-9f) * 0.1f);
+  f4[7] = make_float4(tc, log1pf((float)cand_it * py * pz * tc) * 0.1f, log1pf(m_chain) * 0.1f,
+                      1.f);
+  *cand_out = (uint32_t)min(cand_it, (long long)UINT_MAX);
+}
 template <class R>
 __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items, int64_t nitems,
                                                  const int64_t *__restrict__ nitems_dev,
@@ -65,6 +121,18 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
   const int64_t per = (nitems + nwaves - 1) / nwaves;
   const int64_t ib = (int64_t)wave * per;
   const int64_t ie = ib + per < nitems ? ib + per : nitems;
+  __shared__ float park_mem[kT / 64][kWave * kStatPitch];
+  float *park = park_mem[threadIdx.x >> 6];
+  int nslot = 0;       // items parked
+  int64_t pbase = ib;  // item of slot 0
+  auto flush = [&]() __attribute__((always_inline)) {
+    __builtin_amdgcn_wave_barrier();
+    if (lane < nslot)
+      item_features(park + lane * kStatPitch, feat + (pbase + lane) * kFeat, cand + pbase + lane);
+    __builtin_amdgcn_wave_barrier();
+    pbase += nslot;
+    nslot = 0;
+  };
   int prow = -1, pnrows = -1;  // rows of the cached statistics
   bool act = false;
   int p = 0, h = 0;            // this lane's row and its chain end
@@ -86,10 +154,12 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
         a1 = sc.axis == 0 ? 1 : 0;
         a2 = sc.axis == 2 ? 1 : 2;
       }
-      n1 = qp.scale[a1] / kLevelMax;
-      n2 = qp.scale[a2] / kLevelMax;
-      o1 = qp.origin[a1];
-      o2 = qp.origin[a2];
+      // axis-indexed fields by selects (a runtime index would put qp in scratch)
+      auto pick = [](const double *v, int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); };
+      n1 = pick(qp.scale, a1) / kLevelMax;
+      n2 = pick(qp.scale, a2) / kLevelMax;
+      o1 = pick(qp.origin, a1);
+      o2 = pick(qp.origin, a2);
       act = lane < nrows;
       float fr = act ? 1.f : 0.f;
       float le1 = 0, le2 = 0, c1 = 0, c2 = 0, tv = 0, te = 0, tf = 0, chain = 0;
@@ -143,50 +213,44 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
     const float v_sc2 = wave_sum_dpp(sc2 * sc2) / 64.f - m_sc2 * m_sc2;
     const float g_v = wave_sum_dpp(sv) / 64.f, g_e = wave_sum_dpp(se) / 64.f, g_f = wave_sum_dpp(sf) / 64.f;
     if (lane == 0) {
-      const float steps = 64.f * (float)(span + 1);
-      const float e1r = expf(10.f * m_le1) * n1, e1q = expf(10.f * m_se1) * n1;
-      const float e2r = expf(10.f * m_le2) * n2, e2q = expf(10.f * m_se2) * n2;
-      const float sd_c1 = sqrtf(fmaxf(v_c1, 0.f)) + sqrtf(fmaxf(v_sc1, 0.f));
-      const float sd_c2 = sqrtf(fmaxf(v_c2, 0.f)) + sqrtf(fmaxf(v_sc2, 0.f));
-      const float py = fminf(1.f, (e1r + e1q) / (sd_c1 + fabsf(m_c1 - m_sc1) + 1e-12f));
-      const float pz = fminf(1.f, (e2r + e2q) / (sd_c2 + fabsf(m_c2 - m_sc2) + 1e-12f));
-      const float tc = f_v * g_f + f_f * g_v + f_e * g_e;
-      float *f = feat + it * kFeat;
-      f[0] = nr / 64.f;
-      f[1] = log1pf((float)cand_it) * 0.1f;
-      f[2] = log1pf(steps) * 0.1f;
-      f[3] = (float)cand_it / steps;
-      f[4] = m_le1;
-      f[5] = sqrtf(fmaxf(v_le1, 0.f));
-      f[6] = m_le2;
-      f[7] = sqrtf(fmaxf(v_le2, 0.f));
-      f[8] = m_c1;
-      f[9] = sqrtf(fmaxf(v_c1, 0.f));
-      f[10] = m_c2;
-      f[11] = sqrtf(fmaxf(v_c2, 0.f));
-      f[12] = f_v;
-      f[13] = f_e;
-      f[14] = f_f;
-      f[15] = m_se1;
-      f[16] = sqrtf(fmaxf(v_se1, 0.f));
-      f[17] = m_se2;
-      f[18] = sqrtf(fmaxf(v_se2, 0.f));
-      f[19] = m_sc1;
-      f[20] = sqrtf(fmaxf(v_sc1, 0.f));
-      f[21] = m_sc2;
-      f[22] = sqrtf(fmaxf(v_sc2, 0.f));
-      f[23] = g_v;
-      f[24] = g_e;
-      f[25] = g_f;
-      f[26] = logf(py + 1e-9f) * 0.1f;
-      f[27] = logf(pz + 1e-9f) * 0.1f;
-      f[28] = tc;
-      f[29] = log1pf((float)cand_it * py * pz * tc) * 0.1f;
-      f[30] = log1pf(m_chain) * 0.1f;
-      f[31] = 1.f;
-      cand[it] = (uint32_t)min(cand_it, (long long)UINT_MAX);
+      float *w = park + nslot * kStatPitch;
+      const uint64_t cu = (uint64_t)cand_it;
+      const uint64_t n1u = __builtin_bit_cast(uint64_t, n1), n2u = __builtin_bit_cast(uint64_t, n2);
+      w[kSNr] = nr;
+      w[kSCandLo] = __int_as_float((int)(uint32_t)cu);
+      w[kSCandHi] = __int_as_float((int)(uint32_t)(cu >> 32));
+      w[kSSpan] = __int_as_float(span);
+      w[kSMle1] = m_le1;
+      w[kSVle1] = v_le1;
+      w[kSMle2] = m_le2;
+      w[kSVle2] = v_le2;
+      w[kSMc1] = m_c1;
+      w[kSVc1] = v_c1;
+      w[kSMc2] = m_c2;
+      w[kSVc2] = v_c2;
+      w[kSFv] = f_v;
+      w[kSFe] = f_e;
+      w[kSFf] = f_f;
+      w[kSMse1] = m_se1;
+      w[kSVse1] = v_se1;
+      w[kSMse2] = m_se2;
+      w[kSVse2] = v_se2;
+      w[kSMsc1] = m_sc1;
+      w[kSVsc1] = v_sc1;
+      w[kSMsc2] = m_sc2;
+      w[kSVsc2] = v_sc2;
+      w[kSGv] = g_v;
+      w[kSGe] = g_e;
+      w[kSGf] = g_f;
+      w[kSMchain] = m_chain;
+      w[kSN1Lo] = __int_as_float((int)(uint32_t)n1u);
+      w[kSN1Hi] = __int_as_float((int)(uint32_t)(n1u >> 32));
+      w[kSN2Lo] = __int_as_float((int)(uint32_t)n2u);
+      w[kSN2Hi] = __int_as_float((int)(uint32_t)(n2u >> 32));
     }
+    if (++nslot == kWave) flush();
   }
+  if (nslot) flush();
 }
 
 // ---- MLP forward -----------------------------------------------------------------
