@@ -143,7 +143,9 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   void *d_rec = geo.get((size_t)n * (f32 ? sizeof(Exact64) : sizeof(Rec64)));
   QRec *d_qrec = f32 ? meta.as<QRec>((size_t)n) : nullptr;
   int *d_hi = hi.as<int>((size_t)n);
-  launch_gather(f32, boxes, order, n, axis, qp, d_smin, d_smax, d_rec, d_qrec, st);
+  ColStats *d_cols = learned ? col_stats.as<ColStats>((size_t)(n + kColBlock - 1) / kColBlock)
+                             : nullptr;
+  launch_gather(f32, boxes, order, n, axis, qp, d_smin, d_smax, d_rec, d_qrec, d_cols, st);
   launch_chain_end(f32, d_smin, d_smax, n, d_hi, &ctr[1], st);
   STQ_HIP(hipEventRecord(ev[E_GATHER], st));
   unsigned long long hc[4];
@@ -211,7 +213,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     sizer_ensure();
     d_feat = feat.as<float>((size_t)nitems * kFeat);
     d_cand = cand.as<uint32_t>((size_t)nitems);
-    launch_features(f32, sa, qp, axis, nullptr, nullptr, 0, d_feat, d_cand, st);
+    launch_features(f32, sa, d_cols, qp, axis, nullptr, nullptr, 0, d_feat, d_cand, st);
     ++scenes_seen;
   }
   if (learned_pass) {
@@ -607,6 +609,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   int64_t *d_pairoff = b_pairoff.as<int64_t>((size_t)nscenes + 1);
   unsigned long long *ctr = counters.as<unsigned long long>(kCounters);
   float *d_feat = learned ? feat.as<float>((size_t)bound * kFeat) : nullptr;
+  ColStats *d_cols =
+      learned ? col_stats.as<ColStats>((size_t)(n + kColBlock - 1) / kColBlock) : nullptr;
   uint32_t *d_cand = learned ? cand.as<uint32_t>((size_t)bound) : nullptr;
   float *d_pred = learned ? pred.as<float>((size_t)bound) : nullptr;
   uint32_t *d_caps = single_pass ? caps.as<uint32_t>((size_t)bound) : nullptr;
@@ -640,7 +644,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
         radix_sort_u32_sync_free(d_skey, d_vals, n, sbits, sort, st);
       }
     }
-    launch_gather_seg(f32, boxes, d_vals, t, bp.dev, d_smin, d_smax, d_rec, d_qrec, d_orig, st);
+    launch_gather_seg(f32, boxes, d_vals, t, bp.dev, d_smin, d_smax, d_rec, d_qrec, d_orig, d_cols,
+                      st);
     launch_chain_end_seg(f32, d_smin, d_smax, t, bp.dev, d_hi, &ctr[1], st);
     launch_plan_seg(d_hi, t, bp.dev, W, chunk, tail_cost(opt), d_rowlist, d_groups, d_tg, d_gchunk,
                     d_nch, &ctr[2], st);
@@ -662,8 +667,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     sa.xcd_split = opt.xcd_split == 2 ? 0 : 1;
     sa.work = opt.claim != 2 ? claim_counters(ctr, 0) : nullptr;
     if (learned)
-      launch_features(f32, sa, QParams{}, 0, bp.dev.params, bp.dev.seg_off, nscenes, d_feat, d_cand,
-                      st);
+      launch_features(f32, sa, d_cols, QParams{}, 0, bp.dev.params, bp.dev.seg_off, nscenes, d_feat,
+                      d_cand, st);
     const int64_t *total = d_off + bound;
     if (!single_pass) {
       launch_sweep(f32, kCount, batch, nblocks, sa, st);

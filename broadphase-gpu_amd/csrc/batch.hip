@@ -143,48 +143,64 @@ __global__ __launch_bounds__(kT) void k_gather_seg(const stq_aabb_f64 *__restric
                                                    const SceneParams *__restrict__ sp, int f32,
                                                    void *__restrict__ smin, void *__restrict__ smax,
                                                    void *__restrict__ rec, QRec *__restrict__ qrec,
-                                                   uint32_t *__restrict__ orig_local) {
+                                                   uint32_t *__restrict__ orig_local,
+                                                   ColStats *__restrict__ cols) {
   const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (s >= n) return;
-  const int sc = scene_of(seg_off, nscenes, s);
-  const uint32_t o = order[s];
-  orig_local[s] = (uint32_t)((int64_t)o - seg_off[sc]);
-  const stq_aabb_f64 &b = boxes[o];
-  const SceneParams p = sp[sc];
-  const int axis = p.axis;
-  int a1, a2;
-  other_axes_b(axis, a1, a2);
-  if (f32) {
-    static_cast<float *>(smin)[s] = round_down_f32(b.min[axis]);
-    static_cast<float *>(smax)[s] = round_up_f32(b.max[axis]);
-    Exact64 r;
-    r.lo0 = b.min[axis];
-    r.hi0 = b.max[axis];
-    r.lo1 = b.min[a1];
-    r.hi1 = b.max[a1];
-    r.lo2 = b.min[a2];
-    r.hi2 = b.max[a2];
-    r.v0 = b.vertex_ids[0];
-    r.v1 = b.vertex_ids[1];
-    r.v2 = b.vertex_ids[2];
-    r.type = type_bit(r.v1, r.v2);
-    static_cast<Exact64 *>(rec)[s] = r;
-    qrec[s] = make_qrec(r.lo1, r.hi1, r.lo2, r.hi2, r.type, p.qp.origin[a1], p.qp.scale[a1],
-                        p.qp.origin[a2], p.qp.scale[a2]);
-  } else {
-    static_cast<double *>(smin)[s] = b.min[axis];
-    static_cast<double *>(smax)[s] = b.max[axis];
-    Rec64 r;
-    r.lo1 = b.min[a1];
-    r.hi1 = b.max[a1];
-    r.lo2 = b.min[a2];
-    r.hi2 = b.max[a2];
-    r.v0 = b.vertex_ids[0];
-    r.v1 = b.vertex_ids[1];
-    r.v2 = b.vertex_ids[2];
-    r.type = type_bit(r.v1, r.v2);
-    static_cast<Rec64 *>(rec)[s] = r;
+  const bool valid = s < n;
+  double lo1 = 0, hi1 = 0, lo2 = 0, hi2 = 0, o1 = 0, n1 = 0, o2 = 0, n2 = 0;
+  int type = 0;
+  if (valid) {
+    const int sc = scene_of(seg_off, nscenes, s);
+    const uint32_t o = order[s];
+    orig_local[s] = (uint32_t)((int64_t)o - seg_off[sc]);
+    const stq_aabb_f64 &b = boxes[o];
+    const SceneParams p = sp[sc];
+    const int axis = p.axis;
+    int a1, a2;
+    other_axes_b(axis, a1, a2);
+    lo1 = b.min[a1];
+    hi1 = b.max[a1];
+    lo2 = b.min[a2];
+    hi2 = b.max[a2];
+    type = type_bit(b.vertex_ids[1], b.vertex_ids[2]);
+    o1 = p.qp.origin[a1];
+    n1 = p.qp.scale[a1] / kLevelMax;
+    o2 = p.qp.origin[a2];
+    n2 = p.qp.scale[a2] / kLevelMax;
+    if (f32) {
+      static_cast<float *>(smin)[s] = round_down_f32(b.min[axis]);
+      static_cast<float *>(smax)[s] = round_up_f32(b.max[axis]);
+      Exact64 r;
+      r.lo0 = b.min[axis];
+      r.hi0 = b.max[axis];
+      r.lo1 = lo1;
+      r.hi1 = hi1;
+      r.lo2 = lo2;
+      r.hi2 = hi2;
+      r.v0 = b.vertex_ids[0];
+      r.v1 = b.vertex_ids[1];
+      r.v2 = b.vertex_ids[2];
+      r.type = type;
+      static_cast<Exact64 *>(rec)[s] = r;
+      qrec[s] = make_qrec(r.lo1, r.hi1, r.lo2, r.hi2, r.type, p.qp.origin[a1], p.qp.scale[a1],
+                          p.qp.origin[a2], p.qp.scale[a2]);
+    } else {
+      static_cast<double *>(smin)[s] = b.min[axis];
+      static_cast<double *>(smax)[s] = b.max[axis];
+      Rec64 r;
+      r.lo1 = lo1;
+      r.hi1 = hi1;
+      r.lo2 = lo2;
+      r.hi2 = hi2;
+      r.v0 = b.vertex_ids[0];
+      r.v1 = b.vertex_ids[1];
+      r.v2 = b.vertex_ids[2];
+      r.type = type;
+      static_cast<Rec64 *>(rec)[s] = r;
+    }
   }
+  // blocks that straddle two scenes mix both (the features are estimates)
+  if (cols) col_stats_put(valid, s, lo1, hi1, lo2, hi2, type, o1, n1, o2, n2, cols);
 }
 
 // ---- chain ends, bounded by the scene ---------------------------------------
@@ -362,11 +378,11 @@ void launch_scene_keys(const uint32_t *vals, const BatchTables &t, const BatchDe
 
 void launch_gather_seg(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order,
                        const BatchTables &t, const BatchDev &d, void *smin, void *smax, void *rec,
-                       QRec *qrec, uint32_t *orig_local, hipStream_t st) {
+                       QRec *qrec, uint32_t *orig_local, ColStats *cols, hipStream_t st) {
   if (t.n <= 0) return;
   hipLaunchKernelGGL(k_gather_seg, dim3(nblk(t.n)), dim3(kT), 0, st, boxes, order, t.n, d.seg_off,
                      t.nscenes, (const SceneParams *)d.params, f32 ? 1 : 0, smin, smax, rec, qrec,
-                     orig_local);
+                     orig_local, cols);
   STQ_HIP(hipGetLastError());
 }
 

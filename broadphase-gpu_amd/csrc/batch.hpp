@@ -52,9 +52,11 @@ void launch_keys_seg(bool f32, const stq_aabb_f64 *boxes, const BatchTables &t, 
 // scene_key[s] = scene of input position vals[s] (for the F64 second sort)
 void launch_scene_keys(const uint32_t *vals, const BatchTables &t, const BatchDev &d,
                        uint32_t *scene_key, hipStream_t st);
+// Sorted records of every scene; with cols, the column statistics of every
+// kColBlock global positions as well (null: not made).
 void launch_gather_seg(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order,
                        const BatchTables &t, const BatchDev &d, void *smin, void *smax, void *rec,
-                       QRec *qrec, uint32_t *orig_local, hipStream_t st);
+                       QRec *qrec, uint32_t *orig_local, ColStats *cols, hipStream_t st);
 void launch_chain_end_seg(bool f32, const void *smin, const void *smax, const BatchTables &t,
                           const BatchDev &d, int *hi, unsigned long long *cand, hipStream_t st);
 // Groups of 64 rows per window, regrouped by chain end and planned as body +

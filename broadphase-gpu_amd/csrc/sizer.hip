@@ -40,9 +40,10 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
 }
 
 // ---- features ----------------------------------------------------------------
-// One wave per item.  Rows are the item's lanes; the q range is sampled at 64
-// evenly spaced positions.  Extents enter as log(extent)/10, centres as the
-// position inside the scene's quantisation range (0..1).
+// One wave per item.  Rows are the item's lanes; the q range is described by
+// the column statistics of the position blocks it meets (one block per lane,
+// loaded whole).  Extents enter as log(extent)/10, centres as the position
+// inside the scene's quantisation range (0..1).
 // With scene tables (batched scenes) the level range and axes are those of the
 // item's scene, and the item count is read from the device.
 //
@@ -53,8 +54,9 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
 // instructions) runs once per 64 items instead of once per item.
 enum StatField {
   kSNr, kSCandLo, kSCandHi, kSSpan, kSMle1, kSVle1, kSMle2, kSVle2, kSMc1, kSVc1, kSMc2, kSVc2,
-  kSFv, kSFe, kSFf, kSMse1, kSVse1, kSMse2, kSVse2, kSMsc1, kSVsc1, kSMsc2, kSVsc2, kSGv, kSGe,
-  kSGf, kSMchain, kSN1Lo, kSN1Hi, kSN2Lo, kSN2Hi, kStat
+  kSFv, kSFe, kSFf, kSMchain, kSN1Lo, kSN1Hi, kSN2Lo, kSN2Hi,
+  // column sums over the item's position blocks (ColStats order)
+  kSCol, kStat = kSCol + 12
 };
 // LDS words per parked item: an odd pitch, so the lanes reading one field of
 // their items hit distinct banks
@@ -75,10 +77,14 @@ __device__ __forceinline__ void item_features(const float *__restrict__ st, floa
   const int span = __float_as_int(st[kSSpan]);
   const float m_le1 = st[kSMle1], v_le1 = st[kSVle1], m_le2 = st[kSMle2], v_le2 = st[kSVle2];
   const float m_c1 = st[kSMc1], v_c1 = st[kSVc1], m_c2 = st[kSMc2], v_c2 = st[kSVc2];
-  const float f_v = st[kSFv], f_e = st[kSFe], f_f = st[kSFf];
-  const float m_se1 = st[kSMse1], v_se1 = st[kSVse1], m_se2 = st[kSMse2], v_se2 = st[kSVse2];
-  const float m_sc1 = st[kSMsc1], v_sc1 = st[kSVsc1], m_sc2 = st[kSMsc2], v_sc2 = st[kSVsc2];
-  const float g_v = st[kSGv], g_e = st[kSGe], g_f = st[kSGf], m_chain = st[kSMchain];
+  const float f_v = st[kSFv], f_e = st[kSFe], f_f = st[kSFf], m_chain = st[kSMchain];
+  const float *cs = st + kSCol;  // le1, le1sq, le2, le2sq, c1, c1sq, c2, c2sq, nv, ne, nf, cnt
+  const float ic = 1.f / fmaxf(cs[11], 1.f);
+  const float m_se1 = cs[0] * ic, v_se1 = cs[1] * ic - m_se1 * m_se1;
+  const float m_se2 = cs[2] * ic, v_se2 = cs[3] * ic - m_se2 * m_se2;
+  const float m_sc1 = cs[4] * ic, v_sc1 = cs[5] * ic - m_sc1 * m_sc1;
+  const float m_sc2 = cs[6] * ic, v_sc2 = cs[7] * ic - m_sc2 * m_sc2;
+  const float g_v = cs[8] * ic, g_e = cs[9] * ic, g_f = cs[10] * ic;
   const double n1 = stat_f64(st, kSN1Lo), n2 = stat_f64(st, kSN2Lo);
   const float steps = 64.f * (float)(span + 1);
   const float e1r = expf(10.f * m_le1) * n1, e1q = expf(10.f * m_se1) * n1;
@@ -106,8 +112,10 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
                                                  const int64_t *__restrict__ nitems_dev,
                                                  const int *__restrict__ rowlist,
                                                  const int *__restrict__ hi,
-                                                 const R *__restrict__ rec, QParams qp0, int a10,
-                                                 int a20, const SceneParams *__restrict__ sp,
+                                                 const R *__restrict__ rec,
+                                                 const ColStats *__restrict__ cols, QParams qp0,
+                                                 int a10, int a20,
+                                                 const SceneParams *__restrict__ sp,
                                                  const int64_t *__restrict__ seg_off, int nscenes,
                                                  float *__restrict__ feat,
                                                  uint32_t *__restrict__ cand) {
@@ -196,22 +204,28 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
       cnd = up >= lo ? (long long)(up - lo + 1) : 0;
     }
     const int span = q1 - q0;
-    const int qs = q0 + (span > 0 ? (int)(((long long)lane * span) / 63) : 0);
-    const R s = rec[qs];
-    const float se1 = logf(fmaxf((float)(s.hi1 - s.lo1), 1e-30f)) * 0.1f;
-    const float se2 = logf(fmaxf((float)(s.hi2 - s.lo2), 1e-30f)) * 0.1f;
-    const float sc1 = (float)((0.5 * (s.lo1 + s.hi1) - o1) * n1);
-    const float sc2 = (float)((0.5 * (s.lo2 + s.hi2) - o2) * n2);
-    const float sv = s.type == 1 ? 1.f : 0.f, se = s.type == 2 ? 1.f : 0.f,
-                sf = s.type == 4 ? 1.f : 0.f;
     const long long cand_it = wave_sum(cnd);
-    const float m_se1 = wave_sum_dpp(se1) / 64.f, m_se2 = wave_sum_dpp(se2) / 64.f;
-    const float v_se1 = wave_sum_dpp(se1 * se1) / 64.f - m_se1 * m_se1;
-    const float v_se2 = wave_sum_dpp(se2 * se2) / 64.f - m_se2 * m_se2;
-    const float m_sc1 = wave_sum_dpp(sc1) / 64.f, m_sc2 = wave_sum_dpp(sc2) / 64.f;
-    const float v_sc1 = wave_sum_dpp(sc1 * sc1) / 64.f - m_sc1 * m_sc1;
-    const float v_sc2 = wave_sum_dpp(sc2 * sc2) / 64.f - m_sc2 * m_sc2;
-    const float g_v = wave_sum_dpp(sv) / 64.f, g_e = wave_sum_dpp(se) / 64.f, g_f = wave_sum_dpp(sf) / 64.f;
+    // column sums over the blocks [q0 / kColBlock, q1 / kColBlock], one per lane
+    float cs[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) cs[k] = 0.f;
+    for (int j = q0 / kColBlock + lane; j <= q1 / kColBlock; j += kWave) {
+      const ColStats c = cols[j];
+      cs[0] += c.le1;
+      cs[1] += c.le1sq;
+      cs[2] += c.le2;
+      cs[3] += c.le2sq;
+      cs[4] += c.c1;
+      cs[5] += c.c1sq;
+      cs[6] += c.c2;
+      cs[7] += c.c2sq;
+      cs[8] += c.nv;
+      cs[9] += c.ne;
+      cs[10] += c.nf;
+      cs[11] += c.cnt;
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) cs[k] = wave_sum_dpp(cs[k]);
     if (lane == 0) {
       float *w = park + nslot * kStatPitch;
       const uint64_t cu = (uint64_t)cand_it;
@@ -231,18 +245,9 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
       w[kSFv] = f_v;
       w[kSFe] = f_e;
       w[kSFf] = f_f;
-      w[kSMse1] = m_se1;
-      w[kSVse1] = v_se1;
-      w[kSMse2] = m_se2;
-      w[kSVse2] = v_se2;
-      w[kSMsc1] = m_sc1;
-      w[kSVsc1] = v_sc1;
-      w[kSMsc2] = m_sc2;
-      w[kSVsc2] = v_sc2;
-      w[kSGv] = g_v;
-      w[kSGe] = g_e;
-      w[kSGf] = g_f;
       w[kSMchain] = m_chain;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) w[kSCol + k] = cs[k];
       w[kSN1Lo] = __int_as_float((int)(uint32_t)n1u);
       w[kSN1Hi] = __int_as_float((int)(uint32_t)(n1u >> 32));
       w[kSN2Lo] = __int_as_float((int)(uint32_t)n2u);
@@ -537,19 +542,19 @@ inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
 
 }  // namespace
 
-void launch_features(bool f32, const SweepArgs &sa, const QParams &qp, int axis,
-                     const SceneParams *sp, const int64_t *seg_off, int nscenes, float *feat,
-                     uint32_t *cand, hipStream_t st) {
+void launch_features(bool f32, const SweepArgs &sa, const ColStats *cols, const QParams &qp,
+                     int axis, const SceneParams *sp, const int64_t *seg_off, int nscenes,
+                     float *feat, uint32_t *cand, hipStream_t st) {
   if (sa.nitems <= 0) return;
   const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
   const unsigned g = (unsigned)std::min<int64_t>((sa.nitems + 3) / 4, 4096);
   if (f32)
     hipLaunchKernelGGL((k_features<Exact64>), dim3(g), dim3(kT), 0, st, sa.items, sa.nitems,
-                       sa.nitems_dev, sa.rowlist, sa.hi, (const Exact64 *)sa.rec, qp, a1, a2, sp,
+                       sa.nitems_dev, sa.rowlist, sa.hi, (const Exact64 *)sa.rec, cols, qp, a1, a2, sp,
                        seg_off, nscenes, feat, cand);
   else
     hipLaunchKernelGGL((k_features<Rec64>), dim3(g), dim3(kT), 0, st, sa.items, sa.nitems,
-                       sa.nitems_dev, sa.rowlist, sa.hi, (const Rec64 *)sa.rec, qp, a1, a2, sp,
+                       sa.nitems_dev, sa.rowlist, sa.hi, (const Rec64 *)sa.rec, cols, qp, a1, a2, sp,
                        seg_off, nscenes, feat, cand);
   STQ_HIP(hipGetLastError());
 }

@@ -26,12 +26,12 @@ constexpr int kOffB3 = kOffW3 + kHid;
 constexpr int kParams = kOffB3 + 1;  // 6337
 
 // Features of every work item (nitems x kFeat floats) and its exact number of
-// sweep candidates (an upper bound on its pairs).  One scene: qp and axis;
-// batched scenes: sp/seg_off/nscenes give each item's scene, and the item
-// count is sa.nitems_dev when set.
-void launch_features(bool f32, const SweepArgs &sa, const QParams &qp, int axis,
-                     const SceneParams *sp, const int64_t *seg_off, int nscenes, float *feat,
-                     uint32_t *cand, hipStream_t st);
+// sweep candidates (an upper bound on its pairs).  cols: the column statistics
+// of the gather.  One scene: qp and axis; batched scenes: sp/seg_off/nscenes
+// give each item's scene, and the item count is sa.nitems_dev when set.
+void launch_features(bool f32, const SweepArgs &sa, const ColStats *cols, const QParams &qp,
+                     int axis, const SceneParams *sp, const int64_t *seg_off, int nscenes,
+                     float *feat, uint32_t *cand, hipStream_t st);
 // pred[i] = MLP(feat[i]) (the log1p of the predicted pair count).  Where an
 // n_dev pointer is given, the count is *n_dev (n is then an upper bound).
 void launch_mlp_forward(const float *params, const float *feat, int64_t n, float *pred,

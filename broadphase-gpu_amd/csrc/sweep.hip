@@ -132,29 +132,37 @@ __device__ inline void other_axes(int axis, int &a1, int &a2) {
   a2 = axis == 2 ? 1 : 2;
 }
 
+// Sorted records of position s; with cols, the column statistics of every
+// kColBlock positions as well.
 __global__ __launch_bounds__(kThreads) void k_gather_f64(const stq_aabb_f64 *__restrict__ boxes,
                                                          const uint32_t *__restrict__ order,
-                                                         int64_t n, int axis,
+                                                         int64_t n, int axis, QParams qp,
                                                          double *__restrict__ smin,
                                                          double *__restrict__ smax,
-                                                         Rec64 *__restrict__ rec) {
+                                                         Rec64 *__restrict__ rec,
+                                                         ColStats *__restrict__ cols) {
   const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (s >= n) return;
-  const stq_aabb_f64 &b = boxes[order[s]];
+  const bool valid = s < n;
   int a1, a2;
   other_axes(axis, a1, a2);
-  smin[s] = b.min[axis];
-  smax[s] = b.max[axis];
-  Rec64 r;
-  r.lo1 = b.min[a1];
-  r.hi1 = b.max[a1];
-  r.lo2 = b.min[a2];
-  r.hi2 = b.max[a2];
-  r.v0 = b.vertex_ids[0];
-  r.v1 = b.vertex_ids[1];
-  r.v2 = b.vertex_ids[2];
-  r.type = type_bit(r.v1, r.v2);
-  rec[s] = r;
+  Rec64 r{};
+  if (valid) {
+    const stq_aabb_f64 &b = boxes[order[s]];
+    smin[s] = b.min[axis];
+    smax[s] = b.max[axis];
+    r.lo1 = b.min[a1];
+    r.hi1 = b.max[a1];
+    r.lo2 = b.min[a2];
+    r.hi2 = b.max[a2];
+    r.v0 = b.vertex_ids[0];
+    r.v1 = b.vertex_ids[1];
+    r.v2 = b.vertex_ids[2];
+    r.type = type_bit(r.v1, r.v2);
+    rec[s] = r;
+  }
+  if (cols)
+    col_stats_put(valid, s, r.lo1, r.hi1, r.lo2, r.hi2, r.type, qp.origin[a1],
+                  qp.scale[a1] / kLevelMax, qp.origin[a2], qp.scale[a2] / kLevelMax, cols);
 }
 
 __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__restrict__ boxes,
@@ -163,28 +171,34 @@ __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__r
                                                          float *__restrict__ smin,
                                                          float *__restrict__ smax,
                                                          Exact64 *__restrict__ rec,
-                                                         QRec *__restrict__ qrec) {
+                                                         QRec *__restrict__ qrec,
+                                                         ColStats *__restrict__ cols) {
   const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (s >= n) return;
-  const stq_aabb_f64 &b = boxes[order[s]];
+  const bool valid = s < n;
   int a1, a2;
   other_axes(axis, a1, a2);
-  smin[s] = round_down_f32(b.min[axis]);
-  smax[s] = round_up_f32(b.max[axis]);
-  Exact64 r;
-  r.lo0 = b.min[axis];
-  r.hi0 = b.max[axis];
-  r.lo1 = b.min[a1];
-  r.hi1 = b.max[a1];
-  r.lo2 = b.min[a2];
-  r.hi2 = b.max[a2];
-  r.v0 = b.vertex_ids[0];
-  r.v1 = b.vertex_ids[1];
-  r.v2 = b.vertex_ids[2];
-  r.type = type_bit(r.v1, r.v2);
-  rec[s] = r;
-  qrec[s] = make_qrec(r.lo1, r.hi1, r.lo2, r.hi2, r.type, qp.origin[a1], qp.scale[a1],
-                      qp.origin[a2], qp.scale[a2]);
+  Exact64 r{};
+  if (valid) {
+    const stq_aabb_f64 &b = boxes[order[s]];
+    smin[s] = round_down_f32(b.min[axis]);
+    smax[s] = round_up_f32(b.max[axis]);
+    r.lo0 = b.min[axis];
+    r.hi0 = b.max[axis];
+    r.lo1 = b.min[a1];
+    r.hi1 = b.max[a1];
+    r.lo2 = b.min[a2];
+    r.hi2 = b.max[a2];
+    r.v0 = b.vertex_ids[0];
+    r.v1 = b.vertex_ids[1];
+    r.v2 = b.vertex_ids[2];
+    r.type = type_bit(r.v1, r.v2);
+    rec[s] = r;
+    qrec[s] = make_qrec(r.lo1, r.hi1, r.lo2, r.hi2, r.type, qp.origin[a1], qp.scale[a1],
+                        qp.origin[a2], qp.scale[a2]);
+  }
+  if (cols)
+    col_stats_put(valid, s, r.lo1, r.hi1, r.lo2, r.hi2, r.type, qp.origin[a1],
+                  qp.scale[a1] / kLevelMax, qp.origin[a2], qp.scale[a2] / kLevelMax, cols);
 }
 
 // ---- chain ends -----------------------------------------------------------------
@@ -843,14 +857,14 @@ void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, void 
 
 void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, int64_t n,
                    int axis, const QParams &qp, void *smin, void *smax, void *rec, QRec *qrec,
-                   hipStream_t st) {
+                   ColStats *cols, hipStream_t st) {
   if (n <= 0) return;
   if (f32)
     hipLaunchKernelGGL(k_gather_f32, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, order,
-                       n, axis, qp, (float *)smin, (float *)smax, (Exact64 *)rec, qrec);
+                       n, axis, qp, (float *)smin, (float *)smax, (Exact64 *)rec, qrec, cols);
   else
     hipLaunchKernelGGL(k_gather_f64, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, order,
-                       n, axis, (double *)smin, (double *)smax, (Rec64 *)rec);
+                       n, axis, qp, (double *)smin, (double *)smax, (Rec64 *)rec, cols);
   STQ_HIP(hipGetLastError());
 }
 

@@ -88,6 +88,71 @@ __device__ inline QRec make_qrec(double lo1, double hi1, double lo2, double hi2,
   return q;
 }
 
+// ---- column statistics of position blocks (learned sizing features) ----
+// Per block of kColBlock consecutive sorted positions: sums of the log-extent
+// (log(extent)/10) and of the centre's place in the scene's level range (0..1)
+// on the two non-sweep axes, of their squares, and the kind counts.  An item's
+// column features are the statistics of the blocks its position range meets.
+constexpr int kColBlock = 32;
+struct alignas(16) ColStats {
+  float le1, le1sq, le2, le2sq;
+  float c1, c1sq, c2, c2sq;
+  float nv, ne, nf, cnt;
+};
+__device__ __forceinline__ float log_extent(double lo, double hi) {
+  return logf(fmaxf((float)(hi - lo), 1e-30f)) * 0.1f;
+}
+__device__ __forceinline__ float centre_place(double lo, double hi, double origin, double norm) {
+  return (float)((0.5 * (lo + hi) - origin) * norm);
+}
+// Adds sorted position s to its block.  Every lane of the wave calls this,
+// with valid = false past the last position; the wave's lanes hold 64
+// consecutive positions from a multiple of 64, so a block is 32 consecutive
+// lanes.  norm = scale / kLevelMax of the scene.
+__device__ inline void col_stats_put(bool valid, int64_t s, double lo1, double hi1, double lo2,
+                                     double hi2, int type, double o1, double n1, double o2,
+                                     double n2, ColStats *__restrict__ out) {
+  float v[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) v[k] = 0.f;
+  if (valid) {
+    const float le1 = log_extent(lo1, hi1), le2 = log_extent(lo2, hi2);
+    const float c1 = centre_place(lo1, hi1, o1, n1), c2 = centre_place(lo2, hi2, o2, n2);
+    v[0] = le1;
+    v[1] = le1 * le1;
+    v[2] = le2;
+    v[3] = le2 * le2;
+    v[4] = c1;
+    v[5] = c1 * c1;
+    v[6] = c2;
+    v[7] = c2 * c2;
+    v[8] = type == 1 ? 1.f : 0.f;
+    v[9] = type == 2 ? 1.f : 0.f;
+    v[10] = type == 4 ? 1.f : 0.f;
+    v[11] = 1.f;
+  }
+#pragma unroll
+  for (int k = 0; k < 12; ++k)
+#pragma unroll
+    for (int o = kColBlock / 2; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+  if (valid && (threadIdx.x & (kColBlock - 1)) == 0) {
+    ColStats c;
+    c.le1 = v[0];
+    c.le1sq = v[1];
+    c.le2 = v[2];
+    c.le2sq = v[3];
+    c.c1 = v[4];
+    c.c1sq = v[5];
+    c.c2 = v[6];
+    c.c2sq = v[7];
+    c.nv = v[8];
+    c.ne = v[9];
+    c.nf = v[10];
+    c.cnt = v[11];
+    out[s / kColBlock] = c;
+  }
+}
+
 // Per-scene sweep axis and level range (batched scenes).
 struct SceneParams {
   int axis;
@@ -152,9 +217,10 @@ void launch_axis_stats(const stq_aabb_f64 *boxes, int64_t n, double *partial, in
                        hipStream_t st);
 void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, void *keys,
                  uint32_t *vals, unsigned *err, hipStream_t st);
+// cols: column statistics of every kColBlock positions (null: not made).
 void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, int64_t n,
                    int axis, const QParams &qp, void *smin, void *smax, void *rec, QRec *qrec,
-                   hipStream_t st);
+                   ColStats *cols, hipStream_t st);
 void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, int *hi,
                       unsigned long long *cand, hipStream_t st);
 // tail_cost: weight of tail tests in the group cut (plan.hpp); 0 = no tails.
