@@ -35,8 +35,16 @@ static bool options_valid(const stq_options &o) {
 }
 
 // Weight of a tail test in the group cut; 0 turns tails off.
+// Auto (tails = 0) turns tails on for the plain FP64 sweep and off for the
+// reduced-precision one, whose body positions mostly cost one union test per
+// 64 (a long body is cheap there, a tail item is not).
 static float tail_cost(const stq_options &o) {
-  return o.tails == 2 ? 0.f : (o.tail_cost > 0 ? (float)o.tail_cost * 0.01f : 4.f);
+  const bool on = o.tails == 1 || (o.tails == 0 && o.precision != STQ_PRECISION_F32_CONFIRM);
+  return !on ? 0.f : (o.tail_cost > 0 ? (float)o.tail_cost * 0.01f : 4.f);
+}
+// Item chunk when none is set.
+static int auto_chunk(const stq_options &o) {
+  return o.chunk > 0 ? o.chunk : (o.precision == STQ_PRECISION_F32_CONFIRM ? kQChunk : 2048);
 }
 
 // Axis choice and, for the reduced-precision form, the quantisation range:
@@ -156,12 +164,13 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
 
   // plan
   int W = opt.window;
+  if (W == 0 && f32) W = kQWindow;
   if (W == 0) {
     const double mean = (double)hc[1] / (double)n;
     W = 256;
     while (W < 2048 && (double)(2 * W) <= mean / 8.0) W *= 2;
   }
-  const int chunk = opt.chunk > 0 ? opt.chunk : 2048;
+  const int chunk = auto_chunk(opt);
   const int64_t ngroups = (n + 63) / 64;
   int *d_rowlist = rowlist.as<int>((size_t)n);
   int4 *d_groups = groups.as<int4>((size_t)ngroups);
@@ -169,8 +178,8 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   int2 *d_gch = gchunks.as<int2>((size_t)ngroups);
   uint32_t *d_nch = nchunks.as<uint32_t>((size_t)ngroups);
   int64_t *d_ioff = ioff.as<int64_t>((size_t)ngroups + 1);
-  launch_plan(d_hi, n, W, chunk, tail_cost(opt), d_rowlist, d_groups, d_tg, d_gch, d_nch, &ctr[2],
-              st);
+  launch_plan(d_hi, d_qrec, n, W, chunk, tail_cost(opt), d_rowlist, d_groups, d_tg, d_gch, d_nch,
+              &ctr[2], st);
   scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
   int64_t nitems = 0;
   STQ_HIP(hipMemcpyAsync(&nitems, d_ioff + ngroups, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -556,8 +565,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   const int64_t n = off[nscenes];
   if (n > INT32_MAX) throw StatusError(STQ_ERR_TOO_LARGE, "more than 2^31-1 boxes in a batch");
   stats.n_boxes = n;
-  const int W = opt.window ? opt.window : kBatchWindow;
-  const int chunk = opt.chunk > 0 ? opt.chunk : 2048;
+  const int W = opt.window ? opt.window : (f32 ? kQWindow : kBatchWindow);
+  const int chunk = auto_chunk(opt);
   batch_plan(off.data(), nscenes, W, chunk);
   BatchPlan &bp = bplan;
   const BatchTables &t = bp.t;
@@ -647,8 +656,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     launch_gather_seg(f32, boxes, d_vals, t, bp.dev, d_smin, d_smax, d_rec, d_qrec, d_orig, d_cols,
                       st);
     launch_chain_end_seg(f32, d_smin, d_smax, t, bp.dev, d_hi, &ctr[1], st);
-    launch_plan_seg(d_hi, t, bp.dev, W, chunk, tail_cost(opt), d_rowlist, d_groups, d_tg, d_gchunk,
-                    d_nch, &ctr[2], st);
+    launch_plan_seg(d_hi, d_qrec, t, bp.dev, W, chunk, tail_cost(opt), d_rowlist, d_groups, d_tg,
+                    d_gchunk, d_nch, &ctr[2], st);
     scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
     launch_items(d_groups, d_tg, d_gchunk, d_ioff, ngroups, d_items, st);
     // item-level arrays are used up to the device item count only: every item

@@ -224,6 +224,7 @@ __global__ __launch_bounds__(kT) void k_chain_end_seg(const T *__restrict__ smin
 constexpr int kMaxWin = 2048;
 
 __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
+                                                 const QRec *__restrict__ qrec,
                                                  const int64_t *__restrict__ win_start,
                                                  const int *__restrict__ win_cnt,
                                                  const int64_t *__restrict__ win_gbase, int nwin,
@@ -243,7 +244,10 @@ __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
     const int64_t gbase = win_gbase[b];
     for (int j = threadIdx.x; j < W; j += kT) {
       uint64_t k = ~0ull;
-      if (j < cnt) k = ((uint64_t)(uint32_t)hi[wbase + j] << 32) | (uint32_t)j;
+      if (j < cnt) {
+        const uint32_t g = qrec ? group_key(qrec[wbase + j]) : (uint32_t)hi[wbase + j];
+        k = ((uint64_t)g << 32) | (uint32_t)j;
+      }
       key[j] = k;
     }
     __syncthreads();
@@ -263,14 +267,21 @@ __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
         __syncthreads();
       }
     }
-    for (int j = threadIdx.x; j < cnt; j += kT)
-      rowlist[wbase + j] = (int)(wbase + (int64_t)(key[j] & 0xffffffffu));
     const int ng = (cnt + 63) / 64;
     for (int k = w; k < ng; k += kT / 64) {
       const int j = 64 * k + lane;
       const bool act = j < cnt;
-      const int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
-      const int h = act ? (int)(key[j] >> 32) : INT_MIN;
+      int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
+      int h = act ? hi[p] : INT_MIN;
+      if (qrec) {  // the group's rows in chain-end order
+        const uint64_t r = wave_sort_u64(act ? ((uint64_t)(uint32_t)h << 32) | (uint32_t)p : ~0ull,
+                                         lane);
+        if (act) {
+          p = (int)(uint32_t)r;
+          h = (int)(r >> 32);
+        }
+      }
+      if (act) rowlist[wbase + j] = p;
       const int nrows = min(64, cnt - 64 * k);
       // capped: the item buffer is sized on the host from the scene sizes
       const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, true,
@@ -399,13 +410,14 @@ void launch_chain_end_seg(bool f32, const void *smin, const void *smax, const Ba
   STQ_HIP(hipGetLastError());
 }
 
-void launch_plan_seg(const int *hi, const BatchTables &t, const BatchDev &d, int W, int chunk,
-                     float tail_cost, int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk,
-                     uint32_t *nchunks, unsigned long long *lane_steps, hipStream_t st) {
+void launch_plan_seg(const int *hi, const QRec *qrec, const BatchTables &t, const BatchDev &d,
+                     int W, int chunk, float tail_cost, int *rowlist, int4 *groups,
+                     int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
+                     unsigned long long *lane_steps, hipStream_t st) {
   const int nw = (int)t.win_start.size();
   if (nw <= 0) return;
   hipLaunchKernelGGL(k_plan_seg, dim3(std::min((unsigned)nw, kReduceGrid)), dim3(kT), 0, st, hi,
-                     d.win_start, d.win_cnt, d.win_gbase, nw, W, chunk, tail_cost, rowlist, groups,
+                     qrec, d.win_start, d.win_cnt, d.win_gbase, nw, W, chunk, tail_cost, rowlist, groups,
                      tgroups, gchunk, nchunks, lane_steps);
   STQ_HIP(hipGetLastError());
 }

@@ -59,13 +59,15 @@ void launch_gather_seg(bool f32, const stq_aabb_f64 *boxes, const uint32_t *orde
                        QRec *qrec, uint32_t *orig_local, ColStats *cols, hipStream_t st);
 void launch_chain_end_seg(bool f32, const void *smin, const void *smax, const BatchTables &t,
                           const BatchDev &d, int *hi, unsigned long long *cand, hipStream_t st);
-// Groups of 64 rows per window, regrouped by chain end and planned as body +
-// tail (plan.hpp); each part is cut into at most 64 items, so the item buffer
-// has a bound the host computes from the scene sizes.  Items are then written
-// by launch_items (sweep.hpp).
-void launch_plan_seg(const int *hi, const BatchTables &t, const BatchDev &d, int W, int chunk,
-                     float tail_cost, int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk,
-                     uint32_t *nchunks, unsigned long long *lane_steps, hipStream_t st);
+// Groups of 64 rows per window (ordered as in launch_plan: by chain end, or
+// with qrec by group_key with each group's rows then by chain end), planned as
+// body + tail (plan.hpp); each part is cut into at most 64 items, so the item
+// buffer has a bound the host computes from the scene sizes.  Items are then
+// written by launch_items (sweep.hpp).
+void launch_plan_seg(const int *hi, const QRec *qrec, const BatchTables &t, const BatchDev &d,
+                     int W, int chunk, float tail_cost, int *rowlist, int4 *groups,
+                     int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
+                     unsigned long long *lane_steps, hipStream_t st);
 // pair_off[s] = off[ioff[scene_first_group[s]]] for s = 0..nscenes.
 void launch_scene_pairs(const int64_t *off, const int64_t *ioff, const BatchTables &t,
                         const BatchDev &d, int64_t *pair_off, hipStream_t st);

@@ -35,8 +35,13 @@ constexpr float kBatchCapSlack = 32.f;
 constexpr int64_t kReplay = 262144;
 constexpr int64_t kReplayPerScene = 8192;
 
-// Batched scenes: planning window when none is set.
+// Planning window when none is set: the FP64 form of batched scenes, and the
+// reduced-precision form (whose groups are cut from windows ordered by
+// group_key: a larger window gives groups a smaller union box).
 constexpr int kBatchWindow = 256;
+constexpr int kQWindow = 2048;
+// Reduced-precision item chunk when none is set (positions per body item).
+constexpr int kQChunk = 16384;
 
 // One batch shape (scene offsets, window, chunk): its host and device tables
 // and the graph captured for it.

@@ -102,6 +102,22 @@ __device__ inline GroupPlan plan_group(int lane, int nrows, int p, int h, int ro
   return g;
 }
 
+// Sorts the keys of the 64 lanes ascending (a bitonic network over lane
+// shuffles); every lane calls this.
+__device__ inline uint64_t wave_sort_u64(uint64_t k, int lane) {
+#pragma unroll
+  for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const uint64_t o = __shfl_xor(k, stride, 64);
+      const bool up = (lane & size) == 0;
+      const bool lower = (lane & stride) == 0;
+      k = (lower == up) ? (k < o ? k : o) : (k < o ? o : k);
+    }
+  }
+  return k;
+}
+
 // Items of one group at items[o .. o + n): the body's chunks, then the tail's.
 __device__ inline void write_group_items(const int4 body, const int4 tail, const int2 ch,
                                          int64_t o, int64_t n, Item *__restrict__ items) {

@@ -222,14 +222,17 @@ __global__ __launch_bounds__(kThreads) void k_chain_end(const T *__restrict__ sm
 
 // ---- work planning ----------------------------------------------------------------
 // Blocks take windows of W sorted rows (W a power of two, 64..2048) in
-// grid-stride order: rows are ordered by (hi, position) with a bitonic sort in
-// LDS and cut into groups of 64.  Group g covers rowlist[64g .. 64g+nrows); plan_group splits it into a
-// body over [min p + 1, cut] and, for rows with much longer chains, a tail.
-// nchunks[g] = number of items of the group.
+// grid-stride order.  The rows of a window are ordered with a bitonic sort in
+// LDS, by (chain end, position), or with qrec by (group_key, position), and
+// cut into groups of 64; with qrec each group's rows are then put in chain-end
+// order by a wave sort.  Group g covers rowlist[64g .. 64g+nrows); plan_group
+// splits it into a body over [min p + 1, cut] and, for rows with much longer
+// chains, a tail.  nchunks[g] = number of items of the group.
 constexpr int kMaxWindow = 2048;
 
-__global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi, int64_t n, int W,
-                                                   int chunk, float tail_cost,
+__global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi,
+                                                   const QRec *__restrict__ qrec, int64_t n,
+                                                   int W, int chunk, float tail_cost,
                                                    int *__restrict__ rowlist,
                                                    int4 *__restrict__ groups,
                                                    int4 *__restrict__ tgroups,
@@ -245,7 +248,10 @@ __global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi, i
     const int cnt = (int)std::min<int64_t>(W, n - wbase);
     for (int j = threadIdx.x; j < W; j += kThreads) {
       uint64_t k = ~0ull;
-      if (j < cnt) k = ((uint64_t)(uint32_t)hi[wbase + j] << 32) | (uint32_t)j;
+      if (j < cnt) {
+        const uint32_t g = qrec ? group_key(qrec[wbase + j]) : (uint32_t)hi[wbase + j];
+        k = ((uint64_t)g << 32) | (uint32_t)j;
+      }
       key[j] = k;
     }
     __syncthreads();
@@ -266,15 +272,22 @@ __global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi, i
         __syncthreads();
       }
     }
-    for (int j = threadIdx.x; j < cnt; j += kThreads)
-      rowlist[wbase + j] = (int)(wbase + (int64_t)(key[j] & 0xffffffffu));
     // one wave per group
     const int ng = (cnt + 63) / 64;
     for (int k = w; k < ng; k += kThreads / 64) {
       const int j = 64 * k + lane;
       const bool act = j < cnt;
-      const int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
-      const int h = act ? (int)(key[j] >> 32) : INT_MIN;
+      int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
+      int h = act ? hi[p] : INT_MIN;
+      if (qrec) {
+        const uint64_t r = wave_sort_u64(act ? ((uint64_t)(uint32_t)h << 32) | (uint32_t)p : ~0ull,
+                                         lane);
+        if (act) {
+          p = (int)(uint32_t)r;
+          h = (int)(r >> 32);
+        }
+      }
+      if (act) rowlist[wbase + j] = p;
       const int nrows = min(64, cnt - 64 * k);
       const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, false,
                                       tail_cost);
@@ -349,14 +362,6 @@ __device__ __forceinline__ unsigned claim_issue(unsigned *ctr) {
 }
 __device__ __forceinline__ int64_t claim_take(const WaveSlice &ws, unsigned v) {
   return ws.kbeg + (int64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)v);
-}
-
-// Rows of a batch of positions q .. q+nb-1 near the ends of an item's range
-// (reduced-precision sweep): the rows whose chain (p, hi] meets the batch.
-// Which of them hold a given position is tested only for positions whose
-// level test passed for some row.
-__device__ __forceinline__ uint64_t batch_rows(int p, int myhi, int q, int nb) {
-  return ballot(p < q + nb - 1) & ballot(myhi >= q);
 }
 
 // Bounds of the unchecked middle part: every active lane has p < q <= hi_p
@@ -583,21 +588,23 @@ __global__ __launch_bounds__(kThreads) void k_sweep64(SweepArgs a, const Rec64 *
 }
 
 // ---- reduced-precision sweep with FP64 confirmation ----
-// Hot loop: the 8-byte level records of q (QRec) arrive by scalar loads, B per
-// batch, one batch ahead as above; the overlap test of both non-sweep axes is
-// two packed 16-bit subtractions and one sign-bit test:
+// The 8-byte level records of q (QRec) are tested by two packed 16-bit
+// subtractions and one sign-bit test:
 //   (hi_p - lo_q) | (hi_q - lo_p) has no negative half  <=>  levels overlap,
 // with the kind codes of row p and column q folded in (make_qrec), so kind
 // pairs that the type filter drops fail here already.
-// The (row, q) candidates of the positions that pass are written straight
-// into the wave's LDS slots (the row lanes of the mask, each at its rank in
-// it, under that mask as the execution mask).  After a batch, while 64 or more
-// wait, 64 of them are confirmed at once, one per lane, each lane fetching
-// both exact records: exact FP64 overlap on all three axes, type filter,
-// shared-vertex filter.  The slots hold fewer than 64 waiting candidates plus
-// a whole batch (B * 64), so no mask outlives its batch.  One load latency is
-// paid per round of 64 candidates, and the emission order is the order of q,
-// then of row lanes, as in the plain sweep.
+// A group's rows are close on axis 1 (the planner groups them by group_key),
+// so the union of their level boxes is small: positions go across the lanes
+// 64 at a time and are tested against the union first (one test for 64
+// positions); only the positions that pass are tested against the rows, one
+// position at a time, its record read out of the lane that loaded it.
+// The (row, q) candidates of a position are written straight into the
+// wave's LDS slots (the row lanes of the mask, each at its rank in it, under
+// that mask as the execution mask).  Whenever 64 or more wait, 64 of them are
+// confirmed at once, one per lane, each lane fetching both exact records:
+// exact FP64 overlap on all three axes, type filter, shared-vertex filter.
+// One load latency is paid per round of 64 candidates, and the emission order
+// is the order of q, then of row lanes, as in the plain sweep.
 typedef short short2v __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ bool level_overlap(uint32_t phi, uint32_t plo, uint32_t qlo,
@@ -661,14 +668,22 @@ __device__ __forceinline__ void tailq(const SweepArgs &a, const Exact64 *__restr
   }
 }
 
-template <int MODE, int B>
+typedef unsigned short ushort2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_min_u16(uint32_t x, uint32_t y) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(ushort2v, x),
+                                                                __builtin_bit_cast(ushort2v, y)));
+}
+__device__ __forceinline__ uint32_t pk_max_u16(uint32_t x, uint32_t y) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(ushort2v, x),
+                                                                __builtin_bit_cast(ushort2v, y)));
+}
+
+template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 *__restrict__ rec,
                                                      const QRec *__restrict__ qrec,
                                                      int2 *__restrict__ out) {
-  // LDS slots per wave: fewer than 64 waiting candidates plus one batch's
-  // B * 64 (4.5 KB at B = 8)
-  __shared__ int2 slot_mem[kThreads / kWave][kWave * (B + 1)];
-  const int zero = a.zero;
+  // LDS slots per wave: fewer than 64 waiting candidates plus one position's 64
+  __shared__ int2 slot_mem[kThreads / kWave][2 * kWave];
   const int lane = threadIdx.x & 63;
   int2 *slots = slot_mem[threadIdx.x >> 6];
   unsigned long long nconf = 0;
@@ -693,10 +708,21 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
       mq.lo = 0x7fff7fffu;  // idle lanes: an empty level box that meets no column (make_qrec)
       mq.hi = 0u;
       if (act) mq = row_form(qrec[p]);
+      // The union of the rows' level boxes (kind codes dropped): a position
+      // whose box misses it meets no row.
+      uint32_t ulo = act ? (mq.lo & ~kCodeBits) : 0x3fff3fffu;
+      uint32_t uhi = act ? (mq.hi & ~kCodeBits) : 0u;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        ulo = pk_min_u16(ulo, (uint32_t)__shfl_xor((int)ulo, o, kWave));
+        uhi = pk_max_u16(uhi, (uint32_t)__shfl_xor((int)uhi, o, kWave));
+      }
+      ulo = (uint32_t)uniform((int)ulo);
+      uhi = (uint32_t)uniform((int)uhi);
 
       // Candidates wait in this wave's LDS slots as (row, q), in emission
-      // order (position, then row lane).  Whenever 64 or more wait, they are
-      // confirmed 64 at a time, one per lane, each lane fetching both exact
+      // order (position, then row lane).  Whenever 64 or more wait, 64 are
+      // confirmed at once, one per lane, each lane fetching both exact
       // records; the rest (fewer than 64) move to the front.
       int T = 0;
       auto test_slots = [&](int base, int n) __attribute__((always_inline)) {
@@ -728,97 +754,60 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
       };
       auto drain_full = [&]() __attribute__((always_inline)) {
         __builtin_amdgcn_wave_barrier();
-        int base = 0;
-        do {
-          test_slots(base, kWave);
-          base += kWave;
-        } while (T - base >= kWave);
+        test_slots(0, kWave);
         __builtin_amdgcn_wave_barrier();
-        const bool mv = lane < T - base;
+        const bool mv = lane < T - kWave;
         int2 v = make_int2(0, 0);
-        if (mv) v = slots[base + lane];
+        if (mv) v = slots[kWave + lane];
         __builtin_amdgcn_wave_barrier();
         if (mv) slots[lane] = v;
         __builtin_amdgcn_wave_barrier();
-        T -= base;
+        T -= kWave;
       };
-      // the rows of m at position q into the next slots: each row lane of m
-      // writes its own slot, at its rank in m
-      auto append = [&](int q, uint64_t m) __attribute__((always_inline)) {
-        if (__builtin_amdgcn_inverse_ballot_w64(m))
-          slots[__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)T))] =
-              make_int2(p, q);
-        T += __popcll(m);
-      };
-      auto batch = [&](const QRec *b, int q, bool check) __attribute__((always_inline)) {
-        uint64_t keep = ~0ull;
-        if (check) {
-          keep = batch_rows(p, myhi, q, B);
-          if (keep == 0) return;
-        }
-        uint64_t m[B];
-        uint64_t any = 0;
-  #pragma unroll
-        for (int j = 0; j < B; ++j) {
-          m[j] = ballot(level_overlap(mq.hi, mq.lo, b[j].lo, b[j].hi)) & keep;
-          any |= m[j];
-        }
-        if (any) {
-  #pragma unroll
-          for (int j = 0; j < B; ++j) {
-            if (check && m[j]) m[j] &= ballot(q + j > p) & ballot(q + j <= myhi);
-            if (m[j]) append(q + j, m[j]);
-          }
-          if (T >= kWave) drain_full();
-        }
-      };
-      auto walk = [&](int qa, int qb, bool check) __attribute__((always_inline)) {
-        int q = qa;
-        int nb = (qb - qa + 1) / B;
-        if (nb > 0) {
-          QRec b0[B], b1[B];
-  #pragma unroll
-          for (int j = 0; j < B; ++j) b0[j] = qrec[q + j];
-          while (true) {
-            q += zero * (int)b0[0].lo;  // waits for b0 on every path
-            if (nb > 1) {
-  #pragma unroll
-              for (int j = 0; j < B; ++j) b1[j] = qrec[q + B + j];
-            }
-            batch(b0, q, check);
-            q += B;
-            if (--nb == 0) break;
-            q += zero * (int)b1[0].lo;
-            if (nb > 1) {
-  #pragma unroll
-              for (int j = 0; j < B; ++j) b0[j] = qrec[q + B + j];
-            }
-            batch(b1, q, check);
-            q += B;
-            if (--nb == 0) break;
-          }
-        }
-        for (; q <= qb; ++q) {
-          const QRec r = qrec[q];
-          uint64_t m = ballot(level_overlap(mq.hi, mq.lo, r.lo, r.hi));
-          if (check && m) m &= ballot(q > p) & ballot(q <= myhi);
-          if (m) {
-            append(q, m);
-            if (T >= kWave) drain_full();
-          }
-        }
-      };
-      int pmax, hmin;
-      row_span(act, p, myhi, pmax, hmin);
+      // Positions go across the lanes, 64 at a time, the next 64 loaded ahead:
+      // each lane tests its position against the union; for each position that
+      // passes, its record is read out of its lane and tested against every
+      // row, and the rows that pass and hold the position in their chain
+      // (p < q <= hi) write (p, q) into the next slots, each at its rank.
       const int q0 = item.q0, q1 = item.q1;
-      const int b0 = max(q0, pmax + 1), b1 = min(q1, hmin);
-      if (b0 <= b1) {
-        if (q0 < b0) walk(q0, b0 - 1, true);
-        walk(b0, b1, false);
-        if (b1 < q1) walk(b1 + 1, q1, true);
-      } else {
-        walk(q0, q1, true);
+      uint32_t xlo = 0x7fff7fffu, xhi = 0u;  // past the range: an empty box
+      if (q0 + lane <= q1) {
+        const QRec x = qrec[q0 + lane];
+        xlo = x.lo;
+        xhi = x.hi;
+      }
+      for (int qc = q0; qc <= q1; qc += kWave) {
+        uint64_t pm = ballot((qc + lane <= q1) &
+                             level_overlap(uhi, ulo, xlo & ~kCodeBits, xhi & ~kCodeBits));
+        // the next 64 positions load while these are tested (issued after the
+        // wait for these, which waits for every load in flight)
+        uint32_t nlo = 0x7fff7fffu, nhi = 0u;
+        if (qc + kWave + lane <= q1) {
+          const QRec x = qrec[qc + kWave + lane];
+          nlo = x.lo;
+          nhi = x.hi;
+        }
+        while (pm) {
+          const int j = (int)__builtin_ctzll(pm);
+          pm &= pm - 1;
+          const uint32_t lo = (uint32_t)readlane_i((int)xlo, j);
+          const uint32_t hi = (uint32_t)readlane_i((int)xhi, j);
+          uint64_t m = ballot(level_overlap(mq.hi, mq.lo, lo, hi));
+          if (m) {
+            const int q = qc + j;
+            m &= ballot(q > p) & ballot(q <= myhi);
+            if (m) {
+              if (__builtin_amdgcn_inverse_ballot_w64(m))
+                slots[__builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)T))] =
+                    make_int2(p, q);
+              T += __popcll(m);
+              if (T >= kWave) drain_full();
+            }
+          }
+        }
+        xlo = nlo;
+        xhi = nhi;
       }
       if (T) {
         __builtin_amdgcn_wave_barrier();
@@ -881,14 +870,14 @@ void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, i
   STQ_HIP(hipGetLastError());
 }
 
-void launch_plan(const int *hi, int64_t n, int W, int chunk, float tail_cost, int *rowlist,
-                 int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
+void launch_plan(const int *hi, const QRec *qrec, int64_t n, int W, int chunk, float tail_cost,
+                 int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
                  unsigned long long *lane_steps, hipStream_t st) {
   if (n <= 0) return;
   const int64_t nwin = (n + W - 1) / W;
   hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(nwin, kReduceGrid)), dim3(kThreads),
-                     0, st, hi, n, W, chunk, tail_cost, rowlist, groups, tgroups, gchunk, nchunks,
-                     lane_steps);
+                     0, st, hi, qrec, n, W, chunk, tail_cost, rowlist, groups, tgroups, gchunk,
+                     nchunks, lane_steps);
   STQ_HIP(hipGetLastError());
 }
 
@@ -907,11 +896,11 @@ static void launch_sweep_b(bool f32, int mode, const dim3 &grid, const SweepArgs
   if (f32) {
     const Exact64 *r = static_cast<const Exact64 *>(args.rec);
     if (mode == kCount)
-      hipLaunchKernelGGL((k_qsweep<kCount, B>), grid, block, 0, st, args, r, args.qrec, args.out);
+      hipLaunchKernelGGL((k_qsweep<kCount>), grid, block, 0, st, args, r, args.qrec, args.out);
     else if (mode == kFill)
-      hipLaunchKernelGGL((k_qsweep<kFill, B>), grid, block, 0, st, args, r, args.qrec, args.out);
+      hipLaunchKernelGGL((k_qsweep<kFill>), grid, block, 0, st, args, r, args.qrec, args.out);
     else
-      hipLaunchKernelGGL((k_qsweep<kLearned, B>), grid, block, 0, st, args, r, args.qrec, args.out);
+      hipLaunchKernelGGL((k_qsweep<kLearned>), grid, block, 0, st, args, r, args.qrec, args.out);
   } else {
     constexpr int B64 = B > 4 ? 4 : B;  // Rec64 batches above 4 exceed the SGPR file
     const Rec64 *r = static_cast<const Rec64 *>(args.rec);
@@ -928,9 +917,9 @@ void launch_sweep(bool f32, int mode, int batch, int nblocks, const SweepArgs &a
                   hipStream_t st) {
   if (args.nitems <= 0) return;
   const dim3 grid(nblocks);
-  if (batch >= 8) launch_sweep_b<8>(f32, mode, grid, args, st);
-  else if (batch >= 4) launch_sweep_b<4>(f32, mode, grid, args, st);
-  else launch_sweep_b<2>(f32, mode, grid, args, st);
+  // the batch of scalar-loaded positions applies to the FP64 sweep only
+  if (f32 || batch <= 2) launch_sweep_b<2>(f32, mode, grid, args, st);
+  else launch_sweep_b<4>(f32, mode, grid, args, st);
   STQ_HIP(hipGetLastError());
 }
 

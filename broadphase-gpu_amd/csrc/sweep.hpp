@@ -78,6 +78,11 @@ __device__ __forceinline__ QRec row_form(QRec c) {
   c.hi = (c.hi & ~kCodeBits) | row_code(t);
   return c;
 }
+// Grouping key of a row for the reduced-precision sweep: the centre of its
+// axis-1 levels (twice the centre, kind code dropped).
+__device__ __forceinline__ uint32_t group_key(QRec r) {
+  return (r.lo & 0x3fffu) + (r.hi & 0x3fffu);
+}
 // Column form of a box: levels of the two non-sweep axes and the kind code.
 __device__ inline QRec make_qrec(double lo1, double hi1, double lo2, double hi2, int typebit,
                                  double o1, double s1, double o2, double s2) {
@@ -224,8 +229,11 @@ void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, i
 void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, int *hi,
                       unsigned long long *cand, hipStream_t st);
 // tail_cost: weight of tail tests in the group cut (plan.hpp); 0 = no tails.
-void launch_plan(const int *hi, int64_t n, int W, int chunk, float tail_cost, int *rowlist,
-                 int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
+// qrec (reduced precision): windows are ordered by group_key, so a group's
+// rows are close on axis 1, and each group's rows then by chain end; null:
+// windows ordered by chain end.
+void launch_plan(const int *hi, const QRec *qrec, int64_t n, int W, int chunk, float tail_cost,
+                 int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
                  unsigned long long *lane_steps, hipStream_t st);
 // Items of every group from the group records (body chunks, then tail chunks).
 void launch_items(const int4 *groups, const int4 *tgroups, const int2 *gchunk, const int64_t *ioff,

@@ -1,8 +1,9 @@
 // tune.cpp -- the autotuner.  For each call shape class (precision, batched
 // or not, floor(log2 boxes), floor(log2 scenes)) it picks the sweep-axis
-// policy, the planning window, the item chunk, the scalar batch of the sweep
-// loop (how many q records are in flight per wave), the resident waves per CU
-// and the weight of tail tests in the group cut, by timing count-only trial runs on the first input of that class:
+// policy, the planning window, the item chunk, the scalar batch of the FP64
+// sweep loop (how many q records are in flight per wave), the resident waves
+// per CU and the weight of tail tests in the group cut (0: no tails), by
+// timing count-only trial runs on the first input of that class:
 // coordinate descent, each candidate timed twice with HIP events, a change
 // kept only when it is at least 2% faster.  The table can be saved to and
 // loaded from a text file.
@@ -53,6 +54,7 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
       opt.chunk = e.chunk;
       opt.batch = e.batch;
       opt.waves_per_cu = e.waves_per_cu;
+      opt.tails = e.tail_cost > 0 ? 1 : 2;
       opt.tail_cost = e.tail_cost;
       double best = INFINITY;
       for (int rep = 0; rep < 2; ++rep) {
@@ -72,7 +74,8 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
       }
       return best;
     };
-    TuneEntry best{STQ_AXIS_CHAIN, batched ? kBatchWindow : 0, 2048, f32 ? 8 : 2, 32, 400, 0};
+    TuneEntry best{STQ_AXIS_CHAIN, batched ? (f32 ? kQWindow : kBatchWindow) : 0,
+                   f32 ? kQChunk : 2048, 2, 32, f32 ? 0 : 400, 0};
     best.ms = trial(best);
     auto consider = [&](TuneEntry e) {
       if (std::memcmp(&e, &best, offsetof(TuneEntry, ms)) == 0) return;
@@ -90,14 +93,15 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
       e.window = w;
       consider(e);
     }
-    for (int c : {1024, 4096}) {
+    for (int c : f32 ? std::initializer_list<int>{4096, 65536} : std::initializer_list<int>{1024, 4096}) {
       TuneEntry e = best;
       e.chunk = c;
       consider(e);
     }
-    for (int b : f32 ? std::initializer_list<int>{4, 8} : std::initializer_list<int>{2, 4}) {
+    // the scalar batch applies to the FP64 sweep only
+    if (!f32) {
       TuneEntry e = best;
-      e.batch = b;
+      e.batch = 4;
       consider(e);
     }
     for (int w : {16, 64}) {
@@ -105,7 +109,7 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
       e.waves_per_cu = w;
       consider(e);
     }
-    for (int tc : {200, 800}) {
+    for (int tc : f32 ? std::initializer_list<int>{400} : std::initializer_list<int>{200, 800}) {
       TuneEntry e = best;
       e.tail_cost = tc;
       consider(e);
@@ -121,11 +125,13 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
   opt.chunk = e.chunk;
   opt.batch = e.batch;
   opt.waves_per_cu = e.waves_per_cu;
+  opt.tails = e.tail_cost > 0 ? 1 : 2;
   opt.tail_cost = e.tail_cost;
 }
 
 // File format: a header line, then one line per entry:
 //   <key hex> <axis_policy> <window> <chunk> <batch> <waves_per_cu> <tail_cost> <ms>
+// (tail_cost 0: no tail items)
 int Context::tune_save(const char *path) const {
   FILE *f = std::fopen(path, "w");
   if (!f) return STQ_ERR_INVALID_ARGUMENT;

@@ -118,11 +118,13 @@ typedef struct stq_options {
   int32_t axis_policy;   /* enum stq_axis_policy                  */
   int32_t sizing;        /* enum stq_sizing                       */
   int32_t autotune;      /* 1: pick launch parameters by timing    */
-  int32_t chunk;         /* q-steps per sweep work item; 0 = auto */
-  int32_t window;        /* rows regrouped per tile; 0 = auto     */
+  int32_t chunk;         /* q-steps per sweep work item; 0 = auto
+                            (2048 plain FP64, 16384 reduced)       */
+  int32_t window;        /* rows regrouped per tile; 0 = auto
+                            (2048 reduced precision)               */
   int32_t waves_per_cu;  /* sweep residency per CU; 0 = auto      */
   int32_t batch;         /* q records per scalar-load batch of the
-                            sweep: 0 = auto, or 2, 4, 8            */
+                            plain FP64 sweep: 0 = auto, or 2, 4, 8 */
   int32_t xcd_split;     /* work split over XCDs: 0 = auto (on),
                             1 = on, 2 = off                        */
   int32_t train_scenes;  /* learned sizing: scenes run count-then-fill
@@ -137,7 +139,8 @@ typedef struct stq_options {
                             items from per-XCD counters), 1 = claim,
                             2 = fixed stride                        */
   int32_t tails;         /* long-chain rows swept as tail items with
-                            positions across lanes: 0 = auto (on),
+                            positions across lanes: 0 = auto (on for
+                            plain FP64, off for reduced precision),
                             1 = on, 2 = off                         */
   int32_t tail_cost;     /* weight of a tail (row, position) test
                             against a body lane step, in percent;
@@ -302,8 +305,9 @@ int stq_recall(const stq_pair *h_pairs, int64_t count, const char *json_path, in
    With options.autotune = 1, the first call of each shape class (precision,
    batched or not, floor(log2 boxes), floor(log2 scenes)) with at least 16384
    boxes runs count-only trials on its own input and picks the sweep-axis
-   policy, window, chunk, batch, waves_per_cu and tail_cost; those values then replace
-   the options' own for every call of that class.  The table belongs to the
+   policy, window, chunk, batch, waves_per_cu and tail_cost (0: no tail items, so
+   the tuned value also sets tails); those values then replace the options' own for
+   every call of that class.  The table belongs to the
    context and can be saved to and loaded from a text file. */
 int stq_autotune_save(void *ctx, const char *path);
 int stq_autotune_load(void *ctx, const char *path);
