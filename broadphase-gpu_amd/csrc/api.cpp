@@ -27,7 +27,8 @@ namespace stqamd {
 static bool options_valid(const stq_options &o) {
   return o.precision >= 0 && o.precision <= 1 && o.axis_policy >= 0 && o.axis_policy <= 4 &&
          o.sizing >= 0 && o.sizing <= 1 && o.chunk >= 0 && o.window >= 0 &&
-         (o.window == 0 || (o.window >= 64 && o.window <= 2048 && (o.window & (o.window - 1)) == 0)) &&
+         (o.window == 0 || (o.window >= 64 && o.window <= kMaxWindow32 &&
+                            (o.window & (o.window - 1)) == 0)) &&
          o.waves_per_cu >= 0 && o.waves_per_cu <= 64 &&
          (o.batch == 0 || o.batch == 2 || o.batch == 4 || o.batch == 8) && o.xcd_split >= 0 &&
          o.xcd_split <= 2 && o.graph >= 0 && o.graph <= 2 && o.claim >= 0 && o.claim <= 2 &&
@@ -170,6 +171,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     W = 256;
     while (W < 2048 && (double)(2 * W) <= mean / 8.0) W *= 2;
   }
+  if (!f32) W = std::min(W, kMaxWindow64);  // 64-bit window keys
   const int chunk = auto_chunk(opt);
   const int64_t ngroups = (n + 63) / 64;
   int *d_rowlist = rowlist.as<int>((size_t)n);
@@ -565,7 +567,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   const int64_t n = off[nscenes];
   if (n > INT32_MAX) throw StatusError(STQ_ERR_TOO_LARGE, "more than 2^31-1 boxes in a batch");
   stats.n_boxes = n;
-  const int W = opt.window ? opt.window : (f32 ? kQWindow : kBatchWindow);
+  const int W = opt.window ? (f32 ? opt.window : std::min(opt.window, kMaxWindow64))
+                           : (f32 ? kQWindow : kBatchWindow);
   const int chunk = auto_chunk(opt);
   batch_plan(off.data(), nscenes, W, chunk);
   BatchPlan &bp = bplan;

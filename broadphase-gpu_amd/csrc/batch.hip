@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 
 #include "batch.hpp"
 #include "plan.hpp"
@@ -221,8 +222,7 @@ __global__ __launch_bounds__(kT) void k_chain_end_seg(const T *__restrict__ smin
 }
 
 // ---- planning -------------------------------------------------------------------
-constexpr int kMaxWin = 2048;
-
+template <bool Q>
 __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
                                                  const QRec *__restrict__ qrec,
                                                  const int64_t *__restrict__ win_start,
@@ -235,7 +235,8 @@ __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
                                                  int2 *__restrict__ gchunk,
                                                  uint32_t *__restrict__ nchunks,
                                                  unsigned long long *__restrict__ lane_steps) {
-  __shared__ uint64_t key[kMaxWin];
+  typedef typename std::conditional<Q, uint32_t, uint64_t>::type K;
+  __shared__ K key[Q ? kMaxWindow32 : kMaxWindow64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   unsigned long long steps = 0;
   for (int b = blockIdx.x; b < nwin; b += gridDim.x) {  // grid-stride over windows
@@ -243,37 +244,25 @@ __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
     const int cnt = win_cnt[b];
     const int64_t gbase = win_gbase[b];
     for (int j = threadIdx.x; j < W; j += kT) {
-      uint64_t k = ~0ull;
+      K k = ~K(0);
       if (j < cnt) {
-        const uint32_t g = qrec ? group_key(qrec[wbase + j]) : (uint32_t)hi[wbase + j];
-        k = ((uint64_t)g << 32) | (uint32_t)j;
+        if (Q)
+          k = (K)(((uint64_t)group_key(qrec[wbase + j]) << kWinPosBits) | (uint32_t)j);
+        else
+          k = (K)(((uint64_t)(uint32_t)hi[wbase + j] << 32) | (uint32_t)j);
       }
       key[j] = k;
     }
     __syncthreads();
-    for (int size = 2; size <= W; size <<= 1) {
-      for (int stride = size >> 1; stride > 0; stride >>= 1) {
-        for (int j = threadIdx.x; j < W; j += kT) {
-          const int pj = j ^ stride;
-          if (pj > j) {
-            const bool upd = (j & size) == 0;
-            const uint64_t a = key[j], c = key[pj];
-            if ((a > c) == upd) {
-              key[j] = c;
-              key[pj] = a;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
+    window_sort<kT>(key, W);
+    const uint32_t jmask = Q ? (1u << kWinPosBits) - 1 : 0xffffffffu;
     const int ng = (cnt + 63) / 64;
     for (int k = w; k < ng; k += kT / 64) {
       const int j = 64 * k + lane;
       const bool act = j < cnt;
-      int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
+      int p = act ? (int)(wbase + (int64_t)((uint32_t)key[j] & jmask)) : INT_MAX;
       int h = act ? hi[p] : INT_MIN;
-      if (qrec) {  // the group's rows in chain-end order
+      if (Q) {  // the group's rows in chain-end order
         const uint64_t r = wave_sort_u64(act ? ((uint64_t)(uint32_t)h << 32) | (uint32_t)p : ~0ull,
                                          lane);
         if (act) {
@@ -416,8 +405,11 @@ void launch_plan_seg(const int *hi, const QRec *qrec, const BatchTables &t, cons
                      unsigned long long *lane_steps, hipStream_t st) {
   const int nw = (int)t.win_start.size();
   if (nw <= 0) return;
-  hipLaunchKernelGGL(k_plan_seg, dim3(std::min((unsigned)nw, kReduceGrid)), dim3(kT), 0, st, hi,
-                     qrec, d.win_start, d.win_cnt, d.win_gbase, nw, W, chunk, tail_cost, rowlist, groups,
+  if (W > (qrec ? kMaxWindow32 : kMaxWindow64))
+    throw StatusError(STQ_ERR_INVALID_ARGUMENT, "planning window too large");
+  hipLaunchKernelGGL(qrec ? k_plan_seg<true> : k_plan_seg<false>,
+                     dim3(std::min((unsigned)nw, kReduceGrid)), dim3(kT), 0, st, hi, qrec,
+                     d.win_start, d.win_cnt, d.win_gbase, nw, W, chunk, tail_cost, rowlist, groups,
                      tgroups, gchunk, nchunks, lane_steps);
   STQ_HIP(hipGetLastError());
 }

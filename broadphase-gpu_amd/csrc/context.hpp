@@ -39,7 +39,7 @@ constexpr int64_t kReplayPerScene = 8192;
 // reduced-precision form (whose groups are cut from windows ordered by
 // group_key: a larger window gives groups a smaller union box).
 constexpr int kBatchWindow = 256;
-constexpr int kQWindow = 2048;
+constexpr int kQWindow = 4096;
 // Reduced-precision item chunk when none is set (positions per body item).
 constexpr int kQChunk = 16384;
 

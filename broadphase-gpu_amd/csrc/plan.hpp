@@ -102,6 +102,35 @@ __device__ inline GroupPlan plan_group(int lane, int nrows, int p, int h, int ro
   return g;
 }
 
+// Planning windows: at most kMaxWindow64 rows with 64-bit keys (chain end,
+// position), kMaxWindow32 with 32-bit keys (group_key, position): 16 KB of
+// LDS either way.
+constexpr int kMaxWindow64 = 2048;
+constexpr int kMaxWindow32 = 4096;
+constexpr int kWinPosBits = 12;  // position bits of a 32-bit window key
+
+// Bitonic sort, ascending, of W keys (W a power of two) in LDS by the NT
+// threads of the block; ends with a barrier.
+template <int NT, class K>
+__device__ inline void window_sort(K *key, int W) {
+  for (int size = 2; size <= W; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int j = threadIdx.x; j < W; j += NT) {
+        const int pj = j ^ stride;
+        if (pj > j) {
+          const bool up = (j & size) == 0;
+          const K a = key[j], c = key[pj];
+          if ((a > c) == up) {
+            key[j] = c;
+            key[pj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // Sorts the keys of the 64 lanes ascending (a bitonic network over lane
 // shuffles); every lane calls this.
 __device__ inline uint64_t wave_sort_u64(uint64_t k, int lane) {

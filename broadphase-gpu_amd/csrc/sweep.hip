@@ -29,6 +29,7 @@
 // so the output order is fixed by the input.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "plan.hpp"
 #include "stq_internal.hpp"
@@ -221,15 +222,14 @@ __global__ __launch_bounds__(kThreads) void k_chain_end(const T *__restrict__ sm
 }
 
 // ---- work planning ----------------------------------------------------------------
-// Blocks take windows of W sorted rows (W a power of two, 64..2048) in
-// grid-stride order.  The rows of a window are ordered with a bitonic sort in
-// LDS, by (chain end, position), or with qrec by (group_key, position), and
-// cut into groups of 64; with qrec each group's rows are then put in chain-end
+// Blocks take windows of W sorted rows (W a power of two) in grid-stride
+// order.  The rows of a window are ordered with a bitonic sort in LDS, by
+// (chain end, position), or with qrec (Q) by (group_key, position), and cut
+// into groups of 64; with qrec each group's rows are then put in chain-end
 // order by a wave sort.  Group g covers rowlist[64g .. 64g+nrows); plan_group
 // splits it into a body over [min p + 1, cut] and, for rows with much longer
 // chains, a tail.  nchunks[g] = number of items of the group.
-constexpr int kMaxWindow = 2048;
-
+template <bool Q>
 __global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi,
                                                    const QRec *__restrict__ qrec, int64_t n,
                                                    int W, int chunk, float tail_cost,
@@ -239,7 +239,8 @@ __global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi,
                                                    int2 *__restrict__ gchunk,
                                                    uint32_t *__restrict__ nchunks,
                                                    unsigned long long *__restrict__ lane_steps) {
-  __shared__ uint64_t key[kMaxWindow];
+  typedef typename std::conditional<Q, uint32_t, uint64_t>::type K;
+  __shared__ K key[Q ? kMaxWindow32 : kMaxWindow64];
   const int64_t nwin = (n + W - 1) / W;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   unsigned long long steps = 0;
@@ -247,39 +248,26 @@ __global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi,
     const int64_t wbase = b * W;
     const int cnt = (int)std::min<int64_t>(W, n - wbase);
     for (int j = threadIdx.x; j < W; j += kThreads) {
-      uint64_t k = ~0ull;
+      K k = ~K(0);
       if (j < cnt) {
-        const uint32_t g = qrec ? group_key(qrec[wbase + j]) : (uint32_t)hi[wbase + j];
-        k = ((uint64_t)g << 32) | (uint32_t)j;
+        if (Q)
+          k = (K)(((uint64_t)group_key(qrec[wbase + j]) << kWinPosBits) | (uint32_t)j);
+        else
+          k = (K)(((uint64_t)(uint32_t)hi[wbase + j] << 32) | (uint32_t)j);
       }
       key[j] = k;
     }
     __syncthreads();
-    // bitonic sort ascending of W keys
-    for (int size = 2; size <= W; size <<= 1) {
-      for (int stride = size >> 1; stride > 0; stride >>= 1) {
-        for (int j = threadIdx.x; j < W; j += kThreads) {
-          const int pj = j ^ stride;
-          if (pj > j) {
-            const bool up = (j & size) == 0;
-            const uint64_t a = key[j], c = key[pj];
-            if ((a > c) == up) {
-              key[j] = c;
-              key[pj] = a;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
+    window_sort<kThreads>(key, W);
+    const uint32_t jmask = Q ? (1u << kWinPosBits) - 1 : 0xffffffffu;
     // one wave per group
     const int ng = (cnt + 63) / 64;
     for (int k = w; k < ng; k += kThreads / 64) {
       const int j = 64 * k + lane;
       const bool act = j < cnt;
-      int p = act ? (int)(wbase + (int64_t)(key[j] & 0xffffffffu)) : INT_MAX;
+      int p = act ? (int)(wbase + (int64_t)((uint32_t)key[j] & jmask)) : INT_MAX;
       int h = act ? hi[p] : INT_MIN;
-      if (qrec) {
+      if (Q) {
         const uint64_t r = wave_sort_u64(act ? ((uint64_t)(uint32_t)h << 32) | (uint32_t)p : ~0ull,
                                          lane);
         if (act) {
@@ -874,10 +862,16 @@ void launch_plan(const int *hi, const QRec *qrec, int64_t n, int W, int chunk, f
                  int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
                  unsigned long long *lane_steps, hipStream_t st) {
   if (n <= 0) return;
+  if (W > (qrec ? kMaxWindow32 : kMaxWindow64))
+    throw StatusError(STQ_ERR_INVALID_ARGUMENT, "planning window too large");
   const int64_t nwin = (n + W - 1) / W;
-  hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(nwin, kReduceGrid)), dim3(kThreads),
-                     0, st, hi, qrec, n, W, chunk, tail_cost, rowlist, groups, tgroups, gchunk,
-                     nchunks, lane_steps);
+  const dim3 grid((unsigned)std::min<int64_t>(nwin, kReduceGrid));
+  if (qrec)
+    hipLaunchKernelGGL(k_plan<true>, grid, dim3(kThreads), 0, st, hi, qrec, n, W, chunk, tail_cost,
+                       rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
+  else
+    hipLaunchKernelGGL(k_plan<false>, grid, dim3(kThreads), 0, st, hi, qrec, n, W, chunk,
+                       tail_cost, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
   STQ_HIP(hipGetLastError());
 }
 

@@ -18,6 +18,7 @@
 #include <cstring>
 
 #include "context.hpp"
+#include "plan.hpp"
 
 namespace stqamd {
 
@@ -87,8 +88,9 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
       e.axis_policy = a;
       consider(e);
     }
-    for (int w : {0, 256, 512, 1024, 2048}) {
+    for (int w : {0, 256, 512, 1024, 2048, 4096}) {
       if (batched && w == 0) continue;  // no automatic window in the batched form
+      if (!f32 && w > kMaxWindow64) continue;
       TuneEntry e = best;
       e.window = w;
       consider(e);
@@ -161,7 +163,7 @@ int Context::tune_load(const char *path) {
   while ((r = std::fscanf(f, "%llx %d %d %d %d %d %d %lf", &key, &e.axis_policy, &e.window,
                           &e.chunk, &e.batch, &e.waves_per_cu, &e.tail_cost, &e.ms)) == 8) {
     const bool ok = e.axis_policy >= 0 && e.axis_policy <= 4 && e.chunk > 0 &&
-                    (e.window == 0 || (e.window >= 64 && e.window <= 2048 &&
+                    (e.window == 0 || (e.window >= 64 && e.window <= kMaxWindow32 &&
                                        (e.window & (e.window - 1)) == 0)) &&
                     (e.batch == 2 || e.batch == 4 || e.batch == 8) && e.waves_per_cu > 0 &&
                     e.waves_per_cu <= 64 && e.tail_cost >= 0 && e.tail_cost <= 100000;

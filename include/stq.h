@@ -120,8 +120,10 @@ typedef struct stq_options {
   int32_t autotune;      /* 1: pick launch parameters by timing    */
   int32_t chunk;         /* q-steps per sweep work item; 0 = auto
                             (2048 plain FP64, 16384 reduced)       */
-  int32_t window;        /* rows regrouped per tile; 0 = auto
-                            (2048 reduced precision)               */
+  int32_t window;        /* rows regrouped per tile: 0 = auto
+                            (4096 reduced precision), or a power
+                            of two in [64, 4096] (at most 2048 is
+                            used by the plain FP64 form)           */
   int32_t waves_per_cu;  /* sweep residency per CU; 0 = auto      */
   int32_t batch;         /* q records per scalar-load batch of the
                             plain FP64 sweep: 0 = auto, or 2, 4, 8 */
