@@ -139,11 +139,15 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   void *kbuf = keys.get((size_t)n * (f32 ? 4 : 8));
   launch_keys(f32, boxes, n, axis, kbuf, order, (unsigned *)&ctr[0], st);
   STQ_HIP(hipEventRecord(ev[E_KEYS], st));
-  int passes = 0;
-  if (f32)
-    radix_sort_u32((uint32_t *)kbuf, order, n, 32, sort, st, &passes);
-  else
+  if (f32) {
+    // all four digits of FP32 keys usually vary: the form without the host
+    // read-back of the digit histogram
+    radix_sort_reserve(n, 4, sort);
+    radix_sort_u32_sync_free((uint32_t *)kbuf, order, n, 32, sort, st);
+  } else {
+    int passes = 0;
     radix_sort_u64((uint64_t *)kbuf, order, n, 64, sort, st, &passes);
+  }
   STQ_HIP(hipEventRecord(ev[E_SORT], st));
 
   const size_t ts = f32 ? 4 : 8;
@@ -157,16 +161,23 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   launch_gather(f32, boxes, order, n, axis, qp, d_smin, d_smax, d_rec, d_qrec, d_cols, st);
   launch_chain_end(f32, d_smin, d_smax, n, d_hi, &ctr[1], st);
   STQ_HIP(hipEventRecord(ev[E_GATHER], st));
+  // The error flags and the candidate count are read with the item count
+  // below, except where the window depends on the mean chain length (the
+  // kernels in between stay in bounds on any input).
   unsigned long long hc[4];
-  STQ_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
-  STQ_HIP(hipStreamSynchronize(st));
-  if (hc[0]) throw StatusError(STQ_ERR_INVALID_BOX, "input box with NaN or min > max");
-  stats.n_candidates = (int64_t)hc[1];
+  auto read_counters = [&]() {
+    STQ_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
+    STQ_HIP(hipStreamSynchronize(st));
+    if (hc[0]) throw StatusError(STQ_ERR_INVALID_BOX, "input box with NaN or min > max");
+    stats.n_candidates = (int64_t)hc[1];
+  };
 
   // plan
   int W = opt.window;
   if (W == 0 && f32) W = kQWindow;
-  if (W == 0) {
+  const bool early = W == 0;
+  if (early) {
+    read_counters();
     const double mean = (double)hc[1] / (double)n;
     W = 256;
     while (W < 2048 && (double)(2 * W) <= mean / 8.0) W *= 2;
@@ -185,7 +196,10 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
   int64_t nitems = 0;
   STQ_HIP(hipMemcpyAsync(&nitems, d_ioff + ngroups, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  STQ_HIP(hipStreamSynchronize(st));
+  if (early)
+    STQ_HIP(hipStreamSynchronize(st));
+  else
+    read_counters();
   stats.n_items = nitems;
   Item *d_items = items.as<Item>((size_t)std::max<int64_t>(nitems, 1));
   launch_items(d_groups, d_tg, d_gch, d_ioff, ngroups, d_items, st);
@@ -289,6 +303,24 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   finish_stats(st);
 }
 
+void Context::enqueue_refill(const SweepArgs &sa, bool f32, int batch, int nblocks, const int *ovf,
+                             const unsigned long long *novf, int64_t nbound, const int64_t *off,
+                             int2 *out, const int64_t *total, int64_t capacity, hipStream_t st) {
+  SweepArgs fa = sa;
+  fa.sel = ovf;
+  fa.nitems = nbound;
+  fa.nitems_dev = reinterpret_cast<const int64_t *>(novf);
+  fa.counts = nullptr;
+  fa.confirms = nullptr;
+  fa.caps = nullptr;
+  fa.offsets = off;
+  fa.out = out;
+  fa.total_dev = total;
+  fa.capacity = capacity;
+  if (fa.work) fa.work = claim_counters(counters.as<unsigned long long>(kCounters), 1);
+  launch_sweep(f32, kFill, batch, nblocks, fa, st);
+}
+
 // The single emitting pass sized by the network.  Every item gets
 // min(candidates, kCapScale * predicted + kCapSlack) slots of a scratch
 // buffer; the pass
@@ -353,21 +385,8 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
   STQ_HIP(hipMemsetAsync(d_novf, 0, sizeof(unsigned long long), st));
   launch_compact(d_counts, d_caps, d_capoff, d_off, nitems, d_scratch,
                  reinterpret_cast<int2 *>(dst), d_ovf, d_novf, st, nullptr, d_total, cap_dev);
-  {
-    SweepArgs fa = sa;
-    fa.sel = d_ovf;
-    fa.nitems = nitems;
-    fa.nitems_dev = reinterpret_cast<const int64_t *>(d_novf);
-    fa.counts = nullptr;
-    fa.confirms = nullptr;
-    fa.caps = nullptr;
-    fa.offsets = d_off;
-    fa.out = reinterpret_cast<int2 *>(dst);
-    fa.total_dev = d_total;
-    fa.capacity = cap_dev;
-    if (fa.work) fa.work = claim_counters(counters.as<unsigned long long>(kCounters), 1);
-    launch_sweep(f32, kFill, batch, nblocks, fa, st);
-  }
+  enqueue_refill(sa, f32, batch, nblocks, d_ovf, d_novf, nitems, d_off,
+                 reinterpret_cast<int2 *>(dst), d_total, cap_dev, st);
   int64_t total = 0;
   unsigned long long novf = 0;
   STQ_HIP(hipMemcpyAsync(&total, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -708,18 +727,9 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
       launch_compact(d_counts, d_caps, d_capoff, d_off, bound, d_scratch,
                      reinterpret_cast<int2 *>(out), d_ovf, &ctr[4], st, sa.nitems_dev, total,
                      capacity);
-      SweepArgs fa = sa;  // the overflowed items, exactly, straight into the output
-      fa.sel = d_ovf;
-      fa.nitems_dev = reinterpret_cast<const int64_t *>(&ctr[4]);
-      fa.counts = nullptr;
-      fa.confirms = nullptr;
-      fa.caps = nullptr;
-      fa.offsets = d_off;
-      fa.out = reinterpret_cast<int2 *>(out);
-      fa.total_dev = total;
-      fa.capacity = capacity;
-      if (fa.work) fa.work = claim_counters(ctr, 1);
-      launch_sweep(f32, kFill, batch, nblocks, fa, st);
+      // the overflowed items, exactly, straight into the output
+      enqueue_refill(sa, f32, batch, nblocks, d_ovf, &ctr[4], bound, d_off,
+                     reinterpret_cast<int2 *>(out), total, capacity, st);
     }
     launch_scene_pairs(d_off, d_ioff, t, bp.dev, d_pairoff, st);
     STQ_HIP(hipMemcpyAsync(hbuf, d_pairoff, sizeof(int64_t) * ((size_t)nscenes + 1),

@@ -93,6 +93,8 @@ struct PinnedBuf {
 // steps, [3] confirmations, [4] overflowed items; [8..12) and [12..16) the
 // item-claim counters (kXcds x u32) of the first and second sweep launch.
 constexpr int kCounters = 16;
+// Per-XCD work claim counters of sweep launch 0 or 1 of one call (zeroed with
+// the other counters at its start).
 inline unsigned *claim_counters(unsigned long long *ctr, int launch) {
   return reinterpret_cast<unsigned *>(ctr + 8 + 4 * launch);
 }
@@ -171,6 +173,13 @@ struct Context {
   int pick_axis(const stq_aabb_f64 *boxes, int64_t n, bool need_q, QParams *qp);
   void finish_stats(hipStream_t st);
   void sizer_ensure();
+  // Enqueues the exact refill of the overflowed items ovf[0 .. *novf) (at
+  // most nbound) of the sweep described by sa: a fill pass over them at their
+  // exact offsets off[item] into out (nothing is written when *total >
+  // capacity).  No host synchronisation; sweep launch 1.
+  void enqueue_refill(const SweepArgs &sa, bool f32, int batch, int nblocks, const int *ovf,
+                      const unsigned long long *novf, int64_t nbound, const int64_t *off,
+                      int2 *out, const int64_t *total, int64_t capacity, hipStream_t st);
   void run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const float *d_feat,
                    const uint32_t *d_cand, stq_pair *out, int64_t capacity, stq_pair **alloc_out,
                    int64_t *count);

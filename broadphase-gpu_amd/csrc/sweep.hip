@@ -715,12 +715,15 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
       int T = 0;
       auto test_slots = [&](int base, int n) __attribute__((always_inline)) {
         bool h = false, hg = false;
-        int cp = 0, cq = 0;
+        uint32_t o = 0, oq = 0;
         if (lane < n) {
           const int2 c = slots[base + lane];
-          cp = c.x;
-          cq = c.y;
-          const Exact64 xp = rec[cp], xq = rec[cq];
+          // the input ids load with the records, not after the test
+          if (MODE != kCount) {
+            o = a.orig[c.x];
+            oq = a.orig[c.y];
+          }
+          const Exact64 xp = rec[c.x], xq = rec[c.y];
           hg = ((accept_bits(xp.type) & xq.type) != 0) & (xp.hi0 >= xq.lo0) & (xp.lo0 <= xq.hi0) &
                (xp.hi1 >= xq.lo1) & (xp.lo1 <= xq.hi1) & (xp.hi2 >= xq.lo2) & (xp.lo2 <= xq.hi2);
           h = hg;
@@ -730,15 +733,7 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
         }
         nconf += (unsigned long long)__popcll(ballot(hg));
         const uint64_t hm = ballot(h);
-        if (hm) {
-          uint32_t o = 0;
-          uint32_t oq = 0;
-          if (MODE != kCount && h) {
-            o = a.orig[cp];
-            oq = a.orig[cq];
-          }
-          em.put(h, hm, o, oq, out);
-        }
+        if (hm) em.put(h, hm, o, oq, out);
       };
       auto drain_full = [&]() __attribute__((always_inline)) {
         __builtin_amdgcn_wave_barrier();
@@ -806,7 +801,7 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
     if (MODE != kFill && lane == 0) a.counts[it] = em.cnt;
     k = wc ? claim_take(ws, pend) : k + ws.nwaves;
   }
-  if (MODE != kFill && lane == 0 && nconf) atomicAdd(a.confirms, nconf);
+  if (MODE != kFill && lane == 0 && nconf && a.confirms) atomicAdd(a.confirms, nconf);
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
