@@ -140,10 +140,10 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   launch_keys(f32, boxes, n, axis, kbuf, order, (unsigned *)&ctr[0], st);
   STQ_HIP(hipEventRecord(ev[E_KEYS], st));
   if (f32) {
-    // all four digits of FP32 keys usually vary: the form without the host
-    // read-back of the digit histogram
+    // all four digits of FP32 keys usually vary: every digit a pass, with no
+    // digit histogram and no host read-back of it
     radix_sort_reserve(n, 4, sort);
-    radix_sort_u32_sync_free((uint32_t *)kbuf, order, n, 32, sort, st);
+    radix_sort_u32_all_digits((uint32_t *)kbuf, order, n, 32, sort, st);
   } else {
     int passes = 0;
     radix_sort_u64((uint64_t *)kbuf, order, n, 64, sort, st, &passes);

@@ -322,7 +322,7 @@ __global__ __launch_bounds__(kSortThreads) void k_zero_u32(unsigned *__restrict_
 // skipped digits cost nothing.
 template <class K>
 void radix_sort_impl(K *keys, uint32_t *vals, int64_t n, int bits, SortScratch &s,
-                     hipStream_t st, int *passes_run, bool sync_free) {
+                     hipStream_t st, int *passes_run, bool sync_free, bool all_digits = false) {
   if (passes_run) *passes_run = 0;
   if (n <= 1) return;
   const int ndigits = (bits + 7) / 8;
@@ -333,32 +333,38 @@ void radix_sort_impl(K *keys, uint32_t *vals, int64_t n, int bits, SortScratch &
   uint32_t *av = s.alt_vals.as<uint32_t>((size_t)n);
   unsigned *counts = s.counts.as<unsigned>((size_t)(256 * ntiles + 1));
   scan_scratch_reserve<unsigned>(256 * ntiles, s.misc);
-  hipLaunchKernelGGL(k_zero_u32, dim3(8), dim3(kSortThreads), 0, st, hist, 8 * 256);
-  const int hb = (int)std::min<int64_t>((n + kSortThreads - 1) / kSortThreads, 2048);
-  hipLaunchKernelGGL(k_hist_all<K>, dim3(hb), dim3(kSortThreads), 0, st, (const K *)keys, n,
-                     ndigits, hist);
-  STQ_HIP(hipGetLastError());
   unsigned hh[8 * 256];
-  if (sync_free) {
-    hipLaunchKernelGGL(k_skip_flags, dim3(1), dim3(kSortThreads), 0, st, (const unsigned *)hist,
-                       ndigits, n, skip);
-  } else {
-    STQ_HIP(hipMemcpyAsync(hh, hist, sizeof(unsigned) * ndigits * 256, hipMemcpyDeviceToHost, st));
-    STQ_HIP(hipStreamSynchronize(st));
+  // Digit histograms find the digits that are equal for all keys: skipped
+  // (read back to the host) or made copy passes (flags on the device).  With
+  // all_digits there is no histogram and every digit is a pass.
+  if (!all_digits) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(8), dim3(kSortThreads), 0, st, hist, 8 * 256);
+    const int hb = (int)std::min<int64_t>((n + kSortThreads - 1) / kSortThreads, 2048);
+    hipLaunchKernelGGL(k_hist_all<K>, dim3(hb), dim3(kSortThreads), 0, st, (const K *)keys, n,
+                       ndigits, hist);
+    STQ_HIP(hipGetLastError());
+    if (sync_free) {
+      hipLaunchKernelGGL(k_skip_flags, dim3(1), dim3(kSortThreads), 0, st,
+                         (const unsigned *)hist, ndigits, n, skip);
+    } else {
+      STQ_HIP(hipMemcpyAsync(hh, hist, sizeof(unsigned) * ndigits * 256, hipMemcpyDeviceToHost,
+                             st));
+      STQ_HIP(hipStreamSynchronize(st));
+    }
   }
 
   K *src_k = keys, *dst_k = ak;
   uint32_t *src_v = vals, *dst_v = av;
   int run = 0;
   for (int d = 0; d < ndigits; ++d) {
-    if (!sync_free) {
+    if (!sync_free && !all_digits) {
       bool trivial = false;
       for (int b = 0; b < 256; ++b)
         if ((int64_t)hh[d * 256 + b] == n) trivial = true;
       if (trivial) continue;
     }
     const int shift = 8 * d;
-    const unsigned *sk = sync_free ? skip : nullptr;
+    const unsigned *sk = sync_free && !all_digits ? skip : nullptr;
     hipLaunchKernelGGL(k_count<K>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, st,
                        (const K *)src_k, n, shift, (int)ntiles, counts, sk);
     scan_impl<unsigned>(counts, counts, 256 * ntiles, s.misc, st);
@@ -394,6 +400,10 @@ void radix_sort_u64_sync_free(uint64_t *keys, uint32_t *vals, int64_t n, int bit
 void radix_sort_u32_sync_free(uint32_t *keys, uint32_t *vals, int64_t n, int bits,
                               SortScratch &s, hipStream_t st) {
   radix_sort_impl<uint32_t>(keys, vals, n, bits, s, st, nullptr, true);
+}
+void radix_sort_u32_all_digits(uint32_t *keys, uint32_t *vals, int64_t n, int bits,
+                               SortScratch &s, hipStream_t st) {
+  radix_sort_impl<uint32_t>(keys, vals, n, bits, s, st, nullptr, true, true);
 }
 void radix_sort_reserve(int64_t n, int key_bytes, SortScratch &s) {
   if (n <= 1) return;

@@ -187,6 +187,10 @@ void radix_sort_u64_sync_free(uint64_t *keys, uint32_t *vals, int64_t n, int bit
                               SortScratch &s, hipStream_t st);
 void radix_sort_u32_sync_free(uint32_t *keys, uint32_t *vals, int64_t n, int bits,
                               SortScratch &s, hipStream_t st);
+// The sync-free sort without the digit histogram: every digit is a pass, for
+// keys whose digits are expected to vary (FP32 images of coordinates).
+void radix_sort_u32_all_digits(uint32_t *keys, uint32_t *vals, int64_t n, int bits,
+                               SortScratch &s, hipStream_t st);
 // Allocates the scratch a sort of n keys of key_bytes needs (nothing is
 // allocated by the sort itself afterwards).
 void radix_sort_reserve(int64_t n, int key_bytes, SortScratch &s);

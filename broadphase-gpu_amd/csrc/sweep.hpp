@@ -182,10 +182,9 @@ __device__ inline int scene_of(const int64_t *__restrict__ seg_off, int nscenes,
 // smin[lo] <= x, up == end or smin[up] > x: galloping with the step growing
 // (K+1)-fold, then (K+1)-ary cuts.  A chain of L takes about 2 log_{K+1} L
 // dependent load rounds instead of the 2 log_2 L of a binary search.
-template <class T>
+template <class T, int K = 4>
 __device__ __forceinline__ int64_t last_le(const T *__restrict__ smin, int64_t s, int64_t end,
                                            T x) {
-  constexpr int K = 4;
   int64_t lo = s, up = end, t = 1;
   while (true) {
     int c = 0;
