@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kT) void k_chain_end_seg(const T *__restrict__ smin
 
 // ---- planning -------------------------------------------------------------------
 template <bool Q>
-__global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
+__global__ __launch_bounds__(kPlanThreads) void k_plan_seg(const int *__restrict__ hi,
                                                  const QRec *__restrict__ qrec,
                                                  const int64_t *__restrict__ win_start,
                                                  const int *__restrict__ win_cnt,
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
     const int64_t wbase = win_start[b];
     const int cnt = win_cnt[b];
     const int64_t gbase = win_gbase[b];
-    for (int j = threadIdx.x; j < W; j += kT) {
+    for (int j = threadIdx.x; j < W; j += kPlanThreads) {
       K k = ~K(0);
       if (j < cnt) {
         if (Q)
@@ -254,10 +254,10 @@ __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
       key[j] = k;
     }
     __syncthreads();
-    window_sort<kT>(key, W);
+    window_sort_any<kPlanThreads, Q ? kMaxWindow32 : kMaxWindow64>(key, W);
     const uint32_t jmask = Q ? (1u << kWinPosBits) - 1 : 0xffffffffu;
     const int ng = (cnt + 63) / 64;
-    for (int k = w; k < ng; k += kT / 64) {
+    for (int k = w; k < ng; k += kPlanThreads / 64) {
       const int j = 64 * k + lane;
       const bool act = j < cnt;
       int p = act ? (int)(wbase + (int64_t)((uint32_t)key[j] & jmask)) : INT_MAX;
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(kT) void k_plan_seg(const int *__restrict__ hi,
     }
     __syncthreads();  // the next window reuses key[]
   }
-  block_atomic_add<kT>(steps, lane_steps);
+  block_atomic_add<kPlanThreads>(steps, lane_steps);
 }
 
 __global__ __launch_bounds__(kT) void k_scene_pairs(const int64_t *__restrict__ off,
@@ -408,7 +408,8 @@ void launch_plan_seg(const int *hi, const QRec *qrec, const BatchTables &t, cons
   if (W > (qrec ? kMaxWindow32 : kMaxWindow64))
     throw StatusError(STQ_ERR_INVALID_ARGUMENT, "planning window too large");
   hipLaunchKernelGGL(qrec ? k_plan_seg<true> : k_plan_seg<false>,
-                     dim3(std::min((unsigned)nw, kReduceGrid)), dim3(kT), 0, st, hi, qrec,
+                     dim3(std::min((unsigned)nw, kReduceGrid)), dim3(kPlanThreads), 0, st, hi,
+                     qrec,
                      d.win_start, d.win_cnt, d.win_gbase, nw, W, chunk, tail_cost, rowlist, groups,
                      tgroups, gchunk, nchunks, lane_steps);
   STQ_HIP(hipGetLastError());

@@ -108,6 +108,9 @@ __device__ inline GroupPlan plan_group(int lane, int nrows, int p, int h, int ro
 constexpr int kMaxWindow64 = 2048;
 constexpr int kMaxWindow32 = 4096;
 constexpr int kWinPosBits = 12;  // position bits of a 32-bit window key
+// Threads per planning block (one window at a time): 16 waves share the
+// window's sort and its 64 groups.
+constexpr int kPlanThreads = 1024;
 
 // Bitonic sort, ascending, of W keys (W a power of two) in LDS by the NT
 // threads of the block; ends with a barrier.
@@ -129,6 +132,68 @@ __device__ inline void window_sort(K *key, int W) {
       __syncthreads();
     }
   }
+}
+
+// The same sort with each thread holding E = W / NT consecutive keys in
+// registers: the compare-exchanges of strides below E stay in registers, the
+// others go through LDS (write, barrier, read the partner, barrier), so a
+// window of 4096 keys takes 36 LDS stages instead of 78.  Ends with the
+// sorted keys in LDS and a barrier.
+template <int NT, int E, class K>
+__device__ inline void window_sort_blocked(K *key) {
+  constexpr int W = NT * E;
+  K r[E];
+  const int base = threadIdx.x * E;
+#pragma unroll
+  for (int i = 0; i < E; ++i) r[i] = key[base + i];
+  for (int size = 2; size <= W; size <<= 1) {
+    for (int stride = size >> 1; stride >= E; stride >>= 1) {  // across threads
+#pragma unroll
+      for (int i = 0; i < E; ++i) key[base + i] = r[i];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int j = base + i, pj = j ^ stride;
+        const K o = key[pj];
+        const bool up = (j & size) == 0;
+        r[i] = ((j < pj) == up) ? (r[i] < o ? r[i] : o) : (r[i] < o ? o : r[i]);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int stride = E / 2; stride > 0; stride >>= 1) {  // within the thread's keys
+      if (stride < size) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          const int pi = i ^ stride;
+          if (pi > i) {
+            const bool up = ((base + i) & size) == 0;
+            const K a = r[i], c = r[pi];
+            const bool sw = (a > c) == up;
+            r[i] = sw ? c : a;
+            r[pi] = sw ? a : c;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < E; ++i) key[base + i] = r[i];
+  __syncthreads();
+}
+
+// Sort of a window of W <= WMAX keys (W a power of two): blocked for W >= NT.
+template <int NT, int WMAX, class K>
+__device__ inline void window_sort_any(K *key, int W) {
+  static_assert(WMAX <= 4 * NT, "at most 4 keys per thread");
+  if (WMAX >= 4 * NT && W == 4 * NT)
+    window_sort_blocked<NT, (WMAX >= 4 * NT ? 4 : 1)>(key);
+  else if (WMAX >= 2 * NT && W == 2 * NT)
+    window_sort_blocked<NT, (WMAX >= 2 * NT ? 2 : 1)>(key);
+  else if (W == NT)
+    window_sort_blocked<NT, 1>(key);
+  else
+    window_sort<NT>(key, W);
 }
 
 // Sorts the keys of the 64 lanes ascending (a bitonic network over lane

@@ -230,7 +230,7 @@ __global__ __launch_bounds__(kThreads) void k_chain_end(const T *__restrict__ sm
 // splits it into a body over [min p + 1, cut] and, for rows with much longer
 // chains, a tail.  nchunks[g] = number of items of the group.
 template <bool Q>
-__global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi,
+__global__ __launch_bounds__(kPlanThreads) void k_plan(const int *__restrict__ hi,
                                                    const QRec *__restrict__ qrec, int64_t n,
                                                    int W, int chunk, float tail_cost,
                                                    int *__restrict__ rowlist,
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi,
   for (int64_t b = blockIdx.x; b < nwin; b += gridDim.x) {  // grid-stride over windows
     const int64_t wbase = b * W;
     const int cnt = (int)std::min<int64_t>(W, n - wbase);
-    for (int j = threadIdx.x; j < W; j += kThreads) {
+    for (int j = threadIdx.x; j < W; j += kPlanThreads) {
       K k = ~K(0);
       if (j < cnt) {
         if (Q)
@@ -258,11 +258,11 @@ __global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi,
       key[j] = k;
     }
     __syncthreads();
-    window_sort<kThreads>(key, W);
+    window_sort_any<kPlanThreads, Q ? kMaxWindow32 : kMaxWindow64>(key, W);
     const uint32_t jmask = Q ? (1u << kWinPosBits) - 1 : 0xffffffffu;
     // one wave per group
     const int ng = (cnt + 63) / 64;
-    for (int k = w; k < ng; k += kThreads / 64) {
+    for (int k = w; k < ng; k += kPlanThreads / 64) {
       const int j = 64 * k + lane;
       const bool act = j < cnt;
       int p = act ? (int)(wbase + (int64_t)((uint32_t)key[j] & jmask)) : INT_MAX;
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kThreads) void k_plan(const int *__restrict__ hi,
     }
     __syncthreads();  // the next window reuses key[]
   }
-  block_atomic_add<kThreads>(steps, lane_steps);
+  block_atomic_add<kPlanThreads>(steps, lane_steps);
 }
 
 // Items of every group: the body's chunks, then the tail's (items[ioff[g]..]).
@@ -862,10 +862,10 @@ void launch_plan(const int *hi, const QRec *qrec, int64_t n, int W, int chunk, f
   const int64_t nwin = (n + W - 1) / W;
   const dim3 grid((unsigned)std::min<int64_t>(nwin, kReduceGrid));
   if (qrec)
-    hipLaunchKernelGGL(k_plan<true>, grid, dim3(kThreads), 0, st, hi, qrec, n, W, chunk, tail_cost,
-                       rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
+    hipLaunchKernelGGL(k_plan<true>, grid, dim3(kPlanThreads), 0, st, hi, qrec, n, W, chunk,
+                       tail_cost, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
   else
-    hipLaunchKernelGGL(k_plan<false>, grid, dim3(kThreads), 0, st, hi, qrec, n, W, chunk,
+    hipLaunchKernelGGL(k_plan<false>, grid, dim3(kPlanThreads), 0, st, hi, qrec, n, W, chunk,
                        tail_cost, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
   STQ_HIP(hipGetLastError());
 }
