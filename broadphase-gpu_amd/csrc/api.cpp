@@ -48,58 +48,6 @@ static int auto_chunk(const stq_options &o) {
   return o.chunk > 0 ? o.chunk : (o.precision == STQ_PRECISION_F32_CONFIRM ? kQChunk : 2048);
 }
 
-// Axis choice and, for the reduced-precision form, the quantisation range:
-// one pass of per-block statistics, combined on the host in block order.
-int Context::pick_axis(const stq_aabb_f64 *boxes, int64_t n, bool need_q, QParams *qp) {
-  const int pol = opt.axis_policy;
-  if (pol <= 2 && !need_q) return pol;
-  const int nb = (int)std::min<int64_t>((n + 255) / 256, 1024);
-  double *part = axis_partial.as<double>((size_t)nb * kAxisStats);
-  launch_axis_stats(boxes, n, part, nb, stream);
-  std::vector<double> h((size_t)nb * kAxisStats);
-  STQ_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
-  STQ_HIP(hipStreamSynchronize(stream));
-  double s[kAxisStats];
-  for (int k = 0; k < kAxisStats; ++k) s[k] = k < 12 ? 0.0 : (k < 15 ? INFINITY : -INFINITY);
-  for (int b = 0; b < nb; ++b)
-    for (int k = 0; k < kAxisStats; ++k) {
-      const double v = h[(size_t)b * kAxisStats + k];
-      s[k] = k < 12 ? s[k] + v : (k < 15 ? std::min(s[k], v) : std::max(s[k], v));
-    }
-  if (qp) {
-    for (int a = 0; a < 3; ++a) {
-      const double lo = s[12 + a], up = s[15 + a];
-      const double range = up - lo;
-      qp->origin[a] = std::isfinite(lo) ? lo : 0.0;
-      // kLevelMax levels across the scene; a degenerate or unbounded range
-      // puts every box on the full level range (scale 0)
-      qp->scale[a] = (range > 0 && std::isfinite(range)) ? kLevelMax / range : 0.0;
-      if (!std::isfinite(qp->scale[a])) qp->scale[a] = 0.0;
-    }
-  }
-  if (pol <= 2) return pol;
-  const double dn = (double)n;
-  int best = 0;
-  double bestv = 0;
-  for (int a = 0; a < 3; ++a) {
-    const double s1 = s[4 * a + 0], s2 = s[4 * a + 1], ext = s[4 * a + 2], cc = s[4 * a + 3];
-    double v;
-    if (pol == STQ_AXIS_VARIANCE) {
-      // sum over end points of (x - mu)^2, mu = mean centre (src/sweep.cu:93-123)
-      const double mu = s1 / (2.0 * dn);
-      v = s2 - 2.0 * mu * s1 + 2.0 * dn * mu * mu;
-      if (a == 0 || v > bestv) { best = a; bestv = v; }
-    } else {
-      const double mu = s1 / (2.0 * dn);
-      const double var = std::max(cc / dn - mu * mu, 0.0);
-      const double sd = std::sqrt(var);
-      v = sd > 0 ? (ext / dn) / sd : INFINITY;
-      if (a == 0 || v < bestv) { best = a; bestv = v; }
-    }
-  }
-  return best;
-}
-
 static float ev_ms(hipEvent_t a, hipEvent_t b) {
   float ms = 0;
   if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
@@ -131,13 +79,24 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   unsigned long long *ctr = counters.as<unsigned long long>(kCounters);
   STQ_HIP(hipMemsetAsync(ctr, 0, kCounters * sizeof(unsigned long long), st));
 
-  QParams qp{};
+  // Axis choice and, for the reduced-precision form and learned sizing, the
+  // quantisation range: per-block statistics, combined in block order on the
+  // device (k_axis_final), which the keys, the gather and the features read.
   const bool learned = opt.sizing == STQ_SIZING_LEARNED;
-  const int axis = pick_axis(boxes, n, f32 || learned, &qp);
+  const int pol = opt.axis_policy;
+  const int axis = pol <= 2 ? pol : 0;  // where no statistics are made
+  SceneParams *d_sp = nullptr;
+  if (pol > 2 || f32 || learned) {
+    const int nb = (int)std::min<int64_t>((n + 255) / 256, 1024);
+    double *part = axis_partial.as<double>((size_t)nb * kAxisStats);
+    d_sp = scene_params.as<SceneParams>(1);
+    launch_axis_stats(boxes, n, part, nb, st);
+    launch_axis_final(part, nb, n, pol, d_sp, st);
+  }
   stats.axis = axis;
   uint32_t *order = vals.as<uint32_t>((size_t)n);
   void *kbuf = keys.get((size_t)n * (f32 ? 4 : 8));
-  launch_keys(f32, boxes, n, axis, kbuf, order, (unsigned *)&ctr[0], st);
+  launch_keys(f32, boxes, n, axis, d_sp, kbuf, order, (unsigned *)&ctr[0], st);
   STQ_HIP(hipEventRecord(ev[E_KEYS], st));
   if (f32) {
     // all four digits of FP32 keys usually vary: every digit a pass, with no
@@ -158,16 +117,19 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   int *d_hi = hi.as<int>((size_t)n);
   ColStats *d_cols = learned ? col_stats.as<ColStats>((size_t)(n + kColBlock - 1) / kColBlock)
                              : nullptr;
-  launch_gather(f32, boxes, order, n, axis, qp, d_smin, d_smax, d_rec, d_qrec, d_cols, st);
+  launch_gather(f32, boxes, order, n, axis, d_sp, d_smin, d_smax, d_rec, d_qrec, d_cols, st);
   launch_chain_end(f32, d_smin, d_smax, n, d_hi, &ctr[1], st);
   STQ_HIP(hipEventRecord(ev[E_GATHER], st));
   // The error flags and the candidate count are read with the item count
   // below, except where the window depends on the mean chain length (the
   // kernels in between stay in bounds on any input).
   unsigned long long hc[4];
+  int axis_dev = axis;
   auto read_counters = [&]() {
     STQ_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
+    if (d_sp) STQ_HIP(hipMemcpyAsync(&axis_dev, &d_sp->axis, sizeof(int), hipMemcpyDeviceToHost, st));
     STQ_HIP(hipStreamSynchronize(st));
+    stats.axis = axis_dev;
     if (hc[0]) throw StatusError(STQ_ERR_INVALID_BOX, "input box with NaN or min > max");
     stats.n_candidates = (int64_t)hc[1];
   };
@@ -238,7 +200,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     sizer_ensure();
     d_feat = feat.as<float>((size_t)nitems * kFeat);
     d_cand = cand.as<uint32_t>((size_t)nitems);
-    launch_features(f32, sa, d_cols, qp, axis, nullptr, nullptr, 0, d_feat, d_cand, st);
+    launch_features(f32, sa, d_cols, QParams{}, 0, d_sp, nullptr, 1, d_feat, d_cand, st);
     ++scenes_seen;
   }
   if (learned_pass) {

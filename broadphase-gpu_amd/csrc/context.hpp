@@ -127,7 +127,8 @@ struct Context {
   SortScratch sort;
   // learned sizing: parameters, Adam state, per-item buffers
   DevBuf sz_params, sz_m, sz_v, sz_grad, sz_partial, sz_loss, feat, cand, pred, caps, cap_off,
-      targets, bfeat, btargets, scratch_pairs, ovf, mae_acc, replay_feat, replay_t, col_stats;
+      targets, bfeat, btargets, scratch_pairs, ovf, mae_acc, replay_feat, replay_t, col_stats,
+      scene_params;
   bool sz_ready = false;
   int adam_t = 0;
   int64_t replay_n = 0, replay_pos = 0, train_calls = 0;
@@ -165,12 +166,11 @@ struct Context {
     sort.misc.release();
     DevBuf *szb[] = {&sz_params, &sz_m, &sz_v, &sz_grad, &sz_partial, &sz_loss, &feat, &cand,
                      &pred, &caps, &cap_off, &targets, &bfeat, &btargets, &scratch_pairs, &ovf,
-                     &mae_acc, &replay_feat, &replay_t, &col_stats};
+                     &mae_acc, &replay_feat, &replay_t, &col_stats, &scene_params};
     for (DevBuf *b : szb) b->release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  int pick_axis(const stq_aabb_f64 *boxes, int64_t n, bool need_q, QParams *qp);
   void finish_stats(hipStream_t st);
   void sizer_ensure();
   // Enqueues the exact refill of the overflowed items ovf[0 .. *novf) (at

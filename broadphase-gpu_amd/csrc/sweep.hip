@@ -94,14 +94,88 @@ __global__ __launch_bounds__(kThreads) void k_axis_stats(const stq_aabb_f64 *__r
   }
 }
 
+// One block: the totals of the per-block statistics (combined in a fixed
+// order, so the result does not depend on timing), the quantisation range of
+// each axis and the sweep axis of the policy:
+//   0..2 that axis; 3 (variance) the largest sum over end points of
+//   (x - mu)^2, mu the mean centre (src/sweep.cu:93-123); 4 (chain) the
+//   smallest mean extent over the spread of the centres.
+// kLevelMax levels cover each axis's range; a degenerate or unbounded range
+// puts every box on the full level range (scale 0).
+__global__ __launch_bounds__(kThreads) void k_axis_final(const double *__restrict__ partial,
+                                                         int nb, int64_t n, int policy,
+                                                         SceneParams *__restrict__ out) {
+  __shared__ double red[kThreads];
+  __shared__ double tot[kAxisStats];
+  // per field: thread t combines blocks t, t + kThreads, ... in order, then a
+  // fixed tree over the threads (the same order on every call)
+  for (int k = 0; k < kAxisStats; ++k) {
+    const int mode = k < 12 ? 0 : (k < 15 ? 1 : 2);  // sum, min, max
+    double v = mode == 0 ? 0.0 : (mode == 1 ? INFINITY : -INFINITY);
+    for (int b = threadIdx.x; b < nb; b += kThreads) {
+      const double x = partial[(int64_t)b * kAxisStats + k];
+      v = mode == 0 ? v + x : (mode == 1 ? fmin(v, x) : fmax(v, x));
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = kThreads / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) {
+        const double x = red[threadIdx.x], y = red[threadIdx.x + h];
+        red[threadIdx.x] = mode == 0 ? x + y : (mode == 1 ? fmin(x, y) : fmax(x, y));
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) tot[k] = red[0];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  SceneParams sp;
+  sp.pad = 0;
+  for (int a = 0; a < 3; ++a) {
+    const double lo = tot[12 + a], up = tot[15 + a], range = up - lo;
+    sp.qp.origin[a] = isfinite(lo) ? lo : 0.0;
+    const double sc = (range > 0 && isfinite(range)) ? kLevelMax / range : 0.0;
+    sp.qp.scale[a] = isfinite(sc) ? sc : 0.0;
+  }
+  int best = policy <= 2 ? policy : 0;
+  if (policy > 2) {
+    const double dn = (double)n;
+    double bestv = 0;
+    for (int a = 0; a < 3; ++a) {
+      const double s1 = tot[4 * a + 0], s2 = tot[4 * a + 1], ext = tot[4 * a + 2],
+                   cc = tot[4 * a + 3];
+      const double mu = s1 / (2.0 * dn);
+      double v;
+      if (policy == STQ_AXIS_VARIANCE) {
+        v = s2 - 2.0 * mu * s1 + 2.0 * dn * mu * mu;
+        if (a == 0 || v > bestv) {
+          best = a;
+          bestv = v;
+        }
+      } else {
+        const double sd = sqrt(fmax(cc / dn - mu * mu, 0.0));
+        v = sd > 0 ? (ext / dn) / sd : INFINITY;
+        if (a == 0 || v < bestv) {
+          best = a;
+          bestv = v;
+        }
+      }
+    }
+  }
+  sp.axis = best;
+  *out = sp;
+}
+
 // ---- keys ---------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void k_keys_f64(const stq_aabb_f64 *__restrict__ boxes,
                                                        int64_t n, int axis,
+                                                       const SceneParams *__restrict__ sp,
                                                        uint64_t *__restrict__ keys,
                                                        uint32_t *__restrict__ vals,
                                                        unsigned *__restrict__ err) {
   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
+  if (sp) axis = sp->axis;
   const stq_aabb_f64 &b = boxes[i];
   const double x0 = b.min[0], x1 = b.min[1], x2 = b.min[2];
   const double y0 = b.max[0], y1 = b.max[1], y2 = b.max[2];
@@ -113,11 +187,13 @@ __global__ __launch_bounds__(kThreads) void k_keys_f64(const stq_aabb_f64 *__res
 
 __global__ __launch_bounds__(kThreads) void k_keys_f32(const stq_aabb_f64 *__restrict__ boxes,
                                                        int64_t n, int axis,
+                                                       const SceneParams *__restrict__ sp,
                                                        uint32_t *__restrict__ keys,
                                                        uint32_t *__restrict__ vals,
                                                        unsigned *__restrict__ err) {
   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
+  if (sp) axis = sp->axis;
   const stq_aabb_f64 &b = boxes[i];
   const double x0 = b.min[0], x1 = b.min[1], x2 = b.min[2];
   const double y0 = b.max[0], y1 = b.max[1], y2 = b.max[2];
@@ -137,13 +213,19 @@ __device__ inline void other_axes(int axis, int &a1, int &a2) {
 // kColBlock positions as well.
 __global__ __launch_bounds__(kThreads) void k_gather_f64(const stq_aabb_f64 *__restrict__ boxes,
                                                          const uint32_t *__restrict__ order,
-                                                         int64_t n, int axis, QParams qp,
+                                                         int64_t n, int axis,
+                                                         const SceneParams *__restrict__ sp,
                                                          double *__restrict__ smin,
                                                          double *__restrict__ smax,
                                                          Rec64 *__restrict__ rec,
                                                          ColStats *__restrict__ cols) {
   const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   const bool valid = s < n;
+  QParams qp{};
+  if (sp) {
+    axis = sp->axis;
+    qp = sp->qp;
+  }
   int a1, a2;
   other_axes(axis, a1, a2);
   Rec64 r{};
@@ -168,7 +250,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_f64(const stq_aabb_f64 *__r
 
 __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__restrict__ boxes,
                                                          const uint32_t *__restrict__ order,
-                                                         int64_t n, int axis, QParams qp,
+                                                         int64_t n, const SceneParams *__restrict__ sp,
                                                          float *__restrict__ smin,
                                                          float *__restrict__ smax,
                                                          Exact64 *__restrict__ rec,
@@ -176,6 +258,8 @@ __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__r
                                                          ColStats *__restrict__ cols) {
   const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   const bool valid = s < n;
+  const int axis = sp->axis;
+  const QParams qp = sp->qp;
   int a1, a2;
   other_axes(axis, a1, a2);
   Exact64 r{};
@@ -815,28 +899,35 @@ void launch_axis_stats(const stq_aabb_f64 *boxes, int64_t n, double *partial, in
   STQ_HIP(hipGetLastError());
 }
 
-void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, void *keys,
-                 uint32_t *vals, unsigned *err, hipStream_t st) {
+void launch_axis_final(const double *partial, int nblocks, int64_t n, int policy,
+                       SceneParams *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_axis_final, dim3(1), dim3(kThreads), 0, st, partial, nblocks, n, policy,
+                     out);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, const SceneParams *sp,
+                 void *keys, uint32_t *vals, unsigned *err, hipStream_t st) {
   if (n <= 0) return;
   if (f32)
-    hipLaunchKernelGGL(k_keys_f32, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, n, axis,
+    hipLaunchKernelGGL(k_keys_f32, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, n, axis, sp,
                        (uint32_t *)keys, vals, err);
   else
-    hipLaunchKernelGGL(k_keys_f64, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, n, axis,
+    hipLaunchKernelGGL(k_keys_f64, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, n, axis, sp,
                        (uint64_t *)keys, vals, err);
   STQ_HIP(hipGetLastError());
 }
 
 void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, int64_t n,
-                   int axis, const QParams &qp, void *smin, void *smax, void *rec, QRec *qrec,
+                   int axis, const SceneParams *sp, void *smin, void *smax, void *rec, QRec *qrec,
                    ColStats *cols, hipStream_t st) {
   if (n <= 0) return;
   if (f32)
     hipLaunchKernelGGL(k_gather_f32, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, order,
-                       n, axis, qp, (float *)smin, (float *)smax, (Exact64 *)rec, qrec, cols);
+                       n, sp, (float *)smin, (float *)smax, (Exact64 *)rec, qrec, cols);
   else
     hipLaunchKernelGGL(k_gather_f64, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, order,
-                       n, axis, qp, (double *)smin, (double *)smax, (Rec64 *)rec, cols);
+                       n, axis, sp, (double *)smin, (double *)smax, (Rec64 *)rec, cols);
   STQ_HIP(hipGetLastError());
 }
 

@@ -219,11 +219,17 @@ __device__ __forceinline__ int64_t last_le(const T *__restrict__ smin, int64_t s
 
 void launch_axis_stats(const stq_aabb_f64 *boxes, int64_t n, double *partial, int nblocks,
                        hipStream_t st);
-void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, void *keys,
-                 uint32_t *vals, unsigned *err, hipStream_t st);
+// The scene's sweep axis (axis policy 0..4) and level range from the
+// statistics of nblocks blocks, into *out on the device.
+void launch_axis_final(const double *partial, int nblocks, int64_t n, int policy,
+                       SceneParams *out, hipStream_t st);
+// The sweep axis is sp->axis where sp is given, else axis.  The FP32 form
+// needs sp (its level range).
+void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, const SceneParams *sp,
+                 void *keys, uint32_t *vals, unsigned *err, hipStream_t st);
 // cols: column statistics of every kColBlock positions (null: not made).
 void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, int64_t n,
-                   int axis, const QParams &qp, void *smin, void *smax, void *rec, QRec *qrec,
+                   int axis, const SceneParams *sp, void *smin, void *smax, void *rec, QRec *qrec,
                    ColStats *cols, hipStream_t st);
 void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, int *hi,
                       unsigned long long *cand, hipStream_t st);
