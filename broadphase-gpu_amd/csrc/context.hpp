@@ -20,14 +20,16 @@ namespace stqamd {
 
 // Learned sizing: Adam steps (minibatches of up to 8192 items) and learning
 // rate per training scene; slots per item = min(candidates, scale * prediction
-// + slack).  One scene: 3 * p + 64, which leaves few items to the exact refill
-// (a short, low-occupancy launch of the densest items) for a scratch of about
-// three times the pairs.  Batched scenes: 2 * p + 32; there overflow is rare
-// and the scratch budget of the whole batch stays at twice its pairs.
+// + slack).  One scene: 4 * p + 256.  Any overflowed item costs the exact
+// refill, a launch as long as the sweep of the densest such item by one wave,
+// while unused slots cost memory only (compaction reads the pairs, not the
+// slots): generous slots for a scratch of about four times the pairs.
+// Batched scenes: 2 * p + 32; there overflow is rare and the scratch budget of
+// the whole batch stays at twice its pairs.
 constexpr int kTrainSteps = 64;
 constexpr float kTrainRate = 2e-3f;
-constexpr float kCapScale = 3.0f;
-constexpr float kCapSlack = 64.f;
+constexpr float kCapScale = 4.0f;
+constexpr float kCapSlack = 256.f;
 constexpr float kBatchCapScale = 2.0f;
 constexpr float kBatchCapSlack = 32.f;
 // Replay ring of training rows: each training scene adds a strided sample of

@@ -86,8 +86,8 @@ def test_learned_pipeline_is_exact(ctx, prec):
 
 
 def test_forced_fallback_is_exact(ctx):
-    """Predictions of zero pairs give every item 64 slots: items with more pairs
-    overflow and are redone exactly."""
+    """Predictions of zero pairs give every item the slack (256 slots): items
+    with more pairs overflow and are redone exactly."""
     ctx.sizer_reset()
     w = np.zeros(mlp.NPARAMS, np.float32)
     w[-1] = -5.0
