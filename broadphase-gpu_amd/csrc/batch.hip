@@ -47,21 +47,7 @@ __global__ __launch_bounds__(kT) void k_seg_stats(const stq_aabb_f64 *__restrict
       up[a] = fmax(up[a], u);
     }
   }
-  __shared__ double red[kT];
-  for (int k = 0; k < kAxisStats; ++k) {
-    const int mode = k < 12 ? 0 : (k < 15 ? 1 : 2);
-    red[threadIdx.x] = k < 12 ? acc[k] : (k < 15 ? lo[k - 12] : up[k - 15]);
-    __syncthreads();
-    for (int s = kT / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) {
-        const double x = red[threadIdx.x], y = red[threadIdx.x + s];
-        red[threadIdx.x] = mode == 0 ? x + y : (mode == 1 ? fmin(x, y) : fmax(x, y));
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * kAxisStats + k] = red[0];
-    __syncthreads();
-  }
+  block_axis_stats<kT>(acc, lo, up, partial + (int64_t)blockIdx.x * kAxisStats);
 }
 
 // One thread per scene: its chunks are combined in order, then the axis is

@@ -77,21 +77,7 @@ __global__ __launch_bounds__(kThreads) void k_axis_stats(const stq_aabb_f64 *__r
       up[a] = fmax(up[a], u);
     }
   }
-  __shared__ double red[kThreads];
-  for (int k = 0; k < kAxisStats; ++k) {
-    const int mode = k < 12 ? 0 : (k < 15 ? 1 : 2);  // sum, min, max
-    red[threadIdx.x] = k < 12 ? acc[k] : (k < 15 ? lo[k - 12] : up[k - 15]);
-    __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) {
-        const double x = red[threadIdx.x], y = red[threadIdx.x + s];
-        red[threadIdx.x] = mode == 0 ? x + y : (mode == 1 ? fmin(x, y) : fmax(x, y));
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * kAxisStats + k] = red[0];
-    __syncthreads();
-  }
+  block_axis_stats<kThreads>(acc, lo, up, partial + (int64_t)blockIdx.x * kAxisStats);
 }
 
 // One block: the totals of the per-block statistics (combined in a fixed
@@ -105,29 +91,29 @@ __global__ __launch_bounds__(kThreads) void k_axis_stats(const stq_aabb_f64 *__r
 __global__ __launch_bounds__(kThreads) void k_axis_final(const double *__restrict__ partial,
                                                          int nb, int64_t n, int policy,
                                                          SceneParams *__restrict__ out) {
-  __shared__ double red[kThreads];
   __shared__ double tot[kAxisStats];
-  // per field: thread t combines blocks t, t + kThreads, ... in order, then a
-  // fixed tree over the threads (the same order on every call)
-  for (int k = 0; k < kAxisStats; ++k) {
-    const int mode = k < 12 ? 0 : (k < 15 ? 1 : 2);  // sum, min, max
-    double v = mode == 0 ? 0.0 : (mode == 1 ? INFINITY : -INFINITY);
-    for (int b = threadIdx.x; b < nb; b += kThreads) {
-      const double x = partial[(int64_t)b * kAxisStats + k];
-      v = mode == 0 ? v + x : (mode == 1 ? fmin(v, x) : fmax(v, x));
-    }
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int h = kThreads / 2; h > 0; h >>= 1) {
-      if ((int)threadIdx.x < h) {
-        const double x = red[threadIdx.x], y = red[threadIdx.x + h];
-        red[threadIdx.x] = mode == 0 ? x + y : (mode == 1 ? fmin(x, y) : fmax(x, y));
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) tot[k] = red[0];
-    __syncthreads();
+  // thread t combines blocks t, t + kThreads, ... in order, then the threads
+  // are combined in a fixed order (the same on every call)
+  double acc[12], lo[3], up[3];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = INFINITY;
+    up[a] = -INFINITY;
   }
+  for (int b = threadIdx.x; b < nb; b += kThreads) {
+    const double *x = partial + (int64_t)b * kAxisStats;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] += x[k];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fmin(lo[a], x[12 + a]);
+      up[a] = fmax(up[a], x[15 + a]);
+    }
+  }
+  block_axis_stats<kThreads>(acc, lo, up, tot);
+  __syncthreads();
   if (threadIdx.x != 0) return;
   SceneParams sp;
   sp.pad = 0;

@@ -35,6 +35,35 @@ struct QParams {
 // Per block: 12 sums (per axis: sum(min+max), sum(min^2+max^2), sum(max-min),
 // sum(centre^2)) then per axis min(min) and max(max): kAxisStats doubles.
 constexpr int kAxisStats = 18;
+// Combines the axis statistics of the block's threads (12 sums, 3 minima,
+// 3 maxima; NT threads, all call this) into out[0 .. kAxisStats): shuffle
+// trees inside each wave, then the waves in order, so the result is the same
+// on every call.
+template <int NT>
+__device__ inline void block_axis_stats(const double *acc, const double *lo, const double *up,
+                                        double *__restrict__ out) {
+  __shared__ double wv[NT / 64][kAxisStats];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kAxisStats; ++k) {
+    double v = k < 12 ? acc[k] : (k < 15 ? lo[k - 12] : up[k - 15]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double y = __shfl_xor(v, o, 64);
+      v = k < 12 ? v + y : (k < 15 ? fmin(v, y) : fmax(v, y));
+    }
+    if (lane == 0) wv[w][k] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < kAxisStats) {
+    const int k = threadIdx.x;
+    double v = wv[0][k];
+    for (int i = 1; i < NT / 64; ++i)
+      v = k < 12 ? v + wv[i][k] : (k < 15 ? fmin(v, wv[i][k]) : fmax(v, wv[i][k]));
+    out[k] = v;
+  }
+}
+
 // Outward 14-bit levels: t = (x - origin) * scale is monotone in x under IEEE
 // rounding, so floor(t(lo)) <= ceil(t(hi')) whenever lo <= hi': a pair that
 // overlaps in FP64 always overlaps in levels (a superset test).
