@@ -123,13 +123,15 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   // The error flags and the candidate count are read with the item count
   // below, except where the window depends on the mean chain length (the
   // kernels in between stay in bounds on any input).
-  unsigned long long hc[4];
-  int axis_dev = axis;
+  int64_t *sh = small_host();  // [0..3] counters, [4] axis, [5] items, [6] total, ...
+  const unsigned long long *hc = reinterpret_cast<const unsigned long long *>(sh);
+  int *axis_h = reinterpret_cast<int *>(sh + 4);
   auto read_counters = [&]() {
-    STQ_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
-    if (d_sp) STQ_HIP(hipMemcpyAsync(&axis_dev, &d_sp->axis, sizeof(int), hipMemcpyDeviceToHost, st));
+    *axis_h = axis;
+    STQ_HIP(hipMemcpyAsync(sh, ctr, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    if (d_sp) STQ_HIP(hipMemcpyAsync(axis_h, &d_sp->axis, sizeof(int), hipMemcpyDeviceToHost, st));
     STQ_HIP(hipStreamSynchronize(st));
-    stats.axis = axis_dev;
+    stats.axis = *axis_h;
     if (hc[0]) throw StatusError(STQ_ERR_INVALID_BOX, "input box with NaN or min > max");
     stats.n_candidates = (int64_t)hc[1];
   };
@@ -156,12 +158,12 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   launch_plan(d_hi, d_qrec, n, W, chunk, tail_cost(opt), d_rowlist, d_groups, d_tg, d_gch, d_nch,
               &ctr[2], st);
   scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
-  int64_t nitems = 0;
-  STQ_HIP(hipMemcpyAsync(&nitems, d_ioff + ngroups, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipMemcpyAsync(sh + 5, d_ioff + ngroups, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   if (early)
     STQ_HIP(hipStreamSynchronize(st));
   else
     read_counters();
+  const int64_t nitems = sh[5];
   stats.n_items = nitems;
   Item *d_items = items.as<Item>((size_t)std::max<int64_t>(nitems, 1));
   launch_items(d_groups, d_tg, d_gch, d_ioff, ngroups, d_items, st);
@@ -211,10 +213,10 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   launch_sweep(f32, kCount, batch, nblocks, sa, st);
   STQ_HIP(hipEventRecord(ev[E_COUNT], st));
   scan_u32_to_i64(d_counts, d_off, nitems, scan_tmp, st);
-  int64_t total = 0;
-  STQ_HIP(hipMemcpyAsync(&total, d_off + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipMemcpyAsync(sh + 6, d_off + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   STQ_HIP(hipEventRecord(ev[E_SCAN], st));
   STQ_HIP(hipStreamSynchronize(st));
+  const int64_t total = sh[6];
   *count = total;
   stats.n_pairs = total;
   if (use_net) {
@@ -232,10 +234,9 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     sizer_remember(d_feat, d_t, nitems);
     sizer_train(replay_feat.as<float>((size_t)kReplay * kFeat), replay_t.as<float>(kReplay), replay_n,
                 kTrainSteps, kTrainRate);
-    double mae = 0;
-    STQ_HIP(hipMemcpyAsync(&mae, acc, sizeof(double), hipMemcpyDeviceToHost, st));
+    STQ_HIP(hipMemcpyAsync(sh + 8, acc, sizeof(double), hipMemcpyDeviceToHost, st));
     STQ_HIP(hipStreamSynchronize(st));
-    stats.sizing_mae = mae / (double)nitems;
+    stats.sizing_mae = *reinterpret_cast<const double *>(sh + 8) / (double)nitems;
   }
   stq_pair *dst = out;
   if (alloc_out) {
@@ -303,10 +304,11 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
   launch_mlp_forward(w, d_feat, nitems, d_pred, st);
   launch_caps(d_pred, d_cand, nitems, kCapScale, kCapSlack, d_caps, st);
   scan_u32_to_i64(d_caps, d_capoff, nitems, scan_tmp, st);
-  int64_t total_cap = 0;
-  STQ_HIP(hipMemcpyAsync(&total_cap, d_capoff + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  int64_t *sh = small_host();
+  STQ_HIP(hipMemcpyAsync(sh + 9, d_capoff + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   STQ_HIP(hipEventRecord(ev[E_PRED], st));
   STQ_HIP(hipStreamSynchronize(st));
+  const int64_t total_cap = sh[9];
   int2 *d_scratch = scratch_pairs.as<int2>((size_t)std::max<int64_t>(total_cap, 1));
   STQ_HIP(hipEventRecord(ev[E_COUNT], st));
   uint32_t *d_counts = sa.counts;
@@ -330,9 +332,9 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
   stq_pair *dst = out;
   int64_t cap_dev = capacity;
   if (alloc_out) {
-    int64_t total = 0;
-    STQ_HIP(hipMemcpyAsync(&total, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    STQ_HIP(hipMemcpyAsync(sh + 6, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     STQ_HIP(hipStreamSynchronize(st));
+    const int64_t total = sh[6];
     void *p = nullptr;
     if (hipMalloc(&p, sizeof(stq_pair) * (size_t)std::max<int64_t>(total, 1)) != hipSuccess) {
       (void)hipGetLastError();
@@ -349,19 +351,18 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
                  reinterpret_cast<int2 *>(dst), d_ovf, d_novf, st, nullptr, d_total, cap_dev);
   enqueue_refill(sa, f32, batch, nblocks, d_ovf, d_novf, nitems, d_off,
                  reinterpret_cast<int2 *>(dst), d_total, cap_dev, st);
-  int64_t total = 0;
-  unsigned long long novf = 0;
-  STQ_HIP(hipMemcpyAsync(&total, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  STQ_HIP(hipMemcpyAsync(&novf, d_novf, sizeof(novf), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipMemcpyAsync(sh + 6, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipMemcpyAsync(sh + 7, d_novf, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   STQ_HIP(hipEventRecord(ev[E_COMPACT], st));
   STQ_HIP(hipEventRecord(ev[E_END], st));
-  double mae = 0;
-  STQ_HIP(hipMemcpyAsync(&mae, acc, sizeof(double), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipMemcpyAsync(sh + 8, acc, sizeof(double), hipMemcpyDeviceToHost, st));
   finish_stats(st);
+  const int64_t total = sh[6];
   *count = total;
   stats.n_pairs = total;
-  stats.n_fallback_tiles = (int64_t)novf;
-  stats.sizing_mae = mae / (double)std::max<int64_t>(nitems, 1);
+  stats.n_fallback_tiles = sh[7];
+  stats.sizing_mae =
+      *reinterpret_cast<const double *>(sh + 8) / (double)std::max<int64_t>(nitems, 1);
   if (!alloc_out && total > capacity) throw StatusError(STQ_ERR_CAPACITY, "output capacity too small");
 }
 
@@ -427,9 +428,9 @@ void Context::sizer_train(const float *d_feat, const float *d_target, int64_t n,
 }
 
 void Context::finish_stats(hipStream_t st) {
-  unsigned long long hc[4];
+  unsigned long long *hc = reinterpret_cast<unsigned long long *>(small_host() + 10);
   unsigned long long *ctr = counters.as<unsigned long long>(kCounters);
-  STQ_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipMemcpyAsync(hc, ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
   STQ_HIP(hipStreamSynchronize(st));
   stats.n_lane_steps = (int64_t)hc[2];
   stats.n_confirm = (int64_t)hc[3];

@@ -140,6 +140,11 @@ struct Context {
   DevBuf b_tables, b_skey, b_orig, b_pairoff, mesh_boxes;
   DevBuf box_kinds;  // narrow-phase hand-off: one kind byte per box
   PinnedBuf b_host;
+  // Small results of a single-scene call (counters, counts, totals) read back
+  // into pinned memory: the copies are plain DMA, with no staging through
+  // pageable memory on the host.
+  PinnedBuf s_host;
+  int64_t *small_host() { return static_cast<int64_t *>(s_host.get(32 * sizeof(int64_t))); }
   BatchPlan bplan;
   int64_t b_budget = 0;      // scratch pairs of the learned batched pass
   int64_t b_last_pairs = 0;  // pairs and items of the last batched call
@@ -151,6 +156,7 @@ struct Context {
   ~Context() {
     bplan.drop_graph();
     b_host.release();
+    s_host.release();
     DevBuf *bb[] = {&b_tables, &b_skey,  &b_orig,     &b_pairoff,
                     &tgroups,  &gchunks, &mesh_boxes, &box_kinds};
     for (DevBuf *b : bb) b->release();
