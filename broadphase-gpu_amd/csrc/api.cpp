@@ -1011,7 +1011,7 @@ int stq_find_pairs_batched_device(void *ctx, const stq_aabb_f64 *d_boxes,
 
 static unsigned *flag_word(Context *c) {
   return reinterpret_cast<unsigned *>(
-      c->counters.as<unsigned long long>(stqamd::kCounters) + stqamd::kCounters - 1);
+      c->counters.as<unsigned long long>(stqamd::kCounters) + stqamd::kFlagCounter);
 }
 
 static void check_flag(Context *c, unsigned *err, const char *what) {

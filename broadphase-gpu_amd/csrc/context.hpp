@@ -92,13 +92,17 @@ struct PinnedBuf {
 };
 
 // Device counters of one call: [0] error flags, [1] candidates, [2] lane
-// steps, [3] confirmations, [4] overflowed items; [8..12) and [12..16) the
-// item-claim counters (kXcds x u32) of the first and second sweep launch.
-constexpr int kCounters = 16;
+// steps, [3] confirmations, [4] overflowed items, [15] the mesh calls' error
+// flag; from [kClaimBase] the item-claim counters of the first and second
+// sweep launch, 8 per launch (one per XCD slice), each alone in a 256-byte
+// line (device-scope atomics on one line serialise across the XCDs).
+constexpr int kFlagCounter = 15;
+constexpr int kClaimBase = 16;
+constexpr int kCounters = kClaimBase + 2 * 8 * kClaimStride / 2;
 // Per-XCD work claim counters of sweep launch 0 or 1 of one call (zeroed with
 // the other counters at its start).
 inline unsigned *claim_counters(unsigned long long *ctr, int launch) {
-  return reinterpret_cast<unsigned *>(ctr + 8 + 4 * launch);
+  return reinterpret_cast<unsigned *>(ctr + kClaimBase) + launch * 8 * kClaimStride;
 }
 
 enum Ev {

@@ -412,14 +412,17 @@ __device__ __forceinline__ WaveSlice wave_slice(int64_t nitems, int xcd_split) {
 
 // Claims the next item of the slice: lane 0's atomic result, read out only
 // when the item is needed (claim_issue at the start of the previous item,
-// claim_take at its end), so the atomic's latency overlaps the sweep.
+// claim_take at its end), so the atomic's latency overlaps the sweep.  Claim
+// v is item kbeg + nwaves + v: items kbeg .. kbeg + nwaves - 1 are the waves'
+// first ones, taken without a claim, so a launch starts without a burst of
+// atomics and a wave with an empty slice issues none.
 __device__ __forceinline__ unsigned claim_issue(unsigned *ctr) {
   unsigned v = 0;
   if ((threadIdx.x & 63) == 0) v = atomicAdd(ctr, 1u);
   return v;
 }
 __device__ __forceinline__ int64_t claim_take(const WaveSlice &ws, unsigned v) {
-  return ws.kbeg + (int64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)v);
+  return ws.kbeg + ws.nwaves + (int64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)v);
 }
 
 // Bounds of the unchecked middle part: every active lane has p < q <= hi_p
@@ -544,8 +547,9 @@ __global__ __launch_bounds__(kThreads) void k_sweep64(SweepArgs a, const Rec64 *
   const int lane = threadIdx.x & 63;
   if (MODE == kFill && a.total_dev && *a.total_dev > a.capacity) return;
   const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split);
-  unsigned *wc = a.work ? a.work + ws.xcd : nullptr;
-  int64_t k = wc ? claim_take(ws, claim_issue(wc)) : ws.kbeg + ws.wave;
+  // the first item is the wave's own; the claims number the rest from nwaves
+  unsigned *wc = a.work ? a.work + ws.xcd * kClaimStride : nullptr;
+  int64_t k = ws.kbeg + ws.wave;
   while (k < ws.kend) {
     const unsigned pend = wc ? claim_issue(wc) : 0u;
     const int64_t it = a.sel ? (int64_t)a.sel[k] : k;
@@ -747,8 +751,9 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
   unsigned long long nconf = 0;
   if (MODE == kFill && a.total_dev && *a.total_dev > a.capacity) return;
   const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split);
-  unsigned *wc = a.work ? a.work + ws.xcd : nullptr;
-  int64_t k = wc ? claim_take(ws, claim_issue(wc)) : ws.kbeg + ws.wave;
+  // the first item is the wave's own; the claims number the rest from nwaves
+  unsigned *wc = a.work ? a.work + ws.xcd * kClaimStride : nullptr;
+  int64_t k = ws.kbeg + ws.wave;
   while (k < ws.kend) {
     const unsigned pend = wc ? claim_issue(wc) : 0u;
     const int64_t it = a.sel ? (int64_t)a.sel[k] : k;

@@ -274,6 +274,9 @@ void launch_items(const int4 *groups, const int4 *tgroups, const int2 *gchunk, c
                   int64_t ngroups, Item *items, hipStream_t st);
 enum SweepMode { kCount = 0, kFill = 1, kLearned = 2 };
 
+// Words between the per-XCD item-claim counters of one sweep launch.
+constexpr int kClaimStride = 64;
+
 struct SweepArgs {
   const Item *items = nullptr;
   int64_t nitems = 0;            // work items (or selected items when sel != null)
@@ -296,8 +299,9 @@ struct SweepArgs {
   const int64_t *nitems_dev = nullptr;
   const int64_t *total_dev = nullptr;
   int64_t capacity = 0;
-  // kXcds zeroed claim counters (one per XCD slice): waves claim items
-  // dynamically; null: a fixed stride
+  // Zeroed claim counters, one per XCD slice, kClaimStride words apart: each
+  // wave takes its first item by its index in the slice and claims the
+  // following ones; null: a fixed stride
   unsigned *work = nullptr;
 };
 
