@@ -75,7 +75,8 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     stats.axis = opt.axis_policy <= 2 ? opt.axis_policy : 0;
     return;
   }
-  // counters: [0] error flags, [1] candidates, [2] lane steps, [3] confirms
+  // counters: [0] error flags, [1] candidates, [2] lane steps; confirmations
+  // per XCD slice in the claim lines
   unsigned long long *ctr = counters.as<unsigned long long>(kCounters);
   STQ_HIP(hipMemsetAsync(ctr, 0, kCounters * sizeof(unsigned long long), st));
 
@@ -184,7 +185,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   sa.qrec = d_qrec;
   sa.orig = order;
   sa.counts = d_counts;
-  sa.confirms = &ctr[3];
+  sa.confirms = confirm_counters(ctr);
   sa.xcd_split = opt.xcd_split == 2 ? 0 : 1;
   const bool claim = opt.claim != 2;
   sa.work = claim ? claim_counters(ctr, 0) : nullptr;
@@ -430,10 +431,13 @@ void Context::sizer_train(const float *d_feat, const float *d_target, int64_t n,
 void Context::finish_stats(hipStream_t st) {
   unsigned long long *hc = reinterpret_cast<unsigned long long *>(small_host() + 10);
   unsigned long long *ctr = counters.as<unsigned long long>(kCounters);
+  unsigned long long *lines = reinterpret_cast<unsigned long long *>(small_host() + 32);
   STQ_HIP(hipMemcpyAsync(hc, ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipMemcpyAsync(lines, ctr + kClaimBase, kClaimLinesU64 * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, st));
   STQ_HIP(hipStreamSynchronize(st));
   stats.n_lane_steps = (int64_t)hc[2];
-  stats.n_confirm = (int64_t)hc[3];
+  stats.n_confirm = (int64_t)confirm_sum(lines);
   stats.ms_total = ev_ms(ev[E_START], ev[E_END]);
   stats.ms_keys = ev_ms(ev[E_START], ev[E_KEYS]);
   stats.ms_sort = ev_ms(ev[E_KEYS], ev[E_SORT]);
@@ -613,8 +617,9 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   int2 *d_scratch = single_pass ? scratch_pairs.as<int2>((size_t)b_budget) : nullptr;
   double *d_acc = single_pass ? mae_acc.as<double>(1) : nullptr;
   const float *d_w = learned ? sz_params.as<float>(kParams) : nullptr;
-  // pinned results: pair offsets, 5 counters, items, total slots, error sum
-  const int64_t nh = (int64_t)nscenes + 1 + 8;
+  // pinned results: pair offsets, 5 counters, items, total slots, error sum,
+  // the claim lines of the first sweep launch (confirmation counters)
+  const int64_t nh = (int64_t)nscenes + 1 + 8 + kClaimLinesU64;
   int64_t *hbuf = static_cast<int64_t *>(b_host.get(sizeof(int64_t) * (size_t)nh));
   int64_t *h_ctr = hbuf + nscenes + 1;
 
@@ -657,7 +662,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     sa.qrec = d_qrec;
     sa.orig = d_orig;
     sa.counts = d_counts;
-    sa.confirms = &ctr[3];
+    sa.confirms = confirm_counters(ctr);
     sa.xcd_split = opt.xcd_split == 2 ? 0 : 1;
     sa.work = opt.claim != 2 ? claim_counters(ctr, 0) : nullptr;
     if (learned)
@@ -699,6 +704,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
                            hipMemcpyDeviceToHost, st));
     STQ_HIP(hipMemcpyAsync(h_ctr, ctr, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     STQ_HIP(hipMemcpyAsync(h_ctr + 5, d_ioff + ngroups, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    STQ_HIP(hipMemcpyAsync(h_ctr + 8, ctr + kClaimBase, kClaimLinesU64 * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, st));
     if (single_pass) {
       STQ_HIP(hipMemcpyAsync(h_ctr + 6, d_capoff + bound, sizeof(int64_t), hipMemcpyDeviceToHost, st));
       STQ_HIP(hipMemcpyAsync(h_ctr + 7, d_acc, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -778,7 +785,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   const int64_t nitems = h_ctr[5], total = pair_off[nscenes];
   stats.n_candidates = (int64_t)hc[1];
   stats.n_lane_steps = (int64_t)hc[2];
-  stats.n_confirm = (int64_t)hc[3];
+  stats.n_confirm = (int64_t)confirm_sum(reinterpret_cast<const unsigned long long *>(h_ctr + 8));
   stats.n_items = nitems;
   stats.n_pairs = total;
   if (hc[0]) throw StatusError(STQ_ERR_INVALID_BOX, "input box with NaN or min > max");

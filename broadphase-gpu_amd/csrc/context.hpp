@@ -92,7 +92,7 @@ struct PinnedBuf {
 };
 
 // Device counters of one call: [0] error flags, [1] candidates, [2] lane
-// steps, [3] confirmations, [4] overflowed items, [15] the mesh calls' error
+// steps, [4] overflowed items, [15] the mesh calls' error
 // flag; from [kClaimBase] the item-claim counters of the first and second
 // sweep launch, 8 per launch (one per XCD slice), each alone in a 256-byte
 // line (device-scope atomics on one line serialise across the XCDs).
@@ -103,6 +103,16 @@ constexpr int kCounters = kClaimBase + 2 * 8 * kClaimStride / 2;
 // the other counters at its start).
 inline unsigned *claim_counters(unsigned long long *ctr, int launch) {
   return reinterpret_cast<unsigned *>(ctr + kClaimBase) + launch * 8 * kClaimStride;
+}
+// u64 words of the 8 claim lines of launch 0, which also hold the per-XCD
+// FP64 confirmation counters (second u64 of each line).
+constexpr int kClaimLinesU64 = 8 * kClaimStride / 2;
+inline unsigned long long *confirm_counters(unsigned long long *ctr) { return ctr + kClaimBase + 1; }
+// Sum of the confirmation counters in a host copy of those lines.
+inline unsigned long long confirm_sum(const unsigned long long *lines) {
+  unsigned long long s = 0;
+  for (int x = 0; x < 8; ++x) s += lines[x * (kClaimStride / 2) + 1];
+  return s;
 }
 
 enum Ev {
@@ -148,7 +158,10 @@ struct Context {
   // into pinned memory: the copies are plain DMA, with no staging through
   // pageable memory on the host.
   PinnedBuf s_host;
-  int64_t *small_host() { return static_cast<int64_t *>(s_host.get(32 * sizeof(int64_t))); }
+  // [0..32) single values, then a copy of the claim lines of launch 0
+  int64_t *small_host() {
+    return static_cast<int64_t *>(s_host.get((32 + kClaimLinesU64) * sizeof(int64_t)));
+  }
   BatchPlan bplan;
   int64_t b_budget = 0;      // scratch pairs of the learned batched pass
   int64_t b_last_pairs = 0;  // pairs and items of the last batched call

@@ -876,7 +876,8 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
     if (MODE != kFill && lane == 0) a.counts[it] = em.cnt;
     k = wc ? claim_take(ws, pend) : k + ws.nwaves;
   }
-  if (MODE != kFill && lane == 0 && nconf && a.confirms) atomicAdd(a.confirms, nconf);
+  if (MODE != kFill && lane == 0 && nconf && a.confirms)
+    atomicAdd(a.confirms + ws.xcd * (kClaimStride / 2), nconf);
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }

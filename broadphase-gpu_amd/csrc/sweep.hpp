@@ -290,7 +290,9 @@ struct SweepArgs {
   const int64_t *offsets = nullptr;  // kFill: output offset; kLearned: scratch offset
   const uint32_t *caps = nullptr;    // kLearned: capacity per item
   int2 *out = nullptr;           // kFill: output; kLearned: scratch
-  unsigned long long *confirms = nullptr;  // kCount, kLearned: FP64 confirmations (null: none)
+  // kCount, kLearned: FP64 confirmations, one counter per XCD slice
+  // kClaimStride / 2 words apart (null: none)
+  unsigned long long *confirms = nullptr;
   int zero = 0;                  // always 0 (an opaque runtime zero)
   int xcd_split = 1;             // 1: each XCD sweeps one contiguous slice of items
   // device-resident sizes (batched pipeline): the item count, and for kFill
