@@ -88,7 +88,8 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   const int axis = pol <= 2 ? pol : 0;  // where no statistics are made
   SceneParams *d_sp = nullptr;
   if (pol > 2 || f32 || learned) {
-    const int nb = (int)std::min<int64_t>((n + 255) / 256, 1024);
+    // about four boxes per thread (the loads of one thread overlap)
+    const int nb = (int)std::min<int64_t>((n + 1023) / 1024, 4096);
     double *part = axis_partial.as<double>((size_t)nb * kAxisStats);
     d_sp = scene_params.as<SceneParams>(1);
     launch_axis_stats(boxes, n, part, nb, st);
@@ -203,7 +204,9 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     sizer_ensure();
     d_feat = feat.as<float>((size_t)nitems * kFeat);
     d_cand = cand.as<uint32_t>((size_t)nitems);
-    launch_features(f32, sa, d_cols, QParams{}, 0, d_sp, nullptr, 1, d_feat, d_cand, st);
+    const int64_t ncols = (n + kColBlock - 1) / kColBlock;
+    launch_features(f32, sa, d_cols, ncols, col_work.as<double>(col_work_doubles(ncols)),
+                    QParams{}, 0, d_sp, nullptr, 1, d_feat, d_cand, st);
     ++scenes_seen;
   }
   if (learned_pass) {
@@ -607,8 +610,9 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   int64_t *d_pairoff = b_pairoff.as<int64_t>((size_t)nscenes + 1);
   unsigned long long *ctr = counters.as<unsigned long long>(kCounters);
   float *d_feat = learned ? feat.as<float>((size_t)bound * kFeat) : nullptr;
-  ColStats *d_cols =
-      learned ? col_stats.as<ColStats>((size_t)(n + kColBlock - 1) / kColBlock) : nullptr;
+  const int64_t ncols = (n + kColBlock - 1) / kColBlock;
+  ColStats *d_cols = learned ? col_stats.as<ColStats>((size_t)ncols) : nullptr;
+  double *d_colwork = learned ? col_work.as<double>(col_work_doubles(ncols)) : nullptr;
   uint32_t *d_cand = learned ? cand.as<uint32_t>((size_t)bound) : nullptr;
   float *d_pred = learned ? pred.as<float>((size_t)bound) : nullptr;
   uint32_t *d_caps = single_pass ? caps.as<uint32_t>((size_t)bound) : nullptr;
@@ -666,8 +670,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     sa.xcd_split = opt.xcd_split == 2 ? 0 : 1;
     sa.work = opt.claim != 2 ? claim_counters(ctr, 0) : nullptr;
     if (learned)
-      launch_features(f32, sa, d_cols, QParams{}, 0, bp.dev.params, bp.dev.seg_off, nscenes, d_feat,
-                      d_cand, st);
+      launch_features(f32, sa, d_cols, ncols, d_colwork, QParams{}, 0, bp.dev.params,
+                      bp.dev.seg_off, nscenes, d_feat, d_cand, st);
     const int64_t *total = d_off + bound;
     if (!single_pass) {
       launch_sweep(f32, kCount, batch, nblocks, sa, st);

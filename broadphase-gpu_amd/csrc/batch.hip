@@ -33,20 +33,7 @@ __global__ __launch_bounds__(kT) void k_seg_stats(const stq_aabb_f64 *__restrict
     lo[a] = INFINITY;
     up[a] = -INFINITY;
   }
-  for (int64_t i = b0 + threadIdx.x; i < b1; i += kT) {
-    const stq_aabb_f64 &b = boxes[i];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const double l = b.min[a], u = b.max[a];
-      const double c = 0.5 * (l + u);
-      acc[4 * a + 0] += l + u;
-      acc[4 * a + 1] += l * l + u * u;
-      acc[4 * a + 2] += u - l;
-      acc[4 * a + 3] += c * c;
-      lo[a] = fmin(lo[a], l);
-      up[a] = fmax(up[a], u);
-    }
-  }
+  axis_stats_range(boxes, b0 + threadIdx.x, b1, kT, acc, lo, up);
   block_axis_stats<kT>(acc, lo, up, partial + (int64_t)blockIdx.x * kAxisStats);
 }
 
@@ -308,7 +295,7 @@ void build_batch_tables(const int64_t *off, int nscenes, int W, BatchTables &t) 
   t.win_gbase.clear();
   t.scene_first_group.assign(nscenes + 1, 0);
   int64_t g = 0;
-  const int64_t kStatChunk = 65536;
+  const int64_t kStatChunk = 8192;  // 32 boxes per thread of a statistics block
   for (int s = 0; s < nscenes; ++s) {
     const int64_t a = off[s], b = off[s + 1];
     t.scene_chunk_begin[s] = (int)t.chunk_scene.size();

@@ -39,10 +39,124 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// ---- prefix sums of the column statistics ---------------------------------------
+// pre[j][k] = sum of field k (ColStats order) over blocks 0 .. j - 1, in
+// FP64, j = 0 .. m; an item's column sums are then two lookups.  Three
+// launches: the sum of every run of kColRun blocks (one wave each), the
+// exclusive scan of the runs (one block), each block's prefix (one wave per
+// run, a wave scan on top of its run's prefix).
+constexpr int kColFields = 12;
+constexpr int kColRun = kWave;
+constexpr int kTopThreads = 1024;
+
+__device__ __forceinline__ void col_fields(const ColStats &c, double *v) {
+  v[0] = c.le1;
+  v[1] = c.le1sq;
+  v[2] = c.le2;
+  v[3] = c.le2sq;
+  v[4] = c.c1;
+  v[5] = c.c1sq;
+  v[6] = c.c2;
+  v[7] = c.c2sq;
+  v[8] = c.nv;
+  v[9] = c.ne;
+  v[10] = c.nf;
+  v[11] = c.cnt;
+}
+
+__global__ __launch_bounds__(kT) void k_col_runs(const ColStats *__restrict__ cols, int64_t m,
+                                                 double *__restrict__ run) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  const int64_t j = w * kColRun + lane;
+  double v[kColFields];
+#pragma unroll
+  for (int k = 0; k < kColFields; ++k) v[k] = 0.0;
+  if (j < m) col_fields(cols[j], v);
+#pragma unroll
+  for (int k = 0; k < kColFields; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0 && w * kColRun < m) {
+#pragma unroll
+    for (int k = 0; k < kColFields; ++k) run[w * kColFields + k] = v[k];
+  }
+}
+
+// Exclusive scan of the nr run sums in place; the total goes to total[].
+// Thread t owns a contiguous slice of runs.
+__global__ __launch_bounds__(kTopThreads) void k_col_top(double *__restrict__ run, int64_t nr,
+                                                         double *__restrict__ total) {
+  __shared__ double wtot[kTopThreads / 64][kColFields];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t per = (nr + kTopThreads - 1) / kTopThreads;
+  const int64_t r0 = (int64_t)t * per < nr ? (int64_t)t * per : nr;
+  const int64_t r1 = r0 + per < nr ? r0 + per : nr;
+  double own[kColFields], inc[kColFields];
+#pragma unroll
+  for (int k = 0; k < kColFields; ++k) own[k] = 0.0;
+  for (int64_t r = r0; r < r1; ++r)
+#pragma unroll
+    for (int k = 0; k < kColFields; ++k) own[k] += run[r * kColFields + k];
+#pragma unroll
+  for (int k = 0; k < kColFields; ++k) {
+    double x = own[k];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const double y = __shfl_up(x, o, kWave);
+      if (lane >= o) x += y;
+    }
+    inc[k] = x;
+    if (lane == 63) wtot[w][k] = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kColFields; ++k) {
+    double before = 0.0;
+    for (int i = 0; i < w; ++i) before += wtot[i][k];
+    inc[k] += before;
+  }
+  if (t == kTopThreads - 1) {
+#pragma unroll
+    for (int k = 0; k < kColFields; ++k) total[k] = inc[k];
+  }
+  double acc[kColFields];
+#pragma unroll
+  for (int k = 0; k < kColFields; ++k) acc[k] = inc[k] - own[k];
+  for (int64_t r = r0; r < r1; ++r)
+#pragma unroll
+    for (int k = 0; k < kColFields; ++k) {
+      const double x = run[r * kColFields + k];
+      run[r * kColFields + k] = acc[k];
+      acc[k] += x;
+    }
+}
+
+__global__ __launch_bounds__(kT) void k_col_pre(const ColStats *__restrict__ cols, int64_t m,
+                                                const double *__restrict__ run,
+                                                double *__restrict__ pre) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (w * kColRun >= m) return;
+  const int64_t j = w * kColRun + lane;
+  double v[kColFields];
+#pragma unroll
+  for (int k = 0; k < kColFields; ++k) v[k] = 0.0;
+  if (j < m) col_fields(cols[j], v);
+#pragma unroll
+  for (int k = 0; k < kColFields; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const double y = __shfl_up(x, o, kWave);
+      if (lane >= o) x += y;
+    }
+    if (j < m) pre[j * kColFields + k] = run[w * kColFields + k] + (x - v[k]);
+  }
+}
+
 // ---- features ----------------------------------------------------------------
 // One wave per item.  Rows are the item's lanes; the q range is described by
-// the column statistics of the position blocks it meets (one block per lane,
-// loaded whole).  Extents enter as log(extent)/10, centres as the position
+// the column statistics of the position blocks it meets (their sums: two
+// lookups in the prefix sums).  Extents enter as log(extent)/10, centres as the position
 // inside the scene's quantisation range (0..1).
 // With scene tables (batched scenes) the level range and axes are those of the
 // item's scene, and the item count is read from the device.
@@ -113,7 +227,7 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
                                                  const int *__restrict__ rowlist,
                                                  const int *__restrict__ hi,
                                                  const R *__restrict__ rec,
-                                                 const ColStats *__restrict__ cols, QParams qp0,
+                                                 const double *__restrict__ pre, QParams qp0,
                                                  int a10, int a20,
                                                  const SceneParams *__restrict__ sp,
                                                  const int64_t *__restrict__ seg_off, int nscenes,
@@ -205,27 +319,13 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
     }
     const int span = q1 - q0;
     const long long cand_it = wave_sum(cnd);
-    // column sums over the blocks [q0 / kColBlock, q1 / kColBlock], one per lane
-    float cs[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) cs[k] = 0.f;
-    for (int j = q0 / kColBlock + lane; j <= q1 / kColBlock; j += kWave) {
-      const ColStats c = cols[j];
-      cs[0] += c.le1;
-      cs[1] += c.le1sq;
-      cs[2] += c.le2;
-      cs[3] += c.le2sq;
-      cs[4] += c.c1;
-      cs[5] += c.c1sq;
-      cs[6] += c.c2;
-      cs[7] += c.c2sq;
-      cs[8] += c.nv;
-      cs[9] += c.ne;
-      cs[10] += c.nf;
-      cs[11] += c.cnt;
-    }
-#pragma unroll
-    for (int k = 0; k < 12; ++k) cs[k] = wave_sum_dpp(cs[k]);
+    // column sums over the blocks [q0 / kColBlock, q1 / kColBlock]: lane k < 12
+    // takes field k
+    const int64_t j0 = q0 / kColBlock, j1 = (int64_t)(q1 / kColBlock) + 1;
+    float cs = 0.f;
+    if (lane < kColFields && j1 > j0)
+      cs = (float)(pre[j1 * kColFields + lane] - pre[j0 * kColFields + lane]);
+    if (lane < kColFields) park[nslot * kStatPitch + kSCol + lane] = cs;
     if (lane == 0) {
       float *w = park + nslot * kStatPitch;
       const uint64_t cu = (uint64_t)cand_it;
@@ -246,8 +346,6 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
       w[kSFe] = f_e;
       w[kSFf] = f_f;
       w[kSMchain] = m_chain;
-#pragma unroll
-      for (int k = 0; k < 12; ++k) w[kSCol + k] = cs[k];
       w[kSN1Lo] = __int_as_float((int)(uint32_t)n1u);
       w[kSN1Hi] = __int_as_float((int)(uint32_t)(n1u >> 32));
       w[kSN2Lo] = __int_as_float((int)(uint32_t)n2u);
@@ -542,19 +640,33 @@ inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
 
 }  // namespace
 
-void launch_features(bool f32, const SweepArgs &sa, const ColStats *cols, const QParams &qp,
-                     int axis, const SceneParams *sp, const int64_t *seg_off, int nscenes,
-                     float *feat, uint32_t *cand, hipStream_t st) {
-  if (sa.nitems <= 0) return;
+size_t col_work_doubles(int64_t ncols) {
+  return (size_t)kColFields * ((size_t)ncols + 1 + (size_t)((ncols + kColRun - 1) / kColRun));
+}
+
+void launch_features(bool f32, const SweepArgs &sa, const ColStats *cols, int64_t ncols,
+                     double *col_work, const QParams &qp, int axis, const SceneParams *sp,
+                     const int64_t *seg_off, int nscenes, float *feat, uint32_t *cand,
+                     hipStream_t st) {
+  if (sa.nitems <= 0 || ncols <= 0) return;
+  // prefix sums of the column statistics: pre[0 .. ncols], then the run sums
+  double *pre = col_work;
+  double *run = col_work + (size_t)kColFields * ((size_t)ncols + 1);
+  const int64_t nr = (ncols + kColRun - 1) / kColRun;
+  const unsigned gr = (unsigned)((nr + kT / 64 - 1) / (kT / 64));
+  hipLaunchKernelGGL(k_col_runs, dim3(gr), dim3(kT), 0, st, cols, ncols, run);
+  hipLaunchKernelGGL(k_col_top, dim3(1), dim3(kTopThreads), 0, st, run, nr,
+                     pre + (size_t)kColFields * (size_t)ncols);
+  hipLaunchKernelGGL(k_col_pre, dim3(gr), dim3(kT), 0, st, cols, ncols, (const double *)run, pre);
   const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
   const unsigned g = (unsigned)std::min<int64_t>((sa.nitems + 3) / 4, 4096);
   if (f32)
     hipLaunchKernelGGL((k_features<Exact64>), dim3(g), dim3(kT), 0, st, sa.items, sa.nitems,
-                       sa.nitems_dev, sa.rowlist, sa.hi, (const Exact64 *)sa.rec, cols, qp, a1, a2, sp,
+                       sa.nitems_dev, sa.rowlist, sa.hi, (const Exact64 *)sa.rec, (const double *)pre, qp, a1, a2, sp,
                        seg_off, nscenes, feat, cand);
   else
     hipLaunchKernelGGL((k_features<Rec64>), dim3(g), dim3(kT), 0, st, sa.items, sa.nitems,
-                       sa.nitems_dev, sa.rowlist, sa.hi, (const Rec64 *)sa.rec, cols, qp, a1, a2, sp,
+                       sa.nitems_dev, sa.rowlist, sa.hi, (const Rec64 *)sa.rec, (const double *)pre, qp, a1, a2, sp,
                        seg_off, nscenes, feat, cand);
   STQ_HIP(hipGetLastError());
 }

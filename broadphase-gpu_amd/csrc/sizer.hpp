@@ -27,11 +27,15 @@ constexpr int kParams = kOffB3 + 1;  // 6337
 
 // Features of every work item (nitems x kFeat floats) and its exact number of
 // sweep candidates (an upper bound on its pairs).  cols: the column statistics
-// of the gather.  One scene: qp and axis; batched scenes: sp/seg_off/nscenes
+// of the gather (ncols blocks of kColBlock positions).  One scene: qp and axis; batched scenes: sp/seg_off/nscenes
 // give each item's scene, and the item count is sa.nitems_dev when set.
-void launch_features(bool f32, const SweepArgs &sa, const ColStats *cols, const QParams &qp,
-                     int axis, const SceneParams *sp, const int64_t *seg_off, int nscenes,
-                     float *feat, uint32_t *cand, hipStream_t st);
+// ncols: the number of column blocks; col_work: col_work_doubles(ncols)
+// doubles for their prefix sums.
+void launch_features(bool f32, const SweepArgs &sa, const ColStats *cols, int64_t ncols,
+                     double *col_work, const QParams &qp, int axis, const SceneParams *sp,
+                     const int64_t *seg_off, int nscenes, float *feat, uint32_t *cand,
+                     hipStream_t st);
+size_t col_work_doubles(int64_t ncols);
 // pred[i] = MLP(feat[i]) (the log1p of the predicted pair count).  Where an
 // n_dev pointer is given, the count is *n_dev (n is then an upper bound).
 void launch_mlp_forward(const float *params, const float *feat, int64_t n, float *pred,

@@ -62,21 +62,8 @@ __global__ __launch_bounds__(kThreads) void k_axis_stats(const stq_aabb_f64 *__r
     lo[a] = INFINITY;
     up[a] = -INFINITY;
   }
-  const int64_t stride = (int64_t)gridDim.x * kThreads;
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
-    const stq_aabb_f64 &b = boxes[i];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const double l = b.min[a], u = b.max[a];
-      const double c = 0.5 * (l + u);
-      acc[4 * a + 0] += l + u;
-      acc[4 * a + 1] += l * l + u * u;
-      acc[4 * a + 2] += u - l;
-      acc[4 * a + 3] += c * c;
-      lo[a] = fmin(lo[a], l);
-      up[a] = fmax(up[a], u);
-    }
-  }
+  axis_stats_range(boxes, (int64_t)blockIdx.x * kThreads + threadIdx.x, n,
+                   (int64_t)gridDim.x * kThreads, acc, lo, up);
   block_axis_stats<kThreads>(acc, lo, up, partial + (int64_t)blockIdx.x * kAxisStats);
 }
 

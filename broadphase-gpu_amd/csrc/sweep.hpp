@@ -35,6 +35,34 @@ struct QParams {
 // Per block: 12 sums (per axis: sum(min+max), sum(min^2+max^2), sum(max-min),
 // sum(centre^2)) then per axis min(min) and max(max): kAxisStats doubles.
 constexpr int kAxisStats = 18;
+// Adds one box to a thread's axis statistics.
+__device__ inline void axis_stats_add(const stq_aabb_f64 &b, double *acc, double *lo, double *up) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double l = b.min[a], u = b.max[a];
+    const double c = 0.5 * (l + u);
+    acc[4 * a + 0] += l + u;
+    acc[4 * a + 1] += l * l + u * u;
+    acc[4 * a + 2] += u - l;
+    acc[4 * a + 3] += c * c;
+    lo[a] = fmin(lo[a], l);
+    up[a] = fmax(up[a], u);
+  }
+}
+// Adds boxes i, i + step, ... below i1, four loads in flight per thread (the
+// reads are latency-bound with one box of 48 bytes per thread at a time).
+__device__ inline void axis_stats_range(const stq_aabb_f64 *__restrict__ boxes, int64_t i,
+                                        int64_t i1, int64_t step, double *acc, double *lo,
+                                        double *up) {
+  for (; i + 3 * step < i1; i += 4 * step) {
+    stq_aabb_f64 b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) b[u] = boxes[i + u * step];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) axis_stats_add(b[u], acc, lo, up);
+  }
+  for (; i < i1; i += step) axis_stats_add(boxes[i], acc, lo, up);
+}
 // Combines the axis statistics of the block's threads (12 sums, 3 minima,
 // 3 maxima; NT threads, all call this) into out[0 .. kAxisStats): shuffle
 // trees inside each wave, then the waves in order, so the result is the same
