@@ -223,6 +223,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   const int64_t total = sh[6];
   *count = total;
   stats.n_pairs = total;
+  s_last_pairs = total;
   if (use_net) {
     // prediction error before this scene's training, then the training steps
     float *d_pred = pred.as<float>((size_t)nitems);
@@ -308,12 +309,17 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
   launch_mlp_forward(w, d_feat, nitems, d_pred, st);
   launch_caps(d_pred, d_cand, nitems, kCapScale, kCapSlack, d_caps, st);
   scan_u32_to_i64(d_caps, d_capoff, nitems, scan_tmp, st);
+  // The scratch buffer holds a budget of slots kept across calls: items whose
+  // slots would pass it get none and are redone by the exact refill, and the
+  // next call gets room for every slot.  So the pass starts without waiting
+  // for the slot total on the host.
+  if (s_budget == 0)
+    s_budget = std::max<int64_t>(int64_t(1) << 22, (int64_t)kCapScale * s_last_pairs +
+                                                       (int64_t)kCapSlack * nitems);
+  launch_cap_budget(d_caps, d_capoff, nitems, s_budget, st);
+  int2 *d_scratch = scratch_pairs.as<int2>((size_t)s_budget);
   int64_t *sh = small_host();
-  STQ_HIP(hipMemcpyAsync(sh + 9, d_capoff + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   STQ_HIP(hipEventRecord(ev[E_PRED], st));
-  STQ_HIP(hipStreamSynchronize(st));
-  const int64_t total_cap = sh[9];
-  int2 *d_scratch = scratch_pairs.as<int2>((size_t)std::max<int64_t>(total_cap, 1));
   STQ_HIP(hipEventRecord(ev[E_COUNT], st));
   uint32_t *d_counts = sa.counts;
   int64_t *d_off = offsets.as<int64_t>((size_t)nitems + 1);
@@ -360,10 +366,13 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
   STQ_HIP(hipEventRecord(ev[E_COMPACT], st));
   STQ_HIP(hipEventRecord(ev[E_END], st));
   STQ_HIP(hipMemcpyAsync(sh + 8, acc, sizeof(double), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipMemcpyAsync(sh + 9, d_capoff + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   finish_stats(st);
-  const int64_t total = sh[6];
+  const int64_t total = sh[6], want = sh[9];
+  if (want > s_budget) s_budget = want + want / 8;
   *count = total;
   stats.n_pairs = total;
+  s_last_pairs = total;
   stats.n_fallback_tiles = sh[7];
   stats.sizing_mae =
       *reinterpret_cast<const double *>(sh + 8) / (double)std::max<int64_t>(nitems, 1);

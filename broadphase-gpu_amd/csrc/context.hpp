@@ -164,6 +164,8 @@ struct Context {
   }
   BatchPlan bplan;
   int64_t b_budget = 0;      // scratch pairs of the learned batched pass
+  int64_t s_budget = 0;      // scratch pairs of the learned single-scene pass
+  int64_t s_last_pairs = 0;  // pairs of the last single-scene call
   int64_t b_last_pairs = 0;  // pairs and items of the last batched call
   int64_t b_last_items = 0;
   // autotuning: launch parameters per call shape class (tune.cpp)

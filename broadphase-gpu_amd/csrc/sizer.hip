@@ -617,7 +617,16 @@ __global__ __launch_bounds__(kT) void k_compact(const uint32_t *__restrict__ cou
     if (c <= caps[it]) {
       const int2 *src = scratch + cap_off[it];
       int2 *dst = out + off[it];
-      for (uint32_t k = lane; k < c; k += 64) dst[k] = src[k];
+      // four loads in flight per lane, then the rest one by one
+      uint32_t k = lane;
+      for (; k + 192 < c; k += 256) {
+        const int2 x0 = src[k], x1 = src[k + 64], x2 = src[k + 128], x3 = src[k + 192];
+        dst[k] = x0;
+        dst[k + 64] = x1;
+        dst[k + 128] = x2;
+        dst[k + 192] = x3;
+      }
+      for (; k < c; k += 64) dst[k] = src[k];
     } else if (lane == 0) {
       ovf[atomicAdd(novf, 1ull)] = (int)it;
     }
