@@ -109,7 +109,9 @@ enum stq_sizing {
      double pass, src/simulation.cu:451-489) */
   STQ_SIZING_COUNT_FILL = 0,
   /* one sweep pass into per-tile buffers sized by the learned per-tile pair
-     predictor; tiles that overflow are redone by the exact count pass */
+     predictor; tiles that overflow are redone by the exact count pass.  The
+     buffers come from a scratch budget the context keeps (and grows) across
+     calls; tiles past it are redone too, so the result never depends on it */
   STQ_SIZING_LEARNED = 1
 };
 
