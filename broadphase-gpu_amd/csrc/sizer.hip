@@ -683,10 +683,10 @@ void launch_features(bool f32, const SweepArgs &sa, const ColStats *cols, int64_
 void launch_mlp_forward(const float *params, const float *feat, int64_t n, float *pred,
                         hipStream_t st, const int64_t *n_dev) {
   if (n <= 0) return;
-  // 16 items per wave step; about as many blocks as are resident at once
-  // (3 waves per SIMD at this register count), each wave looping over groups
+  // 16 items per wave step; one block per CU, each wave looping over groups,
+  // so a wave's weight loads (96 per lane) serve many groups
   const int64_t per_block = (kT / 64) * kMfmaItems;
-  const unsigned g = (unsigned)std::min<int64_t>((n + per_block - 1) / per_block, 768);
+  const unsigned g = (unsigned)std::min<int64_t>((n + per_block - 1) / per_block, 256);
   hipLaunchKernelGGL(k_mlp_forward, dim3(g), dim3(kT), 0, st, params, feat, n, n_dev, pred);
   STQ_HIP(hipGetLastError());
 }
