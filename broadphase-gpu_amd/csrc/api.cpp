@@ -311,7 +311,7 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
   scan_u32_to_i64(d_caps, d_capoff, nitems, scan_tmp, st);
   // The scratch buffer holds a budget of slots kept across calls: items whose
   // slots would pass it get none and are redone by the exact refill, and the
-  // next call gets room for every slot.  So the pass starts without waiting
+  // next call gets room for every slot and half as many again.  So the pass starts without waiting
   // for the slot total on the host.
   if (s_budget == 0)
     s_budget = std::max<int64_t>(int64_t(1) << 22, (int64_t)kCapScale * s_last_pairs +
@@ -369,7 +369,9 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
   STQ_HIP(hipMemcpyAsync(sh + 9, d_capoff + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   finish_stats(st);
   const int64_t total = sh[6], want = sh[9];
-  if (want > s_budget) s_budget = want + want / 8;
+  // geometric growth: a stream of ever larger scenes overflows the budget a
+  // few times, not at every new largest scene
+  if (want > s_budget) s_budget = want + want / 2;
   *count = total;
   stats.n_pairs = total;
   s_last_pairs = total;
