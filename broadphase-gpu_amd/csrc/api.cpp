@@ -311,8 +311,8 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
   scan_u32_to_i64(d_caps, d_capoff, nitems, scan_tmp, st);
   // The scratch buffer holds a budget of slots kept across calls: items whose
   // slots would pass it get none and are redone by the exact refill, and the
-  // next call gets room for every slot and half as many again.  So the pass starts without waiting
-  // for the slot total on the host.
+  // next call gets room for every slot and half as many again.  So the pass
+  // starts without waiting for the slot total on the host.
   if (s_budget == 0)
     s_budget = std::max<int64_t>(int64_t(1) << 22, (int64_t)kCapScale * s_last_pairs +
                                                        (int64_t)kCapSlack * nitems);

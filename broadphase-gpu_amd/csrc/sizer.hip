@@ -156,8 +156,8 @@ __global__ __launch_bounds__(kT) void k_col_pre(const ColStats *__restrict__ col
 // ---- features ----------------------------------------------------------------
 // One wave per item.  Rows are the item's lanes; the q range is described by
 // the column statistics of the position blocks it meets (their sums: two
-// lookups in the prefix sums).  Extents enter as log(extent)/10, centres as the position
-// inside the scene's quantisation range (0..1).
+// lookups in the prefix sums).  Extents enter as log(extent)/10, centres as
+// the position inside the scene's quantisation range (0..1).
 // With scene tables (batched scenes) the level range and axes are those of the
 // item's scene, and the item count is read from the device.
 //
