@@ -42,11 +42,13 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
 // ---- prefix sums of the column statistics ---------------------------------------
 // pre[j][k] = sum of field k (ColStats order) over blocks 0 .. j - 1, in
 // FP64, j = 0 .. m; an item's column sums are then two lookups.  Three
-// launches: the sum of every run of kColRun blocks (one wave each), the
-// exclusive scan of the runs (one block), each block's prefix (one wave per
-// run, a wave scan on top of its run's prefix).
+// launches: the sum of every run of kColRun blocks (one thread block each,
+// kColPer consecutive blocks per thread), the exclusive scan of the runs (one
+// thread block), each block's prefix (a thread-block scan on top of its run's
+// prefix).
 constexpr int kColFields = 12;
-constexpr int kColRun = kWave;
+constexpr int kColPer = 4;
+constexpr int kColRun = kT * kColPer;
 constexpr int kTopThreads = 1024;
 
 __device__ __forceinline__ void col_fields(const ColStats &c, double *v) {
@@ -64,20 +66,46 @@ __device__ __forceinline__ void col_fields(const ColStats &c, double *v) {
   v[11] = c.cnt;
 }
 
+// Sum of the fields over the thread block (fixed order): wave trees, then the
+// waves in order; the result is valid in every thread.
+__device__ __forceinline__ void block_sum_fields(double *v) {
+  __shared__ double wt[kT / 64][kColFields];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kColFields; ++k) {
+    v[k] = wave_sum(v[k]);
+    if (lane == 0) wt[w][k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kColFields; ++k) {
+    double x = 0.0;
+    for (int i = 0; i < kT / 64; ++i) x += wt[i][k];
+    v[k] = x;
+  }
+}
+
 __global__ __launch_bounds__(kT) void k_col_runs(const ColStats *__restrict__ cols, int64_t m,
                                                  double *__restrict__ run) {
-  const int lane = threadIdx.x & 63;
-  const int64_t w = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
-  const int64_t j = w * kColRun + lane;
+  const int64_t j0 = (int64_t)blockIdx.x * kColRun + (int64_t)threadIdx.x * kColPer;
   double v[kColFields];
 #pragma unroll
   for (int k = 0; k < kColFields; ++k) v[k] = 0.0;
-  if (j < m) col_fields(cols[j], v);
 #pragma unroll
-  for (int k = 0; k < kColFields; ++k) v[k] = wave_sum(v[k]);
-  if (lane == 0 && w * kColRun < m) {
+  for (int i = 0; i < kColPer; ++i) {
+    if (j0 + i < m) {
+      double f[kColFields];
+      col_fields(cols[j0 + i], f);
 #pragma unroll
-    for (int k = 0; k < kColFields; ++k) run[w * kColFields + k] = v[k];
+      for (int k = 0; k < kColFields; ++k) v[k] += f[k];
+    }
+  }
+  block_sum_fields(v);
+  if (threadIdx.x < kColFields) {
+    double x = v[0];
+#pragma unroll
+    for (int k = 1; k < kColFields; ++k) x = (int)threadIdx.x == k ? v[k] : x;
+    run[(int64_t)blockIdx.x * kColFields + threadIdx.x] = x;
   }
 }
 
@@ -133,23 +161,51 @@ __global__ __launch_bounds__(kTopThreads) void k_col_top(double *__restrict__ ru
 __global__ __launch_bounds__(kT) void k_col_pre(const ColStats *__restrict__ cols, int64_t m,
                                                 const double *__restrict__ run,
                                                 double *__restrict__ pre) {
-  const int lane = threadIdx.x & 63;
-  const int64_t w = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
-  if (w * kColRun >= m) return;
-  const int64_t j = w * kColRun + lane;
-  double v[kColFields];
+  __shared__ double wt[kT / 64][kColFields];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t j0 = (int64_t)blockIdx.x * kColRun + (int64_t)threadIdx.x * kColPer;
+  // this thread's total, then an exclusive scan of the totals over the block
+  double tot[kColFields], acc[kColFields];
 #pragma unroll
-  for (int k = 0; k < kColFields; ++k) v[k] = 0.0;
-  if (j < m) col_fields(cols[j], v);
+  for (int k = 0; k < kColFields; ++k) tot[k] = 0.0;
+#pragma unroll
+  for (int i = 0; i < kColPer; ++i) {
+    if (j0 + i < m) {
+      double f[kColFields];
+      col_fields(cols[j0 + i], f);
+#pragma unroll
+      for (int k = 0; k < kColFields; ++k) tot[k] += f[k];
+    }
+  }
 #pragma unroll
   for (int k = 0; k < kColFields; ++k) {
-    double x = v[k];
+    double x = tot[k];
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const double y = __shfl_up(x, o, kWave);
       if (lane >= o) x += y;
     }
-    if (j < m) pre[j * kColFields + k] = run[w * kColFields + k] + (x - v[k]);
+    acc[k] = x - tot[k];
+    if (lane == 63) wt[w][k] = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kColFields; ++k) {
+    double before = run[(int64_t)blockIdx.x * kColFields + k];
+    for (int i = 0; i < w; ++i) before += wt[i][k];
+    acc[k] += before;
+  }
+#pragma unroll
+  for (int i = 0; i < kColPer; ++i) {
+    if (j0 + i < m) {
+      double f[kColFields];
+      col_fields(cols[j0 + i], f);
+#pragma unroll
+      for (int k = 0; k < kColFields; ++k) {
+        pre[(j0 + i) * kColFields + k] = acc[k];
+        acc[k] += f[k];
+      }
+    }
   }
 }
 
@@ -662,7 +718,7 @@ void launch_features(bool f32, const SweepArgs &sa, const ColStats *cols, int64_
   double *pre = col_work;
   double *run = col_work + (size_t)kColFields * ((size_t)ncols + 1);
   const int64_t nr = (ncols + kColRun - 1) / kColRun;
-  const unsigned gr = (unsigned)((nr + kT / 64 - 1) / (kT / 64));
+  const unsigned gr = (unsigned)nr;
   hipLaunchKernelGGL(k_col_runs, dim3(gr), dim3(kT), 0, st, cols, ncols, run);
   hipLaunchKernelGGL(k_col_top, dim3(1), dim3(kTopThreads), 0, st, run, nr,
                      pre + (size_t)kColFields * (size_t)ncols);

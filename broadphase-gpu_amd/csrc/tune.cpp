@@ -5,8 +5,10 @@
 // per CU and the weight of tail tests in the group cut (0: no tails), by
 // timing count-only trial runs on the first input of that class:
 // coordinate descent, each candidate timed twice with HIP events, a change
-// kept only when it is at least 2% faster.  The table can be saved to and
-// loaded from a text file.
+// kept only when it is at least 2% faster.  For a caller with learned sizing
+// each trial is charged the per-item cost of the learned stages as well
+// (kLearnedItemMs per item), which count-only runs do not pay.  The table can
+// be saved to and loaded from a text file.
 //
 // Replaces the reference's setup() (src/util.cu:9-55), which derives one
 // block size and shared-memory size from device attributes.
@@ -18,6 +20,13 @@
 #include <cstring>
 
 #include "context.hpp"
+
+namespace {
+// Per-item time of the learned pass's item-proportional stages, measured on
+// the 3M-box cloth: 93k -> 286k items (chunks of 16384 -> 4096 positions)
+// added 0.1 ms of features, network, caps and compaction.
+constexpr double kLearnedItemMs = 5e-7;
+}  // namespace
 #include "plan.hpp"
 
 namespace stqamd {
@@ -71,7 +80,10 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
             throw;
           }
         }
-        best = std::min(best, stats.ms_total);
+        // learned sizing adds per-item work the count-only trial does not do
+        // (features, network, caps, compaction bookkeeping)
+        const double extra = user.sizing == STQ_SIZING_LEARNED ? kLearnedItemMs * stats.n_items : 0.0;
+        best = std::min(best, stats.ms_total + extra);
       }
       return best;
     };
