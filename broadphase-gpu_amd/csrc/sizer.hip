@@ -317,10 +317,22 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
   double n1 = 0, n2 = 0, o1 = 0, o2 = 0;
   float nr = 1.f, m_le1 = 0, m_le2 = 0, v_le1 = 0, v_le2 = 0, m_c1 = 0, m_c2 = 0, v_c1 = 0,
         v_c2 = 0, f_v = 0, f_e = 0, f_f = 0, m_chain = 0;
+  // The next item and this item's prefix-sum lookups are loaded ahead of
+  // the dependent row loads, so their latencies overlap.
+  Item nxt = ib < ie ? items[ib] : Item{};
   for (int64_t it = ib; it < ie; ++it) {
-    const Item item = items[it];
+    const Item item = nxt;
+    if (it + 1 < ie) nxt = items[it + 1];
     const int nrows = item.nrows & kRowsMask;  // tail items: their rows
     const int q0 = item.q0, q1 = item.q1;
+    // column sums over the blocks [q0 / kColBlock, q1 / kColBlock]: lane k < 12
+    // takes field k
+    const int64_t j0 = q0 / kColBlock, j1 = (int64_t)(q1 / kColBlock) + 1;
+    double phi = 0.0, plo = 0.0;
+    if (lane < kColFields && j1 > j0) {
+      phi = pre[j1 * kColFields + lane];
+      plo = pre[j0 * kColFields + lane];
+    }
     if (item.rowoff != prow || nrows != pnrows) {
       prow = item.rowoff;
       pnrows = nrows;
@@ -375,13 +387,7 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
     }
     const int span = q1 - q0;
     const long long cand_it = wave_sum(cnd);
-    // column sums over the blocks [q0 / kColBlock, q1 / kColBlock]: lane k < 12
-    // takes field k
-    const int64_t j0 = q0 / kColBlock, j1 = (int64_t)(q1 / kColBlock) + 1;
-    float cs = 0.f;
-    if (lane < kColFields && j1 > j0)
-      cs = (float)(pre[j1 * kColFields + lane] - pre[j0 * kColFields + lane]);
-    if (lane < kColFields) park[nslot * kStatPitch + kSCol + lane] = cs;
+    if (lane < kColFields) park[nslot * kStatPitch + kSCol + lane] = (float)(phi - plo);
     if (lane == 0) {
       float *w = park + nslot * kStatPitch;
       const uint64_t cu = (uint64_t)cand_it;
