@@ -975,6 +975,10 @@ static void launch_sweep_b(bool f32, int mode, const dim3 &grid, const SweepArgs
 void launch_sweep(bool f32, int mode, int batch, int nblocks, const SweepArgs &args,
                   hipStream_t st) {
   if (args.nitems <= 0) return;
+  // XCD slices: every XCD gets gridDim / kXcds blocks, and each wave's first
+  // item is its index in the slice, so a remainder block would repeat items
+  if (nblocks <= 0 || (args.xcd_split && nblocks % kXcds != 0))
+    throw StatusError(STQ_ERR_INVALID_ARGUMENT, "sweep grid: a positive multiple of 8 blocks");
   const dim3 grid(nblocks);
   // the batch of scalar-loaded positions applies to the FP64 sweep only
   if (f32 || batch <= 2) launch_sweep_b<2>(f32, mode, grid, args, st);
