@@ -108,9 +108,10 @@ __device__ inline GroupPlan plan_group(int lane, int nrows, int p, int h, int ro
 constexpr int kMaxWindow64 = 2048;
 constexpr int kMaxWindow32 = 4096;
 constexpr int kWinPosBits = 12;  // position bits of a 32-bit window key
-// Threads per planning block (one window at a time): 16 waves share the
-// window's sort and its 64 groups.
-constexpr int kPlanThreads = 1024;
+// Threads per planning block (one window at a time): 8 waves share the
+// window's sort and its 64 groups, and four blocks fit on a CU (all 733
+// windows of a 3M-box scene run in one round).
+constexpr int kPlanThreads = 512;
 
 // Bitonic sort, ascending, of W keys (W a power of two) in LDS by the NT
 // threads of the block; ends with a barrier.
@@ -136,9 +137,9 @@ __device__ inline void window_sort(K *key, int W) {
 
 // The same sort with each thread holding E = W / NT consecutive keys in
 // registers: the compare-exchanges of strides below E stay in registers, the
-// others go through LDS (write, barrier, read the partner, barrier), so a
-// window of 4096 keys takes 36 LDS stages instead of 78.  Ends with the
-// sorted keys in LDS and a barrier.
+// others go through LDS (write, barrier, read the partner, barrier): of the
+// 78 stages of a 4096-key window, 55 go through LDS with E = 4 and 45 with
+// E = 8.  Ends with the sorted keys in LDS and a barrier.
 template <int NT, int E, class K>
 __device__ inline void window_sort_blocked(K *key) {
   constexpr int W = NT * E;
@@ -185,8 +186,10 @@ __device__ inline void window_sort_blocked(K *key) {
 // Sort of a window of W <= WMAX keys (W a power of two): blocked for W >= NT.
 template <int NT, int WMAX, class K>
 __device__ inline void window_sort_any(K *key, int W) {
-  static_assert(WMAX <= 4 * NT, "at most 4 keys per thread");
-  if (WMAX >= 4 * NT && W == 4 * NT)
+  static_assert(WMAX <= 8 * NT, "at most 8 keys per thread");
+  if (WMAX >= 8 * NT && W == 8 * NT)
+    window_sort_blocked<NT, (WMAX >= 8 * NT ? 8 : 1)>(key);
+  else if (WMAX >= 4 * NT && W == 4 * NT)
     window_sort_blocked<NT, (WMAX >= 4 * NT ? 4 : 1)>(key);
   else if (WMAX >= 2 * NT && W == 2 * NT)
     window_sort_blocked<NT, (WMAX >= 2 * NT ? 2 : 1)>(key);
