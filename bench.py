@@ -1,32 +1,37 @@
 """Benchmark of the gfx950 STQ broad phase on one MI355X.
 
-One step = one broad-phase call over the 708 x 708 swept cloth (3,001,923
-FP64 boxes, ~48.4M pairs), boxes already resident in HBM, pairs written to a
-device buffer.  Two forms run on the same scene:
+Headline (the JSON line's `value`): the batched workload, 512 independent
+scenes back to back (sizes log-uniform in [1e3, 5e5] boxes, uniform-like or
+cloth-like, density factor U[0.5, 4]; 37,422,181 FP64 boxes, about 4.6e8
+pairs), boxes resident in HBM, pairs written to a device buffer.  One step is
+one batched call over all 512 scenes.  Forms on the same scenes:
 
-  plain  : FP64 sort keys and comparisons, sweep axis x, exact count pass +
-           fill pass, no tail items, default launch parameters (the
-           reference method, src/simulation.cu:555-782, on this library's
-           kernels)
-  chosen : FP32 outward-rounded keys and chain ends, 15-bit outward levels in
-           the sweep's hot loop, FP64 confirmation of every candidate,
-           autotuned launch parameters, tail items for long chains, and
-           learned sizing: one emitting pass into per-item slots predicted by
-           the network (trained online on the exact counts of a stream of
-           other scenes before timing)
+  chosen      : FP32 outward-rounded keys and chain ends, 14-bit outward
+                levels with kind codes in the hot loop, FP64 confirmation of
+                every candidate, learned sizing (one emitting pass into slots
+                predicted by the network, trained online on the first call's
+                exact counts), autotuned launch parameters, hipGraph replay
+  plain       : the reference method on this library's kernels: FP64 keys
+                and comparisons, axis x, count pass + fill pass, no tail
+                items, default launch parameters (src/simulation.cu:555-782)
+  plain_tuned : FP64 as plain, with tail items and autotuned launch
+                parameters (the best this library does without reducing
+                precision)
 
-The JSON line reports the chosen form's boxes/s as `value`, its pairs/s, the
-plain form's numbers and the speedup, whether both pair sets are identical,
-the roofline of the dominant kernel (the emitting sweep) and a CPU baseline
-(the oracle's restatement of the reference CPU sweep on this host, with a
-1-thread figure).  Further legs, reported in the same line: the 8M-box
-clustered heavy-tailed scene (`skewed`), 512 scenes in one batched call
-(`batched`), a bounded sample of the learned-sizing stream
-(`learned_stream`), the PCIe-inclusive host-to-host time
-(`end_to_end_host`), the kernels either side of the broad phase on the
-cloth mesh: swept-box build, pair orientation and narrow-phase query records
-(`handoff`), each against the HBM roofline, and the 100,000-box uniform scene
-with its whole CPU sweep (`small`).
+Timed region: K calls bracketed by device synchronisations on both sides;
+ms_per_step is the host clock over them.  The roofline is that of the
+dominant kernel, the emitting sweep of the chosen form (k_qsweep, learned
+pass), timed by HIP events on the library's stream around that kernel inside
+every timed call (event nodes of the replayed graph).  The CPU baseline is the
+oracle's restatement of the reference CPU sweep (src/cpu/sweep.cpp) over all
+512 scenes on this host's cores.
+
+Further legs in the same line: the 708 x 708 swept cloth (3,001,923 boxes,
+single-scene call), the 8M-box clustered heavy-tailed scene (`skewed`), a
+bounded sample of the learned-sizing stream (`learned_stream`), the
+PCIe-inclusive host-to-host call (`end_to_end_host`), the kernels either side
+of the broad phase on the cloth mesh (`handoff`) and the 100,000-box uniform
+scene with its whole CPU sweep (`small`).
 """
 from __future__ import annotations
 
@@ -58,33 +63,30 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--legs", default="batched,cloth,skewed,stream,handoff,small,host",
+                    help="comma list of legs; the batched leg is the headline")
+    ap.add_argument("--batched-forms", default="chosen,plain,plain_tuned",
+                    help="forms of the batched leg (chosen is the headline)")
     ap.add_argument("--grid", type=int, default=708, help="cloth grid size g (g x g vertices)")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--forms", default="chosen,plain")
+    ap.add_argument("--forms", default="chosen,plain", help="forms of the cloth leg")
     ap.add_argument("--cpu-sample", type=float, default=0.25,
-                    help="fraction of sorted rows the CPU baseline sweeps (0 = skip)")
-    ap.add_argument("--no-check", action="store_true", help="skip the pair-set comparison")
+                    help="cloth: fraction of sorted rows the CPU baseline sweeps (0 = skip)")
+    ap.add_argument("--cpu-batched", type=int, default=1,
+                    help="batched: run the CPU baseline over all scenes (0 = skip)")
+    ap.add_argument("--no-check", action="store_true", help="skip the pair-set comparisons")
     ap.add_argument("--train-scenes", type=int, default=80,
-                    help="scenes of the online training stream run before timing the chosen form")
+                    help="cloth: scenes of the online training stream run before timing the chosen form")
     ap.add_argument("--batched-scenes", type=int, default=512,
-                    help="scenes of the batched leg (sizes log-uniform in [1e3, 5e5]; 0 = skip)")
+                    help="scenes of the batched leg (sizes log-uniform in [1e3, 5e5])")
     ap.add_argument("--stream-scenes", type=int, default=240,
                     help="scenes of the learned-sizing stream leg: the first 200 train, the rest are "
-                         "held out (sizes log-uniform in [5e4, 4e6]; 0 = skip)")
-    ap.add_argument("--skewed", type=int, default=1,
-                    help="run the clustered heavy-tailed 8M-box leg (0 = skip)")
+                         "held out (sizes log-uniform in [5e4, 4e6])")
     ap.add_argument("--cpu-sample-1t", type=float, default=0.02,
-                    help="fraction of sorted rows the 1-thread CPU timing sweeps (0 = skip)")
-    ap.add_argument("--host-e2e", type=int, default=1,
-                    help="time the PCIe-inclusive host-to-host call (0 = skip)")
-    ap.add_argument("--handoff", type=int, default=1,
-                    help="time the swept-box build and the narrow-phase hand-off kernels (1 = on)")
-    ap.add_argument("--small", type=int, default=1,
-                    help="time the 100,000-box uniform scene, GPU forms and CPU oracle (1 = on)")
+                    help="cloth: fraction of sorted rows the 1-thread CPU timing sweeps (0 = skip)")
     ap.add_argument("--tune-table", default="",
-                    help="autotune table of the cloth forms: loaded when the file exists (no tuning), "
-                         "else written after tuning; counter runs of one workload use it to share "
-                         "launch parameters")
+                    help="autotune table: loaded when the file exists (no tuning), else written after "
+                         "tuning; counter runs of one workload use it to share launch parameters")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (traffic calibration)")
     return ap.parse_args()
@@ -135,11 +137,44 @@ def tuned_table(ctx):
     return out
 
 
+def load_traffic(name, form):
+    """HBM bytes and VALU wave instructions per launch of a workload's dominant
+    kernel, from the committed rocprof counter runs (profiles/<name>.json)."""
+    path = os.path.join(ROOT, "profiles", name + ".json")
+    try:
+        with open(path) as fh:
+            entry = json.load(fh).get(form, {})
+        return entry.get("hbm_bytes_per_launch"), entry.get("valu_wave_instructions_per_launch")
+    except (OSError, ValueError):
+        return None, None
+
+
+def roofline(alg_bytes, kernel_ms, kernel, traffic=None, valu=None):
+    achieved = alg_bytes / (kernel_ms * 1e-3)
+    issue = None
+    if valu:
+        rate = valu / (kernel_ms * 1e-3)
+        issue = {"bound": "valu issue", "achieved": rate, "peak": VALU_ISSUE_PEAK,
+                 "unit": "wave instructions/s", "frac": rate / VALU_ISSUE_PEAK, "valu_per_launch": valu}
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": kernel,
+            "algorithmic_bytes_per_launch": alg_bytes, "kernel_ms": kernel_ms, "issue": issue}
+
+
+BATCHED_FORMS = {
+    "chosen": dict(precision=1, axis_policy=4, sizing=1, train_scenes=200, autotune=1),
+    "plain": dict(precision=0, axis_policy=0, tails=2),
+    "plain_tuned": dict(precision=0, axis_policy=0, tails=1, autotune=1),
+}
+
+
 def bench_batched(args, H):
-    """512 scenes back to back (the batched stream), device-resident: the
-    chosen batched form (reduced precision, chain-length axis, learned single
-    pass, hipGraph replay) against plain FP64 count-then-fill, also batched
-    and replayed.  Per-scene pair sets are compared by digest."""
+    """The headline: 512 scenes back to back, device-resident, one batched
+    call per step.  Each form: one sizing call (tunes, and for the chosen
+    form trains the network on the batch's exact counts), warm-up calls (an
+    eager run, then the capture), then `steps` timed calls (graph replays)
+    between device synchronisations.  Per-scene pair sets are compared by
+    digest across forms and with the oracle (the CPU baseline's own run)."""
     import numpy as np
     import stq_amd
     from stq_amd import _lib, scenes
@@ -147,53 +182,107 @@ def bench_batched(args, H):
     items = scenes.batched_scenes(count=args.batched_scenes, seed=3, nmin=1e3, nmax=5e5)
     sc = [b for (_, _, b) in items]
     allb, offs = scenes.concat_scenes(sc)
-    del items, sc
+    del items
     gen_s = time.perf_counter() - t0
     db = _lib.DeviceBuffer.from_array(allb)
     nbox = len(allb)
-    del allb
-    forms = {
-        "chosen": dict(precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN,
-                       sizing=stq_amd.SIZING_LEARNED, train_scenes=200, autotune=1),
-        "plain": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X, tails=2),
-    }
+    want = [f for f in args.batched_forms.split(",") if f]
     out, digests = {}, {}
-    for name, kw in forms.items():
+    for name in want:
+        kw = BATCHED_FORMS[name]
         ctx = stq_amd.Context(0, **kw)
+        if kw.get("autotune") and args.tune_table and os.path.exists(args.tune_table):
+            ctx.autotune_load(args.tune_table)
         rc, poffs = ctx.find_pairs_batched_device(db.ptr, offs, None, 0)  # sizes, tunes, trains
         train = ctx.stats()
+        if kw.get("autotune") and args.tune_table and not os.path.exists(args.tune_table):
+            ctx.autotune_save(args.tune_table)
         tuned = tuned_table(ctx) if kw.get("autotune") else None
         need = int(poffs[-1])
         dout = _lib.DeviceBuffer(8 * max(need, 1))
         for _ in range(max(args.warmup, 2)):  # eager, then capture
             ctx.find_pairs_batched_device(db.ptr, offs, dout.ptr, need)
-        ms = []
+        stats = []
+        H.hipDeviceSynchronize()
+        t1 = time.perf_counter()
         for _ in range(args.steps):
             rc, p2 = ctx.find_pairs_batched_device(db.ptr, offs, dout.ptr, need)
-            assert rc == _lib.STQ_OK and np.array_equal(p2, poffs)
-            ms.append(ctx.stats()["ms_total"])
-        st = ctx.stats()
-        t = float(np.median(ms))
-        out[name] = dict(ms_per_batch=t, boxes_per_s=nbox / (t * 1e-3), pairs_per_s=need / (t * 1e-3),
-                         pairs=need, items=st["n_items"], candidates=st["n_candidates"],
-                         graph_state=st["graph"], fallback_items=st["n_fallback_tiles"],
+            stats.append(ctx.stats())
+        H.hipDeviceSynchronize()
+        t2 = time.perf_counter()
+        assert rc == _lib.STQ_OK and np.array_equal(p2, poffs)
+        ms = (t2 - t1) * 1e3 / max(args.steps, 1)
+        st = stats[-1]
+        sweep_ms = [x["ms_sweep_fill"] for x in stats]
+        kernel_timing = "HIP events around the kernel inside each timed graph replay"
+        if min(sweep_ms) <= 0.0:
+            # the runtime recorded no events inside the graph: time the same
+            # kernel in eager calls of the same batch
+            ctx.set_options(**{**kw, "graph": 2})
+            sweep_ms = []
+            for _ in range(max(args.steps // 2, 3)):
+                ctx.find_pairs_batched_device(db.ptr, offs, dout.ptr, need)
+                sweep_ms.append(ctx.stats()["ms_sweep_fill"])
+            kernel_timing = "HIP events around the kernel in eager calls of the same batch"
+        phases = {k: float(np.mean([x[k] for x in stats])) for k in
+                  ("ms_total", "ms_keys", "ms_sort", "ms_gather", "ms_plan", "ms_predict",
+                   "ms_sweep_count", "ms_scan", "ms_sweep_fill", "ms_compact")}
+        out[name] = dict(ms_per_step=ms, boxes_per_s=nbox / (ms * 1e-3), pairs_per_s=need / (ms * 1e-3),
+                         stream_ms=float(np.median([x["ms_total"] for x in stats])),
+                         sweep_kernel_ms=float(np.mean(sweep_ms)), sweep_kernel_timing=kernel_timing,
+                         phases_ms=phases, pairs=need, items=st["n_items"],
+                         candidates=st["n_candidates"], lane_steps=st["n_lane_steps"],
+                         confirms=st["n_confirm"], graph_state=st["graph"],
+                         fallback_items=st["n_fallback_tiles"],
                          mae_pairs_per_item=st["sizing_mae"] if kw.get("sizing") else None,
                          training_call_mae=train["sizing_mae"] if kw.get("sizing") else None,
-                         tune_ms=train["ms_tune"], autotuned=tuned)
+                         first_call_ms=train["ms_total"], tune_ms=train["ms_tune"], autotuned=tuned,
+                         form=kw)
         if not args.no_check:
             pairs = dout.to_array(np.int32, 2 * need).reshape(-1, 2)
             digests[name] = [pair_digest(pairs[poffs[s]:poffs[s + 1]]) for s in range(len(poffs) - 1)]
             del pairs
         dout.close()
         ctx.close()
+        progress(f"batched form {name}: {ms:.2f} ms per batch, {need} pairs")
     db.close()
-    exact = (digests["chosen"] == digests["plain"]) if len(digests) == 2 else None
-    return {"workload": f"{args.batched_scenes} scenes, sizes log-uniform in [1e3, 5e5], uniform-like or "
-                        f"cloth-like, density factor U[0.5, 4] (seed 3): {nbox} FP64 boxes, device-resident",
-            "boxes": nbox, "generate_s": gen_s,
-            "chosen": out["chosen"], "plain_fp64_batched": out["plain"],
-            "speedup_vs_plain_fp64": out["chosen"]["boxes_per_s"] / out["plain"]["boxes_per_s"],
-            "per_scene_digests_equal": exact}
+    res = {"workload": f"{args.batched_scenes} scenes, sizes log-uniform in [1e3, 5e5], uniform-like or "
+                       f"cloth-like, density factor U[0.5, 4] (seed 3): {nbox} FP64 boxes, device-resident",
+           "boxes": nbox, "scenes": len(offs) - 1, "generate_s": gen_s, "forms": out}
+    if "chosen" in out:
+        for base in ("plain", "plain_tuned"):
+            if base in out:
+                res[f"speedup_vs_{base}"] = out["chosen"]["boxes_per_s"] / out[base]["boxes_per_s"]
+        ref = next(iter(digests.values()), None)
+        res["per_scene_digests_equal_across_forms"] = (all(d == ref for d in digests.values())
+                                                       if len(digests) > 1 else None)
+    cpu = None
+    if args.cpu_batched:
+        from oracle import oracle
+        threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 64))
+        secs, npairs, mism = 0.0, 0, 0
+        for i, b in enumerate(sc):
+            pairs, t = oracle.sweep_timed(b, axis=0, nthreads=threads)
+            secs += t
+            npairs += len(pairs)
+            if digests and not args.no_check:
+                mism += int(pair_digest(pairs) != next(iter(digests.values()))[i])
+        one = None
+        sub = sc[::32]
+        if sub:
+            t1s = sum(oracle.sweep_timed(b, axis=0, nthreads=1)[1] for b in sub)
+            one = {"value": sum(len(b) for b in sub) / t1s, "unit": "boxes/s",
+                   "sample": f"every 32nd scene ({len(sub)} scenes, {sum(len(b) for b in sub)} boxes), "
+                             f"{t1s:.2f} s on 1 thread"}
+        progress(f"batched cpu baseline: {nbox / secs:.3g} boxes/s on {threads} threads")
+        cpu = {"value": nbox / secs, "unit": "boxes/s", "cores": threads, "kind": "port",
+               "one_thread": one, "seconds": secs, "pairs": npairs,
+               "sample": (f"the whole workload: all {len(sc)} scenes ({nbox} boxes) through the oracle's "
+                          f"restatement of src/cpu/sweep.cpp (FP64, axis x; sort, chain sweep, merge per "
+                          f"scene, {secs:.2f} s); CPU {cpu_model()}, {threads} threads"),
+               "per_scene_digest_mismatches_vs_gpu": mism if digests and not args.no_check else None}
+    res["cpu_baseline"] = cpu
+    return res
 
 
 def bench_skewed(args, H):
@@ -255,7 +344,7 @@ def bench_stream(args, H):
     import numpy as np
     import stq_amd
     from stq_amd import _lib, scenes
-    rng = np.random.default_rng(4)
+    params = scenes.learned_stream_params(count=2000)[:args.stream_scenes]
     ntrain = min(200, args.stream_scenes)
     ctx = stq_amd.Context(0, precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN,
                           sizing=stq_amd.SIZING_LEARNED, train_scenes=ntrain)
@@ -268,10 +357,8 @@ def bench_stream(args, H):
     rows = []
     t0 = time.perf_counter()
     checked, mismatches = 0, 0
-    for i in range(args.stream_scenes):
-        nb = int(10 ** rng.uniform(np.log10(5e4), np.log10(4e6)))
-        rho = float(10 ** rng.uniform(np.log10(0.25), np.log10(8.0)))
-        b = scenes.density_scene(i % 2, nb, rho, seed=70_000 + i)
+    for i, (kind, nb, rho, seed) in enumerate(params):
+        b = scenes.density_scene(kind, nb, rho, seed=seed)
         db = _lib.DeviceBuffer.from_array(b)
         try:
             cnt = ctx.find_pairs_device(db.ptr, len(b), dout.ptr, cap)
@@ -313,7 +400,8 @@ def bench_stream(args, H):
                 "fallback_rate": sum(r[4] for r in rs) / max(items, 1),
                 "mae_pairs_per_item": sum(r[5] * r[3] for r in rs) / max(items, 1),
                 "ms_total": sum(r[6] for r in rs)}
-    return {"workload": f"{args.stream_scenes} scenes (seed 4), 5e4..4e6 boxes log-uniform, density factor "
+    return {"workload": f"the first {args.stream_scenes} of the 2000 scenes of scenes.learned_stream_params "
+                        f"(seed 4), 5e4..4e6 boxes log-uniform, density factor "
                         f"log-uniform in [0.25, 8]; first {ntrain} train the network online",
             "seconds": time.perf_counter() - t0, "training": agg(tr), "held_out": agg(held),
             "held_out_checked_vs_plain_fp64": checked, "mismatches": mismatches,
@@ -443,30 +531,18 @@ def bench_handoff(args, H):
     return out
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
+def bench_cloth(args, H):
+    """The 708 x 708 swept cloth as one single-scene call: the chosen form
+    (reduced precision, learned single pass trained on a stream of other
+    scenes, autotuned) and plain FP64 count-then-fill; pair sets compared in
+    full; roofline of the emitting sweep; CPU baseline on a sample of rows."""
     import numpy as np
     import stq_amd
     from stq_amd import _lib, scenes
 
-    H = _lib.hip()
-    if args.copy_probe_mib:
-        nb = args.copy_probe_mib << 20
-        a, b = _lib.DeviceBuffer(nb), _lib.DeviceBuffer(nb)
-        for _ in range(5):
-            _lib._check_hip(H.hipMemcpy(b.ptr, a.ptr, nb, _lib.HIP_D2D), "hipMemcpy D2D")
-        H.hipDeviceSynchronize()
-        print(json.dumps({"copy_probe_bytes": nb, "launches": 5}))
-        return
-
     boxes = scenes.cloth_scene(g=args.grid, seed=args.seed)
     n = len(boxes)
     d_boxes = stq_amd.DeviceBuffer.from_array(boxes)
-
     forms = {
         "chosen": dict(precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN,
                        sizing=stq_amd.SIZING_LEARNED, train_scenes=max(args.train_scenes, 1),
@@ -475,9 +551,7 @@ def main():
                       sizing=stq_amd.SIZING_COUNT_FILL, tails=2),
     }
     want = [f for f in args.forms.split(",") if f]
-    res = {}
-    outputs = {}
-    training = None
+    res, outputs, training = {}, {}, None
     for name in want:
         ctx = stq_amd.Context(0, **forms[name])
         table = args.tune_table if forms[name].get("autotune") else None
@@ -503,8 +577,7 @@ def main():
                 dsc.close()
             training = {"scenes": len(maes), "seconds": time.perf_counter() - t0,
                         "mae_first": maes[0], "mae_last": maes[-1]}
-        # size the output once (the count is part of every step anyway)
-        try:
+        try:  # size the output once (the count is part of every step anyway)
             ctx.find_pairs_device(d_boxes.ptr, n, None, 0)
         except stq_amd.StqError as e:
             if e.status != _lib.STQ_ERR_CAPACITY:
@@ -536,7 +609,8 @@ def main():
             outputs[name] = d_out.to_array(np.int32, 2 * cnt).reshape(-1, 2)
         d_out.close()
         ctx.close()
-        progress(f"form {name}: {ms:.2f} ms per call, {cnt} pairs")
+        progress(f"cloth form {name}: {ms:.2f} ms per call, {cnt} pairs")
+    d_boxes.close()
 
     exact = None
     if not args.no_check and len(outputs) == 2:
@@ -546,31 +620,12 @@ def main():
             return key
         a, b = outputs["chosen"], outputs["plain"]
         exact = bool(len(a) == len(b) and np.array_equal(canon(a), canon(b)))
+    del outputs
 
     main_form = "chosen" if "chosen" in res else want[0]
     r = res[main_form]
     P = r["pairs"]
-    alg_bytes = 60.0 * n + 8.0 * P
-    achieved = alg_bytes / (r["fill_sweep_ms"] * 1e-3)
-    # HBM bytes and VALU wave instructions per launch of this kernel, from the
-    # committed rocprof counter runs of the same workload
-    traffic, valu = None, None
-    tpath = os.path.join(ROOT, "profiles", "sweep_traffic_cloth708.json")
-    if os.path.exists(tpath):
-        try:
-            with open(tpath) as fh:
-                entry = json.load(fh).get(main_form, {})
-            traffic = entry.get("hbm_bytes_per_launch")
-            valu = entry.get("valu_wave_instructions_per_launch")
-        except (OSError, ValueError):
-            traffic, valu = None, None
-    issue = None
-    if valu:
-        rate = valu / (r["fill_sweep_ms"] * 1e-3)
-        issue = {"bound": "valu issue", "achieved": rate, "peak": VALU_ISSUE_PEAK,
-                 "unit": "wave instructions/s", "frac": rate / VALU_ISSUE_PEAK,
-                 "valu_per_launch": valu}
-
+    traffic, valu = load_traffic("sweep_traffic_cloth708", main_form)
     cpu = None
     if args.cpu_sample > 0:
         from oracle import oracle
@@ -590,35 +645,16 @@ def main():
             one = {"value": n / (sort1 + sw1 * (n / nr1)), "unit": "boxes/s",
                    "sample": f"sort of all boxes ({sort1:.2f} s) + sweep of the first {nr1} sorted rows "
                              f"({sw1:.2f} s), scaled to all rows"}
-        progress(f"cpu baseline: {n / est:.3g} boxes/s on {threads} threads")
+        progress(f"cloth cpu baseline: {n / est:.3g} boxes/s on {threads} threads")
         cpu = {"value": n / est, "unit": "boxes/s", "cores": threads, "kind": "port", "one_thread": one,
                "sample": (f"oracle restatement of src/cpu/sweep.cpp (FP64, axis x): parallel sort of all "
                           f"{n} boxes ({sort_s:.2f} s) + chain sweep of the first {nrows} sorted rows "
                           f"({sweep_s:.2f} s, {len(pairs)} pairs), scaled to all rows; "
                           f"CPU {cpu_model()}, {threads} threads")}
-
-    line = {
-        "metric": METRIC,
-        "value": r["boxes_per_s"],
-        "unit": "boxes/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": r["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64" if main_form == "chosen" else "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": f"swept cloth {args.grid}x{args.grid} (z=0.05 sin6x sin6y, step N(0,2e-3)), "
-                        f"{n} FP64 boxes, device-resident in/out",
-            "boxes": n,
-            "pairs": P,
-            "candidates_x": r["stats"]["n_candidates"],
-            "form": main_form,
-        },
-        "pairs_per_s": r["pairs_per_s"],
+    return {
+        "workload": f"swept cloth {args.grid}x{args.grid} (z=0.05 sin6x sin6y, step N(0,2e-3)), "
+                    f"{n} FP64 boxes, device-resident in/out",
+        "boxes": n, "pairs": P, "candidates_x": r["stats"]["n_candidates"],
         "exact_pair_set_vs_plain": exact,
         "forms": {k: {kk: vv for kk, vv in v.items() if kk != "stats"} | {"stats": v["stats"]}
                   for k, v in res.items()},
@@ -627,41 +663,95 @@ def main():
         "sizing": ({"training": training, "items": r["stats"]["n_tiles"],
                     "fallback_items": r["fallback_items"], "mae_pairs_per_item": r["stats"]["sizing_mae"]}
                    if training else None),
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "emitting sweep (k_qsweep learned pass for the chosen form)",
-                     "algorithmic_bytes_per_launch": alg_bytes,
-                     "kernel_ms": r["fill_sweep_ms"],
-                     "issue": issue},
+        "roofline": roofline(60.0 * n + 8.0 * P, r["fill_sweep_ms"],
+                             "emitting sweep (k_qsweep learned pass for the chosen form)", traffic, valu),
         "cpu_baseline": cpu,
     }
-    if args.host_e2e and "chosen" in want:
-        # PCIe-inclusive: host boxes in, host pairs out (not the benchmark value)
-        c2 = stq_amd.Context(0, **forms["chosen"])
-        c2.set_options(**{**forms["chosen"], "sizing": stq_amd.SIZING_COUNT_FILL})
+
+
+def bench_host(args, H):
+    """PCIe-inclusive: host boxes in, host pairs out through
+    stq_find_pairs_host on the cloth (never the benchmark value)."""
+    import numpy as np
+    import stq_amd
+    from stq_amd import scenes
+    boxes = scenes.cloth_scene(g=args.grid, seed=args.seed)
+    c2 = stq_amd.Context(0, precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_CHAIN,
+                         sizing=stq_amd.SIZING_COUNT_FILL, autotune=1)
+    c2.find_pairs_host(boxes)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
         c2.find_pairs_host(boxes)
-        ts = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            c2.find_pairs_host(boxes)
-            ts.append(time.perf_counter() - t0)
-        c2.close()
-        line["end_to_end_host"] = {"ms": 1e3 * float(np.median(ts)),
-                                   "what": "stq_find_pairs_host, reduced precision, count then fill: "
-                                           "H2D of the boxes, the call, D2H of the pairs"}
-    if args.skewed:
+        ts.append(time.perf_counter() - t0)
+    c2.close()
+    return {"ms": 1e3 * float(np.median(ts)),
+            "what": "stq_find_pairs_host on the cloth, reduced precision, count then fill: H2D of the "
+                    "boxes, the call, D2H of the pairs"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only (collectives, sharding and ranks "
+                                   "are not part of it); run with --gpus 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    from stq_amd import _lib
+
+    H = _lib.hip()
+    if args.copy_probe_mib:
+        nb = args.copy_probe_mib << 20
+        a, b = _lib.DeviceBuffer(nb), _lib.DeviceBuffer(nb)
+        for _ in range(5):
+            _lib._check_hip(H.hipMemcpy(b.ptr, a.ptr, nb, _lib.HIP_D2D), "hipMemcpy D2D")
+        H.hipDeviceSynchronize()
+        print(json.dumps({"copy_probe_bytes": nb, "launches": 5}))
+        return
+    legs = [x for x in args.legs.split(",") if x]
+    line = {"metric": METRIC}
+    if "batched" in legs:
+        bt = bench_batched(args, H)
+        forms = bt["forms"]
+        head = "chosen" if "chosen" in forms else next(iter(forms))
+        r = forms[head]
+        traffic, valu = load_traffic("sweep_traffic_batched512", head)
+        line.update({
+            "value": r["boxes_per_s"], "unit": "boxes/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if head == "chosen" else "f64",
+            "data": "synthetic",
+            "config": {"workload": bt["workload"] + " (batched form: one call per step over all scenes)",
+                       "boxes": bt["boxes"], "scenes": bt["scenes"], "pairs": r["pairs"], "form": head},
+            "pairs_per_s": r["pairs_per_s"],
+            "speedup_vs_plain_fp64": bt.get("speedup_vs_plain"),
+            "speedup_vs_plain_fp64_tuned": bt.get("speedup_vs_plain_tuned"),
+            "exact_pair_sets": {"chosen_vs_plain_forms": bt.get("per_scene_digests_equal_across_forms"),
+                                "oracle_digest_mismatches": (bt["cpu_baseline"] or {}).get(
+                                    "per_scene_digest_mismatches_vs_gpu")},
+            "roofline": roofline(60.0 * bt["boxes"] + 8.0 * r["pairs"], r["sweep_kernel_ms"],
+                                 "emitting sweep of the batch (k_qsweep learned pass)", traffic, valu)
+                        | {"kernel_timing": r["sweep_kernel_timing"]},
+            "cpu_baseline": bt["cpu_baseline"],
+            "batched": bt,
+        })
+        progress("batched leg done")
+    if "cloth" in legs:
+        line["cloth"] = bench_cloth(args, H)
+        progress("cloth leg done")
+    if "host" in legs:
+        line["end_to_end_host"] = bench_host(args, H)
+        progress("host leg done")
+    if "skewed" in legs:
         line["skewed"] = bench_skewed(args, H)
         progress("skewed leg done")
-    if args.batched_scenes > 0:
-        line["batched"] = bench_batched(args, H)
-        progress("batched leg done")
-    if args.stream_scenes > 0:
+    if "stream" in legs and args.stream_scenes > 0:
         line["learned_stream"] = bench_stream(args, H)
         progress("learned stream leg done")
-    if args.handoff:
+    if "handoff" in legs:
         line["handoff"] = bench_handoff(args, H)
         progress("hand-off leg done")
-    if args.small:
+    if "small" in legs:
         line["small"] = bench_small(args, H)
         progress("small scene leg done")
     print(json.dumps(line))

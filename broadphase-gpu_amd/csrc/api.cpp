@@ -453,6 +453,10 @@ void Context::finish_stats(hipStream_t st) {
   stats.n_lane_steps = (int64_t)hc[2];
   stats.n_confirm = (int64_t)confirm_sum(lines);
   stats.ms_total = ev_ms(ev[E_START], ev[E_END]);
+  phase_times();
+}
+
+void Context::phase_times() {
   stats.ms_keys = ev_ms(ev[E_START], ev[E_KEYS]);
   stats.ms_sort = ev_ms(ev[E_KEYS], ev[E_SORT]);
   stats.ms_gather = ev_ms(ev[E_SORT], ev[E_GATHER]);
@@ -643,12 +647,23 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
       1, std::min<int64_t>((bound + 31) / 32, (int64_t)cus * std::max(1, wpc / 4) / 8)));
   const int batch = opt.batch ? opt.batch : (f32 ? 8 : 2);
 
+  // Phase events: plain records when eager; inside a capture, event record
+  // nodes of the graph (where the runtime supports them), so a replay times
+  // its phases too.
+  bool capturing = false;
+  auto mark = [&](int e) {
+    if (!capturing)
+      STQ_HIP(hipEventRecord(ev[e], st));
+    else if (graph_events)
+      STQ_HIP(hipEventRecordWithFlags(ev[e], st, hipEventRecordExternal));
+  };
   auto enqueue = [&]() {
     launch_fill_u64(ctr, 0ull, kCounters, st);
     launch_seg_stats(boxes, t, bp.dev, opt.axis_policy, opt.axis_policy <= 2 ? opt.axis_policy : 0,
                      f32, st);
     launch_keys_seg(f32, boxes, t, bp.dev, d_keys, d_vals, reinterpret_cast<unsigned *>(&ctr[0]),
                     st);
+    mark(E_KEYS);
     if (f32) {
       radix_sort_u64_sync_free(d_keys, d_vals, n, 32 + sbits, sort, st);
     } else {
@@ -658,13 +673,16 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
         radix_sort_u32_sync_free(d_skey, d_vals, n, sbits, sort, st);
       }
     }
+    mark(E_SORT);
     launch_gather_seg(f32, boxes, d_vals, t, bp.dev, d_smin, d_smax, d_rec, d_qrec, d_orig, d_cols,
                       st);
     launch_chain_end_seg(f32, d_smin, d_smax, t, bp.dev, d_hi, &ctr[1], st);
+    mark(E_GATHER);
     launch_plan_seg(d_hi, d_qrec, t, bp.dev, W, chunk, tail_cost(opt), d_rowlist, d_groups, d_tg,
                     d_gchunk, d_nch, &ctr[2], st);
     scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
     launch_items(d_groups, d_tg, d_gchunk, d_ioff, ngroups, d_items, st);
+    mark(E_PLAN);
     // item-level arrays are used up to the device item count only: every item
     // below it gets its count from the sweep, and the scans stop there
     SweepArgs sa;
@@ -685,8 +703,11 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
                       bp.dev.seg_off, nscenes, d_feat, d_cand, st);
     const int64_t *total = d_off + bound;
     if (!single_pass) {
+      mark(E_PRED);
       launch_sweep(f32, kCount, batch, nblocks, sa, st);
+      mark(E_COUNT);
       scan_u32_to_i64(d_counts, d_off, bound, scan_tmp, st, sa.nitems_dev);
+      mark(E_SCAN);
       sa.counts = nullptr;
       sa.confirms = nullptr;
       sa.offsets = d_off;
@@ -695,24 +716,31 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
       sa.capacity = capacity;
       if (sa.work) sa.work = claim_counters(ctr, 1);
       launch_sweep(f32, kFill, batch, nblocks, sa, st);
+      mark(E_FILL);
+      mark(E_COMPACT);
     } else {
       launch_mlp_forward(d_w, d_feat, bound, d_pred, st, sa.nitems_dev);
       launch_caps(d_pred, d_cand, bound, kBatchCapScale, kBatchCapSlack, d_caps, st, sa.nitems_dev);
       scan_u32_to_i64(d_caps, d_capoff, bound, scan_tmp, st, sa.nitems_dev);
       launch_cap_budget(d_caps, d_capoff, bound, b_budget, st, sa.nitems_dev);
+      mark(E_PRED);
+      mark(E_COUNT);
       sa.offsets = d_capoff;
       sa.caps = d_caps;
       sa.out = d_scratch;
       launch_sweep(f32, kLearned, batch, nblocks, sa, st);
+      mark(E_FILL);
       scan_u32_to_i64(d_counts, d_off, bound, scan_tmp, st, sa.nitems_dev);
       launch_fill_u64(reinterpret_cast<unsigned long long *>(d_acc), 0ull, 1, st);
       launch_abs_error(d_pred, d_counts, bound, d_acc, st, sa.nitems_dev);
+      mark(E_SCAN);
       launch_compact(d_counts, d_caps, d_capoff, d_off, bound, d_scratch,
                      reinterpret_cast<int2 *>(out), d_ovf, &ctr[4], st, sa.nitems_dev, total,
                      capacity);
       // the overflowed items, exactly, straight into the output
       enqueue_refill(sa, f32, batch, nblocks, d_ovf, &ctr[4], bound, d_off,
                      reinterpret_cast<int2 *>(out), total, capacity, st);
+      mark(E_COMPACT);
     }
     launch_scene_pairs(d_off, d_ioff, t, bp.dev, d_pairoff, st);
     STQ_HIP(hipMemcpyAsync(hbuf, d_pairoff, sizeof(int64_t) * ((size_t)nscenes + 1),
@@ -731,7 +759,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     const void *boxes, *out, *host;
     int64_t capacity, budget;
     uint64_t generation;
-    int32_t precision, axis_policy, batch, nblocks, xcd_split, sizing;
+    int32_t precision, axis_policy, batch, nblocks, xcd_split, sizing, claim, tails;
+    float tail_cost;
   } key;
   std::memset(&key, 0, sizeof(key));
   key.boxes = boxes;
@@ -746,6 +775,9 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   key.nblocks = nblocks;
   key.xcd_split = opt.xcd_split;
   key.sizing = single_pass ? 1 : 0;
+  key.claim = opt.claim;
+  key.tails = opt.tails;
+  key.tail_cost = tail_cost(opt);
   const uint64_t sig = fnv1a(&key, sizeof(key)) | 1ull;
 
   bool done = false;
@@ -759,13 +791,16 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
       bp.drop_graph();
       hipGraph_t g = nullptr;
       STQ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      capturing = true;
       try {
         enqueue();
       } catch (...) {
+        capturing = false;
         (void)hipStreamEndCapture(st, &g);
         if (g) (void)hipGraphDestroy(g);
         throw;
       }
+      capturing = false;
       STQ_HIP(hipStreamEndCapture(st, &g));
       hipGraphExec_t ex = nullptr;
       const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -795,6 +830,9 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   STQ_HIP(hipEventRecord(ev[E_END], st));
   STQ_HIP(hipStreamSynchronize(st));
   stats.ms_total = ev_ms(ev[E_START], ev[E_END]);
+  learned_pass = single_pass;
+  // the phase events ran unless a graph without event nodes was launched
+  if (stats.graph == 0 || graph_events) phase_times();
   const uint64_t *hc = reinterpret_cast<const uint64_t *>(h_ctr);
   for (int s = 0; s <= nscenes; ++s) pair_off[s] = hbuf[s];
   const int64_t nitems = h_ctr[5], total = pair_off[nscenes];
@@ -918,6 +956,16 @@ int stq_create(void **ctx, int device, const stq_options *opt) {
     STQ_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) STQ_HIP(hipEventCreate(&e));
     c->cus = stqamd::device_cu_count(device);
+    // Can a capture hold event record nodes?  Capture only (nothing runs).
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      const hipError_t e1 = hipEventRecordWithFlags(c->ev[stqamd::E_KEYS], c->stream,
+                                                    hipEventRecordExternal);
+      const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+      c->graph_events = e1 == hipSuccess && e2 == hipSuccess && g != nullptr;
+      if (g) (void)hipGraphDestroy(g);
+    }
+    (void)hipGetLastError();
   });
   if (rc != STQ_OK) {
     delete c;
@@ -942,6 +990,9 @@ int stq_set_options(void *ctx, const stq_options *opt) {
   Context *c = static_cast<Context *>(ctx);
   if (!c || !opt || !stqamd::options_valid(*opt)) return STQ_ERR_INVALID_ARGUMENT;
   c->opt = *opt;
+  // a captured batch holds launch arguments of the old options
+  c->bplan.drop_graph();
+  c->bplan.eager_sig = 0;
   return STQ_OK;
 }
 

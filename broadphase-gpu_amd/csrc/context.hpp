@@ -137,6 +137,7 @@ struct Context {
   std::string last_error;
   int cus = 256;
   hipEvent_t ev[E_NUM] = {};
+  bool graph_events = false;  // a stream capture can record the phase events
 
   DevBuf keys, vals, smin, smax, geo, meta, full, hi, rowlist, groups, nchunks, ioff, items,
       counts, offsets, scan_tmp, counters, axis_partial, h2d_boxes, d2h_pairs, tgroups, gchunks;
@@ -199,6 +200,9 @@ struct Context {
   }
 
   void finish_stats(hipStream_t st);
+  // Phase times of the last call from the events (learned_pass selects the
+  // learned layout of the sweep phases).
+  void phase_times();
   void sizer_ensure();
   // Enqueues the exact refill of the overflowed items ovf[0 .. *novf) (at
   // most nbound) of the sweep described by sa: a fill pass over them at their

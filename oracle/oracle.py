@@ -38,6 +38,10 @@ def lib():
         L.oracle_sweep_rows.argtypes = [P, I64, ctypes.c_int, I64, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_void_p),
                                         ctypes.POINTER(ctypes.c_double)]
+        L.oracle_sweep_timed.restype = I64
+        L.oracle_sweep_timed.argtypes = [P, I64, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_void_p),
+                                         ctypes.POINTER(ctypes.c_double)]
         L.oracle_bruteforce.restype = I64
         L.oracle_bruteforce.argtypes = [P, I64, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         L.oracle_candidates.restype = I64
@@ -95,6 +99,17 @@ def sweep(boxes, axis=0, nthreads=None, nrows=None, return_sort_time=False):
                                   nthreads or default_threads(), ctypes.byref(out), ctypes.byref(st))
     res = _take(out, cnt)
     return (res, st.value) if return_sort_time else res
+
+
+def sweep_timed(boxes, axis=0, nthreads=None):
+    """(pairs, seconds): the whole-scene sweep and its wall time measured
+    inside the C code (sort, sweep, merge; the copy to numpy excluded)."""
+    b = _as_boxes(boxes)
+    out = ctypes.c_void_p()
+    secs = ctypes.c_double()
+    cnt = lib().oracle_sweep_timed(b.ctypes.data, len(b), axis, nthreads or default_threads(),
+                                   ctypes.byref(out), ctypes.byref(secs))
+    return _take(out, cnt), secs.value
 
 
 def bruteforce(boxes, nthreads=None):

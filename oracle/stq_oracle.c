@@ -227,6 +227,16 @@ int64_t oracle_sweep(const oracle_aabb *boxes, int64_t n, int axis,
   return oracle_sweep_rows(boxes, n, axis, n, nthreads, out, NULL);
 }
 
+/* The whole-scene sweep with its wall time (sort, sweep and merge: the scope
+   the reference CLI times, app/main.cpp:138-153) in *seconds. */
+int64_t oracle_sweep_timed(const oracle_aabb *boxes, int64_t n, int axis,
+                           int nthreads, int32_t **out, double *seconds) {
+  double t0 = omp_get_wtime();
+  int64_t c = oracle_sweep_rows(boxes, n, axis, n, nthreads, out, NULL);
+  *seconds = omp_get_wtime() - t0;
+  return c;
+}
+
 /* Number of sweep candidates per sorted row on an axis: the length of the
    chain the reference walks before its break (src/cpu/sweep.cpp:102-108).
    Returns the total; rows with a longer chain than *max_chain update it. */
