@@ -20,6 +20,7 @@
 #include "build.hpp"
 #include "narrow.hpp"
 #include "context.hpp"
+#include "noalloc.hpp"
 #include "plan.hpp"
 
 namespace stqamd {
@@ -158,7 +159,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   uint32_t *d_nch = nchunks.as<uint32_t>((size_t)ngroups);
   int64_t *d_ioff = ioff.as<int64_t>((size_t)ngroups + 1);
   launch_plan(d_hi, d_qrec, n, W, chunk, tail_cost(opt), d_rowlist, d_groups, d_tg, d_gch, d_nch,
-              &ctr[2], st);
+              &ctr[2], st, plan_cap);
   scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
   STQ_HIP(hipMemcpyAsync(sh + 5, d_ioff + ngroups, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   if (early)
@@ -269,6 +270,42 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   STQ_HIP(hipEventRecord(ev[E_COMPACT], st));
   STQ_HIP(hipEventRecord(ev[E_END], st));
   finish_stats(st);
+}
+
+void Context::run_noalloc(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t capacity,
+                          int64_t *count) {
+  if (!noalloc_base) throw StatusError(STQ_ERR_INTERNAL, "no fixed scratch on this device");
+  Arena arena;
+  arena.base = static_cast<char *>(noalloc_base);
+  arena.size = kNoAllocScratchBytes;
+  // every buffer a count-then-fill call of run() touches, swapped for an
+  // arena-backed one for this call and put back afterwards
+  DevBuf *bufs[] = {&keys,     &vals,          &smin,           &smax,          &geo,
+                    &meta,     &hi,            &rowlist,        &groups,        &tgroups,
+                    &gchunks,  &nchunks,       &ioff,           &items,         &counts,
+                    &offsets,  &scan_tmp,      &counters,       &axis_partial,  &scene_params,
+                    &sort.alt_keys, &sort.alt_vals, &sort.counts, &sort.hist,    &sort.misc};
+  constexpr int kBufs = sizeof(bufs) / sizeof(bufs[0]);
+  struct Swap {
+    DevBuf **b;
+    DevBuf saved[kBufs];
+    Swap(DevBuf **bb, Arena *a) : b(bb) {
+      for (int i = 0; i < kBufs; ++i) {
+        saved[i] = *b[i];
+        *b[i] = DevBuf{};
+        b[i]->arena = a;
+      }
+    }
+    ~Swap() {
+      for (int i = 0; i < kBufs; ++i) *b[i] = saved[i];
+    }
+  } swap(bufs, &arena);
+  struct Cap {
+    bool &f;
+    explicit Cap(bool &x) : f(x) { f = true; }
+    ~Cap() { f = false; }
+  } cap(plan_cap);
+  run(boxes, n, out, capacity, nullptr, count);
 }
 
 void Context::enqueue_refill(const SweepArgs &sa, bool f32, int batch, int nblocks, const int *ovf,
@@ -966,6 +1003,12 @@ int stq_create(void **ctx, int device, const stq_options *opt) {
       if (g) (void)hipGraphDestroy(g);
     }
     (void)hipGetLastError();
+    // set-up of the allocation-free entry: this device's scratch address and
+    // the pinned buffer of the small read-backs, so that entry allocates
+    // nothing later
+    const int st = stqamd::noalloc_scratch(device, &c->noalloc_base);
+    if (st != STQ_OK) throw stqamd::StatusError(st, "hipGetSymbolAddress of the fixed scratch failed");
+    c->small_host();
   });
   if (rc != STQ_OK) {
     delete c;
@@ -1005,6 +1048,23 @@ int stq_find_pairs_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n, stq
   return stqamd::guarded(c, [&] {
     stqamd::tuned_call(c, d_boxes, n, nullptr, 1, false,
                        [&] { c->run(d_boxes, n, d_out, capacity, nullptr, count); });
+  });
+}
+
+int stq_find_pairs_device_noalloc(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n,
+                                  stq_pair *d_out, int64_t capacity, int64_t *count) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !count || n < 0 || capacity < 0 || (n > 0 && !d_boxes) || (capacity > 0 && !d_out))
+    return STQ_ERR_INVALID_ARGUMENT;
+  if (n > stqamd::kNoAllocMaxBoxes) {
+    c->last_error = "more boxes than the allocation-free entry's maximum (STQ_NOALLOC_MAX_BOXES)";
+    return STQ_ERR_TOO_LARGE;
+  }
+  return stqamd::guarded(c, [&] {
+    stqamd::OptionsScope scope(c);
+    c->autotune_and_apply(d_boxes, n, nullptr, 1, false, false);  // a table entry, no trials
+    c->opt.sizing = STQ_SIZING_COUNT_FILL;
+    c->run_noalloc(d_boxes, n, d_out, capacity, count);
   });
 }
 

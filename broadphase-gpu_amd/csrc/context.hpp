@@ -138,6 +138,11 @@ struct Context {
   int cus = 256;
   hipEvent_t ev[E_NUM] = {};
   bool graph_events = false;  // a stream capture can record the phase events
+  // The allocation-free entry: this device's fixed scratch array (looked up
+  // at stq_create), and whether planning caps the items per group (it does
+  // there, so the item count has a bound that the array is sized for).
+  void *noalloc_base = nullptr;
+  bool plan_cap = false;
 
   DevBuf keys, vals, smin, smax, geo, meta, full, hi, rowlist, groups, nchunks, ioff, items,
       counts, offsets, scan_tmp, counters, axis_partial, h2d_boxes, d2h_pairs, tgroups, gchunks;
@@ -219,14 +224,19 @@ struct Context {
   float sizer_gradient(const float *d_feat, const float *d_target, int64_t n);
   void run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t capacity,
            stq_pair **alloc_out, int64_t *count);
+  // run() with count-then-fill sizing and every scratch buffer carved from the
+  // fixed device array (noalloc_base): no device or host memory is allocated.
+  void run_noalloc(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t capacity,
+                   int64_t *count);
   void batch_plan(const int64_t *off, int nscenes, int W, int chunk);
   void run_batched(const stq_aabb_f64 *boxes, const int64_t *off, int nscenes, stq_pair *out,
                    int64_t capacity, int64_t *pair_off);
   // With opt.autotune: tunes the call's shape class on this input if the
   // table has no entry yet (trial runs, results discarded), then copies the
   // entry into opt.  The caller restores opt after the call.
+  // trials = false: only an existing table entry is applied.
   void autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int64_t *off, int nscenes,
-                          bool batched);
+                          bool batched, bool trials = true);
   int tune_save(const char *path) const;
   int tune_load(const char *path);
 };

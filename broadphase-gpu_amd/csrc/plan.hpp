@@ -25,6 +25,15 @@ constexpr int kRowsMask = 127;      // Item::nrows & kRowsMask = rows of the ite
 constexpr int kCapChunks = 64;      // capped planning: at most this many items per body or tail
 constexpr double kTailFixed = 256.0;  // wave steps charged for making a tail item at all
 
+// Most items a capped plan (plan_group with cap) makes for one scene of n
+// boxes: each 64-row group's body and tail at most min(kCapChunks, chunks of
+// the longest possible span) items.
+inline int64_t capped_item_bound(int64_t n, int chunk) {
+  if (n < 2) return 0;
+  const int64_t spans = (n - 1 + chunk - 1) / chunk;
+  return 2 * ((n + 63) / 64) * (spans < kCapChunks ? spans : (int64_t)kCapChunks);
+}
+
 // Group record: body {rowoff, nrows, q0, q1}, tail {rowoff, rows, q0, q1}
 // (rows = 0: none), chunk sizes {body, tail}.
 struct GroupPlan {

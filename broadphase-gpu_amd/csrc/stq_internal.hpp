@@ -136,12 +136,31 @@ struct StatusError {
 // graph that holds buffer addresses is valid only while it is unchanged.
 inline std::atomic<uint64_t> g_devbuf_generation{0};
 
-// Grow-only device buffer.
+// A fixed device region that buffers are carved from in order (256-byte
+// aligned), instead of being allocated: used by the allocation-free entry.
+struct Arena {
+  char *base = nullptr;
+  size_t size = 0, used = 0;
+};
+
+// Grow-only device buffer.  With an arena, growing carves a new piece of it
+// (nothing is allocated; running out is STQ_ERR_TOO_LARGE).
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  Arena *arena = nullptr;
   void *get(size_t need) {
     if (need == 0) need = 256;
+    if (need > bytes && arena) {
+      const size_t at = (arena->used + 255) & ~(size_t)255;
+      if (at > arena->size || need > arena->size - at)
+        throw StatusError(STQ_ERR_TOO_LARGE, "the call needs more scratch than the fixed array "
+                                             "of the allocation-free entry holds");
+      p = arena->base + at;
+      bytes = need;
+      arena->used = at + need;
+      return p;
+    }
     if (need > bytes) {
       g_devbuf_generation.fetch_add(1);
       if (p) (void)hipFree(p);
@@ -163,8 +182,10 @@ struct DevBuf {
   }
   template <class T> T *as(size_t count) { return static_cast<T *>(get(count * sizeof(T))); }
   void release() {
-    if (p) g_devbuf_generation.fetch_add(1);
-    if (p) (void)hipFree(p);
+    if (p && !arena) {
+      g_devbuf_generation.fetch_add(1);
+      (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
   }

@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kThreads) void k_chain_end(const T *__restrict__ sm
 template <bool Q>
 __global__ __launch_bounds__(kPlanThreads) void k_plan(const int *__restrict__ hi,
                                                    const QRec *__restrict__ qrec, int64_t n,
-                                                   int W, int chunk, float tail_cost,
+                                                   int W, int chunk, float tail_cost, bool cap,
                                                    int *__restrict__ rowlist,
                                                    int4 *__restrict__ groups,
                                                    int4 *__restrict__ tgroups,
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(const int *__restrict__ h
       }
       if (act) rowlist[wbase + j] = p;
       const int nrows = min(64, cnt - 64 * k);
-      const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, false,
+      const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, cap,
                                       tail_cost);
       const int64_t g = (wbase >> 6) + k;
       if (lane == 0) {
@@ -925,7 +925,7 @@ void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, i
 
 void launch_plan(const int *hi, const QRec *qrec, int64_t n, int W, int chunk, float tail_cost,
                  int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
-                 unsigned long long *lane_steps, hipStream_t st) {
+                 unsigned long long *lane_steps, hipStream_t st, bool cap) {
   if (n <= 0) return;
   if (W > (qrec ? kMaxWindow32 : kMaxWindow64))
     throw StatusError(STQ_ERR_INVALID_ARGUMENT, "planning window too large");
@@ -933,10 +933,10 @@ void launch_plan(const int *hi, const QRec *qrec, int64_t n, int W, int chunk, f
   const dim3 grid((unsigned)std::min<int64_t>(nwin, kReduceGrid));
   if (qrec)
     hipLaunchKernelGGL(k_plan<true>, grid, dim3(kPlanThreads), 0, st, hi, qrec, n, W, chunk,
-                       tail_cost, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
+                       tail_cost, cap, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
   else
     hipLaunchKernelGGL(k_plan<false>, grid, dim3(kPlanThreads), 0, st, hi, qrec, n, W, chunk,
-                       tail_cost, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
+                       tail_cost, cap, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
   STQ_HIP(hipGetLastError());
 }
 

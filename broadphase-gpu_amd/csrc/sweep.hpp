@@ -293,10 +293,12 @@ void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, i
 // tail_cost: weight of tail tests in the group cut (plan.hpp); 0 = no tails.
 // qrec (reduced precision): windows are ordered by group_key, so a group's
 // rows are close on axis 1, and each group's rows then by chain end; null:
-// windows ordered by chain end.
+// windows ordered by chain end.  cap: at most kCapChunks items per body and
+// per tail (chunks grow instead), so a scene has at most
+// capped_item_bound(n, chunk) items.
 void launch_plan(const int *hi, const QRec *qrec, int64_t n, int W, int chunk, float tail_cost,
                  int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
-                 unsigned long long *lane_steps, hipStream_t st);
+                 unsigned long long *lane_steps, hipStream_t st, bool cap = false);
 // Items of every group from the group records (body chunks, then tail chunks).
 void launch_items(const int4 *groups, const int4 *tgroups, const int2 *gchunk, const int64_t *ioff,
                   int64_t ngroups, Item *items, hipStream_t st);

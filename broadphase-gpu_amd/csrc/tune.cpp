@@ -44,13 +44,13 @@ uint64_t tune_key(int precision, bool batched, int64_t n, int nscenes) {
 }
 
 void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int64_t *off,
-                                 int nscenes, bool batched) {
+                                 int nscenes, bool batched, bool trials) {
   last_tune_ms = 0;
   if (!opt.autotune) return;
   const uint64_t key = tune_key(opt.precision, batched, n, nscenes);
   auto it = tune_table.find(key);
   if (it == tune_table.end()) {
-    if (n < kTuneMinBoxes) return;
+    if (n < kTuneMinBoxes || !trials) return;
     const auto t0 = std::chrono::steady_clock::now();
     const stq_options user = opt;
     const bool f32 = user.precision == STQ_PRECISION_F32_CONFIRM;

@@ -30,7 +30,8 @@ SIZING_LEARNED = 1
 # every entry point of include/stq.h
 EXPORTS = [
     "stq_options_default", "stq_create", "stq_destroy", "stq_set_options",
-    "stq_find_pairs_device", "stq_find_pairs_device_alloc", "stq_find_pairs_host",
+    "stq_find_pairs_device", "stq_find_pairs_device_noalloc", "stq_find_pairs_device_alloc",
+    "stq_find_pairs_host",
     "stq_find_pairs_batched_device", "stq_build_swept_boxes_device", "stq_find_pairs_mesh_device",
     "stq_orient_pairs_device", "stq_ccd_queries_device", "stq_sizeof_ccd_query",
     "stq_mesh_read", "stq_mesh_edges", "stq_recall",
@@ -127,6 +128,7 @@ def lib():
         "stq_destroy": (ctypes.c_int, [P]),
         "stq_set_options": (ctypes.c_int, [P, ctypes.POINTER(Options)]),
         "stq_find_pairs_device": (ctypes.c_int, [P, P, I64, P, I64, ctypes.POINTER(I64)]),
+        "stq_find_pairs_device_noalloc": (ctypes.c_int, [P, P, I64, P, I64, ctypes.POINTER(I64)]),
         "stq_find_pairs_device_alloc": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
         "stq_find_pairs_host": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
         "stq_find_pairs_batched_device": (ctypes.c_int, [P, P, P, I32, P, I64, P]),

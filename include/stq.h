@@ -204,6 +204,25 @@ int stq_set_options(void *ctx, const stq_options *opt);
 int stq_find_pairs_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n,
                           stq_pair *d_out, int64_t capacity, int64_t *count);
 
+/* ---- allocation-free form (no counterpart in the reference) ----
+   For callers that cannot allocate device memory during the call (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+   the same pairs as stq_find_pairs_device, with every scratch buffer taken
+   from one fixed device array per GPU (STQ_NOALLOC_SCRATCH_BYTES, reserved
+   when the library is loaded on that GPU and held until the process ends).
+   The call allocates no device or host memory.  Sizing is count-then-fill
+   (options.sizing is ignored), autotuning only applies a table entry that
+   already exists, and planning caps the work items per 64-row group, so the
+   pair order may differ from stq_find_pairs_device's (the set is the same).
+   More than STQ_NOALLOC_MAX_BOXES boxes: STQ_ERR_TOO_LARGE, nothing runs.
+   Every call on one GPU shares that GPU's array: calls that overlap on one
+   GPU (from two contexts or two threads) race and must be serialised by the
+   caller. */
+#define STQ_NOALLOC_MAX_BOXES 262144
+#define STQ_NOALLOC_SCRATCH_BYTES (48u << 20)
+int stq_find_pairs_device_noalloc(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n,
+                                  stq_pair *d_out, int64_t capacity, int64_t *count);
+
 /* As stq_find_pairs_device, but the library allocates *d_out with exactly
    *count pairs (at least one), like the reference's cudaMalloc'd d_overlaps
    that the caller owns (src/simulation.cu:686); release with stq_free_device. */
