@@ -121,7 +121,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   ColStats *d_cols = learned ? col_stats.as<ColStats>((size_t)(n + kColBlock - 1) / kColBlock)
                              : nullptr;
   launch_gather(f32, boxes, order, n, axis, d_sp, d_smin, d_smax, d_rec, d_qrec, d_cols, st);
-  launch_chain_end(f32, d_smin, d_smax, n, d_hi, &ctr[1], st);
+  launch_chain_end(f32, d_smin, d_smax, n, d_hi, &ctr[1], st, row_begin, row_end);
   STQ_HIP(hipEventRecord(ev[E_GATHER], st));
   // The error flags and the candidate count are read with the item count
   // below, except where the window depends on the mean chain length (the
@@ -1066,6 +1066,88 @@ int stq_find_pairs_device_noalloc(void *ctx, const stq_aabb_f64 *d_boxes, int64_
     c->opt.sizing = STQ_SIZING_COUNT_FILL;
     c->run_noalloc(d_boxes, n, d_out, capacity, count);
   });
+}
+
+int stq_find_pairs_rows_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n,
+                               int64_t row_begin, int64_t row_end, stq_pair *d_out,
+                               int64_t capacity, int64_t *count) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !count || n < 0 || capacity < 0 || (n > 0 && !d_boxes) || (capacity > 0 && !d_out) ||
+      row_begin < 0 || row_end < row_begin)
+    return STQ_ERR_INVALID_ARGUMENT;
+  if (n > INT32_MAX) return STQ_ERR_TOO_LARGE;
+  return stqamd::guarded(c, [&] {
+    stqamd::OptionsScope scope(c);
+    c->opt.precision = STQ_PRECISION_F64;
+    c->opt.axis_policy = STQ_AXIS_X;
+    c->opt.sizing = STQ_SIZING_COUNT_FILL;
+    c->opt.autotune = 0;
+    struct Rows {
+      Context *c;
+      Rows(Context *x, int64_t b, int64_t e) : c(x) {
+        c->row_begin = b;
+        c->row_end = e;
+      }
+      ~Rows() {
+        c->row_begin = 0;
+        c->row_end = -1;
+      }
+    } rows(c, std::min(row_begin, n), std::min(row_end, n));
+    c->run(d_boxes, n, d_out, capacity, nullptr, count);
+  });
+}
+
+int stq_sort_order_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n, int32_t *d_order) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || n < 0 || (n > 0 && (!d_boxes || !d_order))) return STQ_ERR_INVALID_ARGUMENT;
+  if (n > INT32_MAX) return STQ_ERR_TOO_LARGE;
+  return stqamd::guarded(c, [&] {
+    if (n == 0) return;
+    unsigned long long *ctr = c->counters.as<unsigned long long>(stqamd::kCounters);
+    STQ_HIP(hipMemsetAsync(ctr, 0, sizeof(unsigned long long), c->stream));
+    uint32_t *order = c->vals.as<uint32_t>((size_t)n);
+    uint64_t *keys = c->keys.as<uint64_t>((size_t)n);
+    stqamd::launch_keys(false, d_boxes, n, 0, nullptr, keys, order, reinterpret_cast<unsigned *>(ctr),
+                        c->stream);
+    int passes = 0;
+    stqamd::radix_sort_u64(keys, order, n, 64, c->sort, c->stream, &passes);
+    STQ_HIP(hipMemcpyAsync(d_order, order, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice,
+                           c->stream));
+    unsigned long long err = 0;
+    STQ_HIP(hipMemcpyAsync(&err, ctr, sizeof(err), hipMemcpyDeviceToHost, c->stream));
+    STQ_HIP(hipStreamSynchronize(c->stream));
+    if (err) throw stqamd::StatusError(STQ_ERR_INVALID_BOX, "input box with NaN or min > max");
+  });
+}
+
+int stq_radix_sort_device(void *ctx, void *d_keys, uint32_t *d_vals, int64_t n, int32_t key_bytes,
+                          int32_t key_bits, int32_t form) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || n < 0 || (n > 0 && (!d_keys || !d_vals)) || (key_bytes != 4 && key_bytes != 8) ||
+      key_bits < 1 || key_bits > 8 * key_bytes || form < 0 || form > 2 ||
+      (form == 2 && key_bytes != 4))
+    return STQ_ERR_INVALID_ARGUMENT;
+  if (n > INT32_MAX) return STQ_ERR_TOO_LARGE;
+  return stqamd::guarded(c, [&] {
+    int passes = 0;
+    if (key_bytes == 8) {
+      uint64_t *k = static_cast<uint64_t *>(d_keys);
+      if (form == 0) stqamd::radix_sort_u64(k, d_vals, n, key_bits, c->sort, c->stream, &passes);
+      else stqamd::radix_sort_u64_sync_free(k, d_vals, n, key_bits, c->sort, c->stream);
+    } else {
+      uint32_t *k = static_cast<uint32_t *>(d_keys);
+      if (form == 0) stqamd::radix_sort_u32(k, d_vals, n, key_bits, c->sort, c->stream, &passes);
+      else if (form == 1) stqamd::radix_sort_u32_sync_free(k, d_vals, n, key_bits, c->sort, c->stream);
+      else stqamd::radix_sort_u32_all_digits(k, d_vals, n, key_bits, c->sort, c->stream);
+    }
+    STQ_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int stq_device_memory(void *ctx, size_t *free_bytes, size_t *total_bytes) {
+  Context *c = static_cast<Context *>(ctx);
+  if (!c || !free_bytes || !total_bytes) return STQ_ERR_INVALID_ARGUMENT;
+  return stqamd::guarded(c, [&] { STQ_HIP(hipMemGetInfo(free_bytes, total_bytes)); });
 }
 
 int stq_find_pairs_device_alloc(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n,

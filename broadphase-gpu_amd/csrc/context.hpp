@@ -143,6 +143,9 @@ struct Context {
   // there, so the item count has a bound that the array is sized for).
   void *noalloc_base = nullptr;
   bool plan_cap = false;
+  // Sorted rows whose pairs a single-scene call emits: [row_begin, row_end)
+  // (row_end < 0: all), the reference's tidstart range.
+  int64_t row_begin = 0, row_end = -1;
 
   DevBuf keys, vals, smin, smax, geo, meta, full, hi, rowlist, groups, nchunks, ioff, items,
       counts, offsets, scan_tmp, counters, axis_partial, h2d_boxes, d2h_pairs, tgroups, gchunks;

@@ -262,15 +262,22 @@ __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__r
 // ---- chain ends -----------------------------------------------------------------
 // hi[s] = last q with smin[q] <= smax[s] (q >= s since smin[s] <= smax[s]),
 // by last_le; grid-stride over a capped grid, adding sum(hi - s) to *cand
-// with one atomic per block.
+// with one atomic per block.  Rows outside [r0, r1) get an empty chain
+// (hi = s): they emit nothing, as the reference's rows outside
+// [tidstart, tidstart + MAX_OVERLAP_CUTOFF) (src/sweep.cu:234-240).
 template <class T>
 __global__ __launch_bounds__(kThreads) void k_chain_end(const T *__restrict__ smin,
                                                         const T *__restrict__ smax, int64_t n,
+                                                        int64_t r0, int64_t r1,
                                                         int *__restrict__ hi,
                                                         unsigned long long *__restrict__ cand) {
   unsigned long long len = 0;
   for (int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x; s < n;
        s += (int64_t)gridDim.x * kThreads) {
+    if (s < r0 || s >= r1) {
+      hi[s] = (int)s;
+      continue;
+    }
     const int64_t lo = last_le(smin, s, n, smax[s]);
     hi[s] = (int)lo;
     len += (unsigned long long)(lo - s);
@@ -911,15 +918,16 @@ void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, i
 }
 
 void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, int *hi,
-                      unsigned long long *cand, hipStream_t st) {
+                      unsigned long long *cand, hipStream_t st, int64_t r0, int64_t r1) {
   if (n <= 0) return;
+  if (r1 < 0 || r1 > n) r1 = n;
   const unsigned g = std::min(blocks_for(n), kReduceGrid);
   if (f32)
     hipLaunchKernelGGL(k_chain_end<float>, dim3(g), dim3(kThreads), 0, st, (const float *)smin,
-                       (const float *)smax, n, hi, cand);
+                       (const float *)smax, n, r0, r1, hi, cand);
   else
     hipLaunchKernelGGL(k_chain_end<double>, dim3(g), dim3(kThreads), 0, st, (const double *)smin,
-                       (const double *)smax, n, hi, cand);
+                       (const double *)smax, n, r0, r1, hi, cand);
   STQ_HIP(hipGetLastError());
 }
 

@@ -288,8 +288,9 @@ void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, const
 void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, int64_t n,
                    int axis, const SceneParams *sp, void *smin, void *smax, void *rec, QRec *qrec,
                    ColStats *cols, hipStream_t st);
+// Rows outside [r0, r1) (r1 < 0: n) get empty chains.
 void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, int *hi,
-                      unsigned long long *cand, hipStream_t st);
+                      unsigned long long *cand, hipStream_t st, int64_t r0 = 0, int64_t r1 = -1);
 // tail_cost: weight of tail tests in the group cut (plan.hpp); 0 = no tails.
 // qrec (reduced precision): windows are ordered by group_key, so a group's
 // rows are close on axis 1, and each group's rows then by chain end; null:

@@ -31,7 +31,8 @@ SIZING_LEARNED = 1
 EXPORTS = [
     "stq_options_default", "stq_create", "stq_destroy", "stq_set_options",
     "stq_find_pairs_device", "stq_find_pairs_device_noalloc", "stq_find_pairs_device_alloc",
-    "stq_find_pairs_host",
+    "stq_find_pairs_host", "stq_find_pairs_rows_device", "stq_sort_order_device", "stq_device_memory",
+    "stq_radix_sort_device",
     "stq_find_pairs_batched_device", "stq_build_swept_boxes_device", "stq_find_pairs_mesh_device",
     "stq_orient_pairs_device", "stq_ccd_queries_device", "stq_sizeof_ccd_query",
     "stq_mesh_read", "stq_mesh_edges", "stq_recall",
@@ -131,6 +132,11 @@ def lib():
         "stq_find_pairs_device_noalloc": (ctypes.c_int, [P, P, I64, P, I64, ctypes.POINTER(I64)]),
         "stq_find_pairs_device_alloc": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
         "stq_find_pairs_host": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
+        "stq_find_pairs_rows_device": (ctypes.c_int, [P, P, I64, I64, I64, P, I64, ctypes.POINTER(I64)]),
+        "stq_sort_order_device": (ctypes.c_int, [P, P, I64, P]),
+        "stq_device_memory": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_size_t),
+                                             ctypes.POINTER(ctypes.c_size_t)]),
+        "stq_radix_sort_device": (ctypes.c_int, [P, P, P, I64, I32, I32, I32]),
         "stq_find_pairs_batched_device": (ctypes.c_int, [P, P, P, I32, P, I64, P]),
         "stq_build_swept_boxes_device": (ctypes.c_int, [P, P, P, I64, P, I64, P, I64, ctypes.c_double,
                                                          I32, P]),

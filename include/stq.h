@@ -7,8 +7,10 @@
  *       int2 *&d_overlaps, int *&d_count, int &threads, int &tidstart,
  *       int &devcount, const int memlimit)
  *   (reference src/simulation.cuh:22-27, body src/simulation.cu:555-782)
- * and the CPU entry it mirrors,
- *   stq::cpu::sweep_cpu_single_batch (src/cpu/sweep.hpp:19-20).
+ * and of the reference CPU entries
+ *   stq::cpu::sweep_cpu_single_batch, run_sweep_cpu, sweep, sort_along_xaxis
+ *   (src/cpu/sweep.hpp:16-25), which include/stq/cpu/stq_cpu_dropin.hpp
+ *   implements over this ABI (the pairs are computed on the GPU).
  *
  * Plain C types only: no HIP, CUDA or torch types cross this boundary.  Device
  * pointers are plain `void *`-compatible pointers into the current device's
@@ -222,6 +224,34 @@ int stq_find_pairs_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n,
 #define STQ_NOALLOC_SCRATCH_BYTES (48u << 20)
 int stq_find_pairs_device_noalloc(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n,
                                   stq_pair *d_out, int64_t capacity, int64_t *count);
+
+/* ---- sorted-row ranges (the reference's tidstart batching) ----
+   The pairs whose sorted-earlier box has rank in [row_begin, row_end) of the
+   order the reference sweeps: FP64 minima on x, ascending, ties by input
+   position (thrust::sort_by_key on the x minimum, src/simulation.cu:645-649;
+   rows of twostage_queue in [tidstart, tidstart + MAX_OVERLAP_CUTOFF),
+   src/sweep.cu:234-240).  The union over ranges that cover [0, n) is the
+   whole pair set, each pair in exactly one range.  Runs the plain FP64 form
+   with count-then-fill whatever the context's options; ids and orientation
+   as stq_find_pairs_device; row_end may exceed n. */
+int stq_find_pairs_rows_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n,
+                               int64_t row_begin, int64_t row_end, stq_pair *d_out,
+                               int64_t capacity, int64_t *count);
+/* The input positions of the boxes in that order (n int32 to d_order): the
+   key sort of the sweep.  STQ_ERR_INVALID_BOX as above. */
+int stq_sort_order_device(void *ctx, const stq_aabb_f64 *d_boxes, int64_t n, int32_t *d_order);
+/* Free and total bytes of the context's device (for callers that size their
+   own buffers, as the reference's MemHandler does, src/memory.cuh:78-94). */
+int stq_device_memory(void *ctx, size_t *free_bytes, size_t *total_bytes);
+/* The sweep's key sort (replaces thrust::sort_by_key, src/simulation.cu:
+   645-649) on caller keys: a stable LSD radix sort of n (key, value) pairs
+   by the low key_bits bits of 4- or 8-byte unsigned keys, in place on the
+   device.  form 0: digits equal for every key are skipped (one host
+   read-back of the digit histogram); 1: no host read-back (such digits cost
+   a copy pass; capturable); 2: 4-byte keys, every digit a pass, no
+   histogram. */
+int stq_radix_sort_device(void *ctx, void *d_keys, uint32_t *d_vals, int64_t n, int32_t key_bytes,
+                          int32_t key_bits, int32_t form);
 
 /* As stq_find_pairs_device, but the library allocates *d_out with exactly
    *count pairs (at least one), like the reference's cudaMalloc'd d_overlaps
