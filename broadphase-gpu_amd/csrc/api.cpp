@@ -432,8 +432,7 @@ void Context::sizer_ensure() {
 }
 
 float Context::sizer_gradient(const float *d_feat, const float *d_target, int64_t n) {
-  const int64_t nb = (n + 31) / 32;
-  float *part = sz_partial.as<float>((size_t)nb * (kParams + 1));
+  float *part = sz_partial.as<float>(mlp_partial_floats(n));
   float *g = sz_grad.as<float>(kParams);
   float *loss = sz_loss.as<float>(1);
   launch_mlp_gradient(sz_params.as<float>(kParams), d_feat, d_target, n, part, g, loss, stream);
@@ -464,18 +463,16 @@ void Context::sizer_train(const float *d_feat, const float *d_target, int64_t n,
                           float lr) {
   if (n <= 0) return;
   const int64_t bs = std::min<int64_t>(n, 8192);
-  float *bf = bfeat.as<float>((size_t)bs * kFeat);
-  float *bt = btargets.as<float>((size_t)bs);
+  float *part = sz_partial.as<float>(mlp_partial_floats(bs));
   int64_t stride = 7919;
   while (std::__gcd(stride, n) != 1) ++stride;
   const int64_t call = train_calls++;
   for (int s = 0; s < steps; ++s) {
+    // the gradient kernel gathers the minibatch rows itself
     const int64_t start = ((call * steps + s) * bs) % n;
-    launch_gather_batch(d_feat, d_target, n, start, stride, bs, bf, bt, stream);
-    sizer_gradient(bf, bt, bs);
     ++adam_t;
-    launch_adam(sz_params.as<float>(kParams), sz_grad.as<float>(kParams), sz_m.as<float>(kParams),
-                sz_v.as<float>(kParams), lr, adam_t, stream);
+    launch_train_step(sz_params.as<float>(kParams), sz_m.as<float>(kParams), sz_v.as<float>(kParams),
+                      d_feat, d_target, n, start, stride, bs, part, lr, adam_t, stream);
   }
 }
 

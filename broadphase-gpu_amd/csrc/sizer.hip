@@ -12,6 +12,9 @@ namespace stqamd {
 namespace {
 
 constexpr int kT = 256;
+#ifndef STQ_GRAD_WAVES
+#define STQ_GRAD_WAVES 4
+#endif
 
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -517,107 +520,338 @@ __global__ __launch_bounds__(kT) void k_mlp_forward(const float *__restrict__ pa
   }
 }
 
-// ---- MLP gradient --------------------------------------------------------------------
-// One wave per block, 32 items per block.  Phase 1: lanes 0..31 run forward
-// and backward for one item each and stage the per-item vectors in LDS.
-// Phase 2: every lane sums its share of the parameter gradients over the
-// block's items.  partial[block] holds kParams gradients and the loss.
-constexpr int kGradItems = 32;
+// ---- MLP gradient on the matrix cores ---------------------------------------------
+// One block of four waves per 64 minibatch rows, one 16-row group per wave;
+// every product is an FP32 v_mfma_f32_16x16x4f32 (lane l holds A[l%16][l/16],
+// B[l/16][l%16] and D[4(l/16)+r][l%16], r = 0..3, as in the forward):
+//   forward   H1 = relu(X W1^T + b1), H2 = relu(H1 W2^T + b2), y = H2 w3 + b3
+//   backward  dH2 = dy w3^T * [H2 > 0],  dH1 = (dH2 W2) * [H1 > 0]
+//   weights   dW2 = dH2^T H1, dW1 = dH1^T X (the rows are the K dimension)
+// H1 and dH2 go from the D layout to the A layout through a small LDS tile.
+// The weight products need no transpose: step s of their K loop feeds row
+// 4 (l/16) + s of the group from the D-layout register r = s to both
+// operands, so each of the 16 rows is summed once.  Rows are gathered from
+// the source rows directly: minibatch row k is source row
+// ((start + k) * stride) mod n.  The four waves' gradients are summed in LDS
+// in a fixed order ((w0 + w2) + (w1 + w3)) and the block writes one partial
+// record (kParams gradients, then the loss).
+constexpr int kGradWaves = STQ_GRAD_WAVES;  // waves per block, one 16-row group each
+constexpr int kGradRows = 16 * kGradWaves;   // minibatch rows per block
+constexpr int kGradRec = kParams + 1;    // a partial record: gradients, loss
+constexpr int kTileRow = kHid + 2;       // LDS row stride of a transpose tile
+// The weights' B operands in LDS in lane order, [nt][ks][lane], so every
+// operand read is 64 consecutive floats: W1 as in layer 1, W2 as in layer 2,
+// and W2 as the backward product dH2 W2 reads it.
+constexpr int kB1 = 0, kB2 = kB1 + kHid * kFeat, kB2t = kB2 + kHid * kHid,
+              kBias = kB2t + kHid * kHid,  // b1, b2, w3, b3 (as in the parameter vector)
+              kTiles = kBias + (kParams - kOffB2 + kHid),  // b1 .. b3 end
+              kGradOps = kTiles + kGradWaves * 16 * kTileRow,
+              kGradLds = kGradOps > kGradWaves * kGradRec ? kGradOps : kGradWaves * kGradRec;
 
-__global__ __launch_bounds__(64) void k_mlp_grad(const float *__restrict__ params,
+__global__ __launch_bounds__(64 * kGradWaves) void k_mlp_grad(const float *__restrict__ params,
                                                  const float *__restrict__ feat,
                                                  const float *__restrict__ target, int64_t n,
+                                                 int64_t start, int64_t stride, int64_t bs,
                                                  float *__restrict__ partial) {
-  __shared__ float xs[kGradItems][kFeat + 1];
-  __shared__ float h1s[kGradItems][kHid + 1];
-  __shared__ float h2s[kGradItems][kHid + 1];
-  __shared__ float d1s[kGradItems][kHid + 1];
-  __shared__ float d2s[kGradItems][kHid + 1];
-  __shared__ float douts[kGradItems];
-  __shared__ float losses[kGradItems];
-  const int t = threadIdx.x;
-  const int64_t base = (int64_t)blockIdx.x * kGradItems;
-  const float invn = 1.f / (float)n;
-  if (t < kGradItems) {
-    const int64_t i = base + t;
-    const bool valid = i < n;
-    for (int k = 0; k < kFeat; ++k) xs[t][k] = valid ? feat[i * kFeat + k] : 0.f;
-    for (int j = 0; j < kHid; ++j) {
-      float s = params[kOffB1 + j];
-      for (int k = 0; k < kFeat; ++k) s += params[kOffW1 + j * kFeat + k] * xs[t][k];
-      h1s[t][j] = fmaxf(s, 0.f);
-    }
-    float out = params[kOffB3];
-    for (int j = 0; j < kHid; ++j) {
-      float s = params[kOffB2 + j];
-      for (int k = 0; k < kHid; ++k) s += params[kOffW2 + j * kHid + k] * h1s[t][k];
-      const float h2 = fmaxf(s, 0.f);
-      h2s[t][j] = h2;
-      out += params[kOffW3 + j] * h2;
-    }
-    const float err = valid ? out - target[i] : 0.f;
-    const float dout = 2.f * err * invn;
-    douts[t] = dout;
-    losses[t] = err * err * invn;
-    for (int j = 0; j < kHid; ++j)
-      d2s[t][j] = h2s[t][j] > 0.f ? dout * params[kOffW3 + j] : 0.f;
-    for (int k = 0; k < kHid; ++k) {
-      float s = 0.f;
-      for (int j = 0; j < kHid; ++j) s += d2s[t][j] * params[kOffW2 + j * kHid + k];
-      d1s[t][k] = h1s[t][k] > 0.f ? s : 0.f;
-    }
+  // weight operands and transpose tiles (16 x kTileRow floats per wave) while
+  // computing, then two partial records for the reduction over the waves
+  __shared__ float lds[kGradLds];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 15, quad = lane >> 4;
+  float(*tile)[kTileRow] = reinterpret_cast<float(*)[kTileRow]>(lds + kTiles + wv * 16 * kTileRow);
+  // coalesced reads of W1 and W2, scattered into the operand order
+  for (int i = threadIdx.x; i < kHid * kFeat; i += 64 * kGradWaves) {
+    const int j = i / kFeat, k = i % kFeat;
+    lds[kB1 + ((j >> 4) * (kFeat / 4) + (k >> 2)) * 64 + (k & 3) * 16 + (j & 15)] = params[kOffW1 + i];
   }
+  for (int i = threadIdx.x; i < kHid * kHid; i += 64 * kGradWaves) {
+    const int j = i / kHid, k = i % kHid;
+    const float x = params[kOffW2 + i];
+    lds[kB2 + ((j >> 4) * (kHid / 4) + (k >> 2)) * 64 + (k & 3) * 16 + (j & 15)] = x;
+    lds[kB2t + ((k >> 4) * (kHid / 4) + (j >> 2)) * 64 + (j & 3) * 16 + (k & 15)] = x;
+  }
+  // b1, then b2, w3 and b3 (W2 skipped)
+  for (int i = threadIdx.x; i < kHid + (kParams - kOffB2); i += 64 * kGradWaves)
+    lds[kBias + i] = params[i < kHid ? kOffB1 + i : kOffB2 + (i - kHid)];
   __syncthreads();
-  float *out = partial + (int64_t)blockIdx.x * (kParams + 1);
-  for (int pidx = t; pidx <= kParams; pidx += 64) {
-    float g = 0.f;
-    if (pidx < kOffB1) {
-      const int j = pidx / kFeat, k = pidx % kFeat;
-      for (int s = 0; s < kGradItems; ++s) g += d1s[s][j] * xs[s][k];
-    } else if (pidx < kOffW2) {
-      const int j = pidx - kOffB1;
-      for (int s = 0; s < kGradItems; ++s) g += d1s[s][j];
-    } else if (pidx < kOffB2) {
-      const int j = (pidx - kOffW2) / kHid, k = (pidx - kOffW2) % kHid;
-      for (int s = 0; s < kGradItems; ++s) g += d2s[s][j] * h1s[s][k];
-    } else if (pidx < kOffW3) {
-      const int j = pidx - kOffB2;
-      for (int s = 0; s < kGradItems; ++s) g += d2s[s][j];
-    } else if (pidx < kOffB3) {
-      const int j = pidx - kOffW3;
-      for (int s = 0; s < kGradItems; ++s) g += douts[s] * h2s[s][j];
-    } else if (pidx == kOffB3) {
-      for (int s = 0; s < kGradItems; ++s) g += douts[s];
-    } else {
-      for (int s = 0; s < kGradItems; ++s) g += losses[s];
+  const float *b1s = lds + kBias, *b2s = lds + kBias + kHid, *w3s = lds + kBias + 2 * kHid;
+  const float b3 = lds[kBias + 3 * kHid];
+  const float invn = 1.f / (float)bs;
+  const int64_t k0 = (int64_t)blockIdx.x * kGradRows + wv * 16;
+  // ((start + k) * stride) mod n: the product stays below 2^53, so its double
+  // is exact and the double quotient is off by at most one
+  const double dn = (double)n;
+  auto src = [&](int64_t k) {
+    const int64_t a = (start + k) * stride;
+    int64_t r = a - (int64_t)((double)a / dn) * n;
+    r += r < 0 ? n : 0;
+    r -= r >= n ? n : 0;
+    return r;
+  };
+
+  // X in the A layout (row col, features 4 ks + quad) and in the D layout
+  // (rows 4 quad + s, features 16 nt + col), with the targets of those rows
+  float xa[kFeat / 4], xd[2][4], td[4];
+  bool valid[4];
+  {
+    const int64_t ka = k0 + col;
+    const int64_t ia = ka < bs ? src(ka) : 0;
+#pragma unroll
+    for (int ks = 0; ks < kFeat / 4; ++ks) xa[ks] = ka < bs ? feat[ia * kFeat + 4 * ks + quad] : 0.f;
+#pragma unroll
+    for (int sr = 0; sr < 4; ++sr) {
+      const int64_t kd = k0 + 4 * quad + sr;
+      valid[sr] = kd < bs;
+      const int64_t id = valid[sr] ? src(kd) : 0;
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) xd[nt][sr] = valid[sr] ? feat[id * kFeat + 16 * nt + col] : 0.f;
+      td[sr] = valid[sr] ? target[id] : 0.f;
     }
-    out[pidx] = g;
+  }
+
+  // layer 1: H1 (D layout), passed to the A layout through the tile; the K
+  // loop is outermost so the four column tiles' MFMAs are independent
+  float h1[kHid / 16][4];
+  {
+    floatx4 acc[kHid / 16] = {};
+#pragma unroll
+    for (int ks = 0; ks < kFeat / 4; ++ks)
+#pragma unroll
+      for (int nt = 0; nt < kHid / 16; ++nt)
+        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+            xa[ks], lds[kB1 + (nt * (kFeat / 4) + ks) * 64 + lane], acc[nt], 0, 0, 0);
+#pragma unroll
+    for (int nt = 0; nt < kHid / 16; ++nt) {
+      const int j = 16 * nt + col;
+      const float bj = b1s[j];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        h1[nt][r] = fmaxf(acc[nt][r] + bj, 0.f);
+        tile[4 * quad + r][j] = h1[nt][r];
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  float ha[kHid / 4];
+#pragma unroll
+  for (int ks = 0; ks < kHid / 4; ++ks) ha[ks] = tile[col][4 * ks + quad];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  // layer 2 and the output
+  float h2[kHid / 16][4], w3[kHid / 16], part[4] = {0.f, 0.f, 0.f, 0.f};
+  {
+    floatx4 acc[kHid / 16] = {};
+#pragma unroll
+    for (int ks = 0; ks < kHid / 4; ++ks)
+#pragma unroll
+      for (int nt = 0; nt < kHid / 16; ++nt)
+        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+            ha[ks], lds[kB2 + (nt * (kHid / 4) + ks) * 64 + lane], acc[nt], 0, 0, 0);
+#pragma unroll
+    for (int nt = 0; nt < kHid / 16; ++nt) {
+      const int j = 16 * nt + col;
+      const float bj = b2s[j];
+      w3[nt] = w3s[j];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        h2[nt][r] = fmaxf(acc[nt][r] + bj, 0.f);
+        part[r] += w3[nt] * h2[nt][r];
+      }
+    }
+  }
+  float dy[4], loss = 0.f;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    float y = part[r];
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) y += __shfl_xor(y, o, kWave);  // the 16 lanes of the quad
+    const float err = valid[r] ? y + b3 - td[r] : 0.f;
+    dy[r] = 2.f * err * invn;
+    loss += err * err * invn;
+  }
+
+  // backward: dH2 (D layout), then through the tile to the A layout
+  float dh2[kHid / 16][4];
+#pragma unroll
+  for (int nt = 0; nt < kHid / 16; ++nt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      dh2[nt][r] = h2[nt][r] > 0.f ? dy[r] * w3[nt] : 0.f;
+      tile[4 * quad + r][16 * nt + col] = dh2[nt][r];
+    }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  float da[kHid / 4];
+#pragma unroll
+  for (int ks = 0; ks < kHid / 4; ++ks) da[ks] = tile[col][4 * ks + quad];
+  // dH1 = (dH2 W2) * [H1 > 0], with B[k = j][n = kk] = W2[j][kk]
+  float dh1[kHid / 16][4];
+  {
+    floatx4 acc[kHid / 16] = {};
+#pragma unroll
+    for (int ks = 0; ks < kHid / 4; ++ks)
+#pragma unroll
+      for (int nt = 0; nt < kHid / 16; ++nt)
+        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+            da[ks], lds[kB2t + (nt * (kHid / 4) + ks) * 64 + lane], acc[nt], 0, 0, 0);
+#pragma unroll
+    for (int nt = 0; nt < kHid / 16; ++nt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dh1[nt][r] = h1[nt][r] > 0.f ? acc[nt][r] : 0.f;
+  }
+
+  // weight gradients over the group's 16 rows (step s: rows 4 quad + s)
+  float gw2[kHid / 16][kHid / 16][4], gw1[kHid / 16][kFeat / 16][4];
+  {
+    // the step loop outermost: the 24 tiles' MFMAs are independent
+    floatx4 a2[kHid / 16][kHid / 16] = {}, a1[kHid / 16][kFeat / 16] = {};
+#pragma unroll
+    for (int sr = 0; sr < 4; ++sr)
+#pragma unroll
+      for (int mt = 0; mt < kHid / 16; ++mt) {
+#pragma unroll
+        for (int nt = 0; nt < kHid / 16; ++nt)
+          a2[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(dh2[mt][sr], h1[nt][sr], a2[mt][nt], 0, 0, 0);
+#pragma unroll
+        for (int nt = 0; nt < kFeat / 16; ++nt)
+          a1[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(dh1[mt][sr], xd[nt][sr], a1[mt][nt], 0, 0, 0);
+      }
+#pragma unroll
+    for (int mt = 0; mt < kHid / 16; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int nt = 0; nt < kHid / 16; ++nt) gw2[mt][nt][r] = a2[mt][nt][r];
+#pragma unroll
+        for (int nt = 0; nt < kFeat / 16; ++nt) gw1[mt][nt][r] = a1[mt][nt][r];
+      }
+  }
+  // biases, w3, b3 and the loss: sums over the rows (r, then the quads)
+  float gb1[kHid / 16], gb2[kHid / 16], gw3[kHid / 16];
+#pragma unroll
+  for (int nt = 0; nt < kHid / 16; ++nt) {
+    float s1 = 0.f, s2 = 0.f, s3 = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      s1 += dh1[nt][r];
+      s2 += dh2[nt][r];
+      s3 += dy[r] * h2[nt][r];
+    }
+#pragma unroll
+    for (int o = 16; o < 64; o <<= 1) {
+      s1 += __shfl_xor(s1, o, kWave);
+      s2 += __shfl_xor(s2, o, kWave);
+      s3 += __shfl_xor(s3, o, kWave);
+    }
+    gb1[nt] = s1;
+    gb2[nt] = s2;
+    gw3[nt] = s3;
+  }
+  float gb3 = dy[0] + dy[1] + dy[2] + dy[3];
+#pragma unroll
+  for (int o = 16; o < 64; o <<= 1) {
+    gb3 += __shfl_xor(gb3, o, kWave);
+    loss += __shfl_xor(loss, o, kWave);
+  }
+
+  // every entry of the record has one owning lane; the same visiting order
+  // serves the LDS exchange and the global store
+  auto visit = [&](auto &&f) __attribute__((always_inline)) {
+#pragma unroll
+    for (int mt = 0; mt < kHid / 16; ++mt) {
+#pragma unroll
+      for (int nt = 0; nt < kHid / 16; ++nt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          f(kOffW2 + (16 * mt + 4 * quad + r) * kHid + 16 * nt + col, gw2[mt][nt][r]);
+#pragma unroll
+      for (int nt = 0; nt < kFeat / 16; ++nt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          f(kOffW1 + (16 * mt + 4 * quad + r) * kFeat + 16 * nt + col, gw1[mt][nt][r]);
+    }
+    if (quad == 0) {
+#pragma unroll
+      for (int nt = 0; nt < kHid / 16; ++nt) {
+        f(kOffB1 + 16 * nt + col, gb1[nt]);
+        f(kOffB2 + 16 * nt + col, gb2[nt]);
+        f(kOffW3 + 16 * nt + col, gw3[nt]);
+      }
+      if (col == 0) {
+        f(kOffB3, gb3);
+        f(kParams, loss);
+      }
+    }
+  };
+  // each wave's record into its own LDS record, then the block sums them in a
+  // fixed tree order (four waves: (w0 + w2) + (w1 + w3)) and writes the
+  // partial record with consecutive threads on consecutive entries
+  __syncthreads();  // operands and tiles are done with: the records reuse the LDS
+  visit([&](int i, float &v) { lds[wv * kGradRec + i] = v; });
+  __syncthreads();
+  float *out = partial + (int64_t)blockIdx.x * kGradRec;
+  for (int i = threadIdx.x; i < kGradRec; i += 64 * kGradWaves) {
+    float v[kGradWaves];
+#pragma unroll
+    for (int w = 0; w < kGradWaves; ++w) v[w] = lds[w * kGradRec + i];
+#pragma unroll
+    for (int h = kGradWaves / 2; h >= 1; h /= 2)
+#pragma unroll
+      for (int w = 0; w < h; ++w) v[w] += v[w + h];
+    out[i] = v[0];
   }
 }
 
-__global__ __launch_bounds__(kT) void k_grad_reduce(const float *__restrict__ partial, int nblocks,
+// Sum of the partial records of nb blocks in a fixed order: thread (p, g) of
+// a block sums records g, g + 16, ... of entry p (64 consecutive entries per
+// wave, so each record is read in whole lines; eight loads in flight per
+// thread), then the 16 sums are added in order.  With ADAM the sum is applied as one Adam step to the
+// parameters; otherwise it is stored as the gradient (and the loss).
+constexpr int kRedP = 64, kRedG = 16;
+template <bool ADAM>
+__global__ __launch_bounds__(kRedP * kRedG) void k_grad_reduce(const float *__restrict__ partial, int nb,
                                                     float *__restrict__ grad,
-                                                    float *__restrict__ loss) {
-  const int pidx = blockIdx.x * kT + threadIdx.x;
-  if (pidx > kParams) return;
-  float g = 0.f;
-  for (int b = 0; b < nblocks; ++b) g += partial[(int64_t)b * (kParams + 1) + pidx];
-  if (pidx < kParams) grad[pidx] = g;
-  else *loss = g;
-}
-
-__global__ __launch_bounds__(kT) void k_adam(float *__restrict__ w, const float *__restrict__ g,
-                                             float *__restrict__ m, float *__restrict__ v, float lr,
-                                             float c1, float c2) {
-  const int i = blockIdx.x * kT + threadIdx.x;
-  if (i >= kParams) return;
+                                                    float *__restrict__ loss,
+                                                    float *__restrict__ w, float *__restrict__ m,
+                                                    float *__restrict__ v, float lr, float c1,
+                                                    float c2) {
+  __shared__ float sums[kRedG][kRedP];
+  const int pl = threadIdx.x % kRedP, g = threadIdx.x / kRedP;
+  const int p = blockIdx.x * kRedP + pl;
+  float s = 0.f;
+  if (p < kGradRec) {
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int b = g;
+    for (; b + 7 * kRedG < nb; b += 8 * kRedG) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] += partial[(int64_t)(b + u * kRedG) * kGradRec + p];
+    }
+    for (int u = 0; b < nb; b += kRedG, ++u) acc[u & 7] += partial[(int64_t)b * kGradRec + p];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += acc[u];
+  }
+  sums[g][pl] = s;
+  __syncthreads();
+  if (g != 0 || p >= kGradRec) return;
+  float gi = sums[0][pl];
+  for (int k = 1; k < kRedG; ++k) gi += sums[k][pl];
+  if (!ADAM) {
+    if (p < kParams) grad[p] = gi;
+    else *loss = gi;
+    return;
+  }
+  if (p >= kParams) return;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.f - b1) * gi;
-  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  const float mi = b1 * m[p] + (1.f - b1) * gi;
+  const float vi = b2 * v[p] + (1.f - b2) * gi * gi;
+  m[p] = mi;
+  v[p] = vi;
+  w[p] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
 __global__ __launch_bounds__(kT) void k_caps(const float *__restrict__ pred,
@@ -775,21 +1009,33 @@ void launch_targets(const uint32_t *counts, int64_t n, float *targets, hipStream
   STQ_HIP(hipGetLastError());
 }
 
+size_t mlp_partial_floats(int64_t bs) {
+  return (size_t)((bs + kGradRows - 1) / kGradRows) * kGradRec;
+}
+
 void launch_mlp_gradient(const float *params, const float *feat, const float *targets, int64_t n,
                          float *partial, float *grad, float *loss, hipStream_t st) {
   if (n <= 0) return;
-  const int nb = (int)((n + kGradItems - 1) / kGradItems);
-  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(64), 0, st, params, feat, targets, n, partial);
-  hipLaunchKernelGGL(k_grad_reduce, dim3((kParams + 1 + kT - 1) / kT), dim3(kT), 0, st,
-                     (const float *)partial, nb, grad, loss);
+  const int nb = (int)((n + kGradRows - 1) / kGradRows);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(64 * kGradWaves), 0, st, params, feat, targets, n,
+                     (int64_t)0,
+                     (int64_t)1, n, partial);
+  hipLaunchKernelGGL(k_grad_reduce<false>, dim3((kGradRec + kRedP - 1) / kRedP), dim3(kRedP * kRedG), 0, st,
+                     (const float *)partial, nb, grad, loss, nullptr, nullptr, nullptr, 0.f, 1.f, 1.f);
   STQ_HIP(hipGetLastError());
 }
 
-void launch_adam(float *params, const float *grad, float *m, float *v, float lr, int t,
-                 hipStream_t st) {
+void launch_train_step(float *params, float *m, float *v, const float *feat, const float *targets,
+                       int64_t n, int64_t start, int64_t stride, int64_t bs, float *partial, float lr,
+                       int t, hipStream_t st) {
+  if (n <= 0 || bs <= 0) return;
+  const int nb = (int)((bs + kGradRows - 1) / kGradRows);
   const float c1 = 1.f - powf(0.9f, (float)t), c2 = 1.f - powf(0.999f, (float)t);
-  hipLaunchKernelGGL(k_adam, dim3((kParams + kT - 1) / kT), dim3(kT), 0, st, params, grad, m, v,
-                     lr, c1, c2);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(64 * kGradWaves), 0, st, (const float *)params, feat,
+                     targets, n,
+                     start % n, stride, bs, partial);
+  hipLaunchKernelGGL(k_grad_reduce<true>, dim3((kGradRec + kRedP - 1) / kRedP), dim3(kRedP * kRedG), 0, st,
+                     (const float *)partial, nb, nullptr, nullptr, params, m, v, lr, c1, c2);
   STQ_HIP(hipGetLastError());
 }
 

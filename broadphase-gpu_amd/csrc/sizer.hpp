@@ -51,12 +51,15 @@ void launch_cap_budget(uint32_t *caps, const int64_t *cap_off, int64_t n, int64_
 // targets[i] = log1p(counts[i]).
 void launch_targets(const uint32_t *counts, int64_t n, float *targets, hipStream_t st);
 // Mean squared error of MLP(feat) against targets and its gradient (kParams
-// floats), reduced in a fixed order; partial needs ceil(n/64) * (kParams+1).
+// floats), reduced in a fixed order; partial needs mlp_partial_floats(n).
 void launch_mlp_gradient(const float *params, const float *feat, const float *targets, int64_t n,
                          float *partial, float *grad, float *loss, hipStream_t st);
-// One Adam step (t is the 1-based step number).
-void launch_adam(float *params, const float *grad, float *m, float *v, float lr, int t,
-                 hipStream_t st);
+size_t mlp_partial_floats(int64_t bs);
+// One Adam step (t is the 1-based step number) on the minibatch of bs rows
+// whose row k is source row ((start + k) * stride) mod n of feat/targets.
+void launch_train_step(float *params, float *m, float *v, const float *feat, const float *targets,
+                       int64_t n, int64_t start, int64_t stride, int64_t bs, float *partial, float lr,
+                       int t, hipStream_t st);
 // sum |expm1(pred) - counts| over items, added to *acc.
 void launch_abs_error(const float *pred, const uint32_t *counts, int64_t n, double *acc,
                       hipStream_t st, const int64_t *n_dev = nullptr);

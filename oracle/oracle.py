@@ -47,6 +47,8 @@ def lib():
         L.oracle_candidates.restype = I64
         L.oracle_candidates.argtypes = [P, I64, ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(I64)]
+        L.oracle_sort_u64.restype = None
+        L.oracle_sort_u64.argtypes = [P, I64, ctypes.c_int]
         L.oracle_canonicalize.restype = I64
         L.oracle_canonicalize.argtypes = [P, I64]
         L.oracle_round_outward.restype = None
@@ -132,6 +134,13 @@ def canonical(pairs):
     p = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2)).copy()
     m = lib().oracle_canonicalize(p.ctypes.data, len(p))
     return p[:m].copy()
+
+
+def sort_u64(keys, nthreads=None):
+    """Ascending copy of uint64 keys (the radix sort under canonical())."""
+    k = np.ascontiguousarray(np.asarray(keys, dtype=np.uint64)).copy()
+    lib().oracle_sort_u64(k.ctypes.data, len(k), nthreads or default_threads())
+    return k
 
 
 def round_outward(values):

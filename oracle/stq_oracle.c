@@ -298,28 +298,71 @@ int64_t oracle_bruteforce(const oracle_aabb *boxes, int64_t n, int nthreads,
   return total;
 }
 
-/* ---- canonical form: (min,max) orientation, lexicographic sort, unique ----- */
-static int pair_cmp(const void *x, const void *y) {
-  const int32_t *a = (const int32_t *)x, *b = (const int32_t *)y;
-  if (a[0] != b[0]) return a[0] < b[0] ? -1 : 1;
-  if (a[1] != b[1]) return a[1] < b[1] ? -1 : 1;
-  return 0;
+/* ---- canonical form: (min,max) orientation, lexicographic sort, unique -----
+   Each pair becomes the 64-bit key (min, max), each half with its sign bit
+   flipped so unsigned order is the signed lexicographic order; the keys are
+   sorted by an LSD radix sort (four 16-bit digits, per-thread histograms) and
+   written back without repeats.  Pinned against numpy's sort in
+   tests/test_oracle.py. */
+static inline uint64_t pair_key(int32_t a, int32_t b) {
+  const int32_t lo = a < b ? a : b, hi = a < b ? b : a;
+  return ((uint64_t)((uint32_t)lo ^ 0x80000000u) << 32) | (uint64_t)((uint32_t)hi ^ 0x80000000u);
+}
+void oracle_sort_u64(uint64_t *keys, int64_t n, int nthreads) {
+  if (n < 2) return;
+  if (nthreads < 1) nthreads = 1;
+  uint64_t *tmp = (uint64_t *)malloc((size_t)n * sizeof(uint64_t));
+  int64_t *hist = (int64_t *)malloc((size_t)nthreads * 65536 * sizeof(int64_t));
+  uint64_t *src = keys, *dst = tmp;
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = 16 * pass;
+#pragma omp parallel num_threads(nthreads)
+    {
+      const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+      int64_t *h = hist + (size_t)t * 65536;
+      memset(h, 0, 65536 * sizeof(int64_t));
+      const int64_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
+      for (int64_t i = i0; i < i1; ++i) h[(src[i] >> shift) & 0xffff]++;
+#pragma omp barrier
+#pragma omp single
+      {
+        /* digit-major, thread-minor exclusive offsets: a stable scatter */
+        int64_t acc = 0;
+        for (int d = 0; d < 65536; ++d)
+          for (int u = 0; u < nt; ++u) {
+            const int64_t c = hist[(size_t)u * 65536 + d];
+            hist[(size_t)u * 65536 + d] = acc;
+            acc += c;
+          }
+      }
+      for (int64_t i = i0; i < i1; ++i) dst[h[(src[i] >> shift) & 0xffff]++] = src[i];
+    }
+    uint64_t *x = src;
+    src = dst;
+    dst = x;
+  }
+  /* four passes: the sorted keys are back in `keys` */
+  free(tmp);
+  free(hist);
 }
 int64_t oracle_canonicalize(int32_t *pairs, int64_t n) {
-  for (int64_t i = 0; i < n; ++i) {
-    int32_t a = pairs[2 * i], b = pairs[2 * i + 1];
-    if (a > b) { pairs[2 * i] = b; pairs[2 * i + 1] = a; }
-  }
-  qsort(pairs, (size_t)n, 2 * sizeof(int32_t), pair_cmp);
+  if (n <= 0) return 0;
+  int nthreads = omp_get_max_threads();
+  if (nthreads > 64) nthreads = 64;
+  uint64_t *k = (uint64_t *)malloc((size_t)n * sizeof(uint64_t));
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int64_t i = 0; i < n; ++i) k[i] = pair_key(pairs[2 * i], pairs[2 * i + 1]);
+  oracle_sort_u64(k, n, n < 65536 ? 1 : nthreads);
   int64_t m = 0;
   for (int64_t i = 0; i < n; ++i) {
-    if (m > 0 && pairs[2 * (m - 1)] == pairs[2 * i] &&
-        pairs[2 * (m - 1) + 1] == pairs[2 * i + 1])
-      continue;
-    pairs[2 * m] = pairs[2 * i];
-    pairs[2 * m + 1] = pairs[2 * i + 1];
-    ++m;
+    if (m > 0 && k[m - 1] == k[i]) continue;
+    k[m++] = k[i];
   }
+  for (int64_t i = 0; i < m; ++i) {
+    pairs[2 * i] = (int32_t)((uint32_t)(k[i] >> 32) ^ 0x80000000u);
+    pairs[2 * i + 1] = (int32_t)((uint32_t)k[i] ^ 0x80000000u);
+  }
+  free(k);
   return m;
 }
 

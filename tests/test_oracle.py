@@ -263,3 +263,20 @@ def test_orient_pairs_known_answers():
     for bad in ([[0, 4]], [[1, 5]], [[0, 2]], [[2, 5]], [[0, 6]], [[-1, 0]]):
         with pytest.raises(ValueError):
             oracle.orient_pairs(b, bad)
+
+
+def test_radix_canonical_form_matches_numpy():
+    """oracle.canonical (the radix sort under every set comparison) against
+    numpy's sort and unique: negative ids, reversed pairs, repeats, sizes on
+    both sides of the threaded path."""
+    rng = np.random.default_rng(31)
+    for n in (0, 1, 7, 65535, 65536, 300_001):
+        k = rng.integers(0, 2 ** 64 - 1, n, dtype=np.uint64)
+        assert np.array_equal(oracle.sort_u64(k), np.sort(k)), n
+        p = rng.integers(-2 ** 31, 2 ** 31 - 1, (n, 2)).astype(np.int32)
+        if n > 10:
+            p[: n // 3] = p[n // 3: 2 * (n // 3)][:, ::-1]  # the same pairs reversed
+        a = np.minimum(p[:, 0], p[:, 1]).astype(np.int64)
+        b = np.maximum(p[:, 0], p[:, 1]).astype(np.int64)
+        ref = np.unique(np.stack([a, b], 1), axis=0).astype(np.int32).reshape(-1, 2)
+        assert np.array_equal(oracle.canonical(p), ref), n
