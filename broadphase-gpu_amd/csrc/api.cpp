@@ -26,7 +26,7 @@
 namespace stqamd {
 
 static bool options_valid(const stq_options &o) {
-  return o.precision >= 0 && o.precision <= 1 && o.axis_policy >= 0 && o.axis_policy <= 4 &&
+  return o.scratch_limit_mb >= 0 && o.precision >= 0 && o.precision <= 1 && o.axis_policy >= 0 && o.axis_policy <= 4 &&
          o.sizing >= 0 && o.sizing <= 1 && o.chunk >= 0 && o.window >= 0 &&
          (o.window == 0 || (o.window >= 64 && o.window <= kMaxWindow32 &&
                             (o.window & (o.window - 1)) == 0)) &&
@@ -346,16 +346,17 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
   launch_mlp_forward(w, d_feat, nitems, d_pred, st);
   launch_caps(d_pred, d_cand, nitems, kCapScale, kCapSlack, d_caps, st);
   scan_u32_to_i64(d_caps, d_capoff, nitems, scan_tmp, st);
-  // The scratch buffer holds a budget of slots kept across calls: items whose
-  // slots would pass it get none and are redone by the exact refill, and the
-  // next call gets room for every slot and half as many again.  So the pass
-  // starts without waiting for the slot total on the host.
-  if (s_budget == 0)
-    s_budget = std::max<int64_t>(int64_t(1) << 22, (int64_t)kCapScale * s_last_pairs +
-                                                       (int64_t)kCapSlack * nitems);
-  launch_cap_budget(d_caps, d_capoff, nitems, s_budget, st);
-  int2 *d_scratch = scratch_pairs.as<int2>((size_t)s_budget);
+  // The scratch buffer holds exactly the predicted slots (their total is read
+  // back once), so an item is redone by the exact refill only when it has
+  // more pairs than its prediction allows -- or, with options.scratch_limit_mb,
+  // when its slots end beyond that limit.
   int64_t *sh = small_host();
+  STQ_HIP(hipMemcpyAsync(sh + 9, d_capoff + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  STQ_HIP(hipStreamSynchronize(st));
+  const int64_t slots = sh[9];
+  const int64_t budget = std::min<int64_t>(std::max<int64_t>(slots, 1), scratch_limit_pairs());
+  if (budget < slots) launch_cap_budget(d_caps, d_capoff, nitems, budget, st);
+  int2 *d_scratch = scratch_pairs.as<int2>((size_t)budget);
   STQ_HIP(hipEventRecord(ev[E_PRED], st));
   STQ_HIP(hipEventRecord(ev[E_COUNT], st));
   uint32_t *d_counts = sa.counts;
@@ -403,12 +404,8 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
   STQ_HIP(hipEventRecord(ev[E_COMPACT], st));
   STQ_HIP(hipEventRecord(ev[E_END], st));
   STQ_HIP(hipMemcpyAsync(sh + 8, acc, sizeof(double), hipMemcpyDeviceToHost, st));
-  STQ_HIP(hipMemcpyAsync(sh + 9, d_capoff + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   finish_stats(st);
-  const int64_t total = sh[6], want = sh[9];
-  // geometric growth: a stream of ever larger scenes overflows the budget a
-  // few times, not at every new largest scene
-  if (want > s_budget) s_budget = want + want / 2;
+  const int64_t total = sh[6];
   *count = total;
   stats.n_pairs = total;
   s_last_pairs = total;
@@ -633,6 +630,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     b_budget = std::max<int64_t>(int64_t(1) << 22,
                                  (int64_t)kBatchCapScale * b_last_pairs +
                                      (int64_t)kBatchCapSlack * b_last_items);
+  if (single_pass) b_budget = std::min(b_budget, scratch_limit_pairs());
 
   // every buffer is reserved here: the launches below allocate nothing
   const size_t ts = f32 ? 4 : 8;
@@ -890,7 +888,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     // predicted slots beyond the scratch were redone exactly; the next call
     // gets room for all of them (a new buffer, hence a new graph)
     const int64_t want = h_ctr[6];
-    if (want > b_budget) b_budget = want + want / 8;
+    if (want > b_budget) b_budget = std::min(want + want / 8, scratch_limit_pairs());
   }
   if (training && nitems > 0) {
     // error of the prediction before training, then a few training rounds

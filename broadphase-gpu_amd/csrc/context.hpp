@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <string>
@@ -173,7 +174,11 @@ struct Context {
   }
   BatchPlan bplan;
   int64_t b_budget = 0;      // scratch pairs of the learned batched pass
-  int64_t s_budget = 0;      // scratch pairs of the learned single-scene pass
+  // Most scratch pairs of a learned pass under options.scratch_limit_mb.
+  int64_t scratch_limit_pairs() const {
+    return opt.scratch_limit_mb > 0 ? std::max<int64_t>(((int64_t)opt.scratch_limit_mb << 20) / 8, 1)
+                                    : INT64_MAX;
+  }
   int64_t s_last_pairs = 0;  // pairs of the last single-scene call
   int64_t b_last_pairs = 0;  // pairs and items of the last batched call
   int64_t b_last_items = 0;

@@ -73,7 +73,8 @@ class Options(ctypes.Structure):
         ("claim", ctypes.c_int32),
         ("tails", ctypes.c_int32),
         ("tail_cost", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("scratch_limit_mb", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

@@ -111,9 +111,11 @@ enum stq_sizing {
      double pass, src/simulation.cu:451-489) */
   STQ_SIZING_COUNT_FILL = 0,
   /* one sweep pass into per-tile buffers sized by the learned per-tile pair
-     predictor; tiles that overflow are redone by the exact count pass.  The
-     buffers come from a scratch budget the context keeps (and grows) across
-     calls; tiles past it are redone too, so the result never depends on it */
+     predictor; tiles that overflow are redone by the exact count pass.  One
+     scene: the scratch holds exactly the predicted slots.  Batched scenes:
+     the buffers come from a scratch budget the context keeps (and grows)
+     across calls, so a call needs no host read-back; tiles past it are redone
+     too, so the result never depends on it */
   STQ_SIZING_LEARNED = 1
 };
 
@@ -151,7 +153,12 @@ typedef struct stq_options {
   int32_t tail_cost;     /* weight of a tail (row, position) test
                             against a body lane step, in percent;
                             0 = 400                                 */
-  int32_t reserved[2];
+  int32_t scratch_limit_mb; /* learned sizing: most MiB of the pair
+                            scratch (0 = no limit); items whose
+                            predicted slots end beyond it are redone
+                            by the exact refill, so the pairs do not
+                            depend on it                           */
+  int32_t reserved;
 } stq_options;
 
 typedef struct stq_stats {

@@ -103,30 +103,30 @@ def test_forced_fallback_is_exact(ctx):
     assert np.array_equal(ctx.find_pairs(b), got)
 
 
-def test_scratch_budget_overflow_is_exact():
-    """The single learned pass holds its slots in a scratch budget kept across
-    calls, so it starts without reading the slot total back.  On a fresh
-    context (no earlier call) the first budget, max(4M, 256 per item) slots,
-    is too small for predictions of about 400 pairs per item on a 540k-box
-    cloth: the items past it get no slots and are redone exactly; the next
-    call has room for all.  Both outputs equal the count-then-fill path's,
-    order included."""
+def test_scratch_limit_overflow_is_exact():
+    """One scene's learned pass holds exactly its predicted slots; with
+    options.scratch_limit_mb below that, the items whose slots end beyond the
+    limit get none and are redone exactly.  Outputs with and without the
+    limit equal the count-then-fill path's, order included."""
     c = stq_amd.Context(0)
     try:
         w = np.zeros(mlp.NPARAMS, np.float32)
         w[-1] = 6.0  # log1p of the prediction: about 400 pairs per item
         c.sizer_set_params(w)
         b = scenes.cloth_scene(g=300, seed=8, sigma=2e-3)
-        c.set_options(precision=stq_amd.PRECISION_F32_CONFIRM, sizing=stq_amd.SIZING_LEARNED, train_scenes=-1)
-        first = c.find_pairs(b)
-        f1 = c.stats()["n_fallback_tiles"]
-        second = c.find_pairs(b)
-        f2 = c.stats()["n_fallback_tiles"]
+        kw = dict(precision=stq_amd.PRECISION_F32_CONFIRM, sizing=stq_amd.SIZING_LEARNED, train_scenes=-1)
+        c.set_options(**kw)
+        free = c.find_pairs(b)
+        f_free = c.stats()["n_fallback_tiles"]
+        c.set_options(**kw, scratch_limit_mb=8)  # one million pair slots
+        limited = c.find_pairs(b)
+        f_lim = c.stats()["n_fallback_tiles"]
         c.set_options(precision=stq_amd.PRECISION_F32_CONFIRM)
         ref = c.find_pairs(b)
-        assert np.array_equal(first, ref)
-        assert np.array_equal(second, ref)
-        assert f1 > f2
+        print(f"fallback items: {f_free} without a limit, {f_lim} with 8 MiB", flush=True)
+        assert np.array_equal(free, ref)
+        assert np.array_equal(limited, ref)
+        assert f_lim > f_free
     finally:
         c.close()
 
