@@ -530,7 +530,8 @@ void Context::batch_plan(const int64_t *off, int nscenes, int W, int chunk) {
   }
   bp.item_bound = bound;
 
-  const size_t nc = t.chunk_start.size(), nw = t.win_start.size(), ns1 = (size_t)nscenes + 1;
+  const size_t nc = t.chunk_start.size(), nw = t.win_start.size(), ns1 = (size_t)nscenes + 1,
+               nst = t.sort_tiles.size();
   size_t at = 0;
   auto place = [&](size_t bytes) {
     const size_t o = at;
@@ -539,7 +540,8 @@ void Context::batch_plan(const int64_t *off, int nscenes, int W, int chunk) {
   };
   const size_t o_seg = place(8 * ns1), o_cs = place(8 * nc), o_ce = place(8 * nc),
                o_scb = place(4 * ns1), o_ws = place(8 * nw), o_wc = place(4 * nw),
-               o_wg = place(8 * nw), o_sfg = place(8 * ns1), upload = at;
+               o_wg = place(8 * nw), o_sfg = place(8 * ns1),
+               o_st = place(sizeof(SegTile) * std::max<size_t>(nst, 1)), upload = at;
   const size_t o_par = place(sizeof(SceneParams) * (size_t)nscenes),
                o_part = place(sizeof(double) * kAxisStats * std::max<size_t>(nc, 1));
   std::vector<char> h(upload);
@@ -554,6 +556,7 @@ void Context::batch_plan(const int64_t *off, int nscenes, int W, int chunk) {
   put(o_wc, t.win_cnt.data(), 4 * nw);
   put(o_wg, t.win_gbase.data(), 8 * nw);
   put(o_sfg, t.scene_first_group.data(), 8 * ns1);
+  put(o_st, t.sort_tiles.data(), sizeof(SegTile) * nst);
   char *d = static_cast<char *>(b_tables.get(at));
   STQ_HIP(hipMemcpyAsync(d, h.data(), upload, hipMemcpyHostToDevice, stream));
   STQ_HIP(hipStreamSynchronize(stream));
@@ -566,6 +569,7 @@ void Context::batch_plan(const int64_t *off, int nscenes, int W, int chunk) {
   dv.win_cnt = reinterpret_cast<int *>(d + o_wc);
   dv.win_gbase = reinterpret_cast<int64_t *>(d + o_wg);
   dv.scene_first_group = reinterpret_cast<int64_t *>(d + o_sfg);
+  dv.sort_tiles = reinterpret_cast<SegTile *>(d + o_st);
   dv.params = reinterpret_cast<SceneParams *>(d + o_par);
   dv.partial = reinterpret_cast<double *>(d + o_part);
   bp.offsets.assign(off, off + nscenes + 1);
@@ -634,10 +638,13 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
 
   // every buffer is reserved here: the launches below allocate nothing
   const size_t ts = f32 ? 4 : 8;
-  uint64_t *d_keys = keys.as<uint64_t>((size_t)n);
+  void *d_keys = keys.get((size_t)n * (f32 ? 4 : 8));
   uint32_t *d_vals = vals.as<uint32_t>((size_t)n);
   uint32_t *d_skey = f32 ? nullptr : b_skey.as<uint32_t>((size_t)n);
-  radix_sort_reserve(n, 8, sort);
+  if (f32)
+    radix_sort_segmented_reserve(n, (int64_t)t.sort_tiles.size(), sort);
+  else
+    radix_sort_reserve(n, 8, sort);
   void *d_smin = smin.get((size_t)n * ts);
   void *d_smax = smax.get((size_t)n * ts);
   void *d_rec = geo.get((size_t)n * (f32 ? sizeof(Exact64) : sizeof(Rec64)));
@@ -697,9 +704,11 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
                     st);
     mark(E_KEYS);
     if (f32) {
-      radix_sort_u64_sync_free(d_keys, d_vals, n, 32 + sbits, sort, st);
+      // FP32 keys sorted within each scene (the scene order is kept)
+      radix_sort_u32_segmented(static_cast<uint32_t *>(d_keys), d_vals, n, 32, bp.dev.sort_tiles,
+                               (int64_t)t.sort_tiles.size(), sort, st);
     } else {
-      radix_sort_u64_sync_free(d_keys, d_vals, n, 64, sort, st);
+      radix_sort_u64_sync_free(static_cast<uint64_t *>(d_keys), d_vals, n, 64, sort, st);
       if (sbits > 0) {
         launch_scene_keys(d_vals, t, bp.dev, d_skey, st);
         radix_sort_u32_sync_free(d_skey, d_vals, n, sbits, sort, st);

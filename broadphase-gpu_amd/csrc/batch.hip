@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kT) void k_seg_finalize(const double *__restrict__ 
 __global__ __launch_bounds__(kT) void k_keys_seg(const stq_aabb_f64 *__restrict__ boxes, int64_t n,
                                                  const int64_t *__restrict__ seg_off, int nscenes,
                                                  const SceneParams *__restrict__ sp, int f32,
-                                                 uint64_t *__restrict__ keys,
+                                                 void *__restrict__ keys,
                                                  uint32_t *__restrict__ vals,
                                                  unsigned *__restrict__ err) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
@@ -98,7 +98,8 @@ __global__ __launch_bounds__(kT) void k_keys_seg(const stq_aabb_f64 *__restrict_
   const double y0 = b.max[0], y1 = b.max[1], y2 = b.max[2];
   if (!(x0 <= y0) || !(x1 <= y1) || !(x2 <= y2)) atomicOr(err, kErrInverted);
   const double m = axis == 0 ? x0 : (axis == 1 ? x1 : x2);
-  keys[i] = f32 ? (((uint64_t)(uint32_t)s << 32) | key_of(round_down_f32(m))) : key_of(m);
+  if (f32) static_cast<uint32_t *>(keys)[i] = key_of(round_down_f32(m));
+  else static_cast<uint64_t *>(keys)[i] = key_of(m);
   vals[i] = (uint32_t)i;
 }
 
@@ -294,6 +295,20 @@ void build_batch_tables(const int64_t *off, int nscenes, int W, BatchTables &t) 
   t.win_cnt.clear();
   t.win_gbase.clear();
   t.scene_first_group.assign(nscenes + 1, 0);
+  t.sort_tiles.clear();
+  const int tk = sort_tile_keys();
+  for (int s = 0; s < nscenes; ++s) {
+    const int64_t a = off[s], m = off[s + 1] - off[s];
+    const int64_t nt = (m + tk - 1) / tk, t0 = (int64_t)t.sort_tiles.size();
+    for (int64_t i = 0; i < nt; ++i) {
+      SegTile x;
+      x.start = a + i * tk;
+      x.count = (int32_t)std::min<int64_t>(tk, m - i * tk);
+      x.cbase = 256 * t0 + i;
+      x.stride = (int32_t)nt;
+      t.sort_tiles.push_back(x);
+    }
+  }
   int64_t g = 0;
   const int64_t kStatChunk = 8192;  // 32 boxes per thread of a statistics block
   for (int s = 0; s < nscenes; ++s) {
@@ -334,7 +349,7 @@ void launch_seg_stats(const stq_aabb_f64 *boxes, const BatchTables &t, const Bat
 }
 
 void launch_keys_seg(bool f32, const stq_aabb_f64 *boxes, const BatchTables &t, const BatchDev &d,
-                     uint64_t *keys, uint32_t *vals, unsigned *err, hipStream_t st) {
+                     void *keys, uint32_t *vals, unsigned *err, hipStream_t st) {
   if (t.n <= 0) return;
   hipLaunchKernelGGL(k_keys_seg, dim3(nblk(t.n)), dim3(kT), 0, st, boxes, t.n, d.seg_off, t.nscenes,
                      (const SceneParams *)d.params, f32 ? 1 : 0, keys, vals, err);

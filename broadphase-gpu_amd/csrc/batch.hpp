@@ -26,6 +26,7 @@ struct BatchTables {
   std::vector<int64_t> win_gbase;      // first group of each window
   std::vector<int64_t> scene_first_group;  // nscenes + 1 (a group id, or ngroups)
   int64_t ngroups = 0;
+  std::vector<SegTile> sort_tiles;     // tiles of the segmented key sort
 };
 void build_batch_tables(const int64_t *h_scene_offsets, int nscenes, int W, BatchTables &t);
 
@@ -41,14 +42,16 @@ struct BatchDev {
   int64_t *scene_first_group = nullptr;
   SceneParams *params = nullptr;
   double *partial = nullptr;  // nchunks * kAxisStats
+  SegTile *sort_tiles = nullptr;
 };
 
 void launch_seg_stats(const stq_aabb_f64 *boxes, const BatchTables &t, const BatchDev &d,
                       int policy, int fixed_axis, bool need_q, hipStream_t st);
-// F32 form: key = scene << 32 | image of the rounded-down minimum; F64 form:
-// key = image of the FP64 minimum (a second, stable sort by scene follows).
+// F32 form: 4-byte keys, the image of the rounded-down minimum, sorted per
+// scene by the segmented sort; F64 form: 8-byte keys, the image of the FP64
+// minimum (a second, stable sort by scene follows).
 void launch_keys_seg(bool f32, const stq_aabb_f64 *boxes, const BatchTables &t, const BatchDev &d,
-                     uint64_t *keys, uint32_t *vals, unsigned *err, hipStream_t st);
+                     void *keys, uint32_t *vals, unsigned *err, hipStream_t st);
 // scene_key[s] = scene of input position vals[s] (for the F64 second sort)
 void launch_scene_keys(const uint32_t *vals, const BatchTables &t, const BatchDev &d,
                        uint32_t *scene_key, hipStream_t st);

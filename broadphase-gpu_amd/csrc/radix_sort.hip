@@ -52,23 +52,50 @@ __global__ __launch_bounds__(kSortThreads) void k_hist_all(const K *__restrict__
 }
 
 // ---- per-tile digit counts --------------------------------------------------
+// Tile t covers keys [t * kTile, min(n, (t + 1) * kTile)) and its count of
+// digit d goes to counts[d * ntiles + t]; with a segment table (SEG) tile t is
+// tiles[t] and its count goes to counts[cbase + d * stride], so one exclusive
+// scan over the counts gives every (tile, digit) its place inside the tile's
+// own segment (segment-major, then digit, then tile).
+__device__ __forceinline__ void tile_of(const SegTile *tiles, int64_t n, int64_t &base, int &cnt) {
+  if (tiles) {
+    const SegTile t = tiles[blockIdx.x];
+    base = t.start;
+    cnt = t.count;
+  } else {
+    base = (int64_t)blockIdx.x * kTile;
+    const int64_t rem = n - base;
+    cnt = rem < kTile ? (int)rem : kTile;
+  }
+}
+__device__ __forceinline__ int64_t count_index(const SegTile *tiles, int ntiles, int d) {
+  if (tiles) {
+    const SegTile t = tiles[blockIdx.x];
+    return t.cbase + (int64_t)d * t.stride;
+  }
+  return (int64_t)d * ntiles + blockIdx.x;
+}
+
 template <class K>
 __global__ __launch_bounds__(kSortThreads) void k_count(const K *__restrict__ keys, int64_t n,
                                                          int shift, int ntiles,
                                                          unsigned *__restrict__ counts,
-                                                         const unsigned *__restrict__ skip) {
+                                                         const unsigned *__restrict__ skip,
+                                                         const SegTile *__restrict__ tiles) {
   if (skip && skip[shift >> 3]) return;
   __shared__ unsigned h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kTile;
+  int64_t base;
+  int cnt;
+  tile_of(tiles, n, base, cnt);
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    int64_t i = base + r * kSortThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[(unsigned)(keys[i] >> shift) & 255u], 1u);
+    const int j = r * kSortThreads + threadIdx.x;
+    if (j < cnt) atomicAdd(&h[(unsigned)(keys[base + j] >> shift) & 255u], 1u);
   }
   __syncthreads();
-  counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+  counts[count_index(tiles, ntiles, threadIdx.x)] = h[threadIdx.x];
 }
 
 // ---- stable scatter -----------------------------------------------------------
@@ -79,15 +106,17 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const K *__restrict__ 
                                                            uint32_t *__restrict__ vout, int64_t n,
                                                            int shift, int ntiles,
                                                            const unsigned *__restrict__ offsets,
-                                                           const unsigned *__restrict__ skip) {
+                                                           const unsigned *__restrict__ skip,
+                                                           const SegTile *__restrict__ tiles) {
+  int64_t tile0;
+  int tcnt;
+  tile_of(tiles, n, tile0, tcnt);
   if (skip && skip[shift >> 3]) {
     // the digit is equal for every key: the stable order is the identity
-    const int64_t b = (int64_t)blockIdx.x * kTile;
-    for (int j = threadIdx.x; j < kTile; j += kSortThreads)
-      if (b + j < n) {
-        kout[b + j] = kin[b + j];
-        vout[b + j] = vin[b + j];
-      }
+    for (int j = threadIdx.x; j < tcnt; j += kSortThreads) {
+      kout[tile0 + j] = kin[tile0 + j];
+      vout[tile0 + j] = vin[tile0 + j];
+    }
     return;
   }
   __shared__ K sk[kTile];
@@ -103,15 +132,16 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const K *__restrict__ 
   __syncthreads();
 
   // Wave w ranks tile positions [w*1024, (w+1)*1024) in 16 rounds of 64.
-  const int64_t tbase = (int64_t)blockIdx.x * kTile + (int64_t)w * (kTile / kWavesPerBlock);
+  const int wbase = w * (kTile / kWavesPerBlock);
   K key[kSortItems];
   uint32_t val[kSortItems];
   unsigned rank[kSortItems];
   const uint64_t lt = lanemask_lt();
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const int64_t i = tbase + r * kWave + lane;
-    const bool valid = i < n;
+    const int j = wbase + r * kWave + lane;
+    const bool valid = j < tcnt;
+    const int64_t i = tile0 + j;
     key[r] = valid ? kin[i] : (K)0;
     val[r] = valid ? vin[i] : 0u;
     const unsigned dig = (unsigned)(key[r] >> shift) & 255u;
@@ -153,7 +183,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const K *__restrict__ 
     unsigned add = 0;
     for (int k = 0; k < w; ++k) add += wsum[k];
     tstart[t] = x - tot + add;
-    gbase[t] = offsets[(int64_t)t * ntiles + blockIdx.x];
+    gbase[t] = offsets[count_index(tiles, ntiles, t)];
   }
   __syncthreads();
 #pragma unroll
@@ -166,9 +196,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const K *__restrict__ 
     }
   }
   __syncthreads();
-  const int64_t rem = n - (int64_t)blockIdx.x * kTile;
-  const int cnt = rem < kTile ? (int)rem : kTile;
-  for (int j = t; j < cnt; j += kSortThreads) {
+  for (int j = t; j < tcnt; j += kSortThreads) {
     const K k = sk[j];
     const unsigned dig = (unsigned)(k >> shift) & 255u;
     const unsigned dst = gbase[dig] + (unsigned)j - tstart[dig];
@@ -366,11 +394,12 @@ void radix_sort_impl(K *keys, uint32_t *vals, int64_t n, int bits, SortScratch &
     const int shift = 8 * d;
     const unsigned *sk = sync_free && !all_digits ? skip : nullptr;
     hipLaunchKernelGGL(k_count<K>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, st,
-                       (const K *)src_k, n, shift, (int)ntiles, counts, sk);
+                       (const K *)src_k, n, shift, (int)ntiles, counts, sk,
+                       (const SegTile *)nullptr);
     scan_impl<unsigned>(counts, counts, 256 * ntiles, s.misc, st);
     hipLaunchKernelGGL(k_scatter<K>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, st,
                        (const K *)src_k, (const uint32_t *)src_v, dst_k, dst_v, n, shift,
-                       (int)ntiles, (const unsigned *)counts, sk);
+                       (int)ntiles, (const unsigned *)counts, sk, (const SegTile *)nullptr);
     STQ_HIP(hipGetLastError());
     std::swap(src_k, dst_k);
     std::swap(src_v, dst_v);
@@ -384,6 +413,45 @@ void radix_sort_impl(K *keys, uint32_t *vals, int64_t n, int bits, SortScratch &
 }
 
 }  // namespace
+
+// Every digit a pass, tiles from the segment table: the keys of each segment
+// are sorted stably among themselves and stay in the segment.
+void radix_sort_u32_segmented(uint32_t *keys, uint32_t *vals, int64_t n, int bits,
+                              const SegTile *tiles, int64_t ntiles, SortScratch &s,
+                              hipStream_t st) {
+  if (n <= 1 || ntiles <= 0) return;
+  const int ndigits = (bits + 7) / 8;
+  uint32_t *ak = s.alt_keys.as<uint32_t>((size_t)n);
+  uint32_t *av = s.alt_vals.as<uint32_t>((size_t)n);
+  unsigned *counts = s.counts.as<unsigned>((size_t)(256 * ntiles + 1));
+  scan_scratch_reserve<unsigned>(256 * ntiles, s.misc);
+  uint32_t *src_k = keys, *dst_k = ak, *src_v = vals, *dst_v = av;
+  for (int d = 0; d < ndigits; ++d) {
+    const int shift = 8 * d;
+    hipLaunchKernelGGL(k_count<uint32_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, st,
+                       (const uint32_t *)src_k, n, shift, (int)ntiles, counts,
+                       (const unsigned *)nullptr, tiles);
+    scan_impl<unsigned>(counts, counts, 256 * ntiles, s.misc, st);
+    hipLaunchKernelGGL(k_scatter<uint32_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, st,
+                       (const uint32_t *)src_k, (const uint32_t *)src_v, dst_k, dst_v, n, shift,
+                       (int)ntiles, (const unsigned *)counts, (const unsigned *)nullptr, tiles);
+    STQ_HIP(hipGetLastError());
+    std::swap(src_k, dst_k);
+    std::swap(src_v, dst_v);
+  }
+  if (src_k != keys) {
+    STQ_HIP(hipMemcpyAsync(keys, src_k, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+    STQ_HIP(hipMemcpyAsync(vals, src_v, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+  }
+}
+void radix_sort_segmented_reserve(int64_t n, int64_t ntiles, SortScratch &s) {
+  if (n <= 1) return;
+  s.alt_keys.as<uint32_t>((size_t)n);
+  s.alt_vals.as<uint32_t>((size_t)n);
+  s.counts.as<unsigned>((size_t)(256 * ntiles + 1));
+  scan_scratch_reserve<unsigned>(256 * ntiles, s.misc);
+}
+int sort_tile_keys() { return kTile; }
 
 void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bits, SortScratch &s,
                     hipStream_t st, int *passes_run) {

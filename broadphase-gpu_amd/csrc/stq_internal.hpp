@@ -215,6 +215,25 @@ void radix_sort_u32_all_digits(uint32_t *keys, uint32_t *vals, int64_t n, int bi
 // Allocates the scratch a sort of n keys of key_bytes needs (nothing is
 // allocated by the sort itself afterwards).
 void radix_sort_reserve(int64_t n, int key_bytes, SortScratch &s);
+// A tile of the segmented sort: keys [start, start + count) of one segment
+// (count <= sort_tile_keys()); its count of digit d is counts[cbase + d *
+// stride] (cbase = 256 * the segment's first tile + the tile's index in the
+// segment, stride = the segment's number of tiles).
+struct SegTile {
+  int64_t start;
+  int64_t cbase;
+  int32_t count;
+  int32_t stride;
+};
+// Keys per sort tile.
+int sort_tile_keys();
+// Segmented sort of 4-byte keys (every digit a pass, no host read-back): the
+// keys of each segment, whose tiles are listed in `tiles`, are sorted stably
+// among themselves and stay in place as a segment.
+void radix_sort_u32_segmented(uint32_t *keys, uint32_t *vals, int64_t n, int bits,
+                              const SegTile *tiles, int64_t ntiles, SortScratch &s,
+                              hipStream_t st);
+void radix_sort_segmented_reserve(int64_t n, int64_t ntiles, SortScratch &s);
 
 // Exclusive prefix sum of n uint32 counts into n+1 int64 offsets (out[n] is
 // the total).  Scratch is taken from `tmp`.  With n_dev, only the first
