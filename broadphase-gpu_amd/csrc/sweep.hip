@@ -734,14 +734,26 @@ __device__ __forceinline__ uint32_t pk_max_u16(uint32_t x, uint32_t y) {
                                                                 __builtin_bit_cast(ushort2v, y)));
 }
 
+// A row of the current item as the confirmation reads it, staged in the
+// wave's LDS when the item starts: FP64 bounds (sweep axis first), vertex
+// ids and the input id (64 bytes).
+struct alignas(16) RowRec {
+  double lo0, hi0, lo1, hi1, lo2, hi2;
+  int v0, v1, v2;
+  uint32_t orig;
+};
+
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 *__restrict__ rec,
                                                      const QRec *__restrict__ qrec,
                                                      int2 *__restrict__ out) {
-  // LDS slots per wave: fewer than 64 waiting candidates plus one position's 64
+  // LDS slots per wave: fewer than 64 waiting candidates plus one position's
+  // 64, as (row lane, position); and the item's rows
   __shared__ int2 slot_mem[kThreads / kWave][2 * kWave];
+  __shared__ RowRec row_mem[kThreads / kWave][kWave];
   const int lane = threadIdx.x & 63;
   int2 *slots = slot_mem[threadIdx.x >> 6];
+  RowRec *rows = row_mem[threadIdx.x >> 6];
   unsigned long long nconf = 0;
   if (MODE == kFill && a.total_dev && *a.total_dev > a.capacity) return;
   const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split);
@@ -764,7 +776,25 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
       QRec mq;
       mq.lo = 0x7fff7fffu;  // idle lanes: an empty level box that meets no column (make_qrec)
       mq.hi = 0u;
-      if (act) mq = row_form(qrec[p]);
+      if (act) {
+        mq = row_form(qrec[p]);
+        const Exact64 x = rec[p];
+        RowRec r;
+        r.lo0 = x.lo0;
+        r.hi0 = x.hi0;
+        r.lo1 = x.lo1;
+        r.hi1 = x.hi1;
+        r.lo2 = x.lo2;
+        r.hi2 = x.hi2;
+        r.v0 = x.v0;
+        r.v1 = x.v1;
+        r.v2 = x.v2;
+        r.orig = MODE != kCount ? a.orig[p] : 0u;
+        rows[lane] = r;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       // The union of the rows' level boxes (kind codes dropped): a position
       // whose box misses it meets no row.
       uint32_t ulo = act ? (mq.lo & ~kCodeBits) : 0x3fff3fffu;
@@ -777,24 +807,26 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
       ulo = (uint32_t)uniform((int)ulo);
       uhi = (uint32_t)uniform((int)uhi);
 
-      // Candidates wait in this wave's LDS slots as (row, q), in emission
-      // order (position, then row lane).  Whenever 64 or more wait, 64 are
-      // confirmed at once, one per lane, each lane fetching both exact
-      // records; the rest (fewer than 64) move to the front.
+      // Candidates wait in this wave's LDS slots as (row lane, q), in
+      // emission order (position, then row lane).  Whenever 64 or more wait,
+      // 64 are confirmed at once, one per lane: the row's record from the
+      // LDS, the position's exact record from memory; the rest (fewer than 64)
+      // move to the front.  The kind codes of the level test have already
+      // applied the type filter exactly, so the confirmation tests the FP64
+      // bounds and the shared vertices.
       int T = 0;
       auto test_slots = [&](int base, int n) __attribute__((always_inline)) {
         bool h = false, hg = false;
         uint32_t o = 0, oq = 0;
         if (lane < n) {
           const int2 c = slots[base + lane];
-          // the input ids load with the records, not after the test
-          if (MODE != kCount) {
-            o = a.orig[c.x];
-            oq = a.orig[c.y];
-          }
-          const Exact64 xp = rec[c.x], xq = rec[c.y];
-          hg = ((accept_bits(xp.type) & xq.type) != 0) & (xp.hi0 >= xq.lo0) & (xp.lo0 <= xq.hi0) &
-               (xp.hi1 >= xq.lo1) & (xp.lo1 <= xq.hi1) & (xp.hi2 >= xq.lo2) & (xp.lo2 <= xq.hi2);
+          // the input id loads with the record, not after the test
+          if (MODE != kCount) oq = a.orig[c.y];
+          const Exact64 xq = rec[c.y];
+          const RowRec xp = rows[c.x];
+          o = xp.orig;
+          hg = (xp.hi0 >= xq.lo0) & (xp.lo0 <= xq.hi0) & (xp.hi1 >= xq.lo1) & (xp.lo1 <= xq.hi1) &
+               (xp.hi2 >= xq.lo2) & (xp.lo2 <= xq.hi2);
           h = hg;
           h = h & (xp.v0 != xq.v0) & (xp.v0 != xq.v1) & (xp.v0 != xq.v2) & (xp.v1 != xq.v0) &
               (xp.v1 != xq.v1) & (xp.v1 != xq.v2) & (xp.v2 != xq.v0) & (xp.v2 != xq.v1) &
@@ -852,7 +884,7 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
               if (__builtin_amdgcn_inverse_ballot_w64(m))
                 slots[__builtin_amdgcn_mbcnt_hi(
                     (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)T))] =
-                    make_int2(p, q);
+                    make_int2(lane, q);
               T += __popcll(m);
               if (T >= kWave) drain_full();
             }
