@@ -65,11 +65,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--legs", default="batched,cloth,skewed,stream,handoff,small,host",
                     help="comma list of legs; the batched leg is the headline")
-    ap.add_argument("--batched-forms", default="chosen,plain,plain_tuned",
+    ap.add_argument("--batched-forms", default="chosen,plain,plain_tuned,fp64_grouped",
                     help="forms of the batched leg (chosen is the headline)")
     ap.add_argument("--grid", type=int, default=708, help="cloth grid size g (g x g vertices)")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--forms", default="chosen,plain", help="forms of the cloth leg")
+    ap.add_argument("--forms", default="chosen,plain,plain_tuned,fp64_grouped", help="forms of the cloth leg")
     ap.add_argument("--cpu-sample", type=float, default=0.25,
                     help="cloth: fraction of sorted rows the CPU baseline sweeps (0 = skip)")
     ap.add_argument("--cpu-batched", type=int, default=1,
@@ -165,6 +165,9 @@ BATCHED_FORMS = {
     "chosen": dict(precision=1, axis_policy=4, sizing=1, train_scenes=200, autotune=1),
     "plain": dict(precision=0, axis_policy=0, tails=2),
     "plain_tuned": dict(precision=0, axis_policy=0, tails=1, autotune=1),
+    # the chosen form's organisation and sizing with every test in FP64: what
+    # the reduced precision alone gains
+    "fp64_grouped": dict(precision=2, axis_policy=4, sizing=1, train_scenes=200, autotune=1),
 }
 
 
@@ -250,7 +253,7 @@ def bench_batched(args, H):
                        f"cloth-like, density factor U[0.5, 4] (seed 3): {nbox} FP64 boxes, device-resident",
            "boxes": nbox, "scenes": len(offs) - 1, "generate_s": gen_s, "forms": out}
     if "chosen" in out:
-        for base in ("plain", "plain_tuned"):
+        for base in ("plain", "plain_tuned", "fp64_grouped"):
             if base in out:
                 res[f"speedup_vs_{base}"] = out["chosen"]["boxes_per_s"] / out[base]["boxes_per_s"]
         ref = next(iter(digests.values()), None)
@@ -298,7 +301,10 @@ def bench_skewed(args, H):
     db = _lib.DeviceBuffer.from_array(b)
     del b
     forms = {"chosen": dict(precision=stq_amd.PRECISION_F32_CONFIRM, autotune=1),
-             "plain": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X, tails=2)}
+             "plain": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X, tails=2),
+             "plain_tuned": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X, tails=1,
+                                 autotune=1),
+             "fp64_grouped": dict(precision=stq_amd.PRECISION_F64_GROUPED, autotune=1)}
     out, dig = {}, {}
     for name, kw in forms.items():
         ctx = stq_amd.Context(0, **kw)
@@ -328,9 +334,10 @@ def bench_skewed(args, H):
     db.close()
     return {"workload": f"{n} boxes, 64 Gaussian clusters (sigma 0.02), Pareto(1.5) half-extents in "
                         f"[5e-5, 0.05] (seed 2), device-resident",
-            "chosen": out["chosen"], "plain_fp64": out["plain"],
-            "speedup_vs_plain_fp64": out["chosen"]["boxes_per_s"] / out["plain"]["boxes_per_s"],
-            "pair_sets_equal": (dig["chosen"] == dig["plain"]) if len(dig) == 2 else None}
+            "forms": out,
+            "speedup_vs": {k: out["chosen"]["boxes_per_s"] / v["boxes_per_s"] for k, v in out.items()
+                           if k != "chosen"},
+            "pair_sets_equal": (all(d == dig["plain"] for d in dig.values()) if len(dig) > 1 else None)}
 
 
 def bench_stream(args, H):
@@ -549,6 +556,11 @@ def bench_cloth(args, H):
                        autotune=1),
         "plain": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X,
                       sizing=stq_amd.SIZING_COUNT_FILL, tails=2),
+        "plain_tuned": dict(precision=stq_amd.PRECISION_F64, axis_policy=stq_amd.AXIS_X,
+                            sizing=stq_amd.SIZING_COUNT_FILL, tails=1, autotune=1),
+        "fp64_grouped": dict(precision=stq_amd.PRECISION_F64_GROUPED, axis_policy=stq_amd.AXIS_CHAIN,
+                             sizing=stq_amd.SIZING_LEARNED, train_scenes=max(args.train_scenes, 1),
+                             autotune=1),
     }
     want = [f for f in args.forms.split(",") if f]
     res, outputs, training = {}, {}, None
@@ -575,8 +587,9 @@ def bench_cloth(args, H):
                         raise
                 maes.append(ctx.stats()["sizing_mae"])
                 dsc.close()
-            training = {"scenes": len(maes), "seconds": time.perf_counter() - t0,
-                        "mae_first": maes[0], "mae_last": maes[-1]}
+            if name == "chosen":
+                training = {"scenes": len(maes), "seconds": time.perf_counter() - t0,
+                            "mae_first": maes[0], "mae_last": maes[-1]}
         try:  # size the output once (the count is part of every step anyway)
             ctx.find_pairs_device(d_boxes.ptr, n, None, 0)
         except stq_amd.StqError as e:
@@ -613,13 +626,14 @@ def bench_cloth(args, H):
     d_boxes.close()
 
     exact = None
-    if not args.no_check and len(outputs) == 2:
+    if not args.no_check and "plain" in outputs and len(outputs) > 1:
         def canon(p):
             key = (p[:, 0].astype(np.int64) << 32) | p[:, 1].astype(np.int64)
             key.sort()
             return key
-        a, b = outputs["chosen"], outputs["plain"]
-        exact = bool(len(a) == len(b) and np.array_equal(canon(a), canon(b)))
+        ref = canon(outputs["plain"])
+        exact = all(bool(len(v) == len(ref) and np.array_equal(canon(v), ref))
+                    for k, v in outputs.items() if k != "plain")
     del outputs
 
     main_form = "chosen" if "chosen" in res else want[0]
@@ -660,6 +674,8 @@ def bench_cloth(args, H):
                   for k, v in res.items()},
         "speedup_vs_plain_fp64": (res["chosen"]["boxes_per_s"] / res["plain"]["boxes_per_s"]
                                   if "chosen" in res and "plain" in res else None),
+        "speedup_vs": {k: res["chosen"]["boxes_per_s"] / v["boxes_per_s"] for k, v in res.items()
+                       if k != "chosen"} if "chosen" in res else None,
         "sizing": ({"training": training, "items": r["stats"]["n_tiles"],
                     "fallback_items": r["fallback_items"], "mae_pairs_per_item": r["stats"]["sizing_mae"]}
                    if training else None),
@@ -726,6 +742,7 @@ def main():
             "pairs_per_s": r["pairs_per_s"],
             "speedup_vs_plain_fp64": bt.get("speedup_vs_plain"),
             "speedup_vs_plain_fp64_tuned": bt.get("speedup_vs_plain_tuned"),
+            "speedup_vs_fp64_grouped": bt.get("speedup_vs_fp64_grouped"),
             "exact_pair_sets": {"chosen_vs_plain_forms": bt.get("per_scene_digests_equal_across_forms"),
                                 "oracle_digest_mismatches": (bt["cpu_baseline"] or {}).get(
                                     "per_scene_digest_mismatches_vs_gpu")},

@@ -26,7 +26,7 @@
 namespace stqamd {
 
 static bool options_valid(const stq_options &o) {
-  return o.scratch_limit_mb >= 0 && o.precision >= 0 && o.precision <= 1 && o.axis_policy >= 0 && o.axis_policy <= 4 &&
+  return o.scratch_limit_mb >= 0 && o.precision >= 0 && o.precision <= 2 && o.axis_policy >= 0 && o.axis_policy <= 4 &&
          o.sizing >= 0 && o.sizing <= 1 && o.chunk >= 0 && o.window >= 0 &&
          (o.window == 0 || (o.window >= 64 && o.window <= kMaxWindow32 &&
                             (o.window & (o.window - 1)) == 0)) &&
@@ -40,13 +40,16 @@ static bool options_valid(const stq_options &o) {
 // Auto (tails = 0) turns tails on for the plain FP64 sweep and off for the
 // reduced-precision one, whose body positions mostly cost one union test per
 // 64 (a long body is cheap there, a tail item is not).
+// The forms whose planner groups rows close on axis 1 (union tests per 64
+// positions): reduced precision and grouped FP64.
+static bool grouped_form(const stq_options &o) { return o.precision != STQ_PRECISION_F64; }
 static float tail_cost(const stq_options &o) {
-  const bool on = o.tails == 1 || (o.tails == 0 && o.precision != STQ_PRECISION_F32_CONFIRM);
+  const bool on = o.tails == 1 || (o.tails == 0 && !grouped_form(o));
   return !on ? 0.f : (o.tail_cost > 0 ? (float)o.tail_cost * 0.01f : 4.f);
 }
 // Item chunk when none is set.
 static int auto_chunk(const stq_options &o) {
-  return o.chunk > 0 ? o.chunk : (o.precision == STQ_PRECISION_F32_CONFIRM ? kQChunk : 2048);
+  return o.chunk > 0 ? o.chunk : (grouped_form(o) ? kQChunk : 2048);
 }
 
 static float ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -66,6 +69,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   stats.precision = opt.precision;
   stats.sizing = opt.sizing;
   const bool f32 = opt.precision == STQ_PRECISION_F32_CONFIRM;
+  const bool grouped = grouped_form(opt);
   hipStream_t st = stream;
   STQ_HIP(hipEventRecord(ev[E_START], st));
   if (n <= 1) {
@@ -88,7 +92,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   const int pol = opt.axis_policy;
   const int axis = pol <= 2 ? pol : 0;  // where no statistics are made
   SceneParams *d_sp = nullptr;
-  if (pol > 2 || f32 || learned) {
+  if (pol > 2 || grouped || learned) {
     // about four boxes per thread (the loads of one thread overlap)
     const int nb = (int)std::min<int64_t>((n + 1023) / 1024, 4096);
     double *part = axis_partial.as<double>((size_t)nb * kAxisStats);
@@ -116,7 +120,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   void *d_smin = smin.get((size_t)n * ts);
   void *d_smax = smax.get((size_t)n * ts);
   void *d_rec = geo.get((size_t)n * (f32 ? sizeof(Exact64) : sizeof(Rec64)));
-  QRec *d_qrec = f32 ? meta.as<QRec>((size_t)n) : nullptr;
+  QRec *d_qrec = grouped ? meta.as<QRec>((size_t)n) : nullptr;
   int *d_hi = hi.as<int>((size_t)n);
   ColStats *d_cols = learned ? col_stats.as<ColStats>((size_t)(n + kColBlock - 1) / kColBlock)
                              : nullptr;
@@ -141,7 +145,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
 
   // plan
   int W = opt.window;
-  if (W == 0 && f32) W = kQWindow;
+  if (W == 0 && grouped) W = kQWindow;
   const bool early = W == 0;
   if (early) {
     read_counters();
@@ -149,7 +153,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     W = 256;
     while (W < 2048 && (double)(2 * W) <= mean / 8.0) W *= 2;
   }
-  if (!f32) W = std::min(W, kMaxWindow64);  // 64-bit window keys
+  if (!grouped) W = std::min(W, kMaxWindow64);  // 64-bit window keys
   const int chunk = auto_chunk(opt);
   const int64_t ngroups = (n + 63) / 64;
   int *d_rowlist = rowlist.as<int>((size_t)n);
@@ -599,6 +603,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   stats.sizing = opt.sizing;
   stats.axis = opt.axis_policy <= 2 ? opt.axis_policy : -1;
   const bool f32 = opt.precision == STQ_PRECISION_F32_CONFIRM;
+  const bool grouped = grouped_form(opt);
   hipStream_t st = stream;
   std::vector<int64_t> off((size_t)nscenes + 1);
   for (int s = 0; s <= nscenes; ++s) off[s] = off_in[s] - off_in[0];
@@ -606,8 +611,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   const int64_t n = off[nscenes];
   if (n > INT32_MAX) throw StatusError(STQ_ERR_TOO_LARGE, "more than 2^31-1 boxes in a batch");
   stats.n_boxes = n;
-  const int W = opt.window ? (f32 ? opt.window : std::min(opt.window, kMaxWindow64))
-                           : (f32 ? kQWindow : kBatchWindow);
+  const int W = opt.window ? (grouped ? opt.window : std::min(opt.window, kMaxWindow64))
+                           : (grouped ? kQWindow : kBatchWindow);
   const int chunk = auto_chunk(opt);
   batch_plan(off.data(), nscenes, W, chunk);
   BatchPlan &bp = bplan;
@@ -648,7 +653,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   void *d_smin = smin.get((size_t)n * ts);
   void *d_smax = smax.get((size_t)n * ts);
   void *d_rec = geo.get((size_t)n * (f32 ? sizeof(Exact64) : sizeof(Rec64)));
-  QRec *d_qrec = f32 ? meta.as<QRec>((size_t)n) : nullptr;
+  QRec *d_qrec = grouped ? meta.as<QRec>((size_t)n) : nullptr;
   uint32_t *d_orig = b_orig.as<uint32_t>((size_t)n);
   int *d_hi = hi.as<int>((size_t)n);
   int *d_rowlist = rowlist.as<int>((size_t)n);

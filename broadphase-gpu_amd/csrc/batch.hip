@@ -172,6 +172,10 @@ __global__ __launch_bounds__(kT) void k_gather_seg(const stq_aabb_f64 *__restric
       r.v2 = b.vertex_ids[2];
       r.type = type;
       static_cast<Rec64 *>(rec)[s] = r;
+      // levels for the axis-1 grouping of the grouped FP64 form
+      if (qrec)
+        qrec[s] = make_qrec(r.lo1, r.hi1, r.lo2, r.hi2, r.type, p.qp.origin[a1], p.qp.scale[a1],
+                            p.qp.origin[a2], p.qp.scale[a2]);
     }
   }
   // blocks that straddle two scenes mix both (the features are estimates)

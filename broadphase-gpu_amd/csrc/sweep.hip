@@ -191,6 +191,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_f64(const stq_aabb_f64 *__r
                                                          double *__restrict__ smin,
                                                          double *__restrict__ smax,
                                                          Rec64 *__restrict__ rec,
+                                                         QRec *__restrict__ qrec,
                                                          ColStats *__restrict__ cols) {
   const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   const bool valid = s < n;
@@ -215,6 +216,10 @@ __global__ __launch_bounds__(kThreads) void k_gather_f64(const stq_aabb_f64 *__r
     r.v2 = b.vertex_ids[2];
     r.type = type_bit(r.v1, r.v2);
     rec[s] = r;
+    // levels for the axis-1 grouping of the grouped FP64 form
+    if (qrec)
+      qrec[s] = make_qrec(r.lo1, r.hi1, r.lo2, r.hi2, r.type, qp.origin[a1], qp.scale[a1],
+                          qp.origin[a2], qp.scale[a2]);
   }
   if (cols)
     col_stats_put(valid, s, r.lo1, r.hi1, r.lo2, r.hi2, r.type, qp.origin[a1],
@@ -906,6 +911,102 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
     atomicAdd(a.confirms + ws.xcd * (kClaimStride / 2), nconf);
 }
 
+// ---- grouped FP64 sweep ----
+// The reduced-precision sweep's organisation with every test in FP64: the
+// planner groups rows close on axis 1 (levels only order the rows), positions
+// go across the lanes 64 at a time and are tested against the FP64 union box
+// of the group's rows on axes 1 and 2, and each position that passes is read
+// out of its lane and tested against every row in FP64 (both axes, type,
+// shared vertices, p < q <= hi).  Tail items as in the plain FP64 sweep.
+// Emission order: position, then row lane.
+__device__ __forceinline__ double wave_min_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+__device__ __forceinline__ double wave_max_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_gsweep64(SweepArgs a, const Rec64 *__restrict__ rec,
+                                                       int2 *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  if (MODE == kFill && a.total_dev && *a.total_dev > a.capacity) return;
+  const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split);
+  unsigned *wc = a.work ? a.work + ws.xcd * kClaimStride : nullptr;
+  int64_t k = ws.kbeg + ws.wave;
+  while (k < ws.kend) {
+    const unsigned pend = wc ? claim_issue(wc) : 0u;
+    const int64_t it = a.sel ? (int64_t)a.sel[k] : k;
+    const Item item = a.items[it];
+    Emit<MODE> em;
+    if (MODE != kCount) em.base0 = em.base = a.offsets[it];
+    if (MODE == kLearned) em.cap = a.caps[it];
+    if (item.nrows & kTailFlag) {
+      tail64<MODE>(a, rec, out, item, lane, em);
+    } else {
+      const bool act = lane < item.nrows;
+      const int p = act ? a.rowlist[item.rowoff + lane] : INT_MAX;
+      const int myhi = act ? a.hi[p] : INT_MIN;
+      Rec64 mine;
+      mine.lo1 = mine.lo2 = INFINITY;  // idle lanes: an empty box
+      mine.hi1 = mine.hi2 = -INFINITY;
+      mine.v0 = mine.v1 = mine.v2 = mine.type = 0;
+      uint32_t op = 0;
+      if (act) {
+        mine = rec[p];
+        if (MODE != kCount) op = a.orig[p];
+      }
+      const int acc = act ? accept_bits(mine.type) : 0;
+      // the union of the rows' boxes on axes 1 and 2
+      const double ulo1 = wave_min_f64(mine.lo1), uhi1 = wave_max_f64(mine.hi1);
+      const double ulo2 = wave_min_f64(mine.lo2), uhi2 = wave_max_f64(mine.hi2);
+      const int q0 = item.q0, q1 = item.q1;
+      Rec64 x;
+      x.lo1 = x.lo2 = INFINITY;
+      x.hi1 = x.hi2 = -INFINITY;
+      x.v0 = x.v1 = x.v2 = x.type = 0;
+      if (q0 + lane <= q1) x = rec[q0 + lane];
+      for (int qc = q0; qc <= q1; qc += kWave) {
+        uint64_t pm = ballot((qc + lane <= q1) & (x.lo1 <= uhi1) & (x.hi1 >= ulo1) &
+                             (x.lo2 <= uhi2) & (x.hi2 >= ulo2));
+        // the next 64 positions load while these are tested
+        Rec64 nx;
+        nx.lo1 = nx.lo2 = INFINITY;
+        nx.hi1 = nx.hi2 = -INFINITY;
+        nx.v0 = nx.v1 = nx.v2 = nx.type = 0;
+        if (qc + kWave + lane <= q1) nx = rec[qc + kWave + lane];
+        uint32_t oql = 0;
+        if (MODE != kCount && pm && qc + lane <= q1) oql = a.orig[qc + lane];
+        while (pm) {
+          const int j = (int)__builtin_ctzll(pm);
+          pm &= pm - 1;
+          const int q = qc + j;
+          const double lo1 = readlane_f64(x.lo1, j), hi1 = readlane_f64(x.hi1, j);
+          const double lo2 = readlane_f64(x.lo2, j), hi2 = readlane_f64(x.hi2, j);
+          const int qt = readlane_i(x.type, j);
+          const bool g = (q > p) & (q <= myhi) & (mine.lo1 <= hi1) & (mine.hi1 >= lo1) &
+                         (mine.lo2 <= hi2) & (mine.hi2 >= lo2) & ((acc & qt) != 0);
+          if (ballot(g)) {
+            const int v0 = readlane_i(x.v0, j), v1 = readlane_i(x.v1, j), v2 = readlane_i(x.v2, j);
+            const bool h = g & (mine.v0 != v0) & (mine.v0 != v1) & (mine.v0 != v2) &
+                           (mine.v1 != v0) & (mine.v1 != v1) & (mine.v1 != v2) & (mine.v2 != v0) &
+                           (mine.v2 != v1) & (mine.v2 != v2);
+            const uint64_t hm = ballot(h);
+            if (hm) em.put(h, hm, op, MODE != kCount ? (uint32_t)readlane_i((int)oql, j) : 0u, out);
+          }
+        }
+        x = nx;
+      }
+    }
+    if (MODE != kFill && lane == 0) a.counts[it] = em.cnt;
+    k = wc ? claim_take(ws, pend) : k + ws.nwaves;
+  }
+}
+
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
 }  // namespace
@@ -945,7 +1046,7 @@ void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, i
                        n, sp, (float *)smin, (float *)smax, (Exact64 *)rec, qrec, cols);
   else
     hipLaunchKernelGGL(k_gather_f64, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, order,
-                       n, axis, sp, (double *)smin, (double *)smax, (Rec64 *)rec, cols);
+                       n, axis, sp, (double *)smin, (double *)smax, (Rec64 *)rec, qrec, cols);
   STQ_HIP(hipGetLastError());
 }
 
@@ -1000,6 +1101,14 @@ static void launch_sweep_b(bool f32, int mode, const dim3 &grid, const SweepArgs
       hipLaunchKernelGGL((k_qsweep<kFill>), grid, block, 0, st, args, r, args.qrec, args.out);
     else
       hipLaunchKernelGGL((k_qsweep<kLearned>), grid, block, 0, st, args, r, args.qrec, args.out);
+  } else if (args.qrec) {  // grouped FP64
+    const Rec64 *r = static_cast<const Rec64 *>(args.rec);
+    if (mode == kCount)
+      hipLaunchKernelGGL((k_gsweep64<kCount>), grid, block, 0, st, args, r, args.out);
+    else if (mode == kFill)
+      hipLaunchKernelGGL((k_gsweep64<kFill>), grid, block, 0, st, args, r, args.out);
+    else
+      hipLaunchKernelGGL((k_gsweep64<kLearned>), grid, block, 0, st, args, r, args.out);
   } else {
     constexpr int B64 = B > 4 ? 4 : B;  // Rec64 batches above 4 exceed the SGPR file
     const Rec64 *r = static_cast<const Rec64 *>(args.rec);

@@ -315,7 +315,7 @@ struct SweepArgs {
   const int *rowlist = nullptr;
   const int *hi = nullptr;
   const void *rec = nullptr;     // Rec64 (FP64 form) or Exact64 by sorted position
-  const QRec *qrec = nullptr;    // reduced-precision form only
+  const QRec *qrec = nullptr;    // reduced-precision form; with Rec64 records: the grouped FP64 form
   const uint32_t *orig = nullptr;
   uint32_t *counts = nullptr;    // kCount, kLearned: exact pairs per item
   const int64_t *offsets = nullptr;  // kFill: output offset; kLearned: scratch offset

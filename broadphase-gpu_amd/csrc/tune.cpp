@@ -53,7 +53,8 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
     if (n < kTuneMinBoxes || !trials) return;
     const auto t0 = std::chrono::steady_clock::now();
     const stq_options user = opt;
-    const bool f32 = user.precision == STQ_PRECISION_F32_CONFIRM;
+    // the forms with the axis-1 grouped planner (reduced precision, grouped FP64)
+    const bool grouped = user.precision != STQ_PRECISION_F64;
     std::vector<int64_t> pair_off(batched ? (size_t)nscenes + 1 : 0);
     auto trial = [&](const TuneEntry &e) {
       opt = user;
@@ -87,8 +88,8 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
       }
       return best;
     };
-    TuneEntry best{STQ_AXIS_CHAIN, batched ? (f32 ? kQWindow : kBatchWindow) : 0,
-                   f32 ? kQChunk : 2048, 2, 32, f32 ? 0 : 400, 0};
+    TuneEntry best{STQ_AXIS_CHAIN, batched ? (grouped ? kQWindow : kBatchWindow) : 0,
+                   grouped ? kQChunk : 2048, 2, 32, grouped ? 0 : 400, 0};
     best.ms = trial(best);
     auto consider = [&](TuneEntry e) {
       if (std::memcmp(&e, &best, offsetof(TuneEntry, ms)) == 0) return;
@@ -102,18 +103,18 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
     }
     for (int w : {0, 256, 512, 1024, 2048, 4096}) {
       if (batched && w == 0) continue;  // no automatic window in the batched form
-      if (!f32 && w > kMaxWindow64) continue;
+      if (!grouped && w > kMaxWindow64) continue;
       TuneEntry e = best;
       e.window = w;
       consider(e);
     }
-    for (int c : f32 ? std::initializer_list<int>{4096, 65536} : std::initializer_list<int>{1024, 4096}) {
+    for (int c : grouped ? std::initializer_list<int>{4096, 65536} : std::initializer_list<int>{1024, 4096}) {
       TuneEntry e = best;
       e.chunk = c;
       consider(e);
     }
-    // the scalar batch applies to the FP64 sweep only
-    if (!f32) {
+    // the scalar batch applies to the plain FP64 sweep only
+    if (!grouped) {
       TuneEntry e = best;
       e.batch = 4;
       consider(e);
@@ -123,7 +124,7 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
       e.waves_per_cu = w;
       consider(e);
     }
-    for (int tc : f32 ? std::initializer_list<int>{400} : std::initializer_list<int>{200, 800}) {
+    for (int tc : grouped ? std::initializer_list<int>{400} : std::initializer_list<int>{200, 800}) {
       TuneEntry e = best;
       e.tail_cost = tc;
       consider(e);

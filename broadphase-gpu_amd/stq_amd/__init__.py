@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (AXIS_CHAIN, AXIS_VARIANCE, AXIS_X, AXIS_Y, AXIS_Z,  # noqa: F401
-                   PRECISION_F32_CONFIRM, PRECISION_F64, SIZING_COUNT_FILL,
+                   PRECISION_F32_CONFIRM, PRECISION_F64, PRECISION_F64_GROUPED, SIZING_COUNT_FILL,
                    SIZING_LEARNED, DeviceBuffer, Options, Stats)
 from .scenes import AABB_DTYPE  # noqa: F401
 

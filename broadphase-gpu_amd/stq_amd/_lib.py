@@ -23,6 +23,7 @@ STQ_ERR_INTERNAL = 7
 
 PRECISION_F64 = 0
 PRECISION_F32_CONFIRM = 1
+PRECISION_F64_GROUPED = 2
 AXIS_X, AXIS_Y, AXIS_Z, AXIS_VARIANCE, AXIS_CHAIN = 0, 1, 2, 3, 4
 SIZING_COUNT_FILL = 0
 SIZING_LEARNED = 1

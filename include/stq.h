@@ -88,7 +88,12 @@ enum stq_precision {
      outward: the reference rule of src/aabb.cu:56-61), so its candidates are a
      superset; every FP32 candidate is re-tested in FP64 against the original
      boxes.  The pair set equals the plain FP64 set exactly. */
-  STQ_PRECISION_F32_CONFIRM = 1
+  STQ_PRECISION_F32_CONFIRM = 1,
+  /* FP64 keys and every test in FP64, with the work organisation of the
+     reduced-precision form (rows grouped close on axis 1, a union test per 64
+     positions, then the row tests): the ablation that separates that
+     organisation from the reduced precision.  The same pair set. */
+  STQ_PRECISION_F64_GROUPED = 2
 };
 
 /* Choice of the sweep axis.  The pair set never depends on it. */

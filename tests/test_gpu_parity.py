@@ -18,6 +18,7 @@ pytestmark = pytest.mark.gpu
 FORMS = [
     ("f64", dict(precision=stq_amd.PRECISION_F64)),
     ("f32c", dict(precision=stq_amd.PRECISION_F32_CONFIRM)),
+    ("f64g", dict(precision=stq_amd.PRECISION_F64_GROUPED)),
 ]
 
 
