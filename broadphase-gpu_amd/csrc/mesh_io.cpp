@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cctype>
 #include <cerrno>
+#include <climits>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -68,7 +70,10 @@ bool read_obj(const std::string &path, std::vector<double> &V, std::vector<int32
         if (head.empty() || *end != '\0' || errno != 0 || v == 0)
           fail("OBJ line " + std::to_string(lineno) + ": bad face index");
         const int64_t nv = (int64_t)(V.size() / 3);
-        idx.push_back(v > 0 ? v - 1 : nv + v);
+        const int64_t i = v > 0 ? (int64_t)v - 1 : nv + (int64_t)v;
+        // narrowed to int32 only in range; the vertex count is checked at the end
+        if (i < 0 || i > INT32_MAX) fail("OBJ line " + std::to_string(lineno) + ": face index out of range");
+        idx.push_back(i);
       }
       if (idx.size() != 3) fail("OBJ line " + std::to_string(lineno) + ": only triangles are read");
       for (int64_t i : idx) F.push_back((int32_t)i);
@@ -200,7 +205,12 @@ bool read_ply(const std::string &path, std::vector<double> &V, std::vector<int32
           if ((int)k == iface && is_face && m != 3) fail("PLY: only triangles are read");
           for (int64_t j = 0; j < m; ++j) {
             const double v = next_value(p.type);
-            if ((int)k == iface && is_face) F.push_back((int32_t)v);
+            if ((int)k == iface && is_face) {
+              // an integral value in int32 range, checked before the conversion
+              if (!(v >= 0.0 && v <= (double)INT32_MAX) || v != std::floor(v))
+                fail("PLY: face index out of range");
+              F.push_back((int32_t)v);
+            }
           }
         } else {
           const double v = next_value(p.type);
@@ -256,11 +266,16 @@ std::vector<std::pair<int64_t, int64_t>> read_pair_json(const std::string &path)
     ws();
     size_t j = i;
     if (j < s.size() && (s[j] == '-' || s[j] == '+')) ++j;
+    const size_t d0 = j;
     while (j < s.size() && std::isdigit((unsigned char)s[j])) ++j;
-    if (j == i) fail("JSON: expected an integer");
-    const int64_t v = std::stoll(s.substr(i, j - i));
+    if (j == d0) fail("JSON: expected an integer");
+    const std::string tok = s.substr(i, j - i);
+    errno = 0;
+    char *end = nullptr;
+    const long long v = std::strtoll(tok.c_str(), &end, 10);
+    if (errno != 0 || *end != '\0') fail("JSON: integer out of range");
     i = j;
-    return v;
+    return (int64_t)v;
   };
   std::vector<std::pair<int64_t, int64_t>> out;
   expect('[');
