@@ -739,13 +739,22 @@ __device__ __forceinline__ uint32_t pk_max_u16(uint32_t x, uint32_t y) {
                                                                 __builtin_bit_cast(ushort2v, y)));
 }
 
-// A row of the current item as the confirmation reads it, staged in the
-// wave's LDS when the item starts: FP64 bounds (sweep axis first), vertex
-// ids and the input id (64 bytes).
-struct alignas(16) RowRec {
-  double lo0, hi0, lo1, hi1, lo2, hi2;
+// The rows of the current item as the confirmation reads them, staged in the
+// wave's LDS when the item starts, in four 16-byte planes of 64 rows: the
+// FP64 bounds of each axis (sweep axis first), then the vertex ids and the
+// input id.  A lane reads one 16-byte entry per plane at 16 * row bytes, so
+// 64 lanes reading distinct rows spread evenly over the banks (a 64-byte row
+// stride would put every row of the same parity on the same banks).
+struct alignas(16) RowBounds {
+  double lo, hi;
+};
+struct alignas(16) RowIds {
   int v0, v1, v2;
   uint32_t orig;
+};
+struct RowPlanes {
+  RowBounds b[3][kWave];
+  RowIds id[kWave];
 };
 
 template <int MODE>
@@ -755,10 +764,10 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
   // LDS slots per wave: fewer than 64 waiting candidates plus one position's
   // 64, as (row lane, position); and the item's rows
   __shared__ int2 slot_mem[kThreads / kWave][2 * kWave];
-  __shared__ RowRec row_mem[kThreads / kWave][kWave];
+  __shared__ RowPlanes row_mem[kThreads / kWave];
   const int lane = threadIdx.x & 63;
   int2 *slots = slot_mem[threadIdx.x >> 6];
-  RowRec *rows = row_mem[threadIdx.x >> 6];
+  RowPlanes &rows = row_mem[threadIdx.x >> 6];
   unsigned long long nconf = 0;
   if (MODE == kFill && a.total_dev && *a.total_dev > a.capacity) return;
   const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split);
@@ -784,18 +793,10 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
       if (act) {
         mq = row_form(qrec[p]);
         const Exact64 x = rec[p];
-        RowRec r;
-        r.lo0 = x.lo0;
-        r.hi0 = x.hi0;
-        r.lo1 = x.lo1;
-        r.hi1 = x.hi1;
-        r.lo2 = x.lo2;
-        r.hi2 = x.hi2;
-        r.v0 = x.v0;
-        r.v1 = x.v1;
-        r.v2 = x.v2;
-        r.orig = MODE != kCount ? a.orig[p] : 0u;
-        rows[lane] = r;
+        rows.b[0][lane] = RowBounds{x.lo0, x.hi0};
+        rows.b[1][lane] = RowBounds{x.lo1, x.hi1};
+        rows.b[2][lane] = RowBounds{x.lo2, x.hi2};
+        rows.id[lane] = RowIds{x.v0, x.v1, x.v2, MODE != kCount ? a.orig[p] : 0u};
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -828,10 +829,11 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
           // the input id loads with the record, not after the test
           if (MODE != kCount) oq = a.orig[c.y];
           const Exact64 xq = rec[c.y];
-          const RowRec xp = rows[c.x];
+          const RowBounds b0 = rows.b[0][c.x], b1 = rows.b[1][c.x], b2 = rows.b[2][c.x];
+          const RowIds xp = rows.id[c.x];
           o = xp.orig;
-          hg = (xp.hi0 >= xq.lo0) & (xp.lo0 <= xq.hi0) & (xp.hi1 >= xq.lo1) & (xp.lo1 <= xq.hi1) &
-               (xp.hi2 >= xq.lo2) & (xp.lo2 <= xq.hi2);
+          hg = (b0.hi >= xq.lo0) & (b0.lo <= xq.hi0) & (b1.hi >= xq.lo1) & (b1.lo <= xq.hi1) &
+               (b2.hi >= xq.lo2) & (b2.lo <= xq.hi2);
           h = hg;
           h = h & (xp.v0 != xq.v0) & (xp.v0 != xq.v1) & (xp.v0 != xq.v2) & (xp.v1 != xq.v0) &
               (xp.v1 != xq.v1) & (xp.v1 != xq.v2) & (xp.v2 != xq.v0) & (xp.v2 != xq.v1) &

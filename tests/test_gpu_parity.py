@@ -73,14 +73,37 @@ def test_scenes_vs_oracle(ctx, form, kw, name, maker):
         assert same_set(got, ref), (name, form, axis)
 
 
+# Windows 64..256 sort each group's rows in LDS; 512 and up use the
+# register-blocked window sort (1, 2, 4 and 8 keys per thread at 512, 1024,
+# 2048 and 4096).
 @pytest.mark.parametrize("window,chunk,batch,xcd", [(64, 1, 2, 1), (64, 7, 4, 2), (256, 64, 8, 1),
+                                                   (512, 16, 2, 2), (1024, 300, 4, 1),
                                                    (2048, 100000, 0, 2), (128, 33, 8, 2)])
 def test_planning_parameters_do_not_change_the_set(ctx, window, chunk, batch, xcd):
     b = scenes.uniform_scene(20000, seed=8, extent_scale=4.0)
     ref = oracle.reference_set(b)
-    for kw in (dict(), dict(precision=stq_amd.PRECISION_F32_CONFIRM)):
+    for kw in (dict(), dict(precision=stq_amd.PRECISION_F32_CONFIRM),
+               dict(precision=stq_amd.PRECISION_F64_GROUPED)):
         got = run(ctx, b, window=window, chunk=chunk, batch=batch, xcd_split=xcd, **kw)
         assert same_set(got, ref), (window, chunk, batch, xcd, kw)
+
+
+@pytest.mark.parametrize("window", [512, 1024])
+def test_planning_windows_of_the_batched_planner(window):
+    """The batched planner's register-blocked window sort at 512 and 1024."""
+    sc = [scenes.uniform_scene(n, seed=80 + i, extent_scale=4.0) for i, n in enumerate((3000, 17000, 40))]
+    sc.append(scenes.cloth_scene(g=60, seed=84, sigma=1e-2))
+    allb, offs = scenes.concat_scenes(sc)
+    c = stq_amd.Context(0)
+    try:
+        for kw in (dict(), dict(precision=stq_amd.PRECISION_F32_CONFIRM),
+                   dict(precision=stq_amd.PRECISION_F64_GROUPED)):
+            c.set_options(window=window, chunk=64, axis_policy=4, **kw)
+            pairs, poffs = c.find_pairs_batched(allb, offs)
+            for s, b in enumerate(sc):
+                assert same_set(pairs[poffs[s]:poffs[s + 1]], oracle.reference_set(b)), (window, kw, s)
+    finally:
+        c.close()
 
 
 def test_deterministic_output_order(ctx):
