@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--legs", default="batched,cloth,skewed,stream,handoff,small,host",
                     help="comma list of legs; the batched leg is the headline")
-    ap.add_argument("--batched-forms", default="chosen,plain,plain_tuned,fp64_grouped",
+    ap.add_argument("--batched-forms", default="chosen,plain,plain_tuned,fp64_grouped,chosen_item_split",
                     help="forms of the batched leg (chosen is the headline)")
     ap.add_argument("--grid", type=int, default=708, help="cloth grid size g (g x g vertices)")
     ap.add_argument("--seed", type=int, default=1)
@@ -168,6 +168,10 @@ BATCHED_FORMS = {
     # the chosen form's organisation and sizing with every test in FP64: what
     # the reduced precision alone gains
     "fp64_grouped": dict(precision=2, axis_policy=4, sizing=1, train_scenes=200, autotune=1),
+    # the chosen form with the XCD slices cut at equal numbers of items instead
+    # of equal predicted cost: what the prediction's work split gains
+    "chosen_item_split": dict(precision=1, axis_policy=4, sizing=1, train_scenes=200, autotune=1,
+                              balance=2),
 }
 
 
@@ -253,7 +257,7 @@ def bench_batched(args, H):
                        f"cloth-like, density factor U[0.5, 4] (seed 3): {nbox} FP64 boxes, device-resident",
            "boxes": nbox, "scenes": len(offs) - 1, "generate_s": gen_s, "forms": out}
     if "chosen" in out:
-        for base in ("plain", "plain_tuned", "fp64_grouped"):
+        for base in ("plain", "plain_tuned", "fp64_grouped", "chosen_item_split"):
             if base in out:
                 res[f"speedup_vs_{base}"] = out["chosen"]["boxes_per_s"] / out[base]["boxes_per_s"]
         ref = next(iter(digests.values()), None)
@@ -743,6 +747,7 @@ def main():
             "speedup_vs_plain_fp64": bt.get("speedup_vs_plain"),
             "speedup_vs_plain_fp64_tuned": bt.get("speedup_vs_plain_tuned"),
             "speedup_vs_fp64_grouped": bt.get("speedup_vs_fp64_grouped"),
+            "work_split_by_predicted_cost_speedup": bt.get("speedup_vs_chosen_item_split"),
             "exact_pair_sets": {"chosen_vs_plain_forms": bt.get("per_scene_digests_equal_across_forms"),
                                 "oracle_digest_mismatches": (bt["cpu_baseline"] or {}).get(
                                     "per_scene_digest_mismatches_vs_gpu")},

@@ -26,7 +26,8 @@
 namespace stqamd {
 
 static bool options_valid(const stq_options &o) {
-  return o.scratch_limit_mb >= 0 && o.precision >= 0 && o.precision <= 2 && o.axis_policy >= 0 && o.axis_policy <= 4 &&
+  return o.scratch_limit_mb >= 0 && o.balance >= 0 && o.balance <= 2 && o.precision >= 0 &&
+         o.precision <= 2 && o.axis_policy >= 0 && o.axis_policy <= 4 &&
          o.sizing >= 0 && o.sizing <= 1 && o.chunk >= 0 && o.window >= 0 &&
          (o.window == 0 || (o.window >= 64 && o.window <= kMaxWindow32 &&
                             (o.window & (o.window - 1)) == 0)) &&
@@ -219,6 +220,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     return;
   }
   STQ_HIP(hipEventRecord(ev[E_PRED], st));
+  sa.slices = cost_slices(sa, nitems, nullptr, nullptr, nullptr, st);
   launch_sweep(f32, kCount, batch, nblocks, sa, st);
   STQ_HIP(hipEventRecord(ev[E_COUNT], st));
   scan_u32_to_i64(d_counts, d_off, nitems, scan_tmp, st);
@@ -263,6 +265,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     throw StatusError(STQ_ERR_CAPACITY, "output capacity too small");
   }
   if (total > 0) {
+    sa.slices = cost_slices(sa, nitems, nullptr, nullptr, d_counts, st);
     sa.counts = nullptr;
     sa.confirms = nullptr;
     sa.offsets = d_off;
@@ -288,7 +291,8 @@ void Context::run_noalloc(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, i
                     &meta,     &hi,            &rowlist,        &groups,        &tgroups,
                     &gchunks,  &nchunks,       &ioff,           &items,         &counts,
                     &offsets,  &scan_tmp,      &counters,       &axis_partial,  &scene_params,
-                    &sort.alt_keys, &sort.alt_vals, &sort.counts, &sort.hist,    &sort.misc};
+                    &sort.alt_keys, &sort.alt_vals, &sort.counts, &sort.hist,    &sort.misc,
+                    &item_cost, &cost_off, &xcd_slices};
   constexpr int kBufs = sizeof(bufs) / sizeof(bufs[0]);
   struct Swap {
     DevBuf **b;
@@ -312,11 +316,36 @@ void Context::run_noalloc(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, i
   run(boxes, n, out, capacity, nullptr, count);
 }
 
+void Context::reserve_cost_slices(int64_t bound) {
+  item_cost.as<uint32_t>((size_t)std::max<int64_t>(bound, 1));
+  cost_off.as<int64_t>((size_t)bound + 1);
+  xcd_slices.as<int64_t>(kXcds + 1);
+  scan_reserve_i64(bound, scan_tmp);
+}
+
+const int64_t *Context::cost_slices(const SweepArgs &sa, int64_t bound, const float *pred,
+                                    const uint32_t *cand, const uint32_t *counts, hipStream_t st) {
+  // auto: only before the learned pass (measured: the network's estimate
+  // moves 14% off the batched sweep; before a count or fill pass the shape
+  // and exact counts alone misjudge the per-position cost of clustered scenes)
+  if (!sa.xcd_split || opt.balance == 2 || bound <= 0 || (opt.balance == 0 && !pred))
+    return nullptr;
+  reserve_cost_slices(bound);
+  uint32_t *c = item_cost.as<uint32_t>((size_t)bound);
+  int64_t *pre = cost_off.as<int64_t>((size_t)bound + 1);
+  int64_t *sl = xcd_slices.as<int64_t>(kXcds + 1);
+  launch_item_cost(sa.items, bound, sa.nitems_dev, sa.qrec != nullptr, pred, cand, counts, c, st);
+  scan_u32_to_i64(c, pre, bound, scan_tmp, st, sa.nitems_dev);
+  launch_xcd_slices(pre, bound, sa.nitems_dev, sl, st);
+  return sl;
+}
+
 void Context::enqueue_refill(const SweepArgs &sa, bool f32, int batch, int nblocks, const int *ovf,
                              const unsigned long long *novf, int64_t nbound, const int64_t *off,
                              int2 *out, const int64_t *total, int64_t capacity, hipStream_t st) {
   SweepArgs fa = sa;
   fa.sel = ovf;
+  fa.slices = nullptr;
   fa.nitems = nbound;
   fa.nitems_dev = reinterpret_cast<const int64_t *>(novf);
   fa.counts = nullptr;
@@ -361,6 +390,7 @@ void Context::run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const 
   const int64_t budget = std::min<int64_t>(std::max<int64_t>(slots, 1), scratch_limit_pairs());
   if (budget < slots) launch_cap_budget(d_caps, d_capoff, nitems, budget, st);
   int2 *d_scratch = scratch_pairs.as<int2>((size_t)budget);
+  sa.slices = cost_slices(sa, nitems, d_pred, d_cand, nullptr, st);
   STQ_HIP(hipEventRecord(ev[E_PRED], st));
   STQ_HIP(hipEventRecord(ev[E_COUNT], st));
   uint32_t *d_counts = sa.counts;
@@ -682,6 +712,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   const float *d_w = learned ? sz_params.as<float>(kParams) : nullptr;
   // pinned results: pair offsets, 5 counters, items, total slots, error sum,
   // the claim lines of the first sweep launch (confirmation counters)
+  reserve_cost_slices(bound);
   const int64_t nh = (int64_t)nscenes + 1 + 8 + kClaimLinesU64;
   int64_t *hbuf = static_cast<int64_t *>(b_host.get(sizeof(int64_t) * (size_t)nh));
   int64_t *h_ctr = hbuf + nscenes + 1;
@@ -750,10 +781,12 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     const int64_t *total = d_off + bound;
     if (!single_pass) {
       mark(E_PRED);
+      sa.slices = cost_slices(sa, bound, nullptr, nullptr, nullptr, st);
       launch_sweep(f32, kCount, batch, nblocks, sa, st);
       mark(E_COUNT);
       scan_u32_to_i64(d_counts, d_off, bound, scan_tmp, st, sa.nitems_dev);
       mark(E_SCAN);
+      sa.slices = cost_slices(sa, bound, nullptr, nullptr, d_counts, st);
       sa.counts = nullptr;
       sa.confirms = nullptr;
       sa.offsets = d_off;
@@ -769,6 +802,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
       launch_caps(d_pred, d_cand, bound, kBatchCapScale, kBatchCapSlack, d_caps, st, sa.nitems_dev);
       scan_u32_to_i64(d_caps, d_capoff, bound, scan_tmp, st, sa.nitems_dev);
       launch_cap_budget(d_caps, d_capoff, bound, b_budget, st, sa.nitems_dev);
+      sa.slices = cost_slices(sa, bound, d_pred, d_cand, nullptr, st);
       mark(E_PRED);
       mark(E_COUNT);
       sa.offsets = d_capoff;
@@ -805,7 +839,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     const void *boxes, *out, *host;
     int64_t capacity, budget;
     uint64_t generation;
-    int32_t precision, axis_policy, batch, nblocks, xcd_split, sizing, claim, tails;
+    int32_t precision, axis_policy, batch, nblocks, xcd_split, sizing, claim, tails, balance;
     float tail_cost;
   } key;
   std::memset(&key, 0, sizeof(key));
@@ -823,6 +857,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   key.sizing = single_pass ? 1 : 0;
   key.claim = opt.claim;
   key.tails = opt.tails;
+  key.balance = opt.balance == 2 ? 2 : 0;
   key.tail_cost = tail_cost(opt);
   const uint64_t sig = fnv1a(&key, sizeof(key)) | 1ull;
 

@@ -150,6 +150,7 @@ struct Context {
 
   DevBuf keys, vals, smin, smax, geo, meta, full, hi, rowlist, groups, nchunks, ioff, items,
       counts, offsets, scan_tmp, counters, axis_partial, h2d_boxes, d2h_pairs, tgroups, gchunks;
+  DevBuf item_cost, cost_off, xcd_slices;  // the work split by estimated cost
   SortScratch sort;
   // learned sizing: parameters, Adam state, per-item buffers
   DevBuf sz_params, sz_m, sz_v, sz_grad, sz_partial, sz_loss, feat, cand, pred, caps, cap_off,
@@ -198,7 +199,7 @@ struct Context {
     DevBuf *bufs[] = {&keys,  &vals,    &smin,    &smax,     &geo,          &meta,
                       &full,  &hi,      &rowlist, &groups,   &nchunks,      &ioff,
                       &items, &counts,  &offsets, &scan_tmp, &counters,     &axis_partial,
-                      &h2d_boxes, &d2h_pairs};
+                      &h2d_boxes, &d2h_pairs, &item_cost, &cost_off, &xcd_slices};
     for (DevBuf *b : bufs) b->release();
     sort.alt_keys.release();
     sort.alt_vals.release();
@@ -224,6 +225,16 @@ struct Context {
   void enqueue_refill(const SweepArgs &sa, bool f32, int batch, int nblocks, const int *ovf,
                       const unsigned long long *novf, int64_t nbound, const int64_t *off,
                       int2 *out, const int64_t *total, int64_t capacity, hipStream_t st);
+  // XCD slices of equal estimated cost for a sweep launch over sa's items
+  // (at most `bound`, or *sa.nitems_dev): the cost of an item counts its
+  // shape and its expected pairs, the network's predictions (pred, bounded by
+  // the candidates cand) or the exact counts (counts), or neither.  Enqueued
+  // only, on buffers that a capture must have reserved
+  // (reserve_cost_slices); null when the split is by item count
+  // (options.balance = 2 or no XCD split).
+  const int64_t *cost_slices(const SweepArgs &sa, int64_t bound, const float *pred,
+                             const uint32_t *cand, const uint32_t *counts, hipStream_t st);
+  void reserve_cost_slices(int64_t bound);
   void run_learned(SweepArgs sa, bool f32, int batch, int nblocks, const float *d_feat,
                    const uint32_t *d_cand, stq_pair *out, int64_t capacity, stq_pair **alloc_out,
                    int64_t *count);

@@ -25,6 +25,7 @@
 namespace stqamd {
 
 constexpr int kWave = 64;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs of 32 CUs
 
 // Most blocks a grid-stride kernel that ends in one atomic per block launches
 // (8 blocks of 256 threads per CU on 256 CUs): atomics to one address

@@ -40,7 +40,6 @@ namespace stqamd {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kXcds = 8;  // MI355X: 8 XCDs of 32 CUs
 
 __device__ inline unsigned mbcnt64(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -373,6 +372,64 @@ __global__ __launch_bounds__(kThreads) void k_items(const int4 *__restrict__ gro
   write_group_items(groups[g], tgroups[g], gchunk[g], ioff[g], ioff[g + 1] - ioff[g], items);
 }
 
+// ---- work split by predicted cost ----------------------------------------------------
+// Cost of an item in units of about 40 ns of one wave, from a least-squares
+// fit of measured per-item times of the reduced-precision sweep (about 4.6 us
+// per item, 0.41 us per 64 positions, 0.07 us per row and 0.04 us per pair):
+// the grouped sweeps scan 64 positions per step; the plain FP64 sweep one
+// position per step (about a third of a 64-position step); a tail item tests
+// each of its rows against each 64 positions.
+__global__ __launch_bounds__(kThreads) void k_item_cost(const Item *__restrict__ items, int64_t n,
+                                                        const int64_t *__restrict__ n_dev,
+                                                        bool grouped,
+                                                        const float *__restrict__ pred,
+                                                        const uint32_t *__restrict__ cand,
+                                                        const uint32_t *__restrict__ counts,
+                                                        uint32_t *__restrict__ cost) {
+  const int64_t m = n_dev ? std::min(*n_dev, n) : n;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * kThreads) {
+    const Item it = items[i];
+    const int rows = it.nrows & kRowsMask;
+    const int64_t span = std::max<int64_t>(0, (int64_t)it.q1 - it.q0 + 1);
+    const int64_t blocks = (span + 63) >> 6;
+    // the network predicts log(1 + pairs); an item has at most cand[i]
+    int64_t pairs = counts ? (int64_t)counts[i] : 0;
+    if (pred) {
+      const float p = expm1f(fminf(fmaxf(pred[i], 0.f), 30.f));
+      pairs = (int64_t)fminf(p, cand ? (float)cand[i] : 1e9f);
+    }
+    int64_t c;
+    if (it.nrows & kTailFlag)
+      c = 64 + blocks * rows + pairs;
+    else if (grouped)
+      c = 64 + 10 * blocks + 2 * rows + pairs;
+    else
+      c = 64 + 3 * span + 2 * rows + pairs;
+    cost[i] = (uint32_t)std::min<int64_t>(c, int64_t(1) << 30);
+  }
+}
+
+// slices[x] = first item whose exclusive cost prefix reaches x / kXcds of the
+// total (x = 0 .. kXcds); slices[kXcds] = the item count.
+__global__ void k_xcd_slices(const int64_t *__restrict__ prefix, int64_t n,
+                             const int64_t *__restrict__ n_dev, int64_t *__restrict__ slices) {
+  const int x = threadIdx.x;
+  if (x > kXcds) return;
+  const int64_t m = n_dev ? std::min(*n_dev, n) : n;
+  const int64_t total = prefix[m];
+  const int64_t target = total / kXcds * x + total % kXcds * x / kXcds;
+  int64_t lo = 0, hi = m;  // lower bound over prefix[0..m]
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (prefix[mid] < target)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  slices[x] = x == kXcds ? m : lo;
+}
+
 // ---- the sweep ----------------------------------------------------------------------
 // Every test is a v_cmp whose lane mask is combined with s_and: the ballot of
 // each comparison is taken separately, so no short-circuit branches appear.
@@ -389,7 +446,8 @@ struct WaveSlice {
   int64_t kbeg, kend;
   int wave, nwaves, xcd;
 };
-__device__ __forceinline__ WaveSlice wave_slice(int64_t nitems, int xcd_split) {
+__device__ __forceinline__ WaveSlice wave_slice(int64_t nitems, int xcd_split,
+                                               const int64_t *slices) {
   WaveSlice w;
   w.xcd = 0;
   if (xcd_split) {
@@ -398,8 +456,13 @@ __device__ __forceinline__ WaveSlice wave_slice(int64_t nitems, int xcd_split) {
     const int per = (int)(gridDim.x / kXcds);
     w.wave = uniform((int)((blockIdx.x / kXcds) * (kThreads / 64) + (threadIdx.x >> 6)));
     w.nwaves = per * (kThreads / 64);
-    w.kbeg = nitems * xcd / kXcds;
-    w.kend = nitems * (xcd + 1) / kXcds;
+    if (slices) {
+      w.kbeg = slices[xcd];
+      w.kend = slices[xcd + 1];
+    } else {
+      w.kbeg = nitems * xcd / kXcds;
+      w.kend = nitems * (xcd + 1) / kXcds;
+    }
   } else {
     w.wave = uniform((int)(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)));
     w.nwaves = (int)gridDim.x * (kThreads / 64);
@@ -545,7 +608,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep64(SweepArgs a, const Rec64 *
   const int zero = a.zero;
   const int lane = threadIdx.x & 63;
   if (MODE == kFill && a.total_dev && *a.total_dev > a.capacity) return;
-  const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split);
+  const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split, a.slices);
   // the first item is the wave's own; the claims number the rest from nwaves
   unsigned *wc = a.work ? a.work + ws.xcd * kClaimStride : nullptr;
   int64_t k = ws.kbeg + ws.wave;
@@ -770,7 +833,7 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
   RowPlanes &rows = row_mem[threadIdx.x >> 6];
   unsigned long long nconf = 0;
   if (MODE == kFill && a.total_dev && *a.total_dev > a.capacity) return;
-  const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split);
+  const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split, a.slices);
   // the first item is the wave's own; the claims number the rest from nwaves
   unsigned *wc = a.work ? a.work + ws.xcd * kClaimStride : nullptr;
   int64_t k = ws.kbeg + ws.wave;
@@ -937,7 +1000,7 @@ __global__ __launch_bounds__(kThreads) void k_gsweep64(SweepArgs a, const Rec64 
                                                        int2 *__restrict__ out) {
   const int lane = threadIdx.x & 63;
   if (MODE == kFill && a.total_dev && *a.total_dev > a.capacity) return;
-  const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split);
+  const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split, a.slices);
   unsigned *wc = a.work ? a.work + ws.xcd * kClaimStride : nullptr;
   int64_t k = ws.kbeg + ws.wave;
   while (k < ws.kend) {
@@ -1088,6 +1151,21 @@ void launch_items(const int4 *groups, const int4 *tgroups, const int2 *gchunk, c
   if (ngroups <= 0) return;
   hipLaunchKernelGGL(k_items, dim3(blocks_for(ngroups)), dim3(kThreads), 0, st, groups, tgroups,
                      gchunk, ioff, ngroups, items);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_item_cost(const Item *items, int64_t n, const int64_t *n_dev, bool grouped,
+                      const float *pred, const uint32_t *cand, const uint32_t *counts,
+                      uint32_t *cost, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_item_cost, dim3(blocks_for(n)), dim3(kThreads), 0, st, items, n, n_dev,
+                     grouped, pred, cand, counts, cost);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_xcd_slices(const int64_t *prefix, int64_t n, const int64_t *n_dev, int64_t *slices,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_xcd_slices, dim3(1), dim3(64), 0, st, prefix, n, n_dev, slices);
   STQ_HIP(hipGetLastError());
 }
 

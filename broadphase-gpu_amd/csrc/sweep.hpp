@@ -336,11 +336,26 @@ struct SweepArgs {
   // wave takes its first item by its index in the slice and claims the
   // following ones; null: a fixed stride
   unsigned *work = nullptr;
+  // With xcd_split: kXcds + 1 item boundaries of the XCD slices (null: equal
+  // numbers of items per slice)
+  const int64_t *slices = nullptr;
 };
 
 // Pairs come out as canonical (min, max) pairs of input ids (orig[] of both
 // sorted positions), in the emission order of the sweep.
 void launch_sweep(bool f32, int mode, int batch, int nblocks, const SweepArgs &args,
                   hipStream_t st);
+
+// Work split by predicted cost.  Each item's cost is estimated from its shape
+// and its expected pairs (before a learned pass the network's prediction,
+// log(1 + pairs), bounded by the item's candidates cand[]; before a fill pass
+// the exact counts; none before a count pass); the XCD slices then get equal
+// shares of the total, in item order.  cost[] is n (or *n_dev) words;
+// slices[] gets kXcds + 1 boundaries from the exclusive prefix sums.
+void launch_item_cost(const Item *items, int64_t n, const int64_t *n_dev, bool grouped,
+                      const float *pred, const uint32_t *cand, const uint32_t *counts,
+                      uint32_t *cost, hipStream_t st);
+void launch_xcd_slices(const int64_t *prefix, int64_t n, const int64_t *n_dev, int64_t *slices,
+                       hipStream_t st);
 
 }  // namespace stqamd

@@ -29,14 +29,15 @@ class StqError(RuntimeError):
 
 def options(precision=PRECISION_F64, axis_policy=AXIS_X, sizing=SIZING_COUNT_FILL,
             autotune=0, chunk=0, window=0, waves_per_cu=0, batch=0, xcd_split=0,
-            train_scenes=0, graph=0, claim=0, tails=0, tail_cost=0, scratch_limit_mb=0) -> Options:
+            train_scenes=0, graph=0, claim=0, tails=0, tail_cost=0, scratch_limit_mb=0,
+            balance=0) -> Options:
     o = Options()
     _lib.lib().stq_options_default(ctypes.byref(o))
     o.precision, o.axis_policy, o.sizing = precision, axis_policy, sizing
     o.autotune, o.chunk, o.window, o.waves_per_cu = autotune, chunk, window, waves_per_cu
     o.batch, o.xcd_split, o.train_scenes = batch, xcd_split, train_scenes
     o.graph, o.claim, o.tails, o.tail_cost = graph, claim, tails, tail_cost
-    o.scratch_limit_mb = scratch_limit_mb
+    o.scratch_limit_mb, o.balance = scratch_limit_mb, balance
     return o
 
 

@@ -75,7 +75,7 @@ class Options(ctypes.Structure):
         ("tails", ctypes.c_int32),
         ("tail_cost", ctypes.c_int32),
         ("scratch_limit_mb", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("balance", ctypes.c_int32),
     ]
 
 

@@ -163,7 +163,17 @@ typedef struct stq_options {
                             predicted slots end beyond it are redone
                             by the exact refill, so the pairs do not
                             depend on it                           */
-  int32_t reserved;
+  int32_t balance;       /* work split over the XCD slices (with
+                            xcd_split on), by cost: each item's cost
+                            from its positions, rows and expected
+                            pairs (the network's prediction before
+                            the learned pass, the exact counts before
+                            a fill pass, none before a count pass),
+                            and each XCD gets an equal share of the
+                            total.  0 = auto (by cost before the
+                            learned pass, else equal numbers of
+                            items), 1 = by cost before every pass,
+                            2 = equal numbers of items             */
 } stq_options;
 
 typedef struct stq_stats {
