@@ -84,10 +84,15 @@ enum stq_precision {
   /* Plain FP64: sort keys and every comparison in FP64 (the reference built
      with CCD_USE_DOUBLE). */
   STQ_PRECISION_F64 = 0,
-  /* FP32 sweep on boxes rounded outward (round to nearest, then one ulp
-     outward: the reference rule of src/aabb.cu:56-61), so its candidates are a
-     superset; every FP32 candidate is re-tested in FP64 against the original
-     boxes.  The pair set equals the plain FP64 set exactly. */
+  /* Reduced-precision filters, FP64 confirmation.  The sort keys and chain
+     ends use FP32 bounds rounded outward (round to nearest, then one ulp
+     outward: the reference rule of src/aabb.cu:56-61); the two other axes
+     are tested on 14-bit levels per scene, rounded outward, with the kind
+     codes of the type filter folded in.  Both filters keep a superset of the
+     FP64 candidates, and every survivor is re-tested in FP64 on all three
+     axes against the original boxes, so the pair set equals the plain FP64
+     set exactly.  FP64 also computes the axis statistics and the level
+     ranges. */
   STQ_PRECISION_F32_CONFIRM = 1,
   /* FP64 keys and every test in FP64, with the work organisation of the
      reduced-precision form (rows grouped close on axis 1, a union test per 64
