@@ -930,20 +930,30 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
         xlo = x.lo;
         xhi = x.hi;
       }
-      for (int qc = q0; qc <= q1; qc += kWave) {
-        uint64_t pm = ballot((qc + lane <= q1) &
-                             level_overlap(uhi, ulo, xlo & ~kCodeBits, xhi & ~kCodeBits));
+      // Lanes past q1 hold the empty box (lo 0x7fff, hi 0 in each half).  It
+      // may pass the union test (when the union spans every level), but never
+      // a row test: that needs a row with hi 0x7fff (kind code set) and lo 0
+      // (code clear), and a row carries the same code in lo and hi.  So the
+      // union test needs no range check of its own.
+      const QRec *qn = qrec + ((int64_t)q0 + kWave + lane);  // this lane's next position
+      for (int qc = q0; qc <= q1; qc += kWave, qn += kWave) {
+        uint64_t pm = ballot(level_overlap(uhi, ulo, xlo & ~kCodeBits, xhi & ~kCodeBits));
         // the next 64 positions load while these are tested (issued after the
-        // wait for these, which waits for every load in flight)
+        // wait for these, which waits for every load in flight); all but an
+        // item's last block are whole, and load without a per-lane check
         uint32_t nlo = 0x7fff7fffu, nhi = 0u;
-        if (qc + kWave + lane <= q1) {
-          const QRec x = qrec[qc + kWave + lane];
+        if (qc + 2 * kWave - 1 <= q1) {
+          const QRec x = *qn;
+          nlo = x.lo;
+          nhi = x.hi;
+        } else if (qc + kWave + lane <= q1) {
+          const QRec x = *qn;
           nlo = x.lo;
           nhi = x.hi;
         }
         while (pm) {
           const int j = (int)__builtin_ctzll(pm);
-          pm &= pm - 1;
+          pm ^= 1ull << j;
           const uint32_t lo = (uint32_t)readlane_i((int)xlo, j);
           const uint32_t hi = (uint32_t)readlane_i((int)xhi, j);
           uint64_t m = ballot(level_overlap(mq.hi, mq.lo, lo, hi));
@@ -1035,15 +1045,21 @@ __global__ __launch_bounds__(kThreads) void k_gsweep64(SweepArgs a, const Rec64 
       x.hi1 = x.hi2 = -INFINITY;
       x.v0 = x.v1 = x.v2 = x.type = 0;
       if (q0 + lane <= q1) x = rec[q0 + lane];
-      for (int qc = q0; qc <= q1; qc += kWave) {
-        uint64_t pm = ballot((qc + lane <= q1) & (x.lo1 <= uhi1) & (x.hi1 >= ulo1) &
-                             (x.lo2 <= uhi2) & (x.hi2 >= ulo2));
-        // the next 64 positions load while these are tested
+      // Lanes past q1 hold the empty box of type 0, which no row accepts, so
+      // the union test needs no range check of its own.
+      const Rec64 *xn = rec + ((int64_t)q0 + kWave + lane);  // this lane's next position
+      for (int qc = q0; qc <= q1; qc += kWave, xn += kWave) {
+        uint64_t pm = ballot((x.lo1 <= uhi1) & (x.hi1 >= ulo1) & (x.lo2 <= uhi2) & (x.hi2 >= ulo2));
+        // the next 64 positions load while these are tested; all but an
+        // item's last block are whole, and load without a per-lane check
         Rec64 nx;
         nx.lo1 = nx.lo2 = INFINITY;
         nx.hi1 = nx.hi2 = -INFINITY;
         nx.v0 = nx.v1 = nx.v2 = nx.type = 0;
-        if (qc + kWave + lane <= q1) nx = rec[qc + kWave + lane];
+        if (qc + 2 * kWave - 1 <= q1)
+          nx = *xn;
+        else if (qc + kWave + lane <= q1)
+          nx = *xn;
         uint32_t oql = 0;
         if (MODE != kCount && pm && qc + lane <= q1) oql = a.orig[qc + lane];
         while (pm) {
