@@ -215,6 +215,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_seg(const int *__restrict
                                                  unsigned long long *__restrict__ lane_steps) {
   typedef typename std::conditional<Q, uint32_t, uint64_t>::type K;
   __shared__ K key[Q ? kMaxWindow32 : kMaxWindow64];
+  __shared__ unsigned rs_cnt[Q ? (kPlanThreads / 64) * 256 : 1], rs_sum[Q ? 256 : 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   unsigned long long steps = 0;
   for (int b = blockIdx.x; b < nwin; b += gridDim.x) {  // grid-stride over windows
@@ -232,7 +233,14 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_seg(const int *__restrict
       key[j] = k;
     }
     __syncthreads();
-    window_sort_any<kPlanThreads, Q ? kMaxWindow32 : kMaxWindow64>(key, W);
+    if constexpr (Q) {
+      if (W >= kPlanThreads)
+        window_radix_sort<kPlanThreads>(key, W, rs_cnt, rs_sum);
+      else
+        window_sort_any<kPlanThreads, kMaxWindow32>(key, W);
+    } else {
+      window_sort_any<kPlanThreads, kMaxWindow64>(key, W);
+    }
     const uint32_t jmask = Q ? (1u << kWinPosBits) - 1 : 0xffffffffu;
     const int ng = (cnt + 63) / 64;
     for (int k = w; k < ng; k += kPlanThreads / 64) {

@@ -309,6 +309,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(const int *__restrict__ h
                                                    unsigned long long *__restrict__ lane_steps) {
   typedef typename std::conditional<Q, uint32_t, uint64_t>::type K;
   __shared__ K key[Q ? kMaxWindow32 : kMaxWindow64];
+  __shared__ unsigned rs_cnt[Q ? (kPlanThreads / 64) * 256 : 1], rs_sum[Q ? 256 : 1];
   const int64_t nwin = (n + W - 1) / W;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   unsigned long long steps = 0;
@@ -326,7 +327,14 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(const int *__restrict__ h
       key[j] = k;
     }
     __syncthreads();
-    window_sort_any<kPlanThreads, Q ? kMaxWindow32 : kMaxWindow64>(key, W);
+    if constexpr (Q) {
+      if (W >= kPlanThreads)
+        window_radix_sort<kPlanThreads>(key, W, rs_cnt, rs_sum);
+      else
+        window_sort_any<kPlanThreads, kMaxWindow32>(key, W);
+    } else {
+      window_sort_any<kPlanThreads, kMaxWindow64>(key, W);
+    }
     const uint32_t jmask = Q ? (1u << kWinPosBits) - 1 : 0xffffffffu;
     // one wave per group
     const int ng = (cnt + 63) / 64;
