@@ -315,6 +315,7 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
     nslot = 0;
   };
   int prow = -1, pnrows = -1;  // rows of the cached statistics
+  int64_t sc_lo = 0, sc_hi = -1;  // sorted positions of the cached scene
   bool act = false;
   int p = 0, h = 0;            // this lane's row and its chain end
   double n1 = 0, n2 = 0, o1 = 0, o2 = 0;
@@ -339,20 +340,32 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
     if (item.rowoff != prow || nrows != pnrows) {
       prow = item.rowoff;
       pnrows = nrows;
-      QParams qp = qp0;
-      int a1 = a10, a2 = a20;
-      if (sp) {
-        const SceneParams sc = sp[scene_of(seg_off, nscenes, rowlist[item.rowoff])];
-        qp = sc.qp;
-        a1 = sc.axis == 0 ? 1 : 0;
-        a2 = sc.axis == 2 ? 1 : 2;
+      // The scene's level range.  A group's rows lie in its planning window,
+      // and windows never straddle scenes, so the scene is that of sorted
+      // position rowoff; a wave's items are consecutive, so the scene is
+      // looked up only when the position leaves the last one's range.
+      if (sp ? (item.rowoff < sc_lo || item.rowoff >= sc_hi) : sc_hi < 0) {
+        QParams qp = qp0;
+        int a1 = a10, a2 = a20;
+        if (sp) {
+          const int sc = seg_off ? scene_of(seg_off, nscenes, item.rowoff) : 0;
+          sc_lo = seg_off ? seg_off[sc] : 0;
+          sc_hi = seg_off ? seg_off[sc + 1] : INT64_MAX;
+          const SceneParams scp = sp[sc];
+          qp = scp.qp;
+          a1 = scp.axis == 0 ? 1 : 0;
+          a2 = scp.axis == 2 ? 1 : 2;
+        } else {
+          sc_lo = 0;
+          sc_hi = INT64_MAX;
+        }
+        // axis-indexed fields by selects (a runtime index would put qp in scratch)
+        auto pick = [](const double *v, int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); };
+        n1 = pick(qp.scale, a1) / kLevelMax;
+        n2 = pick(qp.scale, a2) / kLevelMax;
+        o1 = pick(qp.origin, a1);
+        o2 = pick(qp.origin, a2);
       }
-      // axis-indexed fields by selects (a runtime index would put qp in scratch)
-      auto pick = [](const double *v, int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); };
-      n1 = pick(qp.scale, a1) / kLevelMax;
-      n2 = pick(qp.scale, a2) / kLevelMax;
-      o1 = pick(qp.origin, a1);
-      o2 = pick(qp.origin, a2);
       act = lane < nrows;
       float fr = act ? 1.f : 0.f;
       float le1 = 0, le2 = 0, c1 = 0, c2 = 0, tv = 0, te = 0, tf = 0, chain = 0;
