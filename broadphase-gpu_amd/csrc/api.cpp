@@ -673,7 +673,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
 
   // every buffer is reserved here: the launches below allocate nothing
   const size_t ts = f32 ? 4 : 8;
-  void *d_keys = keys.get((size_t)n * (f32 ? 4 : 8));
+  // reduced precision: the keys of all three axes (the sort reads its scene's)
+  void *d_keys = keys.get((size_t)n * (f32 ? 12 : 8));
   uint32_t *d_vals = vals.as<uint32_t>((size_t)n);
   uint32_t *d_skey = f32 ? nullptr : b_skey.as<uint32_t>((size_t)n);
   if (f32)
@@ -735,14 +736,21 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   auto enqueue = [&]() {
     launch_fill_u64(ctr, 0ull, kCounters, st);
     launch_seg_stats(boxes, t, bp.dev, opt.axis_policy, opt.axis_policy <= 2 ? opt.axis_policy : 0,
-                     f32, st);
-    launch_keys_seg(f32, boxes, t, bp.dev, d_keys, d_vals, reinterpret_cast<unsigned *>(&ctr[0]),
-                    st);
+                     f32 ? static_cast<uint32_t *>(d_keys) : nullptr,
+                     reinterpret_cast<unsigned *>(&ctr[0]), st);
+    if (!f32)
+      launch_keys_seg(f32, boxes, t, bp.dev, d_keys, d_vals, reinterpret_cast<unsigned *>(&ctr[0]),
+                      st);
     mark(E_KEYS);
     if (f32) {
-      // FP32 keys sorted within each scene (the scene order is kept)
-      radix_sort_u32_segmented(static_cast<uint32_t *>(d_keys), d_vals, n, 32, bp.dev.sort_tiles,
-                               (int64_t)t.sort_tiles.size(), sort, st);
+      // FP32 keys sorted within each scene (the scene order is kept), each
+      // scene's axis picked from the three-axis keys by its parameters
+      static_assert(sizeof(SceneParams) % sizeof(int) == 0 && offsetof(SceneParams, axis) == 0,
+                    "the sort reads a scene's axis as the first word of its parameters");
+      radix_sort_u32_segmented_axes(static_cast<uint32_t *>(d_keys), d_vals, n, 32,
+                                    bp.dev.sort_tiles, (int64_t)t.sort_tiles.size(),
+                                    reinterpret_cast<const int *>(bp.dev.params),
+                                    (int)(sizeof(SceneParams) / sizeof(int)), sort, st);
     } else {
       radix_sort_u64_sync_free(static_cast<uint64_t *>(d_keys), d_vals, n, 64, sort, st);
       if (sbits > 0) {

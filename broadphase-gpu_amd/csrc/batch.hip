@@ -20,10 +20,16 @@ __device__ inline void other_axes_b(int axis, int &a1, int &a2) {
 }
 
 // ---- per-scene statistics ---------------------------------------------------
+// With keys3 (reduced precision) the same pass writes the sort keys of every
+// axis, keys3[a n + i] = image of the rounded-down minimum on axis a, and
+// flags NaN or inverted boxes: the sort then reads its scene's axis, so the
+// boxes are read once before the sort instead of twice.
 __global__ __launch_bounds__(kT) void k_seg_stats(const stq_aabb_f64 *__restrict__ boxes,
                                                   const int64_t *__restrict__ cstart,
                                                   const int64_t *__restrict__ cend,
-                                                  double *__restrict__ partial) {
+                                                  double *__restrict__ partial,
+                                                  uint32_t *__restrict__ keys3, int64_t n,
+                                                  unsigned *__restrict__ err) {
   const int64_t b0 = cstart[blockIdx.x], b1 = cend[blockIdx.x];
   double acc[12], lo[3], up[3];
 #pragma unroll
@@ -33,7 +39,27 @@ __global__ __launch_bounds__(kT) void k_seg_stats(const stq_aabb_f64 *__restrict
     lo[a] = INFINITY;
     up[a] = -INFINITY;
   }
-  axis_stats_range(boxes, b0 + threadIdx.x, b1, kT, acc, lo, up);
+  if (!keys3) {
+    axis_stats_range(boxes, b0 + threadIdx.x, b1, kT, acc, lo, up);
+  } else {
+    bool bad = false;
+    auto put = [&](const stq_aabb_f64 &b, int64_t i) __attribute__((always_inline)) {
+      axis_stats_add(b, acc, lo, up);
+      bad |= !(b.min[0] <= b.max[0]) || !(b.min[1] <= b.max[1]) || !(b.min[2] <= b.max[2]);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) keys3[a * n + i] = key_of(round_down_f32(b.min[a]));
+    };
+    int64_t i = b0 + threadIdx.x;
+    for (; i + 3 * kT < b1; i += 4 * kT) {
+      stq_aabb_f64 b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) b[u] = boxes[i + u * kT];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) put(b[u], i + u * kT);
+    }
+    for (; i < b1; i += kT) put(boxes[i], i);
+    if (bad) atomicOr(err, kErrInverted);
+  }
   block_axis_stats<kT>(acc, lo, up, partial + (int64_t)blockIdx.x * kAxisStats);
 }
 
@@ -318,6 +344,8 @@ void build_batch_tables(const int64_t *off, int nscenes, int W, BatchTables &t) 
       x.count = (int32_t)std::min<int64_t>(tk, m - i * tk);
       x.cbase = 256 * t0 + i;
       x.stride = (int32_t)nt;
+      x.scene = s;
+      x.pad = 0;
       t.sort_tiles.push_back(x);
     }
   }
@@ -348,11 +376,11 @@ void build_batch_tables(const int64_t *off, int nscenes, int W, BatchTables &t) 
 }
 
 void launch_seg_stats(const stq_aabb_f64 *boxes, const BatchTables &t, const BatchDev &d,
-                      int policy, int fixed_axis, bool, hipStream_t st) {
+                      int policy, int fixed_axis, uint32_t *keys3, unsigned *err, hipStream_t st) {
   const int nc = (int)t.chunk_scene.size();
   if (nc > 0)
     hipLaunchKernelGGL(k_seg_stats, dim3(nc), dim3(kT), 0, st, boxes, d.chunk_start, d.chunk_end,
-                       d.partial);
+                       d.partial, keys3, t.n, err);
   if (t.nscenes > 0)
     hipLaunchKernelGGL(k_seg_finalize, dim3(nblk(t.nscenes)), dim3(kT), 0, st,
                        (const double *)d.partial, d.scene_chunk_begin, d.seg_off, t.nscenes, policy,

@@ -45,8 +45,11 @@ struct BatchDev {
   SegTile *sort_tiles = nullptr;
 };
 
+// keys3 (reduced precision; null: none): the sort keys of all three axes
+// (3 n words), written by the statistics pass itself, which then also sets
+// kErrInverted in *err for NaN or inverted boxes.
 void launch_seg_stats(const stq_aabb_f64 *boxes, const BatchTables &t, const BatchDev &d,
-                      int policy, int fixed_axis, bool need_q, hipStream_t st);
+                      int policy, int fixed_axis, uint32_t *keys3, unsigned *err, hipStream_t st);
 // F32 form: 4-byte keys, the image of the rounded-down minimum, sorted per
 // scene by the segmented sort; F64 form: 8-byte keys, the image of the FP64
 // minimum (a second, stable sort by scene follows).

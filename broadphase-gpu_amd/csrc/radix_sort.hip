@@ -76,12 +76,22 @@ __device__ __forceinline__ int64_t count_index(const SegTile *tiles, int ntiles,
   return (int64_t)d * ntiles + blockIdx.x;
 }
 
+// With scene_axis (segmented first pass over three-axis keys) the tile reads
+// the keys of its scene's axis: keys + axis * n.
+__device__ __forceinline__ int64_t key_source(const SegTile *tiles, const int *scene_axis,
+                                              int axis_stride, int64_t n) {
+  if (!scene_axis) return 0;
+  return (int64_t)scene_axis[(int64_t)tiles[blockIdx.x].scene * axis_stride] * n;
+}
+
 template <class K>
 __global__ __launch_bounds__(kSortThreads) void k_count(const K *__restrict__ keys, int64_t n,
                                                          int shift, int ntiles,
                                                          unsigned *__restrict__ counts,
                                                          const unsigned *__restrict__ skip,
-                                                         const SegTile *__restrict__ tiles) {
+                                                         const SegTile *__restrict__ tiles,
+                                                         const int *__restrict__ scene_axis = nullptr,
+                                                         int axis_stride = 0) {
   if (skip && skip[shift >> 3]) return;
   __shared__ unsigned h[256];
   h[threadIdx.x] = 0;
@@ -89,6 +99,7 @@ __global__ __launch_bounds__(kSortThreads) void k_count(const K *__restrict__ ke
   int64_t base;
   int cnt;
   tile_of(tiles, n, base, cnt);
+  keys += key_source(tiles, scene_axis, axis_stride, n);
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const int j = r * kSortThreads + threadIdx.x;
@@ -107,10 +118,13 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const K *__restrict__ 
                                                            int shift, int ntiles,
                                                            const unsigned *__restrict__ offsets,
                                                            const unsigned *__restrict__ skip,
-                                                           const SegTile *__restrict__ tiles) {
+                                                           const SegTile *__restrict__ tiles,
+                                                           const int *__restrict__ scene_axis = nullptr,
+                                                           int axis_stride = 0) {
   int64_t tile0;
   int tcnt;
   tile_of(tiles, n, tile0, tcnt);
+  kin += key_source(tiles, scene_axis, axis_stride, n);  // vin null: the values are positions
   if (skip && skip[shift >> 3]) {
     // the digit is equal for every key: the stable order is the identity
     for (int j = threadIdx.x; j < tcnt; j += kSortThreads) {
@@ -143,7 +157,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const K *__restrict__ 
     const bool valid = j < tcnt;
     const int64_t i = tile0 + j;
     key[r] = valid ? kin[i] : (K)0;
-    val[r] = valid ? vin[i] : 0u;
+    val[r] = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
     const unsigned dig = (unsigned)(key[r] >> shift) & 255u;
     // lanes holding the same digit: intersect the 8 bit-planes
     uint64_t peers = __ballot(valid);
@@ -444,6 +458,44 @@ void radix_sort_u32_segmented(uint32_t *keys, uint32_t *vals, int64_t n, int bit
     STQ_HIP(hipMemcpyAsync(vals, src_v, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
   }
 }
+void radix_sort_u32_segmented_axes(uint32_t *keys3, uint32_t *vals, int64_t n, int bits,
+                                   const SegTile *tiles, int64_t ntiles, const int *scene_axis,
+                                   int axis_stride, SortScratch &s, hipStream_t st) {
+  if (n <= 0 || ntiles <= 0) return;
+  const int ndigits = (bits + 7) / 8;
+  uint32_t *ak = s.alt_keys.as<uint32_t>((size_t)n);
+  uint32_t *av = s.alt_vals.as<uint32_t>((size_t)n);
+  unsigned *counts = s.counts.as<unsigned>((size_t)(256 * ntiles + 1));
+  scan_scratch_reserve<unsigned>(256 * ntiles, s.misc);
+  uint32_t *src_k = keys3, *dst_k = ak, *src_v = nullptr, *dst_v = av;
+  for (int d = 0; d < ndigits; ++d) {
+    const int shift = 8 * d;
+    const int *sa = d == 0 ? scene_axis : nullptr;
+    hipLaunchKernelGGL(k_count<uint32_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, st,
+                       (const uint32_t *)src_k, n, shift, (int)ntiles, counts,
+                       (const unsigned *)nullptr, tiles, sa, axis_stride);
+    scan_impl<unsigned>(counts, counts, 256 * ntiles, s.misc, st);
+    hipLaunchKernelGGL(k_scatter<uint32_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, st,
+                       (const uint32_t *)src_k, (const uint32_t *)src_v, dst_k, dst_v, n, shift,
+                       (int)ntiles, (const unsigned *)counts, (const unsigned *)nullptr, tiles, sa,
+                       axis_stride);
+    STQ_HIP(hipGetLastError());
+    if (d == 0) {  // from here on the keys ping-pong between alt and keys3[0 .. n)
+      src_k = ak;
+      dst_k = keys3;
+      src_v = av;
+      dst_v = vals;
+    } else {
+      std::swap(src_k, dst_k);
+      std::swap(src_v, dst_v);
+    }
+  }
+  if (src_k != keys3) {
+    STQ_HIP(hipMemcpyAsync(keys3, src_k, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+    STQ_HIP(hipMemcpyAsync(vals, src_v, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+  }
+}
+
 void radix_sort_segmented_reserve(int64_t n, int64_t ntiles, SortScratch &s) {
   if (n <= 1) return;
   s.alt_keys.as<uint32_t>((size_t)n);

@@ -225,6 +225,8 @@ struct SegTile {
   int64_t cbase;
   int32_t count;
   int32_t stride;
+  int32_t scene;  // the segment's scene (picks its sweep axis's keys in the first pass)
+  int32_t pad;
 };
 // Keys per sort tile.
 int sort_tile_keys();
@@ -235,6 +237,14 @@ void radix_sort_u32_segmented(uint32_t *keys, uint32_t *vals, int64_t n, int bit
                               const SegTile *tiles, int64_t ntiles, SortScratch &s,
                               hipStream_t st);
 void radix_sort_segmented_reserve(int64_t n, int64_t ntiles, SortScratch &s);
+// The same sort with the keys of all three axes as input: keys3 holds 3 n
+// keys (axis a at keys3[a n ..]); tile t reads the axis scene_axis[scene *
+// axis_stride] of its scene (the first word of the scene's parameters) in the
+// first pass, and the values start as the positions 0 .. n-1.  The sorted keys
+// end in keys3[0 .. n), their positions in vals.
+void radix_sort_u32_segmented_axes(uint32_t *keys3, uint32_t *vals, int64_t n, int bits,
+                                   const SegTile *tiles, int64_t ntiles, const int *scene_axis,
+                                   int axis_stride, SortScratch &s, hipStream_t st);
 
 // Exclusive prefix sum of n uint32 counts into n+1 int64 offsets (out[n] is
 // the total).  Scratch is taken from `tmp`.  With n_dev, only the first
