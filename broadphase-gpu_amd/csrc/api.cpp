@@ -93,24 +93,28 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   const int pol = opt.axis_policy;
   const int axis = pol <= 2 ? pol : 0;  // where no statistics are made
   SceneParams *d_sp = nullptr;
+  uint32_t *order = vals.as<uint32_t>((size_t)n);
+  // reduced precision: the statistics pass writes the keys of all three axes
+  // and the sort reads those of the chosen axis
+  void *kbuf = keys.get((size_t)n * (f32 ? 12 : 8));
   if (pol > 2 || grouped || learned) {
     // about four boxes per thread (the loads of one thread overlap)
     const int nb = (int)std::min<int64_t>((n + 1023) / 1024, 4096);
     double *part = axis_partial.as<double>((size_t)nb * kAxisStats);
     d_sp = scene_params.as<SceneParams>(1);
-    launch_axis_stats(boxes, n, part, nb, st);
+    launch_axis_stats(boxes, n, part, nb, st, f32 ? static_cast<uint32_t *>(kbuf) : nullptr,
+                      (unsigned *)&ctr[0]);
     launch_axis_final(part, nb, n, pol, d_sp, st);
   }
   stats.axis = axis;
-  uint32_t *order = vals.as<uint32_t>((size_t)n);
-  void *kbuf = keys.get((size_t)n * (f32 ? 4 : 8));
-  launch_keys(f32, boxes, n, axis, d_sp, kbuf, order, (unsigned *)&ctr[0], st);
+  if (!f32) launch_keys(f32, boxes, n, axis, d_sp, kbuf, order, (unsigned *)&ctr[0], st);
   STQ_HIP(hipEventRecord(ev[E_KEYS], st));
   if (f32) {
     // all four digits of FP32 keys usually vary: every digit a pass, with no
     // digit histogram and no host read-back of it
-    radix_sort_reserve(n, 4, sort);
-    radix_sort_u32_all_digits((uint32_t *)kbuf, order, n, 32, sort, st);
+    const int64_t ntiles = (n + sort_tile_keys() - 1) / sort_tile_keys();
+    radix_sort_u32_segmented_axes(static_cast<uint32_t *>(kbuf), order, n, 32, nullptr, ntiles,
+                                  &d_sp->axis, 0, sort, st);
   } else {
     int passes = 0;
     radix_sort_u64((uint64_t *)kbuf, order, n, 64, sort, st, &passes);

@@ -43,21 +43,7 @@ __global__ __launch_bounds__(kT) void k_seg_stats(const stq_aabb_f64 *__restrict
     axis_stats_range(boxes, b0 + threadIdx.x, b1, kT, acc, lo, up);
   } else {
     bool bad = false;
-    auto put = [&](const stq_aabb_f64 &b, int64_t i) __attribute__((always_inline)) {
-      axis_stats_add(b, acc, lo, up);
-      bad |= !(b.min[0] <= b.max[0]) || !(b.min[1] <= b.max[1]) || !(b.min[2] <= b.max[2]);
-#pragma unroll
-      for (int a = 0; a < 3; ++a) keys3[a * n + i] = key_of(round_down_f32(b.min[a]));
-    };
-    int64_t i = b0 + threadIdx.x;
-    for (; i + 3 * kT < b1; i += 4 * kT) {
-      stq_aabb_f64 b[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) b[u] = boxes[i + u * kT];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) put(b[u], i + u * kT);
-    }
-    for (; i < b1; i += kT) put(boxes[i], i);
+    axis_stats_keys_range(boxes, b0 + threadIdx.x, b1, kT, acc, lo, up, keys3, n, bad);
     if (bad) atomicOr(err, kErrInverted);
   }
   block_axis_stats<kT>(acc, lo, up, partial + (int64_t)blockIdx.x * kAxisStats);

@@ -81,7 +81,8 @@ __device__ __forceinline__ int64_t count_index(const SegTile *tiles, int ntiles,
 __device__ __forceinline__ int64_t key_source(const SegTile *tiles, const int *scene_axis,
                                               int axis_stride, int64_t n) {
   if (!scene_axis) return 0;
-  return (int64_t)scene_axis[(int64_t)tiles[blockIdx.x].scene * axis_stride] * n;
+  const int64_t scene = tiles ? tiles[blockIdx.x].scene : 0;
+  return (int64_t)scene_axis[scene * axis_stride] * n;
 }
 
 template <class K>

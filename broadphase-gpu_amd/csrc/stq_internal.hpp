@@ -241,7 +241,8 @@ void radix_sort_segmented_reserve(int64_t n, int64_t ntiles, SortScratch &s);
 // keys (axis a at keys3[a n ..]); tile t reads the axis scene_axis[scene *
 // axis_stride] of its scene (the first word of the scene's parameters) in the
 // first pass, and the values start as the positions 0 .. n-1.  The sorted keys
-// end in keys3[0 .. n), their positions in vals.
+// end in keys3[0 .. n), their positions in vals.  With tiles null: one scene,
+// tiles of sort_tile_keys() keys over all n (ntiles of them).
 void radix_sort_u32_segmented_axes(uint32_t *keys3, uint32_t *vals, int64_t n, int bits,
                                    const SegTile *tiles, int64_t ntiles, const int *scene_axis,
                                    int axis_stride, SortScratch &s, hipStream_t st);

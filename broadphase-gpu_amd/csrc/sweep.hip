@@ -52,7 +52,9 @@ __device__ inline bool bad(double a, double b) { return !(a <= b); }  // NaN or 
 // Per axis: sum(min+max), sum(min^2+max^2), sum(max-min), sum(((min+max)/2)^2),
 // then min(min) and max(max) for the quantisation range.
 __global__ __launch_bounds__(kThreads) void k_axis_stats(const stq_aabb_f64 *__restrict__ boxes,
-                                                         int64_t n, double *__restrict__ partial) {
+                                                         int64_t n, double *__restrict__ partial,
+                                                         uint32_t *__restrict__ keys3,
+                                                         unsigned *__restrict__ err) {
   double acc[12], lo[3], up[3];
 #pragma unroll
   for (int k = 0; k < 12; ++k) acc[k] = 0.0;
@@ -61,8 +63,15 @@ __global__ __launch_bounds__(kThreads) void k_axis_stats(const stq_aabb_f64 *__r
     lo[a] = INFINITY;
     up[a] = -INFINITY;
   }
-  axis_stats_range(boxes, (int64_t)blockIdx.x * kThreads + threadIdx.x, n,
-                   (int64_t)gridDim.x * kThreads, acc, lo, up);
+  if (!keys3) {
+    axis_stats_range(boxes, (int64_t)blockIdx.x * kThreads + threadIdx.x, n,
+                     (int64_t)gridDim.x * kThreads, acc, lo, up);
+  } else {
+    bool bad = false;
+    axis_stats_keys_range(boxes, (int64_t)blockIdx.x * kThreads + threadIdx.x, n,
+                          (int64_t)gridDim.x * kThreads, acc, lo, up, keys3, n, bad);
+    if (bad) atomicOr(err, kErrInverted);
+  }
   block_axis_stats<kThreads>(acc, lo, up, partial + (int64_t)blockIdx.x * kAxisStats);
 }
 
@@ -1102,8 +1111,9 @@ inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kThreads - 1) / k
 
 // ---- host wrappers -------------------------------------------------------------------
 void launch_axis_stats(const stq_aabb_f64 *boxes, int64_t n, double *partial, int nblocks,
-                       hipStream_t st) {
-  hipLaunchKernelGGL(k_axis_stats, dim3(nblocks), dim3(kThreads), 0, st, boxes, n, partial);
+                       hipStream_t st, uint32_t *keys3, unsigned *err) {
+  hipLaunchKernelGGL(k_axis_stats, dim3(nblocks), dim3(kThreads), 0, st, boxes, n, partial, keys3,
+                     err);
   STQ_HIP(hipGetLastError());
 }
 

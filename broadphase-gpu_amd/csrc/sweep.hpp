@@ -63,6 +63,28 @@ __device__ inline void axis_stats_range(const stq_aabb_f64 *__restrict__ boxes, 
   }
   for (; i < i1; i += step) axis_stats_add(boxes[i], acc, lo, up);
 }
+// The same statistics, also writing the sort keys of every axis (reduced
+// precision): keys3[a n + i] = image of the rounded-down minimum on axis a;
+// *bad is set for a NaN or inverted box.
+__device__ inline void axis_stats_keys_range(const stq_aabb_f64 *__restrict__ boxes, int64_t i,
+                                             int64_t i1, int64_t step, double *acc, double *lo,
+                                             double *up, uint32_t *__restrict__ keys3, int64_t n,
+                                             bool &bad) {
+  auto put = [&](const stq_aabb_f64 &b, int64_t j) __attribute__((always_inline)) {
+    axis_stats_add(b, acc, lo, up);
+    bad |= !(b.min[0] <= b.max[0]) || !(b.min[1] <= b.max[1]) || !(b.min[2] <= b.max[2]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) keys3[a * n + j] = key_of(round_down_f32(b.min[a]));
+  };
+  for (; i + 3 * step < i1; i += 4 * step) {
+    stq_aabb_f64 b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) b[u] = boxes[i + u * step];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) put(b[u], i + u * step);
+  }
+  for (; i < i1; i += step) put(boxes[i], i);
+}
 // Combines the axis statistics of the block's threads (12 sums, 3 minima,
 // 3 maxima; NT threads, all call this) into out[0 .. kAxisStats): shuffle
 // trees inside each wave, then the waves in order, so the result is the same
@@ -276,8 +298,11 @@ __device__ __forceinline__ int64_t last_le(const T *__restrict__ smin, int64_t s
   return lo;
 }
 
+// keys3 (reduced precision; null: none): the pass also writes the sort keys
+// of all three axes (3 n words) and sets kErrInverted in *err for a NaN or
+// inverted box.
 void launch_axis_stats(const stq_aabb_f64 *boxes, int64_t n, double *partial, int nblocks,
-                       hipStream_t st);
+                       hipStream_t st, uint32_t *keys3 = nullptr, unsigned *err = nullptr);
 // The scene's sweep axis (axis policy 0..4) and level range from the
 // statistics of nblocks blocks, into *out on the device.
 void launch_axis_final(const double *partial, int nblocks, int64_t n, int policy,
