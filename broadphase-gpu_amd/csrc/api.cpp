@@ -646,7 +646,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   if (n > INT32_MAX) throw StatusError(STQ_ERR_TOO_LARGE, "more than 2^31-1 boxes in a batch");
   stats.n_boxes = n;
   const int W = opt.window ? (grouped ? opt.window : std::min(opt.window, kMaxWindow64))
-                           : (grouped ? kQWindow : kBatchWindow);
+                           : (grouped ? kBatchQWindow : kBatchWindow);
   const int chunk = auto_chunk(opt);
   batch_plan(off.data(), nscenes, W, chunk);
   BatchPlan &bp = bplan;

@@ -88,7 +88,7 @@ void Context::autotune_and_apply(const stq_aabb_f64 *boxes, int64_t n, const int
       }
       return best;
     };
-    TuneEntry best{STQ_AXIS_CHAIN, batched ? (grouped ? kQWindow : kBatchWindow) : 0,
+    TuneEntry best{STQ_AXIS_CHAIN, batched ? (grouped ? kBatchQWindow : kBatchWindow) : 0,
                    grouped ? kQChunk : 2048, 2, 32, grouped ? 0 : 400, 0};
     best.ms = trial(best);
     auto consider = [&](TuneEntry e) {
