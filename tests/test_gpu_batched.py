@@ -295,3 +295,52 @@ def test_batched_learned_budget_growth():
     finally:
         c.close()
     check_scenes(ref_pairs, ref_offs, sc, "budget")
+
+
+@pytest.mark.parametrize("change", [dict(tails=1, tail_cost=400), dict(claim=2), dict(balance=2),
+                                    dict(sizing=stq_amd.SIZING_COUNT_FILL)])
+def test_graph_follows_option_changes(change):
+    """A captured graph is replayed only for the options it was captured with:
+    after a change of tail items, claiming, work split or sizing, the calls
+    give what an eager context with the new options gives (pairs, order and
+    item count), and the old graph is never replayed."""
+    sc = mixed_scenes(77, 11, nmax=30000)
+    allb, offs = scenes.concat_scenes(sc)
+    base = dict(precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=4,
+                sizing=stq_amd.SIZING_LEARNED, train_scenes=1)
+    c = stq_amd.Context(0, graph=0, **base)
+    db = _lib.DeviceBuffer.from_array(allb)
+    cap = 1 << 22
+    dout = _lib.DeviceBuffer(8 * cap)
+    try:
+        states = []
+        for _ in range(4):  # train/eager, eager, capture, replay
+            rc, poffs = c.find_pairs_batched_device(db.ptr, offs, dout.ptr, cap)
+            assert rc == _lib.STQ_OK
+            states.append(c.stats()["graph"])
+        assert states[-1] == 2, states
+        new = {**base, **change}
+        c.set_options(graph=0, **new)
+        eager = stq_amd.Context(0, graph=2, **new)
+        try:
+            eager.find_pairs_batched(allb, offs)  # trains when learned
+            ref_pairs, ref_offs = eager.find_pairs_batched(allb, offs)
+            ref_items = eager.stats()["n_items"]
+        finally:
+            eager.close()
+        after = []
+        for _ in range(4):
+            rc, poffs = c.find_pairs_batched_device(db.ptr, offs, dout.ptr, cap)
+            assert rc == _lib.STQ_OK
+            st = c.stats()
+            after.append(st["graph"])
+            pairs = dout.to_array(np.int32, 2 * int(poffs[-1])).reshape(-1, 2)
+            assert np.array_equal(poffs, ref_offs), change
+            assert np.array_equal(pairs, ref_pairs), change
+            assert st["n_items"] == ref_items, change
+        assert after[0] != 2, (change, after)
+        print(f"graph states {states} -> {after}", flush=True)
+    finally:
+        dout.close()
+        db.close()
+        c.close()
