@@ -93,7 +93,8 @@ def test_row_ranges_partition_the_pair_set(ctx, cuts):
 
 def radix(ctx, keys, bits, form):
     kb = keys.dtype.itemsize
-    vals = np.arange(len(keys), dtype=np.uint32)
+    # values distinct from the positions, so a sort that drops them shows
+    vals = ((np.arange(len(keys), dtype=np.uint64) * np.uint64(2654435761)) % np.uint64(1 << 32)).astype(np.uint32)
     dk, dv = stq_amd.DeviceBuffer.from_array(keys), stq_amd.DeviceBuffer.from_array(vals)
     rc = ctx._L.stq_radix_sort_device(ctx.ptr, dk.ptr, dv.ptr, len(keys), kb, bits, form)
     assert rc == 0, ctx._L.stq_last_error(ctx.ptr)
@@ -126,7 +127,8 @@ def test_radix_sort_matches_numpy_stable_argsort(ctx, form):
             continue
         k, v = radix(ctx, keys, bits, form)
         ref = np.argsort(keys, kind="stable")
-        assert np.array_equal(v, ref.astype(np.uint32)), (name, form)
+        vals = ((np.arange(len(keys), dtype=np.uint64) * np.uint64(2654435761)) % np.uint64(1 << 32)).astype(np.uint32)
+        assert np.array_equal(v, vals[ref]), (name, form)
         assert np.array_equal(k, keys[ref]), (name, form)
 
 
