@@ -110,11 +110,12 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   if (!f32) launch_keys(f32, boxes, n, axis, d_sp, kbuf, order, (unsigned *)&ctr[0], st);
   STQ_HIP(hipEventRecord(ev[E_KEYS], st));
   if (f32) {
-    // all four digits of FP32 keys usually vary: every digit a pass, with no
-    // digit histogram and no host read-back of it
+    // the sweep keys (sweep_key_bits(n) bits, made from the FP32 keys of the
+    // chosen axis in the first pass): every digit a pass, with no digit
+    // histogram and no host read-back of it
     const int64_t ntiles = (n + sort_tile_keys() - 1) / sort_tile_keys();
-    radix_sort_u32_segmented_axes(static_cast<uint32_t *>(kbuf), order, n, 32, nullptr, ntiles,
-                                  &d_sp->axis, 0, sort, st);
+    radix_sort_u32_segmented_axes(static_cast<uint32_t *>(kbuf), order, n, sweep_key_bits(n),
+                                  nullptr, ntiles, d_sp, sort, st);
   } else {
     int passes = 0;
     radix_sort_u64((uint64_t *)kbuf, order, n, 64, sort, st, &passes);
@@ -669,10 +670,15 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   const bool training = learned && scenes_seen < train_scenes;
   const bool single_pass = learned && !training;
   if (learned) sizer_ensure();
-  if (single_pass && b_budget == 0)
-    b_budget = std::max<int64_t>(int64_t(1) << 22,
-                                 (int64_t)kBatchCapScale * b_last_pairs +
-                                     (int64_t)kBatchCapSlack * b_last_items);
+  // the first budget: the slots the previous batch's exact counts would ask
+  // for, and a quarter more, since the predictions may ask for a little more
+  // and a budget that grows makes a new buffer (the call after it is not
+  // captured)
+  if (single_pass && b_budget == 0) {
+    const int64_t slots = (int64_t)kBatchCapScale * b_last_pairs +
+                          (int64_t)kBatchCapSlack * b_last_items;
+    b_budget = std::max<int64_t>(int64_t(1) << 22, slots + slots / 4);
+  }
   if (single_pass) b_budget = std::min(b_budget, scratch_limit_pairs());
 
   // every buffer is reserved here: the launches below allocate nothing
@@ -747,14 +753,11 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
                       st);
     mark(E_KEYS);
     if (f32) {
-      // FP32 keys sorted within each scene (the scene order is kept), each
-      // scene's axis picked from the three-axis keys by its parameters
-      static_assert(sizeof(SceneParams) % sizeof(int) == 0 && offsetof(SceneParams, axis) == 0,
-                    "the sort reads a scene's axis as the first word of its parameters");
-      radix_sort_u32_segmented_axes(static_cast<uint32_t *>(d_keys), d_vals, n, 32,
-                                    bp.dev.sort_tiles, (int64_t)t.sort_tiles.size(),
-                                    reinterpret_cast<const int *>(bp.dev.params),
-                                    (int)(sizeof(SceneParams) / sizeof(int)), sort, st);
+      // sweep keys sorted within each scene (the scene order is kept), each
+      // scene's made from the three-axis FP32 keys by its parameters
+      radix_sort_u32_segmented_axes(static_cast<uint32_t *>(d_keys), d_vals, n,
+                                    sweep_key_bits(t.max_scene), bp.dev.sort_tiles,
+                                    (int64_t)t.sort_tiles.size(), bp.dev.params, sort, st);
     } else {
       radix_sort_u64_sync_free(static_cast<uint64_t *>(d_keys), d_vals, n, 64, sort, st);
       if (sbits > 0) {

@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kT) void k_seg_finalize(const double *__restrict__ 
                                                      const int *__restrict__ scb,
                                                      const int64_t *__restrict__ seg_off,
                                                      int nscenes, int policy, int fixed_axis,
-                                                     SceneParams *__restrict__ out) {
+                                                     int kbits, SceneParams *__restrict__ out) {
   const int s = blockIdx.x * kT + threadIdx.x;
   if (s >= nscenes) return;
   double v[kAxisStats];
@@ -66,7 +66,6 @@ __global__ __launch_bounds__(kT) void k_seg_finalize(const double *__restrict__ 
       v[k] = k < 12 ? v[k] + x : (k < 15 ? fmin(v[k], x) : fmax(v[k], x));
     }
   SceneParams p;
-  p.pad = 0;
   for (int a = 0; a < 3; ++a) {
     const double lo = v[12 + a], up = v[15 + a], range = up - lo;
     p.qp.origin[a] = isfinite(lo) ? lo : 0.0;
@@ -91,6 +90,7 @@ __global__ __launch_bounds__(kT) void k_seg_finalize(const double *__restrict__ 
     }
   }
   p.axis = best;
+  set_sweep_keys(p, v[12 + best], v[15 + best], kbits);
   out[s] = p;
 }
 
@@ -155,8 +155,8 @@ __global__ __launch_bounds__(kT) void k_gather_seg(const stq_aabb_f64 *__restric
     o2 = p.qp.origin[a2];
     n2 = p.qp.scale[a2] / kLevelMax;
     if (f32) {
-      static_cast<float *>(smin)[s] = round_down_f32(b.min[axis]);
-      static_cast<float *>(smax)[s] = round_up_f32(b.max[axis]);
+      static_cast<uint32_t *>(smin)[s] = skey_down(round_down_f32(b.min[axis]), p);  // sort key
+      static_cast<uint32_t *>(smax)[s] = skey_up(round_up_f32(b.max[axis]), p);
       Exact64 r;
       r.lo0 = b.min[axis];
       r.hi0 = b.max[axis];
@@ -195,20 +195,41 @@ __global__ __launch_bounds__(kT) void k_gather_seg(const stq_aabb_f64 *__restric
 }
 
 // ---- chain ends, bounded by the scene ---------------------------------------
+// As k_chain_end: tiles of rows with their minima in LDS; a row's search
+// stops at the end of its scene.
 template <class T>
-__global__ __launch_bounds__(kT) void k_chain_end_seg(const T *__restrict__ smin,
-                                                      const T *__restrict__ smax, int64_t n,
-                                                      const int64_t *__restrict__ seg_off,
-                                                      int nscenes, int *__restrict__ hi,
-                                                      unsigned long long *__restrict__ cand) {
+__global__ __launch_bounds__(kCeThreads) void k_chain_end_seg(const T *__restrict__ smin,
+                                                              const T *__restrict__ smax,
+                                                              int64_t n,
+                                                              const int64_t *__restrict__ seg_off,
+                                                              int nscenes, int *__restrict__ hi,
+                                                              unsigned long long *__restrict__ cand) {
+  __shared__ T win[kCeWin<T>];
+  __shared__ int sc_range[2];
   unsigned long long len = 0;
-  for (int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x; s < n; s += (int64_t)gridDim.x * kT) {
-    const int64_t end = seg_off[scene_of(seg_off, nscenes, s) + 1];
-    const int64_t lo = last_le(smin, s, end, smax[s]);
-    hi[s] = (int)lo;
-    len += (unsigned long long)(lo - s);
+  const int64_t ntiles = (n + kCeRows<T> - 1) / kCeRows<T>;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int64_t s0 = t * kCeRows<T>;
+    const int64_t s1 = s0 + kCeRows<T> < n ? s0 + kCeRows<T> : n;
+    if (threadIdx.x == 0) {  // the scenes of the tile's first and last rows
+      sc_range[0] = scene_of(seg_off, nscenes, s0);
+      sc_range[1] = scene_of(seg_off, nscenes, s1 - 1);
+    }
+    __syncthreads();
+    const int sa = sc_range[0], sb = sc_range[1];
+    // the end of a row's scene: one lookup when the tile lies in one scene,
+    // else a search among the tile's scenes
+    auto end_of = [=](int64_t s) {
+      int lo = sa, up = sb + 1;
+      while (up - lo > 1) {
+        const int mid = (lo + up) >> 1;
+        if (seg_off[mid] <= s) lo = mid; else up = mid;
+      }
+      return seg_off[lo + 1];
+    };
+    len += chain_end_tile(smin, smax, n, s0, (int64_t)0, n, win, end_of, hi);
   }
-  block_atomic_add<kT>(len, cand);
+  block_atomic_add<kCeThreads>(len, cand);
 }
 
 // ---- planning -------------------------------------------------------------------
@@ -311,6 +332,8 @@ void build_batch_tables(const int64_t *off, int nscenes, int W, BatchTables &t) 
   t.nscenes = nscenes;
   t.seg_off.assign(off, off + nscenes + 1);
   t.n = off[nscenes];
+  t.max_scene = 0;
+  for (int s = 0; s < nscenes; ++s) t.max_scene = std::max(t.max_scene, off[s + 1] - off[s]);
   t.chunk_scene.clear();
   t.chunk_start.clear();
   t.chunk_end.clear();
@@ -370,7 +393,7 @@ void launch_seg_stats(const stq_aabb_f64 *boxes, const BatchTables &t, const Bat
   if (t.nscenes > 0)
     hipLaunchKernelGGL(k_seg_finalize, dim3(nblk(t.nscenes)), dim3(kT), 0, st,
                        (const double *)d.partial, d.scene_chunk_begin, d.seg_off, t.nscenes, policy,
-                       fixed_axis, d.params);
+                       fixed_axis, sweep_key_bits(t.max_scene), d.params);
   STQ_HIP(hipGetLastError());
 }
 
@@ -403,13 +426,17 @@ void launch_gather_seg(bool f32, const stq_aabb_f64 *boxes, const uint32_t *orde
 void launch_chain_end_seg(bool f32, const void *smin, const void *smax, const BatchTables &t,
                           const BatchDev &d, int *hi, unsigned long long *cand, hipStream_t st) {
   if (t.n <= 0) return;
-  const unsigned g = std::min(nblk(t.n), kReduceGrid);
+  // two blocks per CU, persistent over the tiles (one counter atomic per block)
+  const int64_t rows = f32 ? kCeRows<uint32_t> : kCeRows<double>;
+  const unsigned g = (unsigned)std::min<int64_t>((t.n + rows - 1) / rows, 512);
   if (f32)
-    hipLaunchKernelGGL(k_chain_end_seg<float>, dim3(g), dim3(kT), 0, st, (const float *)smin,
-                       (const float *)smax, t.n, d.seg_off, t.nscenes, hi, cand);
+    hipLaunchKernelGGL(k_chain_end_seg<uint32_t>, dim3(g), dim3(kCeThreads), 0, st,
+                       (const uint32_t *)smin, (const uint32_t *)smax, t.n, d.seg_off, t.nscenes,
+                       hi, cand);
   else
-    hipLaunchKernelGGL(k_chain_end_seg<double>, dim3(g), dim3(kT), 0, st, (const double *)smin,
-                       (const double *)smax, t.n, d.seg_off, t.nscenes, hi, cand);
+    hipLaunchKernelGGL(k_chain_end_seg<double>, dim3(g), dim3(kCeThreads), 0, st,
+                       (const double *)smin, (const double *)smax, t.n, d.seg_off, t.nscenes, hi,
+                       cand);
   STQ_HIP(hipGetLastError());
 }
 

@@ -17,6 +17,7 @@ namespace stqamd {
 struct BatchTables {
   int nscenes = 0;
   int64_t n = 0;                       // boxes in all scenes
+  int64_t max_scene = 0;               // boxes in the largest scene
   std::vector<int64_t> seg_off;        // nscenes + 1
   std::vector<int> chunk_scene;        // statistics chunks: scene, [start, end)
   std::vector<int64_t> chunk_start, chunk_end;
@@ -45,9 +46,10 @@ struct BatchDev {
   SegTile *sort_tiles = nullptr;
 };
 
-// keys3 (reduced precision; null: none): the sort keys of all three axes
-// (3 n words), written by the statistics pass itself, which then also sets
-// kErrInverted in *err for NaN or inverted boxes.
+// keys3 (reduced precision; null: none): the FP32 sort keys of all three
+// axes (3 n words), written by the statistics pass itself, which then also
+// sets kErrInverted in *err for NaN or inverted boxes; the scene parameters
+// carry the quantisation of the sweep keys, sweep_key_bits(t.max_scene) bits.
 void launch_seg_stats(const stq_aabb_f64 *boxes, const BatchTables &t, const BatchDev &d,
                       int policy, int fixed_axis, uint32_t *keys3, unsigned *err, hipStream_t st);
 // F32 form: 4-byte keys, the image of the rounded-down minimum, sorted per

@@ -16,6 +16,7 @@
 // A digit that is equal for all keys (one bucket holds all n) is skipped; the
 // per-digit histograms of all passes come from one pre-pass.
 #include "stq_internal.hpp"
+#include "sweep.hpp"
 
 namespace stqamd {
 
@@ -76,13 +77,27 @@ __device__ __forceinline__ int64_t count_index(const SegTile *tiles, int ntiles,
   return (int64_t)d * ntiles + blockIdx.x;
 }
 
-// With scene_axis (segmented first pass over three-axis keys) the tile reads
-// the keys of its scene's axis: keys + axis * n.
-__device__ __forceinline__ int64_t key_source(const SegTile *tiles, const int *scene_axis,
-                                              int axis_stride, int64_t n) {
-  if (!scene_axis) return 0;
-  const int64_t scene = tiles ? tiles[blockIdx.x].scene : 0;
-  return (int64_t)scene_axis[scene * axis_stride] * n;
+// With scene parameters (the first pass over the FP32 keys of all three axes)
+// the tile reads the keys of its scene's sweep axis, keys + axis * n, and
+// sorts their sweep keys: each key is turned into skey_down of its float.
+struct KeySource {
+  int64_t offset = 0;
+  const SceneParams *p = nullptr;
+};
+__device__ __forceinline__ KeySource key_source(const SegTile *tiles, const SceneParams *sp,
+                                                int64_t n) {
+  KeySource k;
+  if (!sp) return k;
+  k.p = sp + (tiles ? tiles[blockIdx.x].scene : 0);
+  k.offset = (int64_t)k.p->axis * n;
+  return k;
+}
+template <class K>
+__device__ __forceinline__ K source_key(K k, const SceneParams *p) {
+  if constexpr (sizeof(K) == 4) {
+    if (p) return (K)skey_down(float_of_key((uint32_t)k), *p);
+  }
+  return k;
 }
 
 template <class K>
@@ -91,8 +106,8 @@ __global__ __launch_bounds__(kSortThreads) void k_count(const K *__restrict__ ke
                                                          unsigned *__restrict__ counts,
                                                          const unsigned *__restrict__ skip,
                                                          const SegTile *__restrict__ tiles,
-                                                         const int *__restrict__ scene_axis = nullptr,
-                                                         int axis_stride = 0) {
+                                                         const SceneParams *__restrict__ sp =
+                                                             nullptr) {
   if (skip && skip[shift >> 3]) return;
   __shared__ unsigned h[256];
   h[threadIdx.x] = 0;
@@ -100,11 +115,13 @@ __global__ __launch_bounds__(kSortThreads) void k_count(const K *__restrict__ ke
   int64_t base;
   int cnt;
   tile_of(tiles, n, base, cnt);
-  keys += key_source(tiles, scene_axis, axis_stride, n);
+  const KeySource src = key_source(tiles, sp, n);
+  keys += src.offset;
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const int j = r * kSortThreads + threadIdx.x;
-    if (j < cnt) atomicAdd(&h[(unsigned)(keys[base + j] >> shift) & 255u], 1u);
+    if (j < cnt)
+      atomicAdd(&h[(unsigned)(source_key(keys[base + j], src.p) >> shift) & 255u], 1u);
   }
   __syncthreads();
   counts[count_index(tiles, ntiles, threadIdx.x)] = h[threadIdx.x];
@@ -120,17 +137,18 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const K *__restrict__ 
                                                            const unsigned *__restrict__ offsets,
                                                            const unsigned *__restrict__ skip,
                                                            const SegTile *__restrict__ tiles,
-                                                           const int *__restrict__ scene_axis = nullptr,
-                                                           int axis_stride = 0) {
+                                                           const SceneParams *__restrict__ sp =
+                                                               nullptr) {
   int64_t tile0;
   int tcnt;
   tile_of(tiles, n, tile0, tcnt);
-  kin += key_source(tiles, scene_axis, axis_stride, n);  // vin null: the values are positions
+  const KeySource src = key_source(tiles, sp, n);
+  kin += src.offset;  // vin null: the values are positions
   if (skip && skip[shift >> 3]) {
     // the digit is equal for every key: the stable order is the identity
     for (int j = threadIdx.x; j < tcnt; j += kSortThreads) {
-      kout[tile0 + j] = kin[tile0 + j];
-      vout[tile0 + j] = vin[tile0 + j];
+      kout[tile0 + j] = source_key(kin[tile0 + j], src.p);
+      vout[tile0 + j] = vin ? vin[tile0 + j] : (uint32_t)(tile0 + j);
     }
     return;
   }
@@ -157,7 +175,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const K *__restrict__ 
     const int j = wbase + r * kWave + lane;
     const bool valid = j < tcnt;
     const int64_t i = tile0 + j;
-    key[r] = valid ? kin[i] : (K)0;
+    key[r] = valid ? source_key(kin[i], src.p) : (K)0;
     val[r] = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
     const unsigned dig = (unsigned)(key[r] >> shift) & 255u;
     // lanes holding the same digit: intersect the 8 bit-planes
@@ -460,8 +478,8 @@ void radix_sort_u32_segmented(uint32_t *keys, uint32_t *vals, int64_t n, int bit
   }
 }
 void radix_sort_u32_segmented_axes(uint32_t *keys3, uint32_t *vals, int64_t n, int bits,
-                                   const SegTile *tiles, int64_t ntiles, const int *scene_axis,
-                                   int axis_stride, SortScratch &s, hipStream_t st) {
+                                   const SegTile *tiles, int64_t ntiles, const SceneParams *sp,
+                                   SortScratch &s, hipStream_t st) {
   if (n <= 0 || ntiles <= 0) return;
   const int ndigits = (bits + 7) / 8;
   uint32_t *ak = s.alt_keys.as<uint32_t>((size_t)n);
@@ -471,15 +489,14 @@ void radix_sort_u32_segmented_axes(uint32_t *keys3, uint32_t *vals, int64_t n, i
   uint32_t *src_k = keys3, *dst_k = ak, *src_v = nullptr, *dst_v = av;
   for (int d = 0; d < ndigits; ++d) {
     const int shift = 8 * d;
-    const int *sa = d == 0 ? scene_axis : nullptr;
+    const SceneParams *ps = d == 0 ? sp : nullptr;
     hipLaunchKernelGGL(k_count<uint32_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, st,
                        (const uint32_t *)src_k, n, shift, (int)ntiles, counts,
-                       (const unsigned *)nullptr, tiles, sa, axis_stride);
+                       (const unsigned *)nullptr, tiles, ps);
     scan_impl<unsigned>(counts, counts, 256 * ntiles, s.misc, st);
     hipLaunchKernelGGL(k_scatter<uint32_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, st,
                        (const uint32_t *)src_k, (const uint32_t *)src_v, dst_k, dst_v, n, shift,
-                       (int)ntiles, (const unsigned *)counts, (const unsigned *)nullptr, tiles, sa,
-                       axis_stride);
+                       (int)ntiles, (const unsigned *)counts, (const unsigned *)nullptr, tiles, ps);
     STQ_HIP(hipGetLastError());
     if (d == 0) {  // from here on the keys ping-pong between alt and keys3[0 .. n)
       src_k = ak;

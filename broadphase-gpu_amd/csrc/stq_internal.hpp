@@ -237,15 +237,17 @@ void radix_sort_u32_segmented(uint32_t *keys, uint32_t *vals, int64_t n, int bit
                               const SegTile *tiles, int64_t ntiles, SortScratch &s,
                               hipStream_t st);
 void radix_sort_segmented_reserve(int64_t n, int64_t ntiles, SortScratch &s);
-// The same sort with the keys of all three axes as input: keys3 holds 3 n
-// keys (axis a at keys3[a n ..]); tile t reads the axis scene_axis[scene *
-// axis_stride] of its scene (the first word of the scene's parameters) in the
-// first pass, and the values start as the positions 0 .. n-1.  The sorted keys
-// end in keys3[0 .. n), their positions in vals.  With tiles null: one scene,
-// tiles of sort_tile_keys() keys over all n (ntiles of them).
+// The same sort with the FP32 keys of all three axes as input: keys3 holds
+// 3 n keys (axis a at keys3[a n ..]); in the first pass tile t reads the keys
+// of its scene's sweep axis (sp[scene].axis) and turns each into the scene's
+// sweep key (skey_down, sweep.hpp), and the values start as the positions
+// 0 .. n-1; `bits` is the width of the sweep keys.  The sorted sweep keys end
+// in keys3[0 .. n), their positions in vals.  With tiles null: one scene
+// (sp[0]), tiles of sort_tile_keys() keys over all n (ntiles of them).
+struct SceneParams;
 void radix_sort_u32_segmented_axes(uint32_t *keys3, uint32_t *vals, int64_t n, int bits,
-                                   const SegTile *tiles, int64_t ntiles, const int *scene_axis,
-                                   int axis_stride, SortScratch &s, hipStream_t st);
+                                   const SegTile *tiles, int64_t ntiles, const SceneParams *sp,
+                                   SortScratch &s, hipStream_t st);
 
 // Exclusive prefix sum of n uint32 counts into n+1 int64 offsets (out[n] is
 // the total).  Scratch is taken from `tmp`.  With n_dev, only the first

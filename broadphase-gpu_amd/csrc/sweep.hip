@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kThreads) void k_axis_stats(const stq_aabb_f64 *__r
 // kLevelMax levels cover each axis's range; a degenerate or unbounded range
 // puts every box on the full level range (scale 0).
 __global__ __launch_bounds__(kThreads) void k_axis_final(const double *__restrict__ partial,
-                                                         int nb, int64_t n, int policy,
+                                                         int nb, int64_t n, int policy, int kbits,
                                                          SceneParams *__restrict__ out) {
   __shared__ double tot[kAxisStats];
   // thread t combines blocks t, t + kThreads, ... in order, then the threads
@@ -111,7 +111,6 @@ __global__ __launch_bounds__(kThreads) void k_axis_final(const double *__restric
   __syncthreads();
   if (threadIdx.x != 0) return;
   SceneParams sp;
-  sp.pad = 0;
   for (int a = 0; a < 3; ++a) {
     const double lo = tot[12 + a], up = tot[15 + a], range = up - lo;
     sp.qp.origin[a] = isfinite(lo) ? lo : 0.0;
@@ -144,6 +143,7 @@ __global__ __launch_bounds__(kThreads) void k_axis_final(const double *__restric
     }
   }
   sp.axis = best;
+  set_sweep_keys(sp, tot[12 + best], tot[15 + best], kbits);
   *out = sp;
 }
 
@@ -237,22 +237,23 @@ __global__ __launch_bounds__(kThreads) void k_gather_f64(const stq_aabb_f64 *__r
 __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__restrict__ boxes,
                                                          const uint32_t *__restrict__ order,
                                                          int64_t n, const SceneParams *__restrict__ sp,
-                                                         float *__restrict__ smin,
-                                                         float *__restrict__ smax,
+                                                         uint32_t *__restrict__ smin,
+                                                         uint32_t *__restrict__ smax,
                                                          Exact64 *__restrict__ rec,
                                                          QRec *__restrict__ qrec,
                                                          ColStats *__restrict__ cols) {
   const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   const bool valid = s < n;
-  const int axis = sp->axis;
-  const QParams qp = sp->qp;
+  const SceneParams p = *sp;
+  const int axis = p.axis;
+  const QParams qp = p.qp;
   int a1, a2;
   other_axes(axis, a1, a2);
   Exact64 r{};
   if (valid) {
     const stq_aabb_f64 &b = boxes[order[s]];
-    smin[s] = round_down_f32(b.min[axis]);
-    smax[s] = round_up_f32(b.max[axis]);
+    smin[s] = skey_down(round_down_f32(b.min[axis]), p);  // = the sort key
+    smax[s] = skey_up(round_up_f32(b.max[axis]), p);
     r.lo0 = b.min[axis];
     r.hi0 = b.max[axis];
     r.lo1 = b.min[a1];
@@ -274,28 +275,24 @@ __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__r
 
 // ---- chain ends -----------------------------------------------------------------
 // hi[s] = last q with smin[q] <= smax[s] (q >= s since smin[s] <= smax[s]),
-// by last_le; grid-stride over a capped grid, adding sum(hi - s) to *cand
-// with one atomic per block.  Rows outside [r0, r1) get an empty chain
+// by binary searches in tiles of rows whose minima are staged in LDS
+// (chain_end_tile); a persistent grid of two blocks per CU adds sum(hi - s)
+// to *cand with one atomic per block.  Rows outside [r0, r1) get an empty chain
 // (hi = s): they emit nothing, as the reference's rows outside
 // [tidstart, tidstart + MAX_OVERLAP_CUTOFF) (src/sweep.cu:234-240).
 template <class T>
-__global__ __launch_bounds__(kThreads) void k_chain_end(const T *__restrict__ smin,
-                                                        const T *__restrict__ smax, int64_t n,
-                                                        int64_t r0, int64_t r1,
-                                                        int *__restrict__ hi,
-                                                        unsigned long long *__restrict__ cand) {
+__global__ __launch_bounds__(kCeThreads) void k_chain_end(const T *__restrict__ smin,
+                                                          const T *__restrict__ smax, int64_t n,
+                                                          int64_t r0, int64_t r1,
+                                                          int *__restrict__ hi,
+                                                          unsigned long long *__restrict__ cand) {
+  __shared__ T win[kCeWin<T>];
   unsigned long long len = 0;
-  for (int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x; s < n;
-       s += (int64_t)gridDim.x * kThreads) {
-    if (s < r0 || s >= r1) {
-      hi[s] = (int)s;
-      continue;
-    }
-    const int64_t lo = last_le(smin, s, n, smax[s]);
-    hi[s] = (int)lo;
-    len += (unsigned long long)(lo - s);
-  }
-  block_atomic_add<kThreads>(len, cand);
+  const int64_t ntiles = (n + kCeRows<T> - 1) / kCeRows<T>;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x)
+    len += chain_end_tile(smin, smax, n, t * kCeRows<T>, r0, r1, win,
+                          [n](int64_t) { return n; }, hi);
+  block_atomic_add<kCeThreads>(len, cand);
 }
 
 // ---- work planning ----------------------------------------------------------------
@@ -837,10 +834,14 @@ struct RowPlanes {
   RowIds id[kWave];
 };
 
+// At most 64 VGPRs (the second bound: 8 waves per SIMD), so 32 waves fit on
+// a CU: with 71 registers 7 fit, and the learned pass took 7.25 instead of
+// 6.81 ms on the 512-scene batch.
 template <int MODE>
-__global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 *__restrict__ rec,
-                                                     const QRec *__restrict__ qrec,
-                                                     int2 *__restrict__ out) {
+__global__ __launch_bounds__(kThreads, 8) void k_qsweep(SweepArgs a,
+                                                        const Exact64 *__restrict__ rec,
+                                                        const QRec *__restrict__ qrec,
+                                                        int2 *__restrict__ out) {
   // LDS slots per wave: fewer than 64 waiting candidates plus one position's
   // 64, as (row lane, position); and the item's rows
   __shared__ int2 slot_mem[kThreads / kWave][2 * kWave];
@@ -1010,6 +1011,7 @@ __global__ __launch_bounds__(kThreads) void k_qsweep(SweepArgs a, const Exact64 
 // of the group's rows on axes 1 and 2, and each position that passes is read
 // out of its lane and tested against every row in FP64 (both axes, type,
 // shared vertices, p < q <= hi).  Tail items as in the plain FP64 sweep.
+// The same 64-VGPR bound as k_qsweep (32 waves per CU).
 // Emission order: position, then row lane.
 __device__ __forceinline__ double wave_min_f64(double v) {
 #pragma unroll
@@ -1023,8 +1025,9 @@ __device__ __forceinline__ double wave_max_f64(double v) {
 }
 
 template <int MODE>
-__global__ __launch_bounds__(kThreads) void k_gsweep64(SweepArgs a, const Rec64 *__restrict__ rec,
-                                                       int2 *__restrict__ out) {
+__global__ __launch_bounds__(kThreads, 8) void k_gsweep64(SweepArgs a,
+                                                          const Rec64 *__restrict__ rec,
+                                                          int2 *__restrict__ out) {
   const int lane = threadIdx.x & 63;
   if (MODE == kFill && a.total_dev && *a.total_dev > a.capacity) return;
   const WaveSlice ws = wave_slice(a.nitems_dev ? *a.nitems_dev : a.nitems, a.xcd_split, a.slices);
@@ -1120,7 +1123,7 @@ void launch_axis_stats(const stq_aabb_f64 *boxes, int64_t n, double *partial, in
 void launch_axis_final(const double *partial, int nblocks, int64_t n, int policy,
                        SceneParams *out, hipStream_t st) {
   hipLaunchKernelGGL(k_axis_final, dim3(1), dim3(kThreads), 0, st, partial, nblocks, n, policy,
-                     out);
+                     sweep_key_bits(n), out);
   STQ_HIP(hipGetLastError());
 }
 
@@ -1142,7 +1145,7 @@ void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, i
   if (n <= 0) return;
   if (f32)
     hipLaunchKernelGGL(k_gather_f32, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, order,
-                       n, sp, (float *)smin, (float *)smax, (Exact64 *)rec, qrec, cols);
+                       n, sp, (uint32_t *)smin, (uint32_t *)smax, (Exact64 *)rec, qrec, cols);
   else
     hipLaunchKernelGGL(k_gather_f64, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, order,
                        n, axis, sp, (double *)smin, (double *)smax, (Rec64 *)rec, qrec, cols);
@@ -1153,12 +1156,14 @@ void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, i
                       unsigned long long *cand, hipStream_t st, int64_t r0, int64_t r1) {
   if (n <= 0) return;
   if (r1 < 0 || r1 > n) r1 = n;
-  const unsigned g = std::min(blocks_for(n), kReduceGrid);
+  // two blocks per CU, persistent over the tiles (one counter atomic per block)
+  const int64_t rows = f32 ? kCeRows<uint32_t> : kCeRows<double>;
+  const unsigned g = (unsigned)std::min<int64_t>((n + rows - 1) / rows, 512);
   if (f32)
-    hipLaunchKernelGGL(k_chain_end<float>, dim3(g), dim3(kThreads), 0, st, (const float *)smin,
-                       (const float *)smax, n, r0, r1, hi, cand);
+    hipLaunchKernelGGL(k_chain_end<uint32_t>, dim3(g), dim3(kCeThreads), 0, st,
+                       (const uint32_t *)smin, (const uint32_t *)smax, n, r0, r1, hi, cand);
   else
-    hipLaunchKernelGGL(k_chain_end<double>, dim3(g), dim3(kThreads), 0, st, (const double *)smin,
+    hipLaunchKernelGGL(k_chain_end<double>, dim3(g), dim3(kCeThreads), 0, st, (const double *)smin,
                        (const double *)smax, n, r0, r1, hi, cand);
   STQ_HIP(hipGetLastError());
 }

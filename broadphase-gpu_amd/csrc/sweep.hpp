@@ -45,8 +45,10 @@ __device__ inline void axis_stats_add(const stq_aabb_f64 &b, double *acc, double
     acc[4 * a + 1] += l * l + u * u;
     acc[4 * a + 2] += u - l;
     acc[4 * a + 3] += c * c;
-    lo[a] = fmin(lo[a], l);
-    up[a] = fmax(up[a], u);
+    // the quantisation ranges cover the finite bounds: an infinite bound
+    // clamps to the end of its range instead of emptying the levels of the scene
+    if (isfinite(l)) lo[a] = fmin(lo[a], l);
+    if (isfinite(u)) up[a] = fmax(up[a], u);
   }
 }
 // Adds boxes i, i + step, ... below i1, four loads in flight per thread (the
@@ -237,12 +239,51 @@ __device__ inline void col_stats_put(bool valid, int64_t s, double lo1, double h
   }
 }
 
-// Per-scene sweep axis and level range (batched scenes).
+// Per-scene sweep axis and level ranges; for the reduced-precision form also
+// the quantisation of the sweep axis's keys (below).
 struct SceneParams {
   int axis;
-  int pad;
+  uint32_t kmax;  // largest sweep key: 2^bits - 1
   QParams qp;
+  double kscale;  // sweep keys per unit of the sweep axis (0: every key equal)
 };
+
+// Sweep keys of the reduced-precision form.  The FP32 outward bounds of the
+// sweep axis (the reference's float rounding) are quantised outward once more,
+// to integers in [0, kmax] over the scene's finite range:
+//   down(f) = floor((f - origin) * kscale),  up(f) = ceil((f - origin) * kscale),
+// clamped.  Both are non-decreasing in f, so a pair that overlaps in FP64
+// overlaps in keys (min_q <= max_p gives down(min_q) <= up(max_p)), and the
+// sort, the chain ends and the planning run on these integers; the FP64
+// confirmation tests the sweep axis again.  The sort key of a box is exactly
+// its down(min), so the keys in sorted order are non-decreasing.
+__device__ inline uint32_t skey_down(float f, const SceneParams &p) {
+  if (p.kscale == 0.0) return 0u;
+  const double t = floor(((double)f - p.qp.origin[p.axis]) * p.kscale);
+  return (uint32_t)fmin(fmax(t, 0.0), (double)p.kmax);
+}
+__device__ inline uint32_t skey_up(float f, const SceneParams &p) {
+  if (p.kscale == 0.0) return p.kmax;
+  const double t = ceil(((double)f - p.qp.origin[p.axis]) * p.kscale);
+  return (uint32_t)fmin(fmax(t, 0.0), (double)p.kmax);
+}
+// Inverse of key_of(float): the float whose order-preserving image is k.
+__device__ inline float float_of_key(uint32_t k) {
+  const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __builtin_bit_cast(float, u);
+}
+// Bits of the sweep keys for a scene of at most n boxes: 16 up to 2^20 boxes
+// (two sort passes; a uniform scene puts at most 16 boxes in a key, which
+// lengthens a chain by as many positions), 24 up to 2^27, else 32.
+inline int sweep_key_bits(int64_t n) { return n <= (1 << 20) ? 16 : (n <= (1 << 27) ? 24 : 32); }
+// The quantisation of the chosen axis: kmax and kscale from the scene's
+// finite range [lo, up] on that axis.
+__device__ inline void set_sweep_keys(SceneParams &p, double lo, double up, int bits) {
+  p.kmax = bits >= 32 ? 0xffffffffu : (1u << bits) - 1u;
+  const double range = up - lo;
+  const double sc = (range > 0 && isfinite(range)) ? (double)p.kmax / range : 0.0;
+  p.kscale = isfinite(sc) ? sc : 0.0;
+}
 
 // Scene of a sorted or input position: the s with seg_off[s] <= pos <
 // seg_off[s+1] (seg_off has nscenes+1 entries, empty scenes are skipped).
@@ -298,13 +339,97 @@ __device__ __forceinline__ int64_t last_le(const T *__restrict__ smin, int64_t s
   return lo;
 }
 
+// Chain ends of a tile of consecutive sorted rows, the minima staged in LDS.
+// A block of kCeThreads threads takes the rows [s0, s0 + kCeRows) and copies
+// the minima of positions [s0, w1) into LDS, w1 = min(n, s0 + kCeWin): a
+// tile's rows see at least 3 kCeRows positions ahead in LDS.  Each thread
+// then runs the binary searches of its rows together (4 float or 2 double
+// rows, their LDS reads in flight at once): the last q in [s, min(end, w1))
+// with smin[q] <= smax[s].  A row whose answer is the last staged position
+// while its scene goes on continues with last_le in memory.  end_of(s) is
+// the end of the row's scene; rows outside [r0, r1) get empty chains
+// (hi = s).  Returns the sum of the thread's chain lengths.
+constexpr int kCeThreads = 1024;
+template <class T>
+constexpr int kCeWin = 65536 / (int)sizeof(T);  // 64 KB of LDS
+template <class T>
+constexpr int kCeRows = kCeWin<T> / 4;
+template <class T, class EndOf>
+__device__ inline unsigned long long chain_end_tile(const T *__restrict__ smin,
+                                                    const T *__restrict__ smax, int64_t n,
+                                                    int64_t s0, int64_t r0, int64_t r1, T *lds,
+                                                    EndOf end_of, int *__restrict__ hi) {
+  constexpr int R = kCeRows<T> / kCeThreads;  // 4 (float) or 2 (double)
+  static_assert(R >= 1 && kCeRows<T> == R * kCeThreads, "whole rows per thread");
+  const int64_t w1 = s0 + kCeWin<T> < n ? s0 + kCeWin<T> : n;
+  const int nw = (int)(w1 - s0);
+  __syncthreads();  // the previous tile's searches are done with the LDS
+  for (int i = threadIdx.x; i < nw; i += kCeThreads) lds[i] = smin[s0 + i];
+  __syncthreads();
+  int lo[R], up[R];
+  int64_t end[R];
+  T x[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t s = s0 + threadIdx.x + (int64_t)r * kCeThreads;
+    lo[r] = 0;
+    up[r] = 1;  // done: lds[0] exists (nw >= 1)
+    end[r] = s;
+    x[r] = T(0);
+    if (s < n && s >= r0 && s < r1) {
+      end[r] = end_of(s);
+      lo[r] = (int)(s - s0);
+      up[r] = (int)((end[r] < w1 ? end[r] : w1) - s0);
+      x[r] = smax[s];
+    }
+  }
+  // invariant: lds[lo] <= x, the answer lies in [lo, up)
+  while (true) {
+    T v[R];
+    int mid[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      mid[r] = (lo[r] + up[r]) >> 1;
+      v[r] = lds[mid[r]];
+    }
+    bool more = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (up[r] - lo[r] > 1) {
+        if (v[r] <= x[r])
+          lo[r] = mid[r];
+        else
+          up[r] = mid[r];
+      }
+      more |= up[r] - lo[r] > 1;
+    }
+    if (!more) break;
+  }
+  unsigned long long len = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t s = s0 + threadIdx.x + (int64_t)r * kCeThreads;
+    if (s >= n) continue;
+    if (s < r0 || s >= r1) {
+      hi[s] = (int)s;
+      continue;
+    }
+    int64_t q = s0 + lo[r];
+    if (q == w1 - 1 && w1 < end[r]) q = last_le(smin, q, end[r], x[r]);
+    hi[s] = (int)q;
+    len += (unsigned long long)(q - s);
+  }
+  return len;
+}
+
 // keys3 (reduced precision; null: none): the pass also writes the sort keys
 // of all three axes (3 n words) and sets kErrInverted in *err for a NaN or
 // inverted box.
 void launch_axis_stats(const stq_aabb_f64 *boxes, int64_t n, double *partial, int nblocks,
                        hipStream_t st, uint32_t *keys3 = nullptr, unsigned *err = nullptr);
-// The scene's sweep axis (axis policy 0..4) and level range from the
-// statistics of nblocks blocks, into *out on the device.
+// The scene's sweep axis (axis policy 0..4), level ranges and the
+// quantisation of its sweep keys (sweep_key_bits(n) bits) from the statistics
+// of nblocks blocks, into *out on the device.
 void launch_axis_final(const double *partial, int nblocks, int64_t n, int policy,
                        SceneParams *out, hipStream_t st);
 // The sweep axis is sp->axis where sp is given, else axis.  The FP32 form

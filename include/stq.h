@@ -85,14 +85,16 @@ enum stq_precision {
      with CCD_USE_DOUBLE). */
   STQ_PRECISION_F64 = 0,
   /* Reduced-precision filters, FP64 confirmation.  The sort keys and chain
-     ends use FP32 bounds rounded outward (round to nearest, then one ulp
-     outward: the reference rule of src/aabb.cu:56-61); the two other axes
+     ends use the sweep axis's FP32 bounds rounded outward (round to nearest,
+     then one ulp outward: the reference rule of src/aabb.cu:56-61),
+     quantised outward once more to 16-, 24- or 32-bit integers over the
+     scene's range (16 bits up to 2^20 boxes per scene); the two other axes
      are tested on 14-bit levels per scene, rounded outward, with the kind
-     codes of the type filter folded in.  Both filters keep a superset of the
-     FP64 candidates, and every survivor is re-tested in FP64 on all three
-     axes against the original boxes, so the pair set equals the plain FP64
-     set exactly.  FP64 also computes the axis statistics and the level
-     ranges. */
+     codes of the type filter folded in.  Every filter keeps a superset of
+     the FP64 candidates, and every survivor is re-tested in FP64 on all
+     three axes against the original boxes, so the pair set equals the plain
+     FP64 set exactly.  FP64 also computes the axis statistics and the
+     quantisation ranges. */
   STQ_PRECISION_F32_CONFIRM = 1,
   /* FP64 keys and every test in FP64, with the work organisation of the
      reduced-precision form (rows grouped close on axis 1, a union test per 64
