@@ -281,7 +281,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_seg(const int *__restrict
       const bool act = j < cnt;
       int p = act ? (int)(wbase + (int64_t)((uint32_t)key[j] & jmask)) : INT_MAX;
       int h = act ? hi[p] : INT_MIN;
-      if (Q) {  // the group's rows in chain-end order
+      if (Q && tail_cost > 0.f) {  // the group's rows in chain-end order (for the tail cut)
         const uint64_t r = wave_sort_u64(act ? ((uint64_t)(uint32_t)h << 32) | (uint32_t)p : ~0ull,
                                          lane);
         if (act) {

@@ -68,7 +68,7 @@ void launch_gather_seg(bool f32, const stq_aabb_f64 *boxes, const uint32_t *orde
 void launch_chain_end_seg(bool f32, const void *smin, const void *smax, const BatchTables &t,
                           const BatchDev &d, int *hi, unsigned long long *cand, hipStream_t st);
 // Groups of 64 rows per window (ordered as in launch_plan: by chain end, or
-// with qrec by group_key with each group's rows then by chain end), planned as
+// with qrec by group_key, with tails each group's rows then by chain end), planned as
 // body + tail (plan.hpp); each part is cut into at most 64 items, so the item
 // buffer has a bound the host computes from the scene sizes.  Items are then
 // written by launch_items (sweep.hpp).

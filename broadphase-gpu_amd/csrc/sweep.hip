@@ -299,8 +299,9 @@ __global__ __launch_bounds__(kCeThreads) void k_chain_end(const T *__restrict__ 
 // Blocks take windows of W sorted rows (W a power of two) in grid-stride
 // order.  The rows of a window are ordered with a bitonic sort in LDS, by
 // (chain end, position), or with qrec (Q) by (group_key, position), and cut
-// into groups of 64; with qrec each group's rows are then put in chain-end
-// order by a wave sort.  Group g covers rowlist[64g .. 64g+nrows); plan_group
+// into groups of 64; with qrec and tails on, each group's rows are then put
+// in chain-end order by a wave sort (the tail cut needs that order; without
+// tails a group's rows stay in group_key order).  Group g covers rowlist[64g .. 64g+nrows); plan_group
 // splits it into a body over [min p + 1, cut] and, for rows with much longer
 // chains, a tail.  nchunks[g] = number of items of the group.
 template <bool Q>
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(const int *__restrict__ h
       const bool act = j < cnt;
       int p = act ? (int)(wbase + (int64_t)((uint32_t)key[j] & jmask)) : INT_MAX;
       int h = act ? hi[p] : INT_MIN;
-      if (Q) {
+      if (Q && tail_cost > 0.f) {  // chain-end order, which only the tail cut needs
         const uint64_t r = wave_sort_u64(act ? ((uint64_t)(uint32_t)h << 32) | (uint32_t)p : ~0ull,
                                          lane);
         if (act) {

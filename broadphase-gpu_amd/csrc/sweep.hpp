@@ -445,7 +445,7 @@ void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, i
                       unsigned long long *cand, hipStream_t st, int64_t r0 = 0, int64_t r1 = -1);
 // tail_cost: weight of tail tests in the group cut (plan.hpp); 0 = no tails.
 // qrec (reduced precision): windows are ordered by group_key, so a group's
-// rows are close on axis 1, and each group's rows then by chain end; null:
+// rows are close on axis 1 (with tails each group's rows then by chain end); null:
 // windows ordered by chain end.  cap: at most kCapChunks items per body and
 // per tail (chunks grow instead), so a scene has at most
 // capped_item_bound(n, chunk) items.
