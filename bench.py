@@ -6,8 +6,10 @@ cloth-like, density factor U[0.5, 4]; 37,422,181 FP64 boxes, about 4.6e8
 pairs), boxes resident in HBM, pairs written to a device buffer.  One step is
 one batched call over all 512 scenes.  Forms on the same scenes:
 
-  chosen      : FP32 outward-rounded keys and chain ends, 14-bit outward
-                levels with kind codes in the hot loop, FP64 confirmation of
+  chosen      : sort keys and chain ends from the FP32 outward-rounded
+                sweep-axis bounds quantised outward per scene (16-bit keys
+                here), 14-bit outward levels with kind codes in the hot
+                loop, FP64 confirmation of
                 every candidate, learned sizing (one emitting pass into slots
                 predicted by the network, trained online on the first call's
                 exact counts), autotuned launch parameters, hipGraph replay

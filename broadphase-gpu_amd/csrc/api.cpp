@@ -130,7 +130,11 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   int *d_hi = hi.as<int>((size_t)n);
   ColStats *d_cols = learned ? col_stats.as<ColStats>((size_t)(n + kColBlock - 1) / kColBlock)
                              : nullptr;
-  launch_gather(f32, boxes, order, n, axis, d_sp, d_smin, d_smax, d_rec, d_qrec, d_cols, st);
+  // learned sizing: row features by sorted position, then in group order
+  uint2 *d_rowf = learned ? row_feat_pos.as<uint2>((size_t)n) : nullptr;
+  uint2 *d_rowfeat = learned ? row_feat_grp.as<uint2>((size_t)n) : nullptr;
+  launch_gather(f32, boxes, order, n, axis, d_sp, d_smin, d_smax, d_rec, d_qrec, d_cols, d_rowf,
+                st);
   launch_chain_end(f32, d_smin, d_smax, n, d_hi, &ctr[1], st, row_begin, row_end);
   STQ_HIP(hipEventRecord(ev[E_GATHER], st));
   // The error flags and the candidate count are read with the item count
@@ -162,14 +166,14 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   if (!grouped) W = std::min(W, kMaxWindow64);  // 64-bit window keys
   const int chunk = auto_chunk(opt);
   const int64_t ngroups = (n + 63) / 64;
-  int *d_rowlist = rowlist.as<int>((size_t)n);
+  int2 *d_rowlist = rowlist.as<int2>((size_t)n);
   int4 *d_groups = groups.as<int4>((size_t)ngroups);
   int4 *d_tg = tgroups.as<int4>((size_t)ngroups);
   int2 *d_gch = gchunks.as<int2>((size_t)ngroups);
   uint32_t *d_nch = nchunks.as<uint32_t>((size_t)ngroups);
   int64_t *d_ioff = ioff.as<int64_t>((size_t)ngroups + 1);
   launch_plan(d_hi, d_qrec, n, W, chunk, tail_cost(opt), d_rowlist, d_groups, d_tg, d_gch, d_nch,
-              &ctr[2], st, plan_cap);
+              &ctr[2], st, plan_cap, d_rowf, d_rowfeat);
   scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
   STQ_HIP(hipMemcpyAsync(sh + 5, d_ioff + ngroups, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   if (early)
@@ -192,7 +196,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   sa.items = d_items;
   sa.nitems = nitems;
   sa.rowlist = d_rowlist;
-  sa.hi = d_hi;
+  sa.rowfeat = d_rowfeat;
   sa.rec = d_rec;
   sa.qrec = d_qrec;
   sa.orig = order;
@@ -697,7 +701,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   QRec *d_qrec = grouped ? meta.as<QRec>((size_t)n) : nullptr;
   uint32_t *d_orig = b_orig.as<uint32_t>((size_t)n);
   int *d_hi = hi.as<int>((size_t)n);
-  int *d_rowlist = rowlist.as<int>((size_t)n);
+  int2 *d_rowlist = rowlist.as<int2>((size_t)n);
   int4 *d_groups = groups.as<int4>((size_t)ngroups);
   int2 *d_gchunk = gchunks.as<int2>((size_t)ngroups);
   int4 *d_tg = tgroups.as<int4>((size_t)ngroups);
@@ -712,6 +716,8 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   float *d_feat = learned ? feat.as<float>((size_t)bound * kFeat) : nullptr;
   const int64_t ncols = (n + kColBlock - 1) / kColBlock;
   ColStats *d_cols = learned ? col_stats.as<ColStats>((size_t)ncols) : nullptr;
+  uint2 *d_rowf = learned ? row_feat_pos.as<uint2>((size_t)n) : nullptr;
+  uint2 *d_rowfeat = learned ? row_feat_grp.as<uint2>((size_t)n) : nullptr;
   double *d_colwork = learned ? col_work.as<double>(col_work_doubles(ncols)) : nullptr;
   uint32_t *d_cand = learned ? cand.as<uint32_t>((size_t)bound) : nullptr;
   float *d_pred = learned ? pred.as<float>((size_t)bound) : nullptr;
@@ -767,11 +773,11 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     }
     mark(E_SORT);
     launch_gather_seg(f32, boxes, d_vals, t, bp.dev, d_smin, d_smax, d_rec, d_qrec, d_orig, d_cols,
-                      st);
+                      d_rowf, st);
     launch_chain_end_seg(f32, d_smin, d_smax, t, bp.dev, d_hi, &ctr[1], st);
     mark(E_GATHER);
     launch_plan_seg(d_hi, d_qrec, t, bp.dev, W, chunk, tail_cost(opt), d_rowlist, d_groups, d_tg,
-                    d_gchunk, d_nch, &ctr[2], st);
+                    d_gchunk, d_nch, &ctr[2], st, d_rowf, d_rowfeat);
     scan_u32_to_i64(d_nch, d_ioff, ngroups, scan_tmp, st);
     launch_items(d_groups, d_tg, d_gchunk, d_ioff, ngroups, d_items, st);
     mark(E_PLAN);
@@ -782,7 +788,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
     sa.nitems = bound;
     sa.nitems_dev = d_ioff + ngroups;
     sa.rowlist = d_rowlist;
-    sa.hi = d_hi;
+    sa.rowfeat = d_rowfeat;
     sa.rec = d_rec;
     sa.qrec = d_qrec;
     sa.orig = d_orig;

@@ -102,8 +102,12 @@ __global__ __launch_bounds__(kT) void k_keys_seg(const stq_aabb_f64 *__restrict_
                                                  uint32_t *__restrict__ vals,
                                                  unsigned *__restrict__ err) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  __shared__ int sc0;  // the scene of the block's first box, found once
+  if (threadIdx.x == 0) sc0 = scene_of(seg_off, nscenes, (int64_t)blockIdx.x * kT);
+  __syncthreads();
   if (i >= n) return;
-  const int s = scene_of(seg_off, nscenes, i);
+  int s = sc0;
+  while (seg_off[s + 1] <= i) ++s;
   const int axis = sp[s].axis;
   const stq_aabb_f64 &b = boxes[i];
   const double x0 = b.min[0], x1 = b.min[1], x2 = b.min[2];
@@ -131,13 +135,20 @@ __global__ __launch_bounds__(kT) void k_gather_seg(const stq_aabb_f64 *__restric
                                                    void *__restrict__ smin, void *__restrict__ smax,
                                                    void *__restrict__ rec, QRec *__restrict__ qrec,
                                                    uint32_t *__restrict__ orig_local,
-                                                   ColStats *__restrict__ cols) {
+                                                   ColStats *__restrict__ cols,
+                                                   uint2 *__restrict__ rowf) {
   const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
   const bool valid = s < n;
+  // the scene of the block's first position, found once; a block of kT
+  // consecutive positions meets at most a few scenes after it
+  __shared__ int sc0;
+  if (threadIdx.x == 0) sc0 = scene_of(seg_off, nscenes, (int64_t)blockIdx.x * kT);
+  __syncthreads();
   double lo1 = 0, hi1 = 0, lo2 = 0, hi2 = 0, o1 = 0, n1 = 0, o2 = 0, n2 = 0;
   int type = 0;
   if (valid) {
-    const int sc = scene_of(seg_off, nscenes, s);
+    int sc = sc0;
+    while (seg_off[sc + 1] <= s) ++sc;
     const uint32_t o = order[s];
     orig_local[s] = (uint32_t)((int64_t)o - seg_off[sc]);
     const stq_aabb_f64 &b = boxes[o];
@@ -190,6 +201,7 @@ __global__ __launch_bounds__(kT) void k_gather_seg(const stq_aabb_f64 *__restric
                             p.qp.origin[a2], p.qp.scale[a2]);
     }
   }
+  if (rowf && valid) rowf[s] = row_feat(lo1, hi1, lo2, hi2, type, o1, n1, o2, n2);
   // blocks that straddle two scenes mix both (the features are estimates)
   if (cols) col_stats_put(valid, s, lo1, hi1, lo2, hi2, type, o1, n1, o2, n2, cols);
 }
@@ -240,12 +252,14 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_seg(const int *__restrict
                                                  const int *__restrict__ win_cnt,
                                                  const int64_t *__restrict__ win_gbase, int nwin,
                                                  int W, int chunk, float tail_cost,
-                                                 int *__restrict__ rowlist,
+                                                 int2 *__restrict__ rowlist,
                                                  int4 *__restrict__ groups,
                                                  int4 *__restrict__ tgroups,
                                                  int2 *__restrict__ gchunk,
                                                  uint32_t *__restrict__ nchunks,
-                                                 unsigned long long *__restrict__ lane_steps) {
+                                                 unsigned long long *__restrict__ lane_steps,
+                                                 const uint2 *__restrict__ rowf,
+                                                 uint2 *__restrict__ rowfeat) {
   typedef typename std::conditional<Q, uint32_t, uint64_t>::type K;
   __shared__ K key[Q ? kMaxWindow32 : kMaxWindow64];
   __shared__ unsigned rs_cnt[Q ? (kPlanThreads / 64) * 256 : 1], rs_sum[Q ? 256 : 1];
@@ -289,7 +303,10 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_seg(const int *__restrict
           h = (int)(r >> 32);
         }
       }
-      if (act) rowlist[wbase + j] = p;
+      if (act) {
+        rowlist[wbase + j] = make_int2(p, h);
+        if (rowf) rowfeat[wbase + j] = rowf[p];
+      }
       const int nrows = min(64, cnt - 64 * k);
       // capped: the item buffer is sized on the host from the scene sizes
       const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, true,
@@ -415,11 +432,12 @@ void launch_scene_keys(const uint32_t *vals, const BatchTables &t, const BatchDe
 
 void launch_gather_seg(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order,
                        const BatchTables &t, const BatchDev &d, void *smin, void *smax, void *rec,
-                       QRec *qrec, uint32_t *orig_local, ColStats *cols, hipStream_t st) {
+                       QRec *qrec, uint32_t *orig_local, ColStats *cols, uint2 *rowf,
+                       hipStream_t st) {
   if (t.n <= 0) return;
   hipLaunchKernelGGL(k_gather_seg, dim3(nblk(t.n)), dim3(kT), 0, st, boxes, order, t.n, d.seg_off,
                      t.nscenes, (const SceneParams *)d.params, f32 ? 1 : 0, smin, smax, rec, qrec,
-                     orig_local, cols);
+                     orig_local, cols, rowf);
   STQ_HIP(hipGetLastError());
 }
 
@@ -441,9 +459,10 @@ void launch_chain_end_seg(bool f32, const void *smin, const void *smax, const Ba
 }
 
 void launch_plan_seg(const int *hi, const QRec *qrec, const BatchTables &t, const BatchDev &d,
-                     int W, int chunk, float tail_cost, int *rowlist, int4 *groups,
+                     int W, int chunk, float tail_cost, int2 *rowlist, int4 *groups,
                      int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
-                     unsigned long long *lane_steps, hipStream_t st) {
+                     unsigned long long *lane_steps, hipStream_t st, const uint2 *rowf,
+                     uint2 *rowfeat) {
   const int nw = (int)t.win_start.size();
   if (nw <= 0) return;
   if (W > (qrec ? kMaxWindow32 : kMaxWindow64))
@@ -452,7 +471,7 @@ void launch_plan_seg(const int *hi, const QRec *qrec, const BatchTables &t, cons
                      dim3(std::min((unsigned)nw, kReduceGrid)), dim3(kPlanThreads), 0, st, hi,
                      qrec,
                      d.win_start, d.win_cnt, d.win_gbase, nw, W, chunk, tail_cost, rowlist, groups,
-                     tgroups, gchunk, nchunks, lane_steps);
+                     tgroups, gchunk, nchunks, lane_steps, rowf, rowfeat);
   STQ_HIP(hipGetLastError());
 }
 

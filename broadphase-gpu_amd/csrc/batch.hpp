@@ -60,11 +60,13 @@ void launch_keys_seg(bool f32, const stq_aabb_f64 *boxes, const BatchTables &t, 
 // scene_key[s] = scene of input position vals[s] (for the F64 second sort)
 void launch_scene_keys(const uint32_t *vals, const BatchTables &t, const BatchDev &d,
                        uint32_t *scene_key, hipStream_t st);
-// Sorted records of every scene; with cols, the column statistics of every
-// kColBlock global positions as well (null: not made).
+// Sorted records of every scene; with cols and rowf, the column statistics
+// of every kColBlock global positions and the row features of every position
+// as well (null: not made).
 void launch_gather_seg(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order,
                        const BatchTables &t, const BatchDev &d, void *smin, void *smax, void *rec,
-                       QRec *qrec, uint32_t *orig_local, ColStats *cols, hipStream_t st);
+                       QRec *qrec, uint32_t *orig_local, ColStats *cols, uint2 *rowf,
+                       hipStream_t st);
 void launch_chain_end_seg(bool f32, const void *smin, const void *smax, const BatchTables &t,
                           const BatchDev &d, int *hi, unsigned long long *cand, hipStream_t st);
 // Groups of 64 rows per window (ordered as in launch_plan: by chain end, or
@@ -73,9 +75,10 @@ void launch_chain_end_seg(bool f32, const void *smin, const void *smax, const Ba
 // buffer has a bound the host computes from the scene sizes.  Items are then
 // written by launch_items (sweep.hpp).
 void launch_plan_seg(const int *hi, const QRec *qrec, const BatchTables &t, const BatchDev &d,
-                     int W, int chunk, float tail_cost, int *rowlist, int4 *groups,
+                     int W, int chunk, float tail_cost, int2 *rowlist, int4 *groups,
                      int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
-                     unsigned long long *lane_steps, hipStream_t st);
+                     unsigned long long *lane_steps, hipStream_t st,
+                     const uint2 *rowf = nullptr, uint2 *rowfeat = nullptr);
 // pair_off[s] = off[ioff[scene_first_group[s]]] for s = 0..nscenes.
 void launch_scene_pairs(const int64_t *off, const int64_t *ioff, const BatchTables &t,
                         const BatchDev &d, int64_t *pair_off, hipStream_t st);

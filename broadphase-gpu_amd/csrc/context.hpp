@@ -160,7 +160,7 @@ struct Context {
   // learned sizing: parameters, Adam state, per-item buffers
   DevBuf sz_params, sz_m, sz_v, sz_grad, sz_partial, sz_loss, feat, cand, pred, caps, cap_off,
       targets, bfeat, btargets, scratch_pairs, ovf, mae_acc, replay_feat, replay_t, col_stats,
-      col_work, scene_params;
+      col_work, scene_params, row_feat_pos, row_feat_grp;
   bool sz_ready = false;
   int adam_t = 0;
   int64_t replay_n = 0, replay_pos = 0, train_calls = 0;
@@ -213,7 +213,8 @@ struct Context {
     sort.misc.release();
     DevBuf *szb[] = {&sz_params, &sz_m, &sz_v, &sz_grad, &sz_partial, &sz_loss, &feat, &cand,
                      &pred, &caps, &cap_off, &targets, &bfeat, &btargets, &scratch_pairs, &ovf,
-                     &mae_acc, &replay_feat, &replay_t, &col_stats, &col_work, &scene_params};
+                     &mae_acc, &replay_feat, &replay_t, &col_stats, &col_work, &scene_params,
+                     &row_feat_pos, &row_feat_grp};
     for (DevBuf *b : szb) b->release();
     if (stream) (void)hipStreamDestroy(stream);
   }

@@ -221,10 +221,10 @@ __global__ __launch_bounds__(kT) void k_col_pre(const ColStats *__restrict__ col
 // item's scene, and the item count is read from the device.
 //
 // The wave reductions of an item leave kStat uniform statistics; they are
-// parked in the wave's LDS and, once 64 items are parked (or the wave's run
-// ends), lane j turns item j's statistics into its 32 features: the
+// parked in the wave's LDS and, once kPark items are parked (or the wave's
+// run ends), lane j turns item j's statistics into its 32 features: the
 // transcendental part (logs, exps, roots, divisions: several hundred
-// instructions) runs once per 64 items instead of once per item.
+// instructions) runs once per kPark items instead of once per item.
 enum StatField {
   kSNr, kSCandLo, kSCandHi, kSSpan, kSMle1, kSVle1, kSMle2, kSVle2, kSMc1, kSVc1, kSMc2, kSVc2,
   kSFv, kSFe, kSFf, kSMchain, kSN1Lo, kSN1Hi, kSN2Lo, kSN2Hi,
@@ -234,6 +234,8 @@ enum StatField {
 // LDS words per parked item: an odd pitch, so the lanes reading one field of
 // their items hit distinct banks
 constexpr int kStatPitch = kStat | 1;
+// Items parked per wave before the feature step.
+constexpr int kPark = 64;
 
 __device__ __forceinline__ double stat_f64(const float *s, int lo) {
   const uint64_t u = (uint64_t)(uint32_t)__float_as_int(s[lo]) |
@@ -280,12 +282,10 @@ __device__ __forceinline__ void item_features(const float *__restrict__ st, floa
                       1.f);
   *cand_out = (uint32_t)min(cand_it, (long long)UINT_MAX);
 }
-template <class R>
 __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items, int64_t nitems,
                                                  const int64_t *__restrict__ nitems_dev,
-                                                 const int *__restrict__ rowlist,
-                                                 const int *__restrict__ hi,
-                                                 const R *__restrict__ rec,
+                                                 const int2 *__restrict__ rowlist,
+                                                 const uint2 *__restrict__ rowfeat,
                                                  const double *__restrict__ pre, QParams qp0,
                                                  int a10, int a20,
                                                  const SceneParams *__restrict__ sp,
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
   const int64_t per = (nitems + nwaves - 1) / nwaves;
   const int64_t ib = (int64_t)wave * per;
   const int64_t ie = ib + per < nitems ? ib + per : nitems;
-  __shared__ float park_mem[kT / 64][kWave * kStatPitch];
+  __shared__ float park_mem[kT / 64][kPark * kStatPitch];
   float *park = park_mem[threadIdx.x >> 6];
   int nslot = 0;       // items parked
   int64_t pbase = ib;  // item of slot 0
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
   int64_t sc_lo = 0, sc_hi = -1;  // sorted positions of the cached scene
   bool act = false;
   int p = 0, h = 0;            // this lane's row and its chain end
-  double n1 = 0, n2 = 0, o1 = 0, o2 = 0;
+  double n1 = 0, n2 = 0;
   float nr = 1.f, m_le1 = 0, m_le2 = 0, v_le1 = 0, v_le2 = 0, m_c1 = 0, m_c2 = 0, v_c1 = 0,
         v_c2 = 0, f_v = 0, f_e = 0, f_f = 0, m_chain = 0;
   // The next item and this item's prefix-sum lookups are loaded ahead of
@@ -363,23 +363,22 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
         auto pick = [](const double *v, int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); };
         n1 = pick(qp.scale, a1) / kLevelMax;
         n2 = pick(qp.scale, a2) / kLevelMax;
-        o1 = pick(qp.origin, a1);
-        o2 = pick(qp.origin, a2);
       }
       act = lane < nrows;
       float fr = act ? 1.f : 0.f;
       float le1 = 0, le2 = 0, c1 = 0, c2 = 0, tv = 0, te = 0, tf = 0, chain = 0;
-      if (act) {
-        p = rowlist[item.rowoff + lane];
-        h = hi[p];
-        const R r = rec[p];
-        le1 = logf(fmaxf((float)(r.hi1 - r.lo1), 1e-30f)) * 0.1f;
-        le2 = logf(fmaxf((float)(r.hi2 - r.lo2), 1e-30f)) * 0.1f;
-        c1 = (float)((0.5 * (r.lo1 + r.hi1) - o1) * n1);
-        c2 = (float)((0.5 * (r.lo2 + r.hi2) - o2) * n2);
-        tv = r.type == 1 ? 1.f : 0.f;
-        te = r.type == 2 ? 1.f : 0.f;
-        tf = r.type == 4 ? 1.f : 0.f;
+      if (act) {  // the rows and their features, in group order (contiguous)
+        const int2 row = rowlist[item.rowoff + lane];
+        const RowFeat f = row_feat_unpack(rowfeat[item.rowoff + lane]);
+        p = row.x;
+        h = row.y;
+        le1 = f.le1;
+        le2 = f.le2;
+        c1 = f.c1;
+        c2 = f.c2;
+        tv = f.kind == 0 ? 1.f : 0.f;
+        te = f.kind == 1 ? 1.f : 0.f;
+        tf = f.kind == 2 ? 1.f : 0.f;
         chain = (float)(h - p);
       }
       nr = fmaxf(wave_sum_dpp(fr), 1.f);
@@ -429,7 +428,7 @@ __global__ __launch_bounds__(kT) void k_features(const Item *__restrict__ items,
       w[kSN2Lo] = __int_as_float((int)(uint32_t)n2u);
       w[kSN2Hi] = __int_as_float((int)(uint32_t)(n2u >> 32));
     }
-    if (++nslot == kWave) flush();
+    if (++nslot == kPark) flush();
   }
   if (nslot) flush();
 }
@@ -978,14 +977,9 @@ void launch_features(bool f32, const SweepArgs &sa, const ColStats *cols, int64_
   hipLaunchKernelGGL(k_col_pre, dim3(gr), dim3(kT), 0, st, cols, ncols, (const double *)run, pre);
   const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
   const unsigned g = (unsigned)std::min<int64_t>((sa.nitems + 3) / 4, 4096);
-  if (f32)
-    hipLaunchKernelGGL((k_features<Exact64>), dim3(g), dim3(kT), 0, st, sa.items, sa.nitems,
-                       sa.nitems_dev, sa.rowlist, sa.hi, (const Exact64 *)sa.rec, (const double *)pre, qp, a1, a2, sp,
-                       seg_off, nscenes, feat, cand);
-  else
-    hipLaunchKernelGGL((k_features<Rec64>), dim3(g), dim3(kT), 0, st, sa.items, sa.nitems,
-                       sa.nitems_dev, sa.rowlist, sa.hi, (const Rec64 *)sa.rec, (const double *)pre, qp, a1, a2, sp,
-                       seg_off, nscenes, feat, cand);
+  hipLaunchKernelGGL(k_features, dim3(g), dim3(kT), 0, st, sa.items, sa.nitems, sa.nitems_dev,
+                     sa.rowlist, sa.rowfeat, (const double *)pre, qp, a1, a2, sp, seg_off, nscenes,
+                     feat, cand);
   STQ_HIP(hipGetLastError());
 }
 

@@ -103,7 +103,7 @@ __host__ __device__ inline float round_down_f32(double d) { return next_down((fl
 __host__ __device__ inline float round_up_f32(double d) { return next_up((float)d); }
 
 struct Item {   // one sweep work item: up to 64 rows against a range of q
-  int rowoff;   // first entry of the group's rows in rowlist[]
+  int rowoff;   // first entry of the group's rows in rowlist[] ({position, chain end})
   int nrows;    // rows in the group (<= 64)
   int q0, q1;   // inclusive q range of this item
 };

@@ -200,7 +200,8 @@ __global__ __launch_bounds__(kThreads) void k_gather_f64(const stq_aabb_f64 *__r
                                                          double *__restrict__ smax,
                                                          Rec64 *__restrict__ rec,
                                                          QRec *__restrict__ qrec,
-                                                         ColStats *__restrict__ cols) {
+                                                         ColStats *__restrict__ cols,
+                                                         uint2 *__restrict__ rowf) {
   const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   const bool valid = s < n;
   QParams qp{};
@@ -228,6 +229,9 @@ __global__ __launch_bounds__(kThreads) void k_gather_f64(const stq_aabb_f64 *__r
     if (qrec)
       qrec[s] = make_qrec(r.lo1, r.hi1, r.lo2, r.hi2, r.type, qp.origin[a1], qp.scale[a1],
                           qp.origin[a2], qp.scale[a2]);
+    if (rowf)
+      rowf[s] = row_feat(r.lo1, r.hi1, r.lo2, r.hi2, r.type, qp.origin[a1],
+                         qp.scale[a1] / kLevelMax, qp.origin[a2], qp.scale[a2] / kLevelMax);
   }
   if (cols)
     col_stats_put(valid, s, r.lo1, r.hi1, r.lo2, r.hi2, r.type, qp.origin[a1],
@@ -241,7 +245,8 @@ __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__r
                                                          uint32_t *__restrict__ smax,
                                                          Exact64 *__restrict__ rec,
                                                          QRec *__restrict__ qrec,
-                                                         ColStats *__restrict__ cols) {
+                                                         ColStats *__restrict__ cols,
+                                                         uint2 *__restrict__ rowf) {
   const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   const bool valid = s < n;
   const SceneParams p = *sp;
@@ -267,6 +272,9 @@ __global__ __launch_bounds__(kThreads) void k_gather_f32(const stq_aabb_f64 *__r
     rec[s] = r;
     qrec[s] = make_qrec(r.lo1, r.hi1, r.lo2, r.hi2, r.type, qp.origin[a1], qp.scale[a1],
                         qp.origin[a2], qp.scale[a2]);
+    if (rowf)
+      rowf[s] = row_feat(r.lo1, r.hi1, r.lo2, r.hi2, r.type, qp.origin[a1],
+                         qp.scale[a1] / kLevelMax, qp.origin[a2], qp.scale[a2] / kLevelMax);
   }
   if (cols)
     col_stats_put(valid, s, r.lo1, r.hi1, r.lo2, r.hi2, r.type, qp.origin[a1],
@@ -308,12 +316,14 @@ template <bool Q>
 __global__ __launch_bounds__(kPlanThreads) void k_plan(const int *__restrict__ hi,
                                                    const QRec *__restrict__ qrec, int64_t n,
                                                    int W, int chunk, float tail_cost, bool cap,
-                                                   int *__restrict__ rowlist,
+                                                   int2 *__restrict__ rowlist,
                                                    int4 *__restrict__ groups,
                                                    int4 *__restrict__ tgroups,
                                                    int2 *__restrict__ gchunk,
                                                    uint32_t *__restrict__ nchunks,
-                                                   unsigned long long *__restrict__ lane_steps) {
+                                                   unsigned long long *__restrict__ lane_steps,
+                                                   const uint2 *__restrict__ rowf,
+                                                   uint2 *__restrict__ rowfeat) {
   typedef typename std::conditional<Q, uint32_t, uint64_t>::type K;
   __shared__ K key[Q ? kMaxWindow32 : kMaxWindow64];
   __shared__ unsigned rs_cnt[Q ? (kPlanThreads / 64) * 256 : 1], rs_sum[Q ? 256 : 1];
@@ -358,7 +368,10 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(const int *__restrict__ h
           h = (int)(r >> 32);
         }
       }
-      if (act) rowlist[wbase + j] = p;
+      if (act) {
+        rowlist[wbase + j] = make_int2(p, h);
+        if (rowf) rowfeat[wbase + j] = rowf[p];
+      }
       const int nrows = min(64, cnt - 64 * k);
       const GroupPlan gp = plan_group(lane, nrows, p, h, (int)(wbase + 64 * k), chunk, cap,
                                       tail_cost);
@@ -578,8 +591,9 @@ __device__ __forceinline__ void tail64(const SweepArgs &a, const Rec64 *__restri
   rr.v0 = rr.v1 = rr.v2 = rr.type = 0;
   uint32_t rop = 0;
   if (lane < R) {
-    rp = a.rowlist[item.rowoff + lane];
-    rh = a.hi[rp];
+    const int2 r = a.rowlist[item.rowoff + lane];
+    rp = r.x;
+    rh = r.y;
     rr = rec[rp];
     if (MODE != kCount) rop = a.orig[rp];
   }
@@ -638,8 +652,8 @@ __global__ __launch_bounds__(kThreads) void k_sweep64(SweepArgs a, const Rec64 *
       tail64<MODE>(a, rec, out, item, lane, em);
     } else {
       const bool act = lane < item.nrows;
-      const int p = act ? a.rowlist[item.rowoff + lane] : INT_MAX;
-      const int myhi = act ? a.hi[p] : INT_MIN;
+      const int2 row = act ? a.rowlist[item.rowoff + lane] : make_int2(INT_MAX, INT_MIN);
+      const int p = row.x, myhi = row.y;
       Rec64 mine;
       mine.lo1 = mine.lo2 = INFINITY;  // empty box for idle lanes
       mine.hi1 = mine.hi2 = -INFINITY;
@@ -764,8 +778,9 @@ __device__ __forceinline__ void tailq(const SweepArgs &a, const Exact64 *__restr
   int rp = INT_MAX, rh = INT_MIN;
   uint32_t rlo = 0x7fff7fffu, rhi = 0u, rop = 0;
   if (lane < R) {
-    rp = a.rowlist[item.rowoff + lane];
-    rh = a.hi[rp];
+    const int2 r = a.rowlist[item.rowoff + lane];
+    rp = r.x;
+    rh = r.y;
     const QRec x = row_form(qrec[rp]);
     rlo = x.lo;
     rhi = x.hi;
@@ -867,8 +882,8 @@ __global__ __launch_bounds__(kThreads, 8) void k_qsweep(SweepArgs a,
       tailq<MODE>(a, rec, qrec, out, item, lane, em, nconf);
     } else {
       const bool act = lane < item.nrows;
-      const int p = act ? a.rowlist[item.rowoff + lane] : INT_MAX;
-      const int myhi = act ? a.hi[p] : INT_MIN;
+      const int2 row = act ? a.rowlist[item.rowoff + lane] : make_int2(INT_MAX, INT_MIN);
+      const int p = row.x, myhi = row.y;
       QRec mq;
       mq.lo = 0x7fff7fffu;  // idle lanes: an empty level box that meets no column (make_qrec)
       mq.hi = 0u;
@@ -1045,8 +1060,8 @@ __global__ __launch_bounds__(kThreads, 8) void k_gsweep64(SweepArgs a,
       tail64<MODE>(a, rec, out, item, lane, em);
     } else {
       const bool act = lane < item.nrows;
-      const int p = act ? a.rowlist[item.rowoff + lane] : INT_MAX;
-      const int myhi = act ? a.hi[p] : INT_MIN;
+      const int2 row = act ? a.rowlist[item.rowoff + lane] : make_int2(INT_MAX, INT_MIN);
+      const int p = row.x, myhi = row.y;
       Rec64 mine;
       mine.lo1 = mine.lo2 = INFINITY;  // idle lanes: an empty box
       mine.hi1 = mine.hi2 = -INFINITY;
@@ -1142,14 +1157,14 @@ void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, const
 
 void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, int64_t n,
                    int axis, const SceneParams *sp, void *smin, void *smax, void *rec, QRec *qrec,
-                   ColStats *cols, hipStream_t st) {
+                   ColStats *cols, uint2 *rowf, hipStream_t st) {
   if (n <= 0) return;
   if (f32)
     hipLaunchKernelGGL(k_gather_f32, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, order,
-                       n, sp, (uint32_t *)smin, (uint32_t *)smax, (Exact64 *)rec, qrec, cols);
+                       n, sp, (uint32_t *)smin, (uint32_t *)smax, (Exact64 *)rec, qrec, cols, rowf);
   else
     hipLaunchKernelGGL(k_gather_f64, dim3(blocks_for(n)), dim3(kThreads), 0, st, boxes, order,
-                       n, axis, sp, (double *)smin, (double *)smax, (Rec64 *)rec, qrec, cols);
+                       n, axis, sp, (double *)smin, (double *)smax, (Rec64 *)rec, qrec, cols, rowf);
   STQ_HIP(hipGetLastError());
 }
 
@@ -1170,8 +1185,9 @@ void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, i
 }
 
 void launch_plan(const int *hi, const QRec *qrec, int64_t n, int W, int chunk, float tail_cost,
-                 int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
-                 unsigned long long *lane_steps, hipStream_t st, bool cap) {
+                 int2 *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
+                 unsigned long long *lane_steps, hipStream_t st, bool cap, const uint2 *rowf,
+                 uint2 *rowfeat) {
   if (n <= 0) return;
   if (W > (qrec ? kMaxWindow32 : kMaxWindow64))
     throw StatusError(STQ_ERR_INVALID_ARGUMENT, "planning window too large");
@@ -1179,10 +1195,12 @@ void launch_plan(const int *hi, const QRec *qrec, int64_t n, int W, int chunk, f
   const dim3 grid((unsigned)std::min<int64_t>(nwin, kReduceGrid));
   if (qrec)
     hipLaunchKernelGGL(k_plan<true>, grid, dim3(kPlanThreads), 0, st, hi, qrec, n, W, chunk,
-                       tail_cost, cap, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
+                       tail_cost, cap, rowlist, groups, tgroups, gchunk, nchunks, lane_steps, rowf,
+                       rowfeat);
   else
     hipLaunchKernelGGL(k_plan<false>, grid, dim3(kPlanThreads), 0, st, hi, qrec, n, W, chunk,
-                       tail_cost, cap, rowlist, groups, tgroups, gchunk, nchunks, lane_steps);
+                       tail_cost, cap, rowlist, groups, tgroups, gchunk, nchunks, lane_steps, rowf,
+                       rowfeat);
   STQ_HIP(hipGetLastError());
 }
 

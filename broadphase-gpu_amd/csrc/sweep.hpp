@@ -191,6 +191,38 @@ __device__ __forceinline__ float log_extent(double lo, double hi) {
 __device__ __forceinline__ float centre_place(double lo, double hi, double origin, double norm) {
   return (float)((0.5 * (lo + hi) - origin) * norm);
 }
+
+// The row features of a box for learned sizing, packed in 8 bytes: the
+// log-extents of both non-sweep axes as FP16, the centre places (0..1) as
+// 14- and 16-bit fractions and the kind (0 vertex, 1 edge, 2 face) in the two
+// top bits of the first.  The gather writes them by sorted position and the
+// planner copies them into group order, so the features of a group's rows
+// are one contiguous 512-byte read instead of 64 scattered FP64 records.
+__device__ inline uint2 row_feat(double lo1, double hi1, double lo2, double hi2, int type,
+                                 double o1, double n1, double o2, double n2) {
+  const _Float16 l1 = (_Float16)log_extent(lo1, hi1), l2 = (_Float16)log_extent(lo2, hi2);
+  const float c1 = fminf(fmaxf(centre_place(lo1, hi1, o1, n1), 0.f), 1.f);
+  const float c2 = fminf(fmaxf(centre_place(lo2, hi2, o2, n2), 0.f), 1.f);
+  const uint32_t kind = type == 1 ? 0u : (type == 2 ? 1u : 2u);
+  uint2 f;
+  f.x = (uint32_t)__builtin_bit_cast(uint16_t, l1) |
+        ((uint32_t)__builtin_bit_cast(uint16_t, l2) << 16);
+  f.y = (uint32_t)(c1 * 16383.f + 0.5f) | (kind << 14) | ((uint32_t)(c2 * 65535.f + 0.5f) << 16);
+  return f;
+}
+struct RowFeat {
+  float le1, le2, c1, c2;
+  int kind;  // 0 vertex, 1 edge, 2 face
+};
+__device__ inline RowFeat row_feat_unpack(uint2 f) {
+  RowFeat r;
+  r.le1 = (float)__builtin_bit_cast(_Float16, (uint16_t)(f.x & 0xffffu));
+  r.le2 = (float)__builtin_bit_cast(_Float16, (uint16_t)(f.x >> 16));
+  r.c1 = (float)(f.y & 0x3fffu) * (1.f / 16383.f);
+  r.kind = (int)((f.y >> 14) & 3u);
+  r.c2 = (float)(f.y >> 16) * (1.f / 65535.f);
+  return r;
+}
 // Adds sorted position s to its block.  Every lane of the wave calls this,
 // with valid = false past the last position; the wave's lanes hold 64
 // consecutive positions from a multiple of 64, so a block is 32 consecutive
@@ -436,10 +468,11 @@ void launch_axis_final(const double *partial, int nblocks, int64_t n, int policy
 // needs sp (its level range).
 void launch_keys(bool f32, const stq_aabb_f64 *boxes, int64_t n, int axis, const SceneParams *sp,
                  void *keys, uint32_t *vals, unsigned *err, hipStream_t st);
-// cols: column statistics of every kColBlock positions (null: not made).
+// cols: column statistics of every kColBlock positions, rowf: the row
+// features of every position (row_feat); null: not made.
 void launch_gather(bool f32, const stq_aabb_f64 *boxes, const uint32_t *order, int64_t n,
                    int axis, const SceneParams *sp, void *smin, void *smax, void *rec, QRec *qrec,
-                   ColStats *cols, hipStream_t st);
+                   ColStats *cols, uint2 *rowf, hipStream_t st);
 // Rows outside [r0, r1) (r1 < 0: n) get empty chains.
 void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, int *hi,
                       unsigned long long *cand, hipStream_t st, int64_t r0 = 0, int64_t r1 = -1);
@@ -449,9 +482,12 @@ void launch_chain_end(bool f32, const void *smin, const void *smax, int64_t n, i
 // windows ordered by chain end.  cap: at most kCapChunks items per body and
 // per tail (chunks grow instead), so a scene has at most
 // capped_item_bound(n, chunk) items.
+// rowf (learned sizing; null: none): the row features by sorted position,
+// copied into rowfeat in the order of rowlist.
 void launch_plan(const int *hi, const QRec *qrec, int64_t n, int W, int chunk, float tail_cost,
-                 int *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
-                 unsigned long long *lane_steps, hipStream_t st, bool cap = false);
+                 int2 *rowlist, int4 *groups, int4 *tgroups, int2 *gchunk, uint32_t *nchunks,
+                 unsigned long long *lane_steps, hipStream_t st, bool cap = false,
+                 const uint2 *rowf = nullptr, uint2 *rowfeat = nullptr);
 // Items of every group from the group records (body chunks, then tail chunks).
 void launch_items(const int4 *groups, const int4 *tgroups, const int2 *gchunk, const int64_t *ioff,
                   int64_t ngroups, Item *items, hipStream_t st);
@@ -464,8 +500,8 @@ struct SweepArgs {
   const Item *items = nullptr;
   int64_t nitems = 0;            // work items (or selected items when sel != null)
   const int *sel = nullptr;      // optional item selection
-  const int *rowlist = nullptr;
-  const int *hi = nullptr;
+  const int2 *rowlist = nullptr;  // the rows in group order: {sorted position, chain end}
+  const uint2 *rowfeat = nullptr;  // learned sizing: their row features (row_feat), same order
   const void *rec = nullptr;     // Rec64 (FP64 form) or Exact64 by sorted position
   const QRec *qrec = nullptr;    // reduced-precision form; with Rec64 records: the grouped FP64 form
   const uint32_t *orig = nullptr;
