@@ -105,6 +105,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
     launch_axis_stats(boxes, n, part, nb, st, f32 ? static_cast<uint32_t *>(kbuf) : nullptr,
                       (unsigned *)&ctr[0]);
     launch_axis_final(part, nb, n, pol, d_sp, st);
+    if (f32) launch_sweep_range(boxes, nullptr, 1, n, d_sp, st);
   }
   stats.axis = axis;
   if (!f32) launch_keys(f32, boxes, n, axis, d_sp, kbuf, order, (unsigned *)&ctr[0], st);

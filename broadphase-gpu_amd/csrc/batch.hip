@@ -412,6 +412,7 @@ void launch_seg_stats(const stq_aabb_f64 *boxes, const BatchTables &t, const Bat
                        (const double *)d.partial, d.scene_chunk_begin, d.seg_off, t.nscenes, policy,
                        fixed_axis, sweep_key_bits(t.max_scene), d.params);
   STQ_HIP(hipGetLastError());
+  if (keys3) launch_sweep_range(boxes, d.seg_off, t.nscenes, t.n, d.params, st);
 }
 
 void launch_keys_seg(bool f32, const stq_aabb_f64 *boxes, const BatchTables &t, const BatchDev &d,

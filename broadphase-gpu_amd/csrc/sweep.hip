@@ -147,6 +147,69 @@ __global__ __launch_bounds__(kThreads) void k_axis_final(const double *__restric
   *out = sp;
 }
 
+// ---- the bulk of each scene's sweep-key range ------------------------------------
+// One block per scene: a strided sample of at most kRangeSample boxes, its
+// minima and maxima sorted together in LDS (one bitonic network, padded with
+// infinities), then the range [qlo - w/8, qhi + w/8] with w = qhi - qlo,
+// kept inside the scene's finite range.
+constexpr int kRangeSample = 512;
+__global__ __launch_bounds__(kThreads) void k_sweep_range(const stq_aabb_f64 *__restrict__ boxes,
+                                                          const int64_t *__restrict__ seg_off,
+                                                          int64_t n,
+                                                          SceneParams *__restrict__ sp) {
+  __shared__ double smn[kRangeSample], smx[kRangeSample];
+  const int sc = blockIdx.x;
+  const int64_t b0 = seg_off ? seg_off[sc] : 0, m = seg_off ? seg_off[sc + 1] - b0 : n;
+  if (m < 2) return;
+  const int axis = sp[sc].axis;
+  const int S = (int)(m < kRangeSample ? m : kRangeSample);
+  for (int i = threadIdx.x; i < kRangeSample; i += kThreads) {
+    double a = INFINITY, c = -INFINITY;  // padding: after the minima, before the maxima
+    if (i < S) {
+      const stq_aabb_f64 &b = boxes[b0 + (int64_t)i * m / S];
+      a = b.min[axis];
+      c = b.max[axis];
+    }
+    smn[i] = a;
+    smx[i] = c;
+  }
+  __syncthreads();
+  for (int size = 2; size <= kRangeSample; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int j = threadIdx.x; j < kRangeSample; j += kThreads) {
+        const int pj = j ^ stride;
+        if (pj > j) {
+          const bool up = (j & size) == 0;
+          const double a = smn[j], c = smn[pj];
+          if ((a > c) == up) {
+            smn[j] = c;
+            smn[pj] = a;
+          }
+          const double x = smx[j], y = smx[pj];
+          if ((x > y) == up) {
+            smx[j] = y;
+            smx[pj] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int k = S / 200;  // 0.5% of the sample on each side
+  if (threadIdx.x == 0) {
+    const double qlo = smn[k], qhi = smx[kRangeSample - 1 - k];
+    const double w = qhi - qlo;
+    const SceneParams p = sp[sc];  // the finite range from the statistics
+    const double flo = p.korigin, fup = p.kscale > 0 ? p.korigin + (double)p.kmax / p.kscale : flo;
+    const double lo = fmax(qlo - 0.125 * w, flo), up = fmin(qhi + 0.125 * w, fup);
+    const double sc2 = (double)sp[sc].kmax / (up - lo);
+    if (isfinite(lo) && isfinite(up) && up > lo && isfinite(sc2)) {
+      sp[sc].korigin = lo;
+      sp[sc].kscale = sc2;
+    }
+  }
+}
+
 // ---- keys ---------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void k_keys_f64(const stq_aabb_f64 *__restrict__ boxes,
                                                        int64_t n, int axis,
@@ -1140,6 +1203,14 @@ void launch_axis_final(const double *partial, int nblocks, int64_t n, int policy
                        SceneParams *out, hipStream_t st) {
   hipLaunchKernelGGL(k_axis_final, dim3(1), dim3(kThreads), 0, st, partial, nblocks, n, policy,
                      sweep_key_bits(n), out);
+  STQ_HIP(hipGetLastError());
+}
+
+void launch_sweep_range(const stq_aabb_f64 *boxes, const int64_t *seg_off, int nscenes, int64_t n,
+                        SceneParams *sp, hipStream_t st) {
+  if (nscenes <= 0) return;
+  hipLaunchKernelGGL(k_sweep_range, dim3((unsigned)nscenes), dim3(kThreads), 0, st, boxes, seg_off,
+                     n, sp);
   STQ_HIP(hipGetLastError());
 }
 

@@ -275,15 +275,16 @@ __device__ inline void col_stats_put(bool valid, int64_t s, double lo1, double h
 // the quantisation of the sweep axis's keys (below).
 struct SceneParams {
   int axis;
-  uint32_t kmax;  // largest sweep key: 2^bits - 1
+  uint32_t kmax;   // largest sweep key: 2^bits - 1
   QParams qp;
-  double kscale;  // sweep keys per unit of the sweep axis (0: every key equal)
+  double korigin;  // sweep-axis coordinate of key 0
+  double kscale;   // sweep keys per unit of the sweep axis (0: every key equal)
 };
 
 // Sweep keys of the reduced-precision form.  The FP32 outward bounds of the
 // sweep axis (the reference's float rounding) are quantised outward once more,
-// to integers in [0, kmax] over the scene's finite range:
-//   down(f) = floor((f - origin) * kscale),  up(f) = ceil((f - origin) * kscale),
+// to integers in [0, kmax] over the scene's key range (set_sweep_keys):
+//   down(f) = floor((f - korigin) * kscale),  up(f) = ceil((f - korigin) * kscale),
 // clamped.  Both are non-decreasing in f, so a pair that overlaps in FP64
 // overlaps in keys (min_q <= max_p gives down(min_q) <= up(max_p)), and the
 // sort, the chain ends and the planning run on these integers; the FP64
@@ -291,12 +292,12 @@ struct SceneParams {
 // its down(min), so the keys in sorted order are non-decreasing.
 __device__ inline uint32_t skey_down(float f, const SceneParams &p) {
   if (p.kscale == 0.0) return 0u;
-  const double t = floor(((double)f - p.qp.origin[p.axis]) * p.kscale);
+  const double t = floor(((double)f - p.korigin) * p.kscale);
   return (uint32_t)fmin(fmax(t, 0.0), (double)p.kmax);
 }
 __device__ inline uint32_t skey_up(float f, const SceneParams &p) {
   if (p.kscale == 0.0) return p.kmax;
-  const double t = ceil(((double)f - p.qp.origin[p.axis]) * p.kscale);
+  const double t = ceil(((double)f - p.korigin) * p.kscale);
   return (uint32_t)fmin(fmax(t, 0.0), (double)p.kmax);
 }
 // Inverse of key_of(float): the float whose order-preserving image is k.
@@ -308,12 +309,13 @@ __device__ inline float float_of_key(uint32_t k) {
 // (two sort passes; a uniform scene puts at most 16 boxes in a key, which
 // lengthens a chain by as many positions), 24 up to 2^27, else 32.
 inline int sweep_key_bits(int64_t n) { return n <= (1 << 20) ? 16 : (n <= (1 << 27) ? 24 : 32); }
-// The quantisation of the chosen axis: kmax and kscale from the scene's
-// finite range [lo, up] on that axis.
+// The quantisation of the chosen axis over the scene's finite range [lo, up]
+// on that axis; launch_sweep_range then narrows it to the bulk of the scene.
 __device__ inline void set_sweep_keys(SceneParams &p, double lo, double up, int bits) {
   p.kmax = bits >= 32 ? 0xffffffffu : (1u << bits) - 1u;
   const double range = up - lo;
   const double sc = (range > 0 && isfinite(range)) ? (double)p.kmax / range : 0.0;
+  p.korigin = isfinite(lo) ? lo : 0.0;
   p.kscale = isfinite(sc) ? sc : 0.0;
 }
 
@@ -462,6 +464,16 @@ void launch_axis_stats(const stq_aabb_f64 *boxes, int64_t n, double *partial, in
 // The scene's sweep axis (axis policy 0..4), level ranges and the
 // quantisation of its sweep keys (sweep_key_bits(n) bits) from the statistics
 // of nblocks blocks, into *out on the device.
+//
+// launch_sweep_range narrows each scene's sweep-key range to its bulk: a
+// sample of at most 512 boxes of the scene, the 0.5% quantile of their
+// minima and the 99.5% quantile of their maxima on the sweep axis, widened
+// by an eighth of their distance on each side (never past the finite range).  A few far-away boxes then no
+// longer spread the keys of all the others over a range they barely use
+// (they share the end keys, which clamping keeps outward).  seg_off null:
+// one scene of n boxes.
+void launch_sweep_range(const stq_aabb_f64 *boxes, const int64_t *seg_off, int nscenes, int64_t n,
+                        SceneParams *sp, hipStream_t st);
 void launch_axis_final(const double *partial, int nblocks, int64_t n, int policy,
                        SceneParams *out, hipStream_t st);
 // The sweep axis is sp->axis where sp is given, else axis.  The FP32 form

@@ -140,6 +140,49 @@ def test_rounding_edges_fp32_confirm_is_exact(ctx):
     assert ctx.stats()["n_confirm"] >= len(ref)
 
 
+def test_sweep_keys_outliers_infinite_bounds_and_key_widths(ctx):
+    """The reduced-precision form sorts quantised sweep keys (16 bits up to
+    2^20 boxes per scene, 24 above).  Far outliers, infinite bounds, a scene
+    whose sweep coordinates are all equal and scenes on both sides of 2^20
+    boxes, single and batched, give the oracle's set exactly; the outliers do
+    not coarsen the other boxes' keys (the candidate count barely moves)."""
+    f32 = dict(precision=stq_amd.PRECISION_F32_CONFIRM, axis_policy=stq_amd.AXIS_X)
+    rng = np.random.default_rng(21)
+    b = scenes.uniform_scene(200000, seed=21)
+    run(ctx, b, **f32)
+    base = ctx.stats()["n_candidates"]
+    assert base >= oracle.candidates(b, axis=0)[0]
+    o = b.copy()
+    k = rng.choice(len(o), size=40, replace=False)
+    far = np.where(rng.random((40, 1)) < 0.5, -1e6, 1e6) * (1.0 + rng.random((40, 3)))
+    o["min"][k] = far
+    o["max"][k] = far + 1.0
+    o["min"][k[:5], 0] = -np.inf          # infinite bounds on the sweep axis
+    o["max"][k[5:10], 0] = np.inf
+    o["min"][k[10:12]] = -np.inf
+    o["max"][k[10:12]] = np.inf           # boxes that meet everything
+    got = run(ctx, o, **f32)
+    assert same_set(got, oracle.reference_set(o))
+    cand = ctx.stats()["n_candidates"]
+    print(f"candidates: {base} without outliers, {cand} with them", flush=True)
+    assert cand <= 1.02 * base + 2 * 12 * len(o)
+    # every sweep coordinate equal: one key for all
+    e = scenes.uniform_scene(3000, seed=22)
+    e["min"][:, 0] = 0.25
+    e["max"][:, 0] = 0.25
+    assert same_set(run(ctx, e, **f32), oracle.reference_set(e))
+    # past 2^20 boxes: 24-bit keys, single and as the largest scene of a batch
+    big = scenes.uniform_scene((1 << 20) + 4096, seed=23, extent_scale=0.5)
+    ref_big = oracle.reference_set(big)
+    assert same_set(run(ctx, big, **f32), ref_big)
+    small = scenes.uniform_scene(30000, seed=24)
+    allb, offs = scenes.concat_scenes([small, big])
+    ctx.set_options(**f32)
+    pairs, poffs = ctx.find_pairs_batched(allb, offs)
+    assert same_set(pairs[poffs[0]:poffs[1]], oracle.reference_set(small))
+    assert same_set(pairs[poffs[1]:poffs[2]], ref_big)
+
+
 def test_invalid_boxes_rejected(ctx):
     b = scenes.uniform_scene(1000, seed=11)
     ctx.set_options()
