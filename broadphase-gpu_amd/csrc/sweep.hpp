@@ -305,10 +305,11 @@ __device__ inline float float_of_key(uint32_t k) {
   const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
   return __builtin_bit_cast(float, u);
 }
-// Bits of the sweep keys for a scene of at most n boxes: 16 up to 2^20 boxes
-// (two sort passes; a uniform scene puts at most 16 boxes in a key, which
-// lengthens a chain by as many positions), 24 up to 2^27, else 32.
-inline int sweep_key_bits(int64_t n) { return n <= (1 << 20) ? 16 : (n <= (1 << 27) ? 24 : 32); }
+// Bits of the sweep keys for a scene of at most n boxes: 16 up to 2^22 boxes
+// (two sort passes; a uniform scene puts at most 64 boxes in a key, which
+// lengthens a chain by about as many positions: 1% on the 3M-box cloth), 24
+// up to 2^28, else 32.
+inline int sweep_key_bits(int64_t n) { return n <= (1 << 22) ? 16 : (n <= (1 << 28) ? 24 : 32); }
 // The quantisation of the chosen axis over the scene's finite range [lo, up]
 // on that axis; launch_sweep_range then narrows it to the bulk of the scene.
 __device__ inline void set_sweep_keys(SceneParams &p, double lo, double up, int bits) {

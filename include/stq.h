@@ -88,7 +88,7 @@ enum stq_precision {
      ends use the sweep axis's FP32 bounds rounded outward (round to nearest,
      then one ulp outward: the reference rule of src/aabb.cu:56-61),
      quantised outward once more to 16-, 24- or 32-bit integers over the
-     scene's range (16 bits up to 2^20 boxes per scene); the two other axes
+     scene's range (16 bits up to 2^22 boxes per scene); the two other axes
      are tested on 14-bit levels per scene, rounded outward, with the kind
      codes of the type filter folded in.  Every filter keeps a superset of
      the FP64 candidates, and every survivor is re-tested in FP64 on all

@@ -142,7 +142,7 @@ def test_rounding_edges_fp32_confirm_is_exact(ctx):
 
 def test_sweep_keys_outliers_infinite_bounds_and_key_widths(ctx):
     """The reduced-precision form sorts quantised sweep keys (16 bits up to
-    2^20 boxes per scene, 24 above).  Far outliers, infinite bounds, a scene
+    2^22 boxes per scene, 24 above).  Far outliers, infinite bounds, a scene
     whose sweep coordinates are all equal and scenes on both sides of 2^20
     boxes, single and batched, give the oracle's set exactly; the outliers do
     not coarsen the other boxes' keys (the candidate count barely moves)."""
@@ -171,8 +171,8 @@ def test_sweep_keys_outliers_infinite_bounds_and_key_widths(ctx):
     e["min"][:, 0] = 0.25
     e["max"][:, 0] = 0.25
     assert same_set(run(ctx, e, **f32), oracle.reference_set(e))
-    # past 2^20 boxes: 24-bit keys, single and as the largest scene of a batch
-    big = scenes.uniform_scene((1 << 20) + 4096, seed=23, extent_scale=0.5)
+    # past 2^22 boxes: 24-bit keys, single and as the largest scene of a batch
+    big = scenes.uniform_scene((1 << 22) + 4096, seed=23, extent_scale=0.05)
     ref_big = oracle.reference_set(big)
     assert same_set(run(ctx, big, **f32), ref_big)
     small = scenes.uniform_scene(30000, seed=24)
