@@ -925,16 +925,17 @@ __global__ __launch_bounds__(kT) void k_compact(const uint32_t *__restrict__ cou
     if (c <= caps[it]) {
       const int2 *src = scratch + cap_off[it];
       int2 *dst = out + off[it];
-      // four loads in flight per lane, then the rest one by one
-      uint32_t k = lane;
-      for (; k + 192 < c; k += 256) {
-        const int2 x0 = src[k], x1 = src[k + 64], x2 = src[k + 128], x3 = src[k + 192];
-        dst[k] = x0;
-        dst[k + 64] = x1;
-        dst[k + 128] = x2;
-        dst[k + 192] = x3;
+      // rounds of up to eight loads in flight per lane (512 pairs), the last
+      // round's loads predicated: one memory latency per 512 pairs
+      for (uint32_t k = lane; k < c; k += 8 * kWave) {
+        int2 x[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          if (k + r * kWave < c) x[r] = src[k + r * kWave];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          if (k + r * kWave < c) dst[k + r * kWave] = x[r];
       }
-      for (; k < c; k += 64) dst[k] = src[k];
     } else if (lane == 0) {
       ovf[atomicAdd(novf, 1ull)] = (int)it;
     }

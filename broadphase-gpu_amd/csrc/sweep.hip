@@ -998,6 +998,11 @@ __global__ __launch_bounds__(kThreads, 8) void k_qsweep(SweepArgs a,
           h = h & (xp.v0 != xq.v0) & (xp.v0 != xq.v1) & (xp.v0 != xq.v2) & (xp.v1 != xq.v0) &
               (xp.v1 != xq.v1) & (xp.v1 != xq.v2) & (xp.v2 != xq.v0) & (xp.v2 != xq.v1) &
               (xp.v2 != xq.v2);
+          // The id is needed only for a pair, and the compiler would load it
+          // inside the emission branch, a second memory latency per round
+          // after the record's; taking its value here keeps its load issued
+          // beside the record's.
+          if (MODE != kCount) asm volatile("" : "+v"(oq));
         }
         nconf += (unsigned long long)__popcll(ballot(hg));
         const uint64_t hm = ballot(h);
