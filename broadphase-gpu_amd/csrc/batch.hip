@@ -210,13 +210,13 @@ __global__ __launch_bounds__(kT) void k_gather_seg(const stq_aabb_f64 *__restric
 // As k_chain_end: tiles of rows with their minima in LDS; a row's search
 // stops at the end of its scene.
 template <class T>
-__global__ __launch_bounds__(kCeThreads) void k_chain_end_seg(const T *__restrict__ smin,
+__global__ __launch_bounds__(kCeThreads, 8) void k_chain_end_seg(const T *__restrict__ smin,
                                                               const T *__restrict__ smax,
                                                               int64_t n,
                                                               const int64_t *__restrict__ seg_off,
                                                               int nscenes, int *__restrict__ hi,
                                                               unsigned long long *__restrict__ cand) {
-  __shared__ T win[kCeWin<T>];
+  __shared__ __align__(16) T win[kCeWin<T>];  // 16-byte window stores
   __shared__ int sc_range[2];
   unsigned long long len = 0;
   const int64_t ntiles = (n + kCeRows<T> - 1) / kCeRows<T>;
@@ -269,17 +269,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_seg(const int *__restrict
     const int64_t wbase = win_start[b];
     const int cnt = win_cnt[b];
     const int64_t gbase = win_gbase[b];
-    for (int j = threadIdx.x; j < W; j += kPlanThreads) {
-      K k = ~K(0);
-      if (j < cnt) {
-        if (Q)
-          k = (K)(((uint64_t)group_key(qrec[wbase + j]) << kWinPosBits) | (uint32_t)j);
-        else
-          k = (K)(((uint64_t)(uint32_t)hi[wbase + j] << 32) | (uint32_t)j);
-      }
-      key[j] = k;
-    }
-    __syncthreads();
+    plan_window_keys<Q, K>(hi, qrec, wbase, cnt, W, key);
     if constexpr (Q) {
       if (W >= kPlanThreads)
         window_radix_sort<kPlanThreads>(key, W, rs_cnt, rs_sum);
@@ -288,14 +278,30 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_seg(const int *__restrict
     } else {
       window_sort_any<kPlanThreads, kMaxWindow64>(key, W);
     }
-    const uint32_t jmask = Q ? (1u << kWinPosBits) - 1 : 0xffffffffu;
     const int ng = (cnt + 63) / 64;
+    const bool tails = Q && tail_cost > 0.f;
+    // with qrec, the rows' window offsets go to the radix counts' LDS (free
+    // again) and their chain ends over their keys
+    uint16_t *off16 = reinterpret_cast<uint16_t *>(rs_cnt);
+    int *hend = reinterpret_cast<int *>(key);
+    if constexpr (Q)
+      plan_rows_q(hi, rowf, tails, reinterpret_cast<const uint32_t *>(key), off16, hend, wbase,
+                  cnt, ng, w, lane, rowlist, rowfeat);
+    // one wave per group
     for (int k = w; k < ng; k += kPlanThreads / 64) {
       const int j = 64 * k + lane;
       const bool act = j < cnt;
-      int p = act ? (int)(wbase + (int64_t)((uint32_t)key[j] & jmask)) : INT_MAX;
-      int h = act ? hi[p] : INT_MIN;
-      if (Q && tail_cost > 0.f) {  // the group's rows in chain-end order (for the tail cut)
+      int p = INT_MAX, h = INT_MIN;
+      if (act) {
+        if constexpr (Q) {
+          p = (int)(wbase + off16[j]);
+          h = hend[j];
+        } else {  // the key is (chain end, window offset)
+          p = (int)(wbase + (uint32_t)key[j]);
+          h = (int)(key[j] >> 32);
+        }
+      }
+      if (tails) {  // chain-end order, which only the tail cut needs
         const uint64_t r = wave_sort_u64(act ? ((uint64_t)(uint32_t)h << 32) | (uint32_t)p : ~0ull,
                                          lane);
         if (act) {
@@ -303,7 +309,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_seg(const int *__restrict
           h = (int)(r >> 32);
         }
       }
-      if (act) {
+      if (act && (tails || !Q)) {
         rowlist[wbase + j] = make_int2(p, h);
         if (rowf) rowfeat[wbase + j] = rowf[p];
       }

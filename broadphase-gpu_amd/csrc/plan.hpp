@@ -17,6 +17,7 @@
 #include <climits>
 
 #include "stq_internal.hpp"
+#include "sweep.hpp"
 
 namespace stqamd {
 
@@ -121,6 +122,90 @@ constexpr int kWinPosBits = 12;  // position bits of a 32-bit window key
 // window's sort and its 64 groups, and four blocks fit on a CU (all 733
 // windows of a 3M-box scene run in one round).
 constexpr int kPlanThreads = 512;
+
+// Keys of one planning window into LDS (key[0 .. W)), ending with a barrier:
+// with qrec (Q) (group_key, position), else (chain end, position); entries
+// past cnt are ~0.  A thread's loads are all in flight at once.
+template <bool Q, class K>
+__device__ inline void plan_window_keys(const int *__restrict__ hi, const QRec *__restrict__ qrec,
+                                        int64_t wbase, int cnt, int W, K *key) {
+  constexpr int J = (Q ? kMaxWindow32 : kMaxWindow64) / kPlanThreads;
+  uint32_t a[J], b[J];
+#pragma unroll
+  for (int i = 0; i < J; ++i) {
+    const int j = threadIdx.x + i * kPlanThreads;
+    a[i] = b[i] = 0u;
+    if (j < cnt) {
+      if constexpr (Q) {
+        const QRec r = qrec[wbase + j];
+        a[i] = r.lo;
+        b[i] = r.hi;
+      } else {
+        a[i] = (uint32_t)hi[wbase + j];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < J; ++i) {
+    const int j = threadIdx.x + i * kPlanThreads;
+    if (j < W) {
+      K k = ~K(0);
+      if (j < cnt) {
+        if constexpr (Q) {
+          QRec r;
+          r.lo = a[i];
+          r.hi = b[i];
+          k = (K)(((uint64_t)group_key(r) << kWinPosBits) | (uint32_t)j);
+        } else {
+          k = (K)(((uint64_t)a[i] << 32) | (uint32_t)j);
+        }
+      }
+      key[j] = k;
+    }
+  }
+  __syncthreads();
+}
+
+// Groups of one wave whose row loads are in flight together.
+constexpr int kPlanBatch = 4;
+// Reduced-precision planner, after the window sort (keys (group_key, window
+// offset)): the rows of the wave's groups (k = w, w + 8, ...), their
+// chain-end loads (and the row features, without tails) issued kPlanBatch
+// groups at a time.  Writes rowlist/rowfeat unless tails (the wave sort
+// reorders the rows first), and leaves each row's window offset in off16[j]
+// and chain end in hend[j] for the group plans.  hend[j] is the LDS word of
+// key[j], read before by the same lane.
+__device__ inline void plan_rows_q(const int *__restrict__ hi, const uint2 *__restrict__ rowf,
+                                   bool tails, const uint32_t *key, uint16_t *off16, int *hend,
+                                   int64_t wbase, int cnt, int ng, int w, int lane,
+                                   int2 *__restrict__ rowlist, uint2 *__restrict__ rowfeat) {
+  for (int k0 = w; k0 < ng; k0 += kPlanBatch * (kPlanThreads / 64)) {
+    int o[kPlanBatch], h[kPlanBatch];
+    uint2 f[kPlanBatch];
+#pragma unroll
+    for (int i = 0; i < kPlanBatch; ++i) {
+      const int j = 64 * (k0 + i * (kPlanThreads / 64)) + lane;
+      o[i] = j < cnt ? (int)(key[j] & ((1u << kWinPosBits) - 1)) : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < kPlanBatch; ++i) {
+      h[i] = o[i] >= 0 ? hi[wbase + o[i]] : 0;
+      f[i] = o[i] >= 0 && rowf && !tails ? rowf[wbase + o[i]] : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int i = 0; i < kPlanBatch; ++i) {
+      const int j = 64 * (k0 + i * (kPlanThreads / 64)) + lane;
+      if (o[i] >= 0) {
+        off16[j] = (uint16_t)o[i];
+        hend[j] = h[i];
+        if (!tails) {
+          rowlist[wbase + j] = make_int2((int)(wbase + o[i]), h[i]);
+          if (rowf) rowfeat[wbase + j] = f[i];
+        }
+      }
+    }
+  }
+}
 
 // Bitonic sort, ascending, of W keys (W a power of two) in LDS by the NT
 // threads of the block; ends with a barrier.

@@ -398,9 +398,7 @@ __device__ inline unsigned long long chain_end_tile(const T *__restrict__ smin,
   static_assert(R >= 1 && kCeRows<T> == R * kCeThreads, "whole rows per thread");
   const int64_t w1 = s0 + kCeWin<T> < n ? s0 + kCeWin<T> : n;
   const int nw = (int)(w1 - s0);
-  __syncthreads();  // the previous tile's searches are done with the LDS
-  for (int i = threadIdx.x; i < nw; i += kCeThreads) lds[i] = smin[s0 + i];
-  __syncthreads();
+  // the rows' own loads first: they stay in flight with the window's
   int lo[R], up[R];
   int64_t end[R];
   T x[R];
@@ -418,6 +416,22 @@ __device__ inline unsigned long long chain_end_tile(const T *__restrict__ smin,
       x[r] = smax[s];
     }
   }
+  __syncthreads();  // the previous tile's searches are done with the LDS
+  if (nw == kCeWin<T>) {
+    // a whole window in 16-byte loads, all of a thread's in flight at once
+    // (s0 is a multiple of kCeRows entries, so they are aligned)
+    constexpr int L = kCeWin<T> * (int)sizeof(T) / 16 / kCeThreads;
+    static_assert(L * 16 * kCeThreads == kCeWin<T> * (int)sizeof(T), "whole loads per thread");
+    const uint4 *src = reinterpret_cast<const uint4 *>(smin + s0);
+    uint4 v[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) v[l] = src[threadIdx.x + l * kCeThreads];
+#pragma unroll
+    for (int l = 0; l < L; ++l) reinterpret_cast<uint4 *>(lds)[threadIdx.x + l * kCeThreads] = v[l];
+  } else {
+    for (int i = threadIdx.x; i < nw; i += kCeThreads) lds[i] = smin[s0 + i];
+  }
+  __syncthreads();
   // invariant: lds[lo] <= x, the answer lies in [lo, up)
   while (true) {
     T v[R];
