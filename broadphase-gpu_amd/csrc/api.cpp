@@ -127,7 +127,7 @@ void Context::run(const stq_aabb_f64 *boxes, int64_t n, stq_pair *out, int64_t c
   void *d_smin = smin.get((size_t)n * ts);
   void *d_smax = smax.get((size_t)n * ts);
   void *d_rec = geo.get((size_t)n * (f32 ? sizeof(Exact64) : sizeof(Rec64)));
-  QRec *d_qrec = grouped ? meta.as<QRec>((size_t)n) : nullptr;
+  QRec *d_qrec = grouped ? meta.as<QRec>((size_t)n + kQPad) : nullptr;
   int *d_hi = hi.as<int>((size_t)n);
   ColStats *d_cols = learned ? col_stats.as<ColStats>((size_t)(n + kColBlock - 1) / kColBlock)
                              : nullptr;
@@ -699,7 +699,7 @@ void Context::run_batched(const stq_aabb_f64 *boxes, const int64_t *off_in, int 
   void *d_smin = smin.get((size_t)n * ts);
   void *d_smax = smax.get((size_t)n * ts);
   void *d_rec = geo.get((size_t)n * (f32 ? sizeof(Exact64) : sizeof(Rec64)));
-  QRec *d_qrec = grouped ? meta.as<QRec>((size_t)n) : nullptr;
+  QRec *d_qrec = grouped ? meta.as<QRec>((size_t)n + kQPad) : nullptr;
   uint32_t *d_orig = b_orig.as<uint32_t>((size_t)n);
   int *d_hi = hi.as<int>((size_t)n);
   int2 *d_rowlist = rowlist.as<int2>((size_t)n);

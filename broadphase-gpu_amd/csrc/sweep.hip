@@ -951,7 +951,10 @@ __global__ __launch_bounds__(kThreads, 8) void k_qsweep(SweepArgs a,
     } else {
       const bool act = lane < item.nrows;
       const int2 row = act ? a.rowlist[item.rowoff + lane] : make_int2(INT_MAX, INT_MIN);
-      const int p = row.x, myhi = row.y;
+      // the chain end within the item: the block loads below read past q1
+      // unchecked, into positions of other items (the next chunk of a body,
+      // the group's tail) or the padding
+      const int p = row.x, myhi = min(row.y, item.q1);
       QRec mq;
       mq.lo = 0x7fff7fffu;  // idle lanes: an empty level box that meets no column (make_qrec)
       mq.hi = 0u;
@@ -1031,36 +1034,16 @@ __global__ __launch_bounds__(kThreads, 8) void k_qsweep(SweepArgs a,
       // row, and the rows that pass and hold the position in their chain
       // (p < q <= hi) write (p, q) into the next slots, each at its rank.
       const int q0 = item.q0, q1 = item.q1;
-      uint32_t xlo = 0x7fff7fffu, xhi = 0u;  // past the range: an empty box
-      if (q0 + lane <= q1) {
-        const QRec x = qrec[q0 + lane];
-        xlo = x.lo;
-        xhi = x.hi;
-      }
-      // Lanes past q1 hold the empty box (lo 0x7fff, hi 0 in each half).  It
-      // may pass the union test (when the union spans every level), but never
-      // a row test: that needs a row with hi 0x7fff (kind code set) and lo 0
-      // (code clear), and a row carries the same code in lo and hi.  So the
-      // union test needs no range check of its own.
-      const QRec *qn = qrec + ((int64_t)q0 + kWave + lane);  // this lane's next position
-      for (int qc = q0; qc <= q1; qc += kWave, qn += kWave) {
+      // Every block loads whole: qrec holds kQPad records past its last
+      // position, and a position past q1 (another item's, or padding) fails
+      // every row's chain test (q <= myhi), whatever its levels.
+      // One block: its positions against the union, then each position that
+      // passes against every row.
+      auto block = [&](int qc, uint32_t xlo, uint32_t xhi) __attribute__((always_inline)) {
         uint64_t pm = ballot(level_overlap(uhi, ulo, xlo & ~kCodeBits, xhi & ~kCodeBits));
-        // the next 64 positions load while these are tested (issued after the
-        // wait for these, which waits for every load in flight); all but an
-        // item's last block are whole, and load without a per-lane check
-        uint32_t nlo = 0x7fff7fffu, nhi = 0u;
-        if (qc + 2 * kWave - 1 <= q1) {
-          const QRec x = *qn;
-          nlo = x.lo;
-          nhi = x.hi;
-        } else if (qc + kWave + lane <= q1) {
-          const QRec x = *qn;
-          nlo = x.lo;
-          nhi = x.hi;
-        }
         while (pm) {
           const int j = (int)__builtin_ctzll(pm);
-          pm ^= 1ull << j;
+          asm("s_bitset0_b64 %0, %1" : "+s"(pm) : "s"(j));  // clears bit j
           const uint32_t lo = (uint32_t)readlane_i((int)xlo, j);
           const uint32_t hi = (uint32_t)readlane_i((int)xhi, j);
           uint64_t m = ballot(level_overlap(mq.hi, mq.lo, lo, hi));
@@ -1077,8 +1060,19 @@ __global__ __launch_bounds__(kThreads, 8) void k_qsweep(SweepArgs a,
             }
           }
         }
-        xlo = nlo;
-        xhi = nhi;
+      };
+      // Blocks go in pairs, each block's successor loading while it is tested
+      // (issued after the wait for its own records, which waits for every
+      // load in flight); in pairs, a loaded block's registers are free when
+      // the load after next arrives, so no copies are made.
+      QRec xa = qrec[q0 + lane];
+      const QRec *qn = qrec + ((int64_t)q0 + kWave + lane);  // this lane's next position
+      for (int qc = q0; qc <= q1; qc += 2 * kWave, qn += 2 * kWave) {
+        const QRec xb = qn[0];
+        block(qc, xa.lo, xa.hi);
+        if (qc + kWave > q1) break;
+        xa = qn[kWave];
+        block(qc + kWave, xb.lo, xb.hi);
       }
       if (T) {
         __builtin_amdgcn_wave_barrier();

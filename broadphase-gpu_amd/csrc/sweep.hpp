@@ -21,6 +21,9 @@ struct alignas(16) Rec64 {
 struct alignas(8) QRec {
   uint32_t lo, hi;
 };
+// Records allocated past the last position of a QRec array: the sweep loads
+// every block of 64 positions whole, up to 127 past an item's last one.
+constexpr int kQPad = 128;
 struct alignas(16) Exact64 {
   double lo0, hi0, lo1, hi1, lo2, hi2;
   int v0, v1, v2, type;
