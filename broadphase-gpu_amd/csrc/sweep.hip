@@ -955,6 +955,9 @@ __global__ __launch_bounds__(kThreads, 8) void k_qsweep(SweepArgs a,
       // unchecked, into positions of other items (the next chunk of a body,
       // the group's tail) or the padding
       const int p = row.x, myhi = min(row.y, item.q1);
+      // p < q <= myhi as one unsigned compare: q - (p + 1) < myhi - p
+      const int pp1 = p + 1;
+      const uint32_t span = act && myhi > p ? (uint32_t)(myhi - p) : 0u;
       QRec mq;
       mq.lo = 0x7fff7fffu;  // idle lanes: an empty level box that meets no column (make_qrec)
       mq.hi = 0u;
@@ -1049,7 +1052,7 @@ __global__ __launch_bounds__(kThreads, 8) void k_qsweep(SweepArgs a,
           uint64_t m = ballot(level_overlap(mq.hi, mq.lo, lo, hi));
           if (m) {
             const int q = qc + j;
-            m &= ballot(q > p) & ballot(q <= myhi);
+            m &= ballot((uint32_t)(q - pp1) < span);
             if (m) {
               if (__builtin_amdgcn_inverse_ballot_w64(m))
                 slots[__builtin_amdgcn_mbcnt_hi(
