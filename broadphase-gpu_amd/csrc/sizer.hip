@@ -571,16 +571,28 @@ __global__ __launch_bounds__(64 * kGradWaves) void k_mlp_grad(const float *__res
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int col = lane & 15, quad = lane >> 4;
   float(*tile)[kTileRow] = reinterpret_cast<float(*)[kTileRow]>(lds + kTiles + wv * 16 * kTileRow);
-  // coalesced reads of W1 and W2, scattered into the operand order
-  for (int i = threadIdx.x; i < kHid * kFeat; i += 64 * kGradWaves) {
-    const int j = i / kFeat, k = i % kFeat;
-    lds[kB1 + ((j >> 4) * (kFeat / 4) + (k >> 2)) * 64 + (k & 3) * 16 + (j & 15)] = params[kOffW1 + i];
-  }
-  for (int i = threadIdx.x; i < kHid * kHid; i += 64 * kGradWaves) {
-    const int j = i / kHid, k = i % kHid;
-    const float x = params[kOffW2 + i];
-    lds[kB2 + ((j >> 4) * (kHid / 4) + (k >> 2)) * 64 + (k & 3) * 16 + (j & 15)] = x;
-    lds[kB2t + ((k >> 4) * (kHid / 4) + (j >> 2)) * 64 + (j & 3) * 16 + (k & 15)] = x;
+  // coalesced reads of W1 and W2, every load of a thread in flight at once,
+  // then scattered into the operand order
+  {
+    constexpr int NT = 64 * kGradWaves;
+    constexpr int N1 = kHid * kFeat / NT, N2 = kHid * kHid / NT;
+    static_assert(N1 * NT == kHid * kFeat && N2 * NT == kHid * kHid, "whole loads per thread");
+    float x1[N1], x2[N2];
+#pragma unroll
+    for (int u = 0; u < N1; ++u) x1[u] = params[kOffW1 + threadIdx.x + u * NT];
+#pragma unroll
+    for (int u = 0; u < N2; ++u) x2[u] = params[kOffW2 + threadIdx.x + u * NT];
+#pragma unroll
+    for (int u = 0; u < N1; ++u) {
+      const int i = threadIdx.x + u * NT, j = i / kFeat, k = i % kFeat;
+      lds[kB1 + ((j >> 4) * (kFeat / 4) + (k >> 2)) * 64 + (k & 3) * 16 + (j & 15)] = x1[u];
+    }
+#pragma unroll
+    for (int u = 0; u < N2; ++u) {
+      const int i = threadIdx.x + u * NT, j = i / kHid, k = i % kHid;
+      lds[kB2 + ((j >> 4) * (kHid / 4) + (k >> 2)) * 64 + (k & 3) * 16 + (j & 15)] = x2[u];
+      lds[kB2t + ((k >> 4) * (kHid / 4) + (j >> 2)) * 64 + (j & 3) * 16 + (k & 15)] = x2[u];
+    }
   }
   // b1, then b2, w3 and b3 (W2 skipped)
   for (int i = threadIdx.x; i < kHid + (kParams - kOffB2); i += 64 * kGradWaves)
